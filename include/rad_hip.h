@@ -1,0 +1,333 @@
+/*
+ * rad_hip.h - C ABI of the MI355X-native `path` / `vol_path` integrators and
+ * the scene intersection they call.
+ *
+ * This is the drop-in boundary for the renderer plugin of RadiantiumPlusPlus:
+ *   - RendererFactory("path") / RendererFactory("vol_path")
+ *       module/rad.offline/include/rad/offline/build/factory.h:84-89,
+ *       module/rad.offline/src/renderer/path.cpp:243-254,
+ *       module/rad.offline/src/renderer/vol_path.cpp:356-367
+ *   - the Renderer contract Start/Wait/Stop/IsComplete/AllTaskCount/
+ *     CompletedTaskCount/ElapsedTime/SaveResult
+ *       module/rad.offline/include/rad/offline/render/renderer.h:19-48
+ *   - the Accel plugin (closest hit / occlusion, world bound)
+ *       module/rad.offline/include/rad/offline/render/accel.h:10-30,
+ *       module/rad.offline/src/accel/embree.cpp:70-111
+ *   - the scene build that feeds them (shapes, BSDFs, lights, media, camera)
+ *       module/rad.offline/src/build/build_context.cpp:88-321
+ *
+ * Conventions: plain C types only, no exceptions cross the ABI, every call that
+ * can fail returns rad_status (0 = OK) and leaves a thread-local message in
+ * rad_hip_last_error().  Host buffers are owned by the caller; a scene handle
+ * owns its device copies.  Matrices are row-major 4x4 (the reference's
+ * `to_world` JSON layout, core/config_node.h:36-55).
+ */
+#ifndef RAD_HIP_H_
+#define RAD_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rad_status;
+
+enum {
+  RAD_OK = 0,
+  RAD_ERR_INVALID_ARG = 1,
+  RAD_ERR_HIP = 2,
+  RAD_ERR_OUT_OF_MEMORY = 3,
+  RAD_ERR_UNSUPPORTED = 4,
+  RAD_ERR_IO = 5,
+  RAD_ERR_CANCELLED = 6,
+  RAD_ERR_STATE = 7,
+  RAD_ERR_ADMISSION = 8
+};
+
+/* ---- scene description (host side, plain arrays) ------------------------ */
+
+enum { /* Shape factories: "mesh", "sphere", "rectangle", "cube" (build/factory.cpp:266-269) */
+  RAD_SHAPE_MESH = 0,
+  RAD_SHAPE_SPHERE = 1,
+  RAD_SHAPE_RECTANGLE = 2,
+  RAD_SHAPE_CUBE = 3
+};
+
+/* One scene entity that carries a shape.  Shapes are listed in the order the
+ * reference's BFS over entities creates them (build_context.cpp:137-288); the
+ * index in this list is the shape index / geomID (embree.cpp:34-36). */
+typedef struct rad_shape_desc {
+  int32_t type;            /* RAD_SHAPE_* */
+  int32_t bsdf;            /* index into rad_scene_desc.bsdfs, -1 = none */
+  int32_t light;           /* index into rad_scene_desc.lights, -1 = none */
+  int32_t medium_inside;   /* index into media, -1 = none */
+  int32_t medium_outside;  /* index into media, -1 = none */
+  int32_t flip_normal;     /* rectangle: "is_flip_normal" */
+  float to_world[16];      /* entity to_world, children composed as Child*Parent */
+  /* mesh: object-space vertex data (world transform applied by the consumer,
+   * as Mesh does at construction, shape/mesh.cpp:28-44) */
+  uint32_t vertex_count;
+  uint32_t triangle_count;
+  const float* positions;  /* vertex_count * 3 */
+  const float* normals;    /* vertex_count * 3, optional (NULL) */
+  const float* uvs;        /* vertex_count * 2, optional (NULL) */
+  const uint32_t* indices; /* triangle_count * 3 */
+  /* sphere */
+  float center[3];
+  float radius;
+} rad_shape_desc;
+
+enum { /* BSDF factories (build/factory.cpp:274-283) on the hot path */
+  RAD_BSDF_DIFFUSE = 0,
+  RAD_BSDF_ROUGH_METAL = 1,
+  RAD_BSDF_ROUGH_GLASS = 2,
+  RAD_BSDF_DISNEY = 3
+};
+
+enum { RAD_MICROFACET_GGX = 0, RAD_MICROFACET_BECKMANN = 1 };
+
+/* Disney "has value" flags: the reference enables a lobe only when the JSON
+ * names the parameter with a non-zero value (bsdf/disney.cpp:459-471). */
+enum {
+  RAD_DISNEY_HAS_ANISOTROPIC = 1u << 0,
+  RAD_DISNEY_HAS_SPEC_TRANS = 1u << 1,
+  RAD_DISNEY_HAS_SHEEN = 1u << 2,
+  RAD_DISNEY_HAS_SHEEN_TINT = 1u << 3,
+  RAD_DISNEY_HAS_FLATNESS = 1u << 4,
+  RAD_DISNEY_HAS_SPEC_TINT = 1u << 5,
+  RAD_DISNEY_HAS_METALLIC = 1u << 6,
+  RAD_DISNEY_HAS_CLEARCOAT = 1u << 7,
+  RAD_DISNEY_HAS_ETA = 1u << 8 /* "eta" given (else "specular" is used) */
+};
+
+/* Constant-parameter BSDF record.  Defaults are the reference constructors'. */
+typedef struct rad_bsdf_desc {
+  int32_t type;            /* RAD_BSDF_* */
+  int32_t distribution;    /* RAD_MICROFACET_* (rough_metal, rough_glass) */
+  int32_t sample_visible;  /* visible-normal sampling (default 1) */
+  uint32_t disney_flags;   /* RAD_DISNEY_HAS_* */
+  float reflectance[3];    /* diffuse: 0.5; rough_metal / rough_glass: 1 */
+  float transmittance[3];  /* rough_glass: 1 */
+  float eta[3];            /* rough_metal complex IOR (real) */
+  float k[3];              /* rough_metal complex IOR (imaginary) */
+  float alpha_u, alpha_v;  /* microfacet roughness (default 0.1) */
+  float int_ior, ext_ior;  /* rough_glass (1.5046, 1.000277) */
+  /* disney (bsdf/disney.cpp:62-110) */
+  float base_color[3];
+  float roughness, anisotropic, spec_trans, sheen, sheen_tint, flatness;
+  float spec_tint, metallic, clearcoat, clearcoat_gloss;
+  float disney_eta;        /* used when RAD_DISNEY_HAS_ETA */
+  float specular;          /* used otherwise (default 0.5) */
+  float specular_sampling_rate, clearcoat_sampling_rate, diffuse_sampling_rate;
+} rad_bsdf_desc;
+
+enum { RAD_LIGHT_AREA = 0, RAD_LIGHT_SKYBOX = 1 };
+
+typedef struct rad_light_desc {
+  int32_t type;            /* RAD_LIGHT_* */
+  int32_t shape;           /* area: emitting shape index (set by the loader) */
+  float radiance[3];       /* constant radiance (area default 1, skybox 1) */
+  float to_world[16];      /* skybox orientation */
+} rad_light_desc;
+
+enum { RAD_MEDIUM_HOMOGENEOUS = 0, RAD_MEDIUM_HETEROGENEOUS = 1 };
+enum { RAD_PHASE_ISOTROPIC = 0, RAD_PHASE_HG = 1 };
+enum { RAD_WRAP_CLAMP = 0, RAD_WRAP_REPEAT = 1 };
+
+typedef struct rad_medium_desc {
+  int32_t type;            /* RAD_MEDIUM_* */
+  int32_t phase;           /* RAD_PHASE_* */
+  float g;                 /* hg_phase g, in (-1, 1) */
+  float sigma_t[3];        /* constant sigma_t (homogeneous; heterogeneous without grid) */
+  float albedo[3];         /* constant single-scatter albedo (default 0.75) */
+  float scale;             /* sigma_t scale (default 1) */
+  float to_world[16];      /* heterogeneous: entity transform (data lives in [-1,1]^3) */
+  int32_t grid_size[3];    /* volume_grid dimensions x, y, z (0 = no grid) */
+  int32_t grid_channels;   /* 1 or 3 */
+  int32_t grid_wrap;       /* RAD_WRAP_* */
+  const float* grid;       /* ((z*Y + y)*X + x)*C floats */
+} rad_medium_desc;
+
+typedef struct rad_camera_desc {
+  float to_world[16];      /* camera-to-world (LookAtLeftHand or explicit) */
+  float fov;               /* degrees, horizontal (camera/perspective.cpp:35) */
+  float near_clip;         /* default 0.001 */
+  float far_clip;          /* default 100 */
+  int32_t width, height;
+} rad_camera_desc;
+
+typedef struct rad_scene_desc {
+  uint32_t shape_count;
+  const rad_shape_desc* shapes;
+  uint32_t bsdf_count;
+  const rad_bsdf_desc* bsdfs;
+  uint32_t light_count;
+  const rad_light_desc* lights;
+  uint32_t medium_count;
+  const rad_medium_desc* media;
+  int32_t global_medium;   /* index into media, -1 = none */
+  rad_camera_desc camera;
+} rad_scene_desc;
+
+/* ---- rays and hits --------------------------------------------------------- */
+
+/* 32-byte ray record (two float4): origin, tmin, direction, tmax. */
+typedef struct rad_ray {
+  float o[3];
+  float tmin;
+  float d[3];
+  float tmax;
+} rad_ray;
+
+/* Closest-hit record: geom == 0xffffffff means miss (embree.cpp:96-108). */
+typedef struct rad_hit {
+  float t, u, v;
+  uint32_t prim;
+  uint32_t geom;
+} rad_hit;
+
+/* ---- render job -------------------------------------------------------------- */
+
+enum { RAD_INTEGRATOR_PATH = 0, RAD_INTEGRATOR_VOL_PATH = 1 };
+
+enum {
+  /* Fix the reference's quirks instead of reproducing them (the
+   * Frame::SinCosPhi clamp of frame.cpp:48-56 and the first-channel-only
+   * Inf/negative sample filter of spectrum.h:30-31).  Off by default. */
+  RAD_RENDER_FIX_QUIRKS = 1u << 0,
+  /* Also return per-pixel sum of squares (for variance / z-tests). */
+  RAD_RENDER_WANT_SUMSQ = 1u << 1,
+  /* Time every wavefront kernel with HIP events on the job's stream
+   * (rad_job_stats.kernel_ms). */
+  RAD_RENDER_PROFILE = 1u << 2
+};
+
+enum { /* rad_job_stats.kernel_ms slots */
+  RAD_KERNEL_REGEN = 0,
+  RAD_KERNEL_EXTEND = 1,
+  RAD_KERNEL_SHADE = 2,
+  RAD_KERNEL_SHADOW = 3,
+  RAD_KERNEL_COUNT = 8
+};
+
+typedef struct rad_render_desc {
+  int32_t integrator;      /* RAD_INTEGRATOR_* ("path" / "vol_path") */
+  int32_t max_depth;       /* renderer.max_depth, -1 = unbounded */
+  int32_t rr_depth;        /* renderer.rr_depth (path 3, vol_path 5) */
+  uint32_t spp;            /* sampler.sample_count */
+  uint32_t seed;           /* sampler.seed */
+  uint32_t flags;          /* RAD_RENDER_* */
+  uint32_t slots_per_pixel;/* wavefront slots per pixel, 0 = autotuned */
+  int32_t guiding;         /* learned path guiding: 0 off, 1 on */
+  uint32_t request_id;     /* RNG stream key for serving (0 for single renders) */
+  uint32_t reserved;
+} rad_render_desc;
+
+typedef struct rad_job_stats {
+  double render_ms;        /* device time of the render loop */
+  double bvh_build_ms;     /* last BVH build of the scene */
+  uint64_t samples;        /* camera samples traced */
+  uint64_t segments;       /* extension rays traced (path segments) */
+  uint64_t shadow_rays;    /* NEE rays traced */
+  uint64_t medium_steps;   /* delta-tracking collisions */
+  uint32_t iterations;     /* wavefront iterations */
+  uint32_t slots;          /* wavefront capacity */
+  double kernel_ms[RAD_KERNEL_COUNT]; /* summed kernel time per stage (RAD_RENDER_PROFILE) */
+  uint32_t kernel_launches[RAD_KERNEL_COUNT];
+} rad_job_stats;
+
+typedef struct rad_bvh_opts {
+  int32_t builder;         /* 0 = device (GPU binned SAH), 1 = host (CPU binned SAH) */
+  int32_t max_leaf_size;   /* 0 = default */
+} rad_bvh_opts;
+
+typedef struct rad_scene_info {
+  uint64_t primitive_count;
+  uint64_t node_count;
+  uint64_t device_bytes;
+  double bvh_build_ms;
+  float world_min[3];
+  float world_max[3];
+  uint32_t bvh_depth;
+  float sah_cost;
+} rad_scene_info;
+
+typedef struct rad_ctx rad_ctx;
+typedef struct rad_scene rad_scene;
+typedef struct rad_job rad_job;
+
+/* ---- entry points -------------------------------------------------------------- */
+
+/* Thread-local description of the last failure on this thread. */
+const char* rad_hip_last_error(void);
+
+/* sizeof() of a struct of this header by name ("rad_shape_desc", ...), 0 if
+ * unknown: lets bindings in other languages verify their layouts. */
+size_t rad_abi_sizeof(const char* struct_name);
+
+/* Open a device (RadCoreInit analogue).  Fails with RAD_ERR_HIP when no GPU. */
+rad_status rad_hip_init(int32_t device_ordinal, rad_ctx** out);
+void rad_hip_shutdown(rad_ctx* ctx);
+
+/* Upload a scene (BuildContext::Build minus the accel, build_context.cpp:88-316). */
+rad_status rad_hip_scene_create(rad_ctx* ctx, const rad_scene_desc* desc, rad_scene** out);
+/* Build the acceleration structure (replaces the Embree ctor, embree.cpp:18-43). */
+rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts);
+rad_status rad_hip_scene_info(const rad_scene* scene, rad_scene_info* info);
+void rad_hip_scene_destroy(rad_scene* scene);
+
+/* Closest hit for n rays (Embree::RayIntersectPreliminary, embree.cpp:87-111).
+ * Host pointers; the call copies in, traces and copies out. */
+rad_status rad_hip_intersect_closest(rad_scene* scene, const rad_ray* rays, uint32_t n, rad_hit* hits);
+/* Occlusion for n rays (Embree::RayIntersect, embree.cpp:79-85): 1 = occluded. */
+rad_status rad_hip_intersect_any(rad_scene* scene, const rad_ray* rays, uint32_t n, uint8_t* occluded);
+/* Device-pointer variants on a caller stream (hipStream_t as void*, NULL = default). */
+rad_status rad_hip_intersect_closest_device(rad_scene* scene, const rad_ray* d_rays, uint32_t n,
+                                            rad_hit* d_hits, void* stream);
+rad_status rad_hip_intersect_any_device(rad_scene* scene, const rad_ray* d_rays, uint32_t n,
+                                        uint8_t* d_occluded, void* stream);
+
+/* Asynchronous render (SampleRenderer::Start, sample_renderer.cpp:60-122). */
+rad_status rad_hip_render_start(rad_scene* scene, const rad_render_desc* desc, rad_job** out);
+/* Progress in camera samples (AllTaskCount / CompletedTaskCount). */
+rad_status rad_hip_job_poll(rad_job* job, uint64_t* done, uint64_t* total, int32_t* complete);
+/* Cooperative cancel (Renderer::Stop, sample_renderer.cpp:30-32). */
+rad_status rad_hip_job_cancel(rad_job* job);
+/* Join and read back: mean radiance W*H*3 (x fastest), optional per-pixel sum
+ * of squared sample values (same layout), stats (any may be NULL). */
+rad_status rad_hip_job_wait(rad_job* job, float* fb_rgb, float* fb_sumsq, rad_job_stats* stats);
+void rad_hip_job_destroy(rad_job* job);
+
+/* ---- JSON scene loading (BuildContext::SetFromJson + Build, host side) -------- */
+
+typedef struct rad_loaded_scene rad_loaded_scene;
+/* Parse a reference scene JSON (and its OBJ / mitsuba .vol assets) into a
+ * scene description and a render description. */
+rad_status rad_scene_load_json(const char* path, rad_loaded_scene** out);
+const rad_scene_desc* rad_loaded_scene_desc(const rad_loaded_scene* s);
+const rad_render_desc* rad_loaded_scene_render(const rad_loaded_scene* s);
+/* renderer.type of the JSON ("path", "vol_path", ...) and renderer.device ("hip" default). */
+const char* rad_loaded_scene_renderer_type(const rad_loaded_scene* s);
+const char* rad_loaded_scene_device(const rad_loaded_scene* s);
+void rad_loaded_scene_free(rad_loaded_scene* s);
+
+/* Write an RGB float image as OpenEXR (Renderer::SaveResult, sample_renderer.cpp:34-46). */
+rad_status rad_write_exr(const char* path, const float* rgb, int32_t width, int32_t height);
+
+/* ---- per-function device probes (same device code the integrators run) ------- */
+
+/* Evaluate BSDF records on the device: in = n x {wi[3], wo[3]} (shading space),
+ * out = n x {f[3], pdf}.  quirks = 1 reproduces the reference's SinCosPhi clamp. */
+rad_status rad_hip_bsdf_eval(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out);
+/* Sample: in = n x {wi[3], lobe_xi, xi[2]}, out = n x {wo[3], pdf, f[3], eta}. */
+rad_status rad_hip_bsdf_sample(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out);
+/* Heterogeneous grid lookup (volume/grid.cpp:41-75): in = n x {p[3]} in [0,1]^3, out = n x 3. */
+rad_status rad_hip_grid_eval(const rad_medium_desc* medium, const float* in, uint32_t n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAD_HIP_H_ */

@@ -1,0 +1,373 @@
+// ORACLE / TEST INFRASTRUCTURE ONLY (see ref_math.h).
+// Scene objects of the restatement: interactions, shapes, BSDFs, lights,
+// media, camera, sampler, accel.  Each piece cites the reference code it
+// restates; paths are relative to module/rad.offline/ unless noted.
+#pragma once
+
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "../../include/rad_hip.h"
+#include "ref_math.h"
+
+namespace ref {
+
+constexpr Float Inf = std::numeric_limits<Float>::infinity();
+
+struct Shape;
+struct Medium;
+struct Bsdf;
+struct Light;
+
+// interaction.h:15-45, interaction.cpp:10-30
+struct Interaction {
+  Float T = Inf;
+  V3 P, N;
+  V3 OffsetP(const V3& d) const {
+    Float mag = (1 + MaxCoeff(Abs(P))) * RayEpsilon;
+    mag = MulSign(mag, Dot(N, d));
+    return Fmadd(V3(mag), N, P);
+  }
+  Ray SpawnRay(const V3& d) const { return Ray{OffsetP(d), d, 0, std::numeric_limits<Float>::max()}; }
+  bool IsValid() const { return T != Inf; }
+};
+
+struct PositionSampleResult {
+  V3 P, N;
+  V2 UV;
+  Float Pdf = 0;
+  bool IsDelta = false;
+};
+struct DirectionSampleResult : PositionSampleResult {
+  V3 Dir;
+  Float Dist = 0;
+};
+
+// interaction.h:47-100
+struct SurfaceInteraction : Interaction {
+  const Shape* shape = nullptr;
+  uint32_t ShapeIndex = 0xffffffffu;
+  Frame Shading;
+  V2 UV;
+  V3 dPdU, dPdV, dNdU, dNdV;
+  V3 Wi;
+  void ComputeShadingFrame() {  // interaction.cpp:45-53
+    if (IsZero(dPdU)) {
+      Shading.S = CoordinateSystem(Shading.N).first;
+    } else {
+      Shading.S = Normalized(Shading.N * (-Dot(Shading.N, dPdU)) + dPdU);
+    }
+    Shading.T = Cross(Shading.N, Shading.S);
+  }
+  V3 ToWorld(const V3& v) const { return Shading.ToWorld(v); }
+  V3 ToLocal(const V3& v) const { return Shading.ToLocal(v); }
+  DirectionSampleResult ToDsr(const Interaction& ref) const {  // interaction.cpp:67-82
+    DirectionSampleResult dsr{};
+    dsr.P = P;
+    dsr.N = Shading.N;
+    dsr.UV = UV;
+    dsr.Pdf = 0;
+    dsr.IsDelta = false;
+    V3 rel = P - ref.P;
+    dsr.Dist = Norm(rel);
+    dsr.Dir = IsValid() ? rel / dsr.Dist : -Wi;
+    return dsr;
+  }
+  const Medium* GetMedium(const Ray& ray) const;
+};
+
+struct MediumInteraction : Interaction {  // interaction.h:102-111
+  const Medium* medium = nullptr;
+  Frame Shading;
+  V3 Wi;
+  Float MinT = 0;
+  Spectrum SigmaS, SigmaN, SigmaT, CombinedExtinction;
+};
+
+struct HitShapeRecord {  // interaction.h:113-122
+  Float T = Inf;
+  V3 GeometryNormal;
+  V2 PrimitiveUV;
+  uint32_t PrimitiveIndex = 0;
+  uint32_t ShapeIndex = 0xffffffffu;
+};
+
+// warp.cpp:10-99
+namespace Warp {
+inline V2 SquareToUniformDisk(const V2& u) {
+  Float r = std::sqrt(u.x);
+  Float angle = 2 * PI * u.y;
+  return V2(r * std::cos(angle), r * std::sin(angle));
+}
+inline V3 SquareToCosineHemisphere(const V2& u) {
+  V2 b = SquareToUniformDisk(u);
+  Float z = SafeSqrt(1 - Sqr(b.x) - Sqr(b.y));
+  return V3(b.x, b.y, z);
+}
+inline Float SquareToCosineHemispherePdf(const V3& v) { return v.z * (1 / PI); }
+inline V2 SquareToUniformTriangle(const V2& u) {
+  Float t = SafeSqrt(1 - u.x);
+  return V2(1 - t, t * u.y);
+}
+inline V3 SquareToUniformSphere(const V2& u) {
+  Float z = Fmadd(-2, u.y, 1);
+  Float r = SafeSqrt(Fmadd(-z, z, 1));
+  Float a = 2 * PI * u.x;
+  return V3(r * std::cos(a), r * std::sin(a), z);
+}
+inline Float SquareToUniformSpherePdf() { return 1 / (4 * PI); }
+inline Float SquareToUniformConePdf(Float cosCutoff) { return 1 / (2 * PI) / (1 - cosCutoff); }
+}  // namespace Warp
+
+// distribution.cpp:7-53
+struct DiscreteDistribution1D {
+  std::vector<Float> pmf, cdf;
+  Float sum = 0, normalization = 0;
+  void Build(const std::vector<Float>& data) {
+    pmf = data;
+    cdf.resize(data.size());
+    double s = 0;
+    for (size_t i = 0; i < data.size(); i++) {
+      s += (double)data[i];
+      cdf[i] = (Float)s;
+    }
+    sum = Float(s);
+    normalization = Float(1.0 / s);
+  }
+  size_t Sample(Float xi) const {
+    Float value = xi * sum;
+    return size_t(std::lower_bound(cdf.begin(), cdf.end(), value) - cdf.begin());
+  }
+  std::pair<size_t, Float> SampleReuse(Float xi) const {
+    size_t index = Sample(xi);
+    Float p = pmf[index] * normalization;
+    Float c = index == 0 ? 0 : cdf[index - 1] * normalization;
+    return {index, (xi - c) / p};
+  }
+};
+
+struct Bounds {
+  V3 lo{Inf, Inf, Inf}, hi{-Inf, -Inf, -Inf};
+  void Extend(const V3& p) { lo = Min(lo, p); hi = Max(hi, p); }
+  void Extend(const Bounds& b) { lo = Min(lo, b.lo); hi = Max(hi, b.hi); }
+  V3 Center() const { return (lo + hi) * 0.5f; }
+  Float Area() const {
+    V3 d = hi - lo;
+    if (d.x < 0) return 0;
+    return 2 * (d.x * d.y + d.y * d.z + d.z * d.x);
+  }
+};
+
+// ---- shapes (shape.h:15-101) ----
+struct Shape {
+  Float surfaceArea = 0;
+  const Bsdf* bsdf = nullptr;
+  const Light* light = nullptr;
+  const Medium* inside = nullptr;
+  const Medium* outside = nullptr;
+  uint32_t index = 0;
+  virtual ~Shape() = default;
+  virtual uint32_t PrimCount() const = 0;
+  virtual Bounds PrimBounds(uint32_t prim) const = 0;
+  // Candidate hit against the current [near, far]; returns true and the hit.
+  virtual bool IntersectPrim(uint32_t prim, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const = 0;
+  virtual SurfaceInteraction ComputeInteraction(const Ray& ray, const HitShapeRecord& rec) const = 0;
+  virtual PositionSampleResult SamplePosition(const V2& xi) const = 0;
+  virtual Float PdfPosition(const PositionSampleResult& psr) const = 0;
+  virtual DirectionSampleResult SampleDirection(const Interaction& ref, const V2& xi) const;
+  virtual Float PdfDirection(const Interaction& ref, const DirectionSampleResult& dsr) const;
+  bool IsLight() const { return light != nullptr; }
+  bool HasBsdf() const { return bsdf != nullptr; }
+  bool HasMedia() const { return inside != nullptr || outside != nullptr; }
+};
+
+// Triangle test shared bit-for-bit with the device kernels: the Moller-
+// Trumbore variant Embree 3 applies to its TriangleM primitives
+// (e1 = v0 - v1, e2 = v2 - v0, Ng = e2 x e1; accept den != 0, U, V >= 0,
+// U + V <= |den|, |den|*tnear < T <= |den|*tfar).  No FMA contraction.
+bool IntersectTriangleMT(const V3& v0, const V3& v1, const V3& v2, const Ray& ray, Float tnear, Float tfar,
+                         Float& t, Float& u, Float& v);
+
+struct MeshShape : Shape {  // shape/mesh_base.cpp, shape/mesh.cpp, shape/cube.cpp
+  Transform toWorld;
+  std::vector<V3> position, normal;
+  std::vector<V2> uv;
+  std::vector<uint32_t> indices;
+  uint32_t triangleCount = 0;
+  DiscreteDistribution1D dist;
+  uint32_t PrimCount() const override { return triangleCount; }
+  Bounds PrimBounds(uint32_t prim) const override;
+  bool IntersectPrim(uint32_t prim, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const override;
+  SurfaceInteraction ComputeInteraction(const Ray& ray, const HitShapeRecord& rec) const override;
+  PositionSampleResult SamplePosition(const V2& xi) const override;
+  Float PdfPosition(const PositionSampleResult&) const override { return dist.normalization; }
+  void UpdateDistribution();
+};
+
+struct SphereShape : Shape {  // shape/sphere.cpp
+  V3 center;
+  Float radius = 1;
+  Transform toWorld;
+  uint32_t PrimCount() const override { return 1; }
+  Bounds PrimBounds(uint32_t) const override;
+  bool IntersectPrim(uint32_t prim, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const override;
+  SurfaceInteraction ComputeInteraction(const Ray& ray, const HitShapeRecord& rec) const override;
+  PositionSampleResult SamplePosition(const V2& xi) const override;
+  Float PdfPosition(const PositionSampleResult&) const override { return 1 / surfaceArea; }
+  DirectionSampleResult SampleDirection(const Interaction& ref, const V2& xi) const override;
+  Float PdfDirection(const Interaction& ref, const DirectionSampleResult& dsr) const override;
+};
+
+struct RectangleShape : Shape {  // shape/rectangle.cpp
+  Transform toWorld;
+  Frame frame;
+  float toObject[4][4];
+  uint32_t PrimCount() const override { return 1; }
+  Bounds PrimBounds(uint32_t) const override;
+  bool IntersectPrim(uint32_t prim, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const override;
+  SurfaceInteraction ComputeInteraction(const Ray& ray, const HitShapeRecord& rec) const override;
+  PositionSampleResult SamplePosition(const V2& xi) const override;
+  Float PdfPosition(const PositionSampleResult&) const override { return 1 / surfaceArea; }
+};
+
+// ---- BSDF (render/bsdf.h) ----
+enum BsdfType : uint32_t {
+  BSDF_DIFFUSE = 1, BSDF_GLOSSY = 2, BSDF_DELTA = 4, BSDF_REFLECTION = 8, BSDF_TRANSMISSION = 16,
+  BSDF_NO_DELTA = BSDF_DIFFUSE | BSDF_GLOSSY
+};
+struct BsdfSampleResult {
+  V3 Wo;
+  Float Pdf = 0;
+  Float Eta = 1;
+  uint32_t TypeMask = 0;
+};
+struct Bsdf {
+  uint32_t flags = 0;
+  bool quirks = true;
+  virtual ~Bsdf() = default;
+  bool HasAnyTypeExceptDelta() const { return (flags & BSDF_NO_DELTA) != 0; }
+  virtual std::pair<BsdfSampleResult, Spectrum> Sample(const SurfaceInteraction& si, Float lobeXi, const V2& dirXi) const = 0;
+  virtual Spectrum Eval(const SurfaceInteraction& si, const V3& wo) const = 0;
+  virtual Float Pdf(const SurfaceInteraction& si, const V3& wo) const = 0;
+};
+std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks);
+
+// ---- lights (render/light.h) ----
+struct Scene;
+struct Light {
+  const Shape* shape = nullptr;
+  virtual ~Light() = default;
+  virtual bool IsEnv() const { return false; }
+  virtual Spectrum Eval(const SurfaceInteraction& si) const = 0;
+  virtual std::pair<DirectionSampleResult, Spectrum> SampleDirection(const Interaction& ref, const V2& xi) const = 0;
+  virtual Float PdfDirection(const Interaction& ref, const DirectionSampleResult& dsr) const = 0;
+  virtual void SetScene(const Scene&) {}
+};
+
+// ---- media (render/medium.h, medium.cpp) ----
+struct Medium {
+  int phaseType = RAD_PHASE_ISOTROPIC;
+  Float g = 0.8f;
+  bool isHomogeneous = true;
+  virtual ~Medium() = default;
+  virtual std::tuple<bool, Float, Float> IntersectAABB(const Ray& ray) const = 0;
+  virtual Spectrum GetMajorant(const MediumInteraction& mi) const = 0;
+  virtual std::tuple<Spectrum, Spectrum, Spectrum> GetScatteringCoefficients(const MediumInteraction& mi) const = 0;
+  MediumInteraction SampleInteraction(const Ray& ray, Float sample, uint32_t channel) const;
+  std::pair<Spectrum, Spectrum> EvalTrAndPdf(const MediumInteraction& mi, const SurfaceInteraction& si) const;
+  // phase_function/henyey_greenstein.cpp:18-40, isotropic.cpp:16-24
+  std::pair<V3, Float> PhaseSample(const MediumInteraction& mi, const V2& xi) const;
+  Float PhaseEval(const MediumInteraction& mi, const V3& wo) const;
+};
+
+// volume/grid.cpp:41-103 (sigma_t volume of a heterogeneous medium)
+struct GridVolume {
+  int sx = 0, sy = 0, sz = 0, channels = 1, wrap = RAD_WRAP_CLAMP;
+  std::vector<float> data;
+  Float maxValue = 0;
+  Spectrum Read(int u, int v, int w) const;
+  Spectrum Eval(const V3& p) const;
+};
+
+std::unique_ptr<Medium> MakeMedium(const rad_medium_desc& d);
+
+// ---- camera (camera/perspective.cpp) ----
+struct Camera {
+  Float nearClip = 0.001f, farClip = 100;
+  int width = 1280, height = 720;
+  V2 rcpResolution;
+  Transform cameraToWorld, cameraToClip;
+  void Init(const rad_camera_desc& d);
+  Ray SampleRay(const V2& screenPosition) const;
+};
+
+// ---- sampler (sampler/independent.cpp) ----
+struct Sampler {
+  std::mt19937 engine;
+  std::uniform_real_distribution<Float> dist;
+  explicit Sampler(uint32_t seed) : engine(seed) {}
+  Float Next1D() { return dist(engine); }
+  V2 Next2D() {
+    Float a = dist(engine);
+    Float b = dist(engine);
+    return V2(a, b);
+  }
+};
+
+// ---- accel: binned-SAH BVH over every primitive of every shape ----
+struct PrimRef {
+  uint32_t shape, prim;
+};
+struct Accel {
+  struct Node {
+    Bounds b;
+    int32_t left = -1, right = -1;
+    uint32_t start = 0, count = 0;
+  };
+  std::vector<Node> nodes;
+  std::vector<PrimRef> prims;
+  const std::vector<std::unique_ptr<Shape>>* shapes = nullptr;
+  Bounds world;
+  double buildMs = 0;
+  void Build(const std::vector<std::unique_ptr<Shape>>& shapes);
+  bool Closest(const Ray& ray, HitShapeRecord& rec) const;
+  bool Occluded(const Ray& ray) const;
+  bool ClosestBrute(const Ray& ray, HitShapeRecord& rec) const;
+  bool OccludedBrute(const Ray& ray) const;
+  int BuildRecursive(std::vector<PrimRef>& refs, std::vector<Bounds>& pb, std::vector<V3>& pc, uint32_t begin, uint32_t end, int depth);
+};
+
+// ---- scene (scene.cpp:11-122) ----
+struct Scene {
+  std::vector<std::unique_ptr<Shape>> shapes;
+  std::vector<std::unique_ptr<Bsdf>> bsdfs;
+  std::vector<std::unique_ptr<Light>> lights;
+  std::vector<std::unique_ptr<Medium>> media;
+  const Medium* globalMedium = nullptr;
+  const Light* envLight = nullptr;
+  Float lightPdf = 0;
+  Camera camera;
+  Accel accel;
+  bool quirks = true;
+  bool brute = false;
+
+  void Create(const rad_scene_desc& desc, bool quirks);
+  bool RayIntersect(const Ray& ray) const { return brute ? accel.OccludedBrute(ray) : accel.Occluded(ray); }
+  bool RayIntersect(const Ray& ray, SurfaceInteraction& si) const;
+  bool IsOcclude(const Interaction& ref, const V3& p) const;
+  std::pair<uint32_t, Float> SampleLight(Float xi) const;
+  std::tuple<const Light*, DirectionSampleResult, Spectrum> SampleLightDirection(const Interaction& ref, Float sl, const V2& xi) const;
+  Float PdfLightDirection(const Light* light, const Interaction& ref, const DirectionSampleResult& dsr) const {
+    return light->PdfDirection(ref, dsr) * lightPdf;
+  }
+  const Light* GetLight(const SurfaceInteraction& si) const {
+    return si.shape == nullptr ? envLight : (si.shape->IsLight() ? si.shape->light : nullptr);
+  }
+};
+
+// ---- integrators (renderer/path.cpp, renderer/vol_path.cpp) ----
+Spectrum PathLi(const Ray& ray, const Scene& scene, Sampler& sampler, int maxDepth, int rrDepth);
+Spectrum VolPathLi(const Ray& ray, const Scene& scene, Sampler& sampler, int maxDepth, int rrDepth);
+
+}  // namespace ref

@@ -1,0 +1,803 @@
+// ORACLE / TEST INFRASTRUCTURE ONLY (see ref_math.h).
+// Shapes, lights, camera, scene assembly and the CPU accel of the restatement.
+#include <chrono>
+#include <functional>
+#include <future>
+
+#include "ref_scene.h"
+
+namespace ref {
+
+// ---------------------------------------------------------------- triangles
+bool IntersectTriangleMT(const V3& v0, const V3& v1, const V3& v2, const Ray& ray, Float tnear, Float tfar,
+                         Float& t, Float& u, Float& v) {
+  V3 e1 = v0 - v1;
+  V3 e2 = v2 - v0;
+  V3 Ng = Cross(e2, e1);
+  V3 C = v0 - ray.O;
+  V3 R = Cross(C, ray.D);
+  Float den = Dot(Ng, ray.D);
+  Float absDen = std::abs(den);
+  Float U = Dot(R, e2);
+  Float V = Dot(R, e1);
+  if (den < 0) { U = -U; V = -V; }
+  if (!(den != 0 && U >= 0 && V >= 0 && U + V <= absDen)) return false;
+  Float T = Dot(Ng, C);
+  if (den < 0) T = -T;
+  if (!(absDen * tnear < T && T <= absDen * tfar)) return false;
+  t = T / absDen;
+  u = U / absDen;
+  v = V / absDen;
+  return true;
+}
+
+Bounds MeshShape::PrimBounds(uint32_t prim) const {
+  Bounds b;
+  for (int k = 0; k < 3; k++) b.Extend(position[indices[prim * 3 + k]]);
+  return b;
+}
+
+bool MeshShape::IntersectPrim(uint32_t prim, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const {
+  const V3& p0 = position[indices[prim * 3 + 0]];
+  const V3& p1 = position[indices[prim * 3 + 1]];
+  const V3& p2 = position[indices[prim * 3 + 2]];
+  Float t, u, v;
+  if (!IntersectTriangleMT(p0, p1, p2, ray, near, far, t, u, v)) return false;
+  rec.T = t;
+  rec.PrimitiveUV = V2(u, v);
+  rec.PrimitiveIndex = prim;
+  rec.ShapeIndex = index;
+  rec.GeometryNormal = Cross(p1 - p0, p2 - p0);
+  return true;
+}
+
+// mesh_base.cpp:38-82
+SurfaceInteraction MeshShape::ComputeInteraction(const Ray& ray, const HitShapeRecord& rec) const {
+  uint32_t face = rec.PrimitiveIndex * 3;
+  uint32_t f0 = indices[face + 0], f1 = indices[face + 1], f2 = indices[face + 2];
+  V3 p0 = position[f0], p1 = position[f1], p2 = position[f2];
+  V2 primUV = rec.PrimitiveUV;
+  V3 bary(1.f - primUV.x - primUV.y, primUV.x, primUV.y);
+  V3 dp0 = p1 - p0, dp1 = p2 - p0;
+  SurfaceInteraction si{};
+  si.P = p0 * bary.x + (p1 * bary.y + (p2 * bary.z));
+  si.T = rec.T;
+  si.N = Normalized(Cross(dp0, dp1));
+  si.shape = this;
+  if (uv.empty()) {
+    si.UV = primUV;
+    auto st = CoordinateSystem(si.N);
+    si.dPdU = st.first;
+    si.dPdV = st.second;
+  } else {
+    V2 uv0 = uv[f0], uv1 = uv[f1], uv2 = uv[f2];
+    si.UV = V2(uv0.x * bary.x + (uv1.x * bary.y + (uv2.x * bary.z)), uv0.y * bary.x + (uv1.y * bary.y + (uv2.y * bary.z)));
+    V2 duv0(uv1.x - uv0.x, uv1.y - uv0.y), duv1(uv2.x - uv0.x, uv2.y - uv0.y);
+    Float det = duv0.x * duv1.y - (duv0.y * duv1.x);
+    Float invDet = det == 0 ? 0 : Rcp(det);
+    if (invDet == 0) {
+      si.dPdU = V3();
+      si.dPdV = V3();
+    } else {
+      si.dPdU = (duv1.y * dp0 - (duv0.y * dp1)) * invDet;
+      si.dPdV = (-duv1.x * dp0 + (duv0.x * dp1)) * invDet;
+    }
+  }
+  if (normal.empty()) {
+    si.Shading.N = si.N;
+  } else {
+    V3 n0 = normal[f0], n1 = normal[f1], n2 = normal[f2];
+    V3 shN = n0 * bary.x + (n1 * bary.y + (n2 * bary.z));
+    Float il = Rsqrt(SquaredNorm(shN));
+    shN *= il;
+    si.Shading.N = shN;
+  }
+  return si;
+}
+
+// mesh_base.cpp:84-112
+PositionSampleResult MeshShape::SamplePosition(const V2& xi) const {
+  V2 txi = xi;
+  auto [index, ry] = dist.SampleReuse(txi.y);
+  txi.y = ry;
+  uint32_t face = uint32_t(index) * 3;
+  uint32_t f0 = indices[face + 0], f1 = indices[face + 1], f2 = indices[face + 2];
+  V3 p0 = position[f0], p1 = position[f1], p2 = position[f2];
+  V3 e0 = p1 - p0, e1 = p2 - p0;
+  V2 b = Warp::SquareToUniformTriangle(txi);
+  PositionSampleResult psr{};
+  psr.P = e0 * b.x + (e1 * b.y + p0);
+  psr.Pdf = dist.normalization;
+  psr.IsDelta = false;
+  if (uv.empty()) {
+    psr.UV = b;
+  } else {
+    V2 uv0 = uv[f0], uv1 = uv[f1], uv2 = uv[f2];
+    Float w = 1 - b.x - b.y;
+    psr.UV = V2(uv0.x * w + (uv1.x * b.x + (uv2.x * b.y)), uv0.y * w + (uv1.y * b.x + (uv2.y * b.y)));
+  }
+  if (normal.empty()) {
+    psr.N = Normalized(Cross(e0, e1));
+  } else {
+    V3 n0 = normal[f0], n1 = normal[f1], n2 = normal[f2];
+    psr.N = Normalized(n0 * (1 - b.x - b.y) + (n1 * b.x + (n2 * b.y)));
+  }
+  return psr;
+}
+
+// mesh_base.cpp:120-131
+void MeshShape::UpdateDistribution() {
+  std::vector<Float> area;
+  area.reserve(triangleCount);
+  for (uint32_t i = 0; i < triangleCount; i++) {
+    V3 p0 = position[indices[i * 3 + 0]], p1 = position[indices[i * 3 + 1]], p2 = position[indices[i * 3 + 2]];
+    area.push_back(Norm(Cross(p1 - p0, p2 - p0)) * Float(0.5));
+  }
+  dist.Build(area);
+  surfaceArea = dist.sum;
+}
+
+// shape.cpp:7-25
+DirectionSampleResult Shape::SampleDirection(const Interaction& ref, const V2& xi) const {
+  PositionSampleResult psr = SamplePosition(xi);
+  DirectionSampleResult dsr{};
+  static_cast<PositionSampleResult&>(dsr) = psr;
+  dsr.Dir = dsr.P - ref.P;
+  Float distanceSquared = SquaredNorm(dsr.Dir);
+  dsr.Dist = std::sqrt(distanceSquared);
+  dsr.Dir /= dsr.Dist;
+  Float dp = std::abs(Dot(dsr.Dir, dsr.N));
+  Float x = distanceSquared / dp;
+  dsr.Pdf *= std::isfinite(x) ? x : 0;
+  return dsr;
+}
+
+Float Shape::PdfDirection(const Interaction& ref, const DirectionSampleResult& dsr) const {
+  Float pdf = PdfPosition(dsr);
+  Float dp = std::abs(Dot(dsr.Dir, dsr.N));
+  pdf *= dp == 0 ? 0 : Sqr(dsr.Dist) / dp;
+  return pdf;
+}
+
+// ---------------------------------------------------------------- sphere
+static std::tuple<bool, Float, Float> SolveQuadratic(Float a, Float b, Float c) {  // math_ext.h:79-97
+  bool linearCase = a == 0.0f;
+  bool validLinear = linearCase && b != 0.0f;
+  Float x0 = -c / b, x1 = -c / b;
+  Float discrim = b * b - Float(4) * a * c;
+  bool validQuadratic = !linearCase && (discrim >= 0.0f);
+  if (validQuadratic) {
+    Float rootDiscrim = std::sqrt(discrim);
+    Float temp = Float(-0.5) * (b + std::copysign(rootDiscrim, b));
+    Float x0p = temp / a;
+    Float x1p = c / temp;
+    Float x0m = std::min(x0p, x1p);
+    Float x1m = std::max(x0p, x1p);
+    x0 = linearCase ? x0 : x0m;
+    x1 = linearCase ? x0 : x1m;
+  }
+  return {validLinear || validQuadratic, x0, x1};
+}
+
+Bounds SphereShape::PrimBounds(uint32_t) const {
+  Bounds b;
+  b.Extend(center - V3(radius));
+  b.Extend(center + V3(radius));
+  return b;
+}
+
+// sphere.cpp:40-54, 71-120
+bool SphereShape::IntersectPrim(uint32_t, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const {
+  V3 o = ray.O - center;
+  Float a = SquaredNorm(ray.D);
+  Float b = 2 * Dot(o, ray.D);
+  Float c = SquaredNorm(o) - Sqr(radius);
+  auto [found, nearT, farT] = SolveQuadratic(a, b, c);
+  bool outBounds = !(nearT <= far && farT >= near);
+  bool inBounds = nearT < near && farT > far;
+  bool isHit = found && !outBounds && !inBounds;
+  if (!isHit) return false;
+  Float t = nearT < 0 ? farT : nearT;
+  rec.T = t;
+  rec.PrimitiveUV = V2(0, 0);
+  rec.PrimitiveIndex = 0;
+  rec.ShapeIndex = index;
+  rec.GeometryNormal = Normalized((ray.D * t + ray.O) - center);
+  return true;
+}
+
+// UnitAngleZ (math_ext.h:101-104)
+static Float UnitAngleZ(const V3& v) {
+  Float temp = std::asin(Float(0.5) * Norm(V3(v.x, v.y, v.z - MulSign(Float(1), v.z)))) * 2;
+  return v.z >= 0 ? temp : PI - temp;
+}
+
+// sphere.cpp:215-249
+SurfaceInteraction SphereShape::ComputeInteraction(const Ray&, const HitShapeRecord& rec) const {
+  SurfaceInteraction si{};
+  si.T = rec.T;
+  si.Shading.N = rec.GeometryNormal;
+  si.P = Fmadd(si.Shading.N, V3(radius), center);
+  V3 local = toWorld.ApplyAffineToLocal(si.P);
+  Float rd2 = Sqr(local.x) + Sqr(local.y);
+  Float theta = UnitAngleZ(local);
+  Float phi = std::atan2(local.y, local.x);
+  if (phi < 0) phi += 2 * PI;
+  si.UV = V2(phi / (2 * PI), theta / PI);
+  si.dPdU = V3(-local.y, local.x, 0.f);
+  Float rd = std::sqrt(rd2);
+  Float invRd = Rcp(rd);
+  Float cosPhi = local.x * invRd;
+  Float sinPhi = local.y * invRd;
+  si.dPdV = V3(local.z * cosPhi, local.z * sinPhi, -rd);
+  if (rd == 0) si.dPdV = V3(1, 0, 0);
+  si.dPdU = toWorld.ApplyLinearToWorld(si.dPdU * (2 * PI));
+  si.dPdV = toWorld.ApplyLinearToWorld(si.dPdV * PI);
+  si.N = si.Shading.N;
+  si.shape = this;
+  return si;
+}
+
+PositionSampleResult SphereShape::SamplePosition(const V2& xi) const {  // sphere.cpp:251-259
+  V3 local = Warp::SquareToUniformSphere(xi);
+  PositionSampleResult psr{};
+  psr.P = Fmadd(local, V3(radius), center);
+  psr.N = local;
+  psr.IsDelta = radius == 0;
+  psr.Pdf = 1 / surfaceArea;
+  return psr;
+}
+
+// sphere.cpp:265-320
+DirectionSampleResult SphereShape::SampleDirection(const Interaction& ref, const V2& xi) const {
+  V3 dcv = center - ref.P;
+  Float dc2 = SquaredNorm(dcv);
+  Float radiusAdj = radius * (Float(1) - RayEpsilon);
+  bool isOut = dc2 > Sqr(radiusAdj);
+  DirectionSampleResult dsr{};
+  if (isOut) {
+    Float invDc = Rsqrt(dc2);
+    Float sinThetaMax = radius * invDc;
+    Float sinThetaMax2 = Sqr(sinThetaMax);
+    Float invSinThetaMax = Rcp(sinThetaMax);
+    Float cosThetaMax = SafeSqrt(1.f - sinThetaMax2);
+    Float sinTheta2 = sinThetaMax2 > Float(0.00068523262) ? Float(1) - Sqr(Fmadd(cosThetaMax - Float(1), xi.x, Float(1)))
+                                                          : sinThetaMax2 * xi.x;
+    Float cosTheta = SafeSqrt(Float(1) - sinTheta2);
+    Float cosAlpha = sinTheta2 * invSinThetaMax + cosTheta * SafeSqrt(Fmadd(-sinTheta2, Sqr(invSinThetaMax), Float(1)));
+    Float sinAlpha = SafeSqrt(Fmadd(-cosAlpha, cosAlpha, Float(1)));
+    Float ang = xi.y * (Float(2) * PI);
+    Float sinPhi = std::sin(ang), cosPhi = std::cos(ang);
+    V3 d = Frame(dcv * -invDc).ToWorld(V3(cosPhi * sinAlpha, sinPhi * sinAlpha, cosAlpha));
+    dsr.P = Fmadd(d, V3(radius), center);
+    dsr.N = d;
+    dsr.Dir = dsr.P - ref.P;
+    Float dist2 = SquaredNorm(dsr.Dir);
+    dsr.Dist = std::sqrt(dist2);
+    dsr.Dir = dsr.Dir / dsr.Dist;
+    dsr.Pdf = Warp::SquareToUniformConePdf(cosThetaMax);
+  } else {
+    V3 d = Warp::SquareToUniformSphere(xi);
+    dsr.P = Fmadd(d, V3(radius), center);
+    dsr.N = d;
+    dsr.Dir = dsr.P - ref.P;
+    Float dist2 = SquaredNorm(dsr.Dir);
+    dsr.Dist = std::sqrt(dist2);
+    dsr.Dir = dsr.Dir / dsr.Dist;
+    dsr.Pdf = (1 / surfaceArea) * dist2 / AbsDot(dsr.Dir, dsr.N);
+  }
+  dsr.IsDelta = radius == 0;
+  return dsr;
+}
+
+Float SphereShape::PdfDirection(const Interaction& ref, const DirectionSampleResult& dsr) const {  // sphere.cpp:322-328
+  Float sinAlpha = radius * Rcp(Norm(center - ref.P));
+  Float cosAlpha = SafeSqrt(Float(1) - sinAlpha * sinAlpha);
+  return sinAlpha < OneMinusEpsilon ? Warp::SquareToUniformConePdf(cosAlpha)
+                                    : (1 / surfaceArea) * Sqr(dsr.Dist) / AbsDot(dsr.Dir, dsr.N);
+}
+
+// ---------------------------------------------------------------- rectangle
+Bounds RectangleShape::PrimBounds(uint32_t) const {  // rectangle.cpp:48-64
+  Bounds b;
+  b.Extend(toWorld.ApplyAffineToWorld(V3(-1, -1, 0)));
+  b.Extend(toWorld.ApplyAffineToWorld(V3(1, -1, 0)));
+  b.Extend(toWorld.ApplyAffineToWorld(V3(1, 1, 0)));
+  b.Extend(toWorld.ApplyAffineToWorld(V3(-1, 1, 0)));
+  return b;
+}
+
+// rectangle.cpp:34-46: ray taken to object space with an unnormalised direction.
+bool RectangleShape::IntersectPrim(uint32_t, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const {
+  const auto& M = toObject;
+  V3 o(M[0][0] * ray.O.x + M[0][1] * ray.O.y + M[0][2] * ray.O.z + M[0][3],
+       M[1][0] * ray.O.x + M[1][1] * ray.O.y + M[1][2] * ray.O.z + M[1][3],
+       M[2][0] * ray.O.x + M[2][1] * ray.O.y + M[2][2] * ray.O.z + M[2][3]);
+  V3 d(M[0][0] * ray.D.x + M[0][1] * ray.D.y + M[0][2] * ray.D.z,
+       M[1][0] * ray.D.x + M[1][1] * ray.D.y + M[1][2] * ray.D.z,
+       M[2][0] * ray.D.x + M[2][1] * ray.D.y + M[2][2] * ray.D.z);
+  Float t = -o.z / d.z;
+  V3 local = d * t + o;
+  bool isHit = t >= near && t <= far && std::abs(local.x) <= 1 && std::abs(local.y) <= 1;
+  if (!isHit) return false;
+  rec.T = t;
+  rec.PrimitiveUV = V2(local.x, local.y);
+  rec.PrimitiveIndex = 0;
+  rec.ShapeIndex = index;
+  rec.GeometryNormal = frame.N;
+  return true;
+}
+
+SurfaceInteraction RectangleShape::ComputeInteraction(const Ray& ray, const HitShapeRecord& rec) const {  // :204-219
+  SurfaceInteraction si{};
+  si.T = rec.T;
+  V3 p = ray(rec.T);
+  si.P = Dot(toWorld.TranslationToWorld() - p, frame.N) * frame.N + p;
+  si.N = frame.N;
+  si.Shading.N = frame.N;
+  si.dPdU = frame.S;
+  si.dPdV = frame.T;
+  si.UV = V2(rec.PrimitiveUV.x * Float(0.5) + Float(0.5), rec.PrimitiveUV.y * Float(0.5) + Float(0.5));
+  si.shape = this;
+  return si;
+}
+
+PositionSampleResult RectangleShape::SamplePosition(const V2& xi) const {  // :221-230
+  PositionSampleResult psr{};
+  V2 s(xi.x * 2 - 1, xi.y * 2 - 1);
+  psr.P = toWorld.ApplyAffineToWorld(V3(s.x, s.y, 0));
+  psr.N = frame.N;
+  psr.Pdf = 1 / surfaceArea;
+  psr.UV = xi;
+  psr.IsDelta = false;
+  return psr;
+}
+
+// ---------------------------------------------------------------- lights
+struct AreaLight final : Light {  // light/area.cpp:20-91 (uniform radiance)
+  Spectrum radiance;
+  Spectrum Eval(const SurfaceInteraction& si) const override {
+    return Dot(si.ToWorld(si.Wi), si.N) >= 0 ? radiance : Spectrum(0);
+  }
+  std::pair<DirectionSampleResult, Spectrum> SampleDirection(const Interaction& ref, const V2& xi) const override {
+    DirectionSampleResult dsr = shape->SampleDirection(ref, xi);
+    Spectrum r = Dot(-dsr.Dir, dsr.N) >= 0 ? radiance : Spectrum(0);
+    return {dsr, r};
+  }
+  Float PdfDirection(const Interaction& ref, const DirectionSampleResult& dsr) const override {
+    return Dot(-dsr.Dir, dsr.N) >= 0 ? shape->PdfDirection(ref, dsr) : 0;
+  }
+};
+
+struct SkyboxLight final : Light {  // light/skybox.cpp:24-141 (constant map)
+  Spectrum radiance;
+  Transform toWorld;
+  V3 sphereCenter;
+  Float sphereRadius = 0;
+  bool IsEnv() const override { return true; }
+  Spectrum Eval(const SurfaceInteraction&) const override { return radiance; }
+  std::pair<DirectionSampleResult, Spectrum> SampleDirection(const Interaction& ref, const V2& xi) const override {
+    Float radius = std::max(sphereRadius, Norm(ref.P - sphereCenter));
+    Float dist = 2 * radius;
+    DirectionSampleResult dsr{};
+    V3 dir = Normalized(Warp::SquareToUniformSphere(xi));
+    V3 worldDir = toWorld.ApplyLinearToWorld(dir);
+    dsr.P = worldDir * dist + ref.P;
+    dsr.N = -worldDir;
+    dsr.Pdf = Warp::SquareToUniformSpherePdf();
+    dsr.Dir = worldDir;
+    dsr.Dist = dist;
+    dsr.IsDelta = false;
+    return {dsr, radiance};
+  }
+  Float PdfDirection(const Interaction&, const DirectionSampleResult&) const override {
+    return Warp::SquareToUniformSpherePdf();
+  }
+  void SetScene(const Scene& scene) override {  // skybox.cpp:136-141, bounding_sphere.cpp:15-18
+    Bounds b = scene.accel.world;
+    V3 c = b.Center();
+    sphereCenter = c;
+    sphereRadius = std::max(RayEpsilon, Norm(c - b.hi) * (1 + ShadowEpsilon));
+  }
+};
+
+// ---------------------------------------------------------------- camera
+void Camera::Init(const rad_camera_desc& d) {  // perspective.cpp:15-70
+  nearClip = d.near_clip;
+  farClip = d.far_clip;
+  width = d.width;
+  height = d.height;
+  rcpResolution = V2(1.f / width, 1.f / height);
+  cameraToWorld = Transform(Mat4::FromRowMajor(d.to_world));
+  Float aspect = width / static_cast<Float>(height);
+  Float recip = 1 / (farClip - nearClip);
+  Float cot = 1 / std::tan(Radian(d.fov / 2));
+  Mat4 perspective, scale, translate;
+  Float P[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, farClip * recip, -nearClip * farClip * recip, 0, 0, 1, 0};
+  Float S[16] = {-0.5f, 0, 0, 0, 0, -0.5f * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  Float T[16] = {1, 0, 0, -1, 0, 1, 0, -1 / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
+  perspective = Mat4::FromRowMajor(P);
+  scale = Mat4::FromRowMajor(S);
+  translate = Mat4::FromRowMajor(T);
+  cameraToClip = Transform(scale * translate * perspective);
+}
+
+Ray Camera::SampleRay(const V2& sp) const {  // perspective.cpp:84-99 (differentials unused)
+  V3 ndc(sp.x * rcpResolution.x, sp.y * rcpResolution.y, 0);
+  V3 nearP = cameraToClip.ApplyAffineToLocal(ndc);
+  V3 dir = Normalized(nearP);
+  Float invZ = Rcp(dir.z);
+  V3 o = cameraToWorld.ApplyAffineToWorld(V3(0, 0, 0));
+  V3 d = Normalized(cameraToWorld.ApplyLinearToWorld(dir));
+  return Ray{o, d, nearClip * invZ, farClip * invZ};
+}
+
+// ---------------------------------------------------------------- scene
+const Medium* SurfaceInteraction::GetMedium(const Ray& ray) const {  // interaction.cpp:113-115
+  return Dot(ray.D, N) > 0 ? shape->outside : shape->inside;
+}
+
+static const float kCubeV[24][3] = {{1, -1, -1}, {1, -1, 1}, {-1, -1, 1}, {-1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                    {-1, 1, 1},  {1, 1, 1},  {1, -1, -1}, {1, 1, -1},   {1, 1, 1},   {1, -1, 1},
+                                    {1, -1, 1},  {1, 1, 1},  {-1, 1, 1},  {-1, -1, 1},  {-1, -1, 1}, {-1, 1, 1},
+                                    {-1, 1, -1}, {-1, -1, -1}, {1, 1, -1}, {1, -1, -1}, {-1, -1, -1}, {-1, 1, -1}};
+static const float kCubeN[6][3] = {{0, -1, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {-1, 0, 0}, {0, 0, -1}};
+static const uint32_t kCubeT[12][3] = {{0, 1, 2},    {3, 0, 2},    {4, 5, 6},    {7, 4, 6},    {8, 9, 10},   {11, 8, 10},
+                                       {12, 13, 14}, {15, 12, 14}, {16, 17, 18}, {19, 16, 18}, {20, 21, 22}, {23, 20, 22}};
+
+void Scene::Create(const rad_scene_desc& desc, bool q) {
+  quirks = q;
+  for (uint32_t i = 0; i < desc.bsdf_count; i++) bsdfs.push_back(MakeBsdf(desc.bsdfs[i], quirks));
+  for (uint32_t i = 0; i < desc.medium_count; i++) media.push_back(MakeMedium(desc.media[i]));
+  globalMedium = desc.global_medium >= 0 ? media[desc.global_medium].get() : nullptr;
+  for (uint32_t i = 0; i < desc.light_count; i++) {
+    const rad_light_desc& ld = desc.lights[i];
+    if (ld.type == RAD_LIGHT_AREA) {
+      auto l = std::make_unique<AreaLight>();
+      l->radiance = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
+      lights.push_back(std::move(l));
+    } else {
+      auto l = std::make_unique<SkyboxLight>();
+      l->radiance = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
+      l->toWorld = Transform(Mat4::FromRowMajor(ld.to_world));
+      lights.push_back(std::move(l));
+    }
+  }
+  for (uint32_t i = 0; i < desc.shape_count; i++) {
+    const rad_shape_desc& sd = desc.shapes[i];
+    Mat4 tw = Mat4::FromRowMajor(sd.to_world);
+    std::unique_ptr<Shape> s;
+    if (sd.type == RAD_SHAPE_MESH || sd.type == RAD_SHAPE_CUBE) {
+      auto m = std::make_unique<MeshShape>();
+      m->toWorld = Transform(tw);
+      bool ident = m->toWorld.IsIdentity();
+      if (sd.type == RAD_SHAPE_MESH) {
+        m->triangleCount = sd.triangle_count;
+        m->indices.assign(sd.indices, sd.indices + 3 * size_t(sd.triangle_count));
+        m->position.resize(sd.vertex_count);
+        for (uint32_t v = 0; v < sd.vertex_count; v++) {
+          V3 p(sd.positions[3 * v], sd.positions[3 * v + 1], sd.positions[3 * v + 2]);
+          m->position[v] = ident ? p : m->toWorld.ApplyAffineToWorld(p);
+        }
+        if (sd.normals) {
+          m->normal.resize(sd.vertex_count);
+          for (uint32_t v = 0; v < sd.vertex_count; v++) {
+            V3 n(sd.normals[3 * v], sd.normals[3 * v + 1], sd.normals[3 * v + 2]);
+            m->normal[v] = ident ? n : m->toWorld.ApplyNormalToWorld(n);
+          }
+        }
+        if (sd.uvs) {
+          m->uv.resize(sd.vertex_count);
+          for (uint32_t v = 0; v < sd.vertex_count; v++) m->uv[v] = V2(sd.uvs[2 * v], sd.uvs[2 * v + 1]);
+        }
+      } else {  // cube.cpp:15-123; UVs are never initialised there, taken as zero here
+        m->triangleCount = 12;
+        m->position.resize(24);
+        m->normal.resize(24);
+        m->uv.assign(24, V2(0, 0));
+        for (int v = 0; v < 24; v++) {
+          m->position[v] = m->toWorld.ApplyAffineToWorld(V3(kCubeV[v][0], kCubeV[v][1], kCubeV[v][2]));
+          const float* n = kCubeN[v / 4];
+          m->normal[v] = m->toWorld.ApplyNormalToWorld(V3(n[0], n[1], n[2]));
+        }
+        for (int t = 0; t < 12; t++)
+          for (int k = 0; k < 3; k++) m->indices.push_back(kCubeT[t][k]);
+      }
+      m->UpdateDistribution();
+      s = std::move(m);
+    } else if (sd.type == RAD_SHAPE_SPHERE) {  // sphere.cpp:173-190
+      auto sp = std::make_unique<SphereShape>();
+      Transform transform(tw);
+      if (transform.HasNonUniformScale()) throw std::runtime_error("Sphere does not support non uniform transformations");
+      sp->center = transform.ApplyAffineToWorld(V3(sd.center[0], sd.center[1], sd.center[2]));
+      sp->radius = Norm(transform.ApplyLinearToWorld(V3(0, 0, sd.radius)));
+      Float TS[16] = {sp->radius, 0, 0, sp->center.x, 0, sp->radius, 0, sp->center.y, 0, 0, sp->radius, sp->center.z, 0, 0, 0, 1};
+      sp->toWorld = Transform(Mat4::FromRowMajor(TS) * tw);
+      sp->surfaceArea = 4 * PI * Sqr(sp->radius);
+      s = std::move(sp);
+    } else if (sd.type == RAD_SHAPE_RECTANGLE) {  // rectangle.cpp:164-179
+      auto r = std::make_unique<RectangleShape>();
+      if (sd.flip_normal) {
+        Float F[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, -1, 0, 0, 0, 0, 1};
+        r->toWorld = Transform(tw * Mat4::FromRowMajor(F));
+      } else {
+        r->toWorld = Transform(tw);
+      }
+      V3 dpdu = r->toWorld.ApplyLinearToWorld(V3(2, 0, 0));
+      V3 dpdv = r->toWorld.ApplyLinearToWorld(V3(0, 2, 0));
+      V3 n = Normalized(r->toWorld.ApplyNormalToWorld(V3(0, 0, 1)));
+      r->frame = Frame(dpdu, dpdv, n);
+      r->surfaceArea = Norm(Cross(dpdu, dpdv));
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) r->toObject[a][b] = r->toWorld.ToLocal.m[a][b];
+      s = std::move(r);
+    } else {
+      throw std::runtime_error("unknown shape type");
+    }
+    s->index = i;
+    if (sd.bsdf >= 0) s->bsdf = bsdfs[sd.bsdf].get();
+    if (sd.light >= 0) {
+      s->light = lights[sd.light].get();
+      lights[sd.light]->shape = s.get();
+    }
+    s->inside = sd.medium_inside >= 0 ? media[sd.medium_inside].get() : nullptr;
+    s->outside = sd.medium_outside >= 0 ? media[sd.medium_outside].get() : nullptr;
+    shapes.push_back(std::move(s));
+  }
+  camera.Init(desc.camera);
+  accel.Build(shapes);
+  lightPdf = Rcp(Float(lights.size()));
+  for (auto& l : lights) {
+    if (l->IsEnv()) {
+      envLight = l.get();
+      l->SetScene(*this);
+    }
+  }
+}
+
+bool Scene::RayIntersect(const Ray& ray, SurfaceInteraction& si) const {  // scene.cpp:42-51
+  HitShapeRecord rec;
+  bool hit = brute ? accel.ClosestBrute(ray, rec) : accel.Closest(ray, rec);
+  if (!hit) {
+    si = {};
+    si.Wi = -ray.D;
+    return false;
+  }
+  si = shapes[rec.ShapeIndex]->ComputeInteraction(ray, rec);  // interaction.cpp:121-127
+  si.ComputeShadingFrame();
+  si.Wi = si.ToLocal(-ray.D);
+  si.ShapeIndex = rec.ShapeIndex;
+  return true;
+}
+
+bool Scene::IsOcclude(const Interaction& ref, const V3& p) const {  // scene.cpp:53-63
+  V3 o = ref.OffsetP(p - ref.P);
+  V3 d = p - o;
+  Float dist = Norm(d);
+  if (dist < ShadowEpsilon) return true;
+  d /= dist;
+  Ray shadowRay{o, d, 0, dist * (1 - ShadowEpsilon)};
+  return RayIntersect(shadowRay);
+}
+
+std::pair<uint32_t, Float> Scene::SampleLight(Float xi) const {  // scene.cpp:65-77
+  if (lights.size() < 2) {
+    if (lights.size() == 1) return {0u, Float(1)};
+    return {0xffffffffu, Float(0)};
+  }
+  uint32_t lightCount = (uint32_t)lights.size();
+  Float lightIndex = xi * lightCount;
+  uint32_t index = std::min(uint32_t(lightIndex), lightCount - 1);
+  return {index, lightPdf};
+}
+
+std::tuple<const Light*, DirectionSampleResult, Spectrum> Scene::SampleLightDirection(const Interaction& ref, Float sl,
+                                                                                       const V2& xi) const {
+  auto [index, pdf] = SampleLight(sl);
+  const Light* light = lights[index].get();
+  auto [dsr, li] = light->SampleDirection(ref, xi);
+  dsr.Pdf *= pdf;
+  return {light, dsr, li};
+}
+
+// ---------------------------------------------------------------- accel
+static inline bool SlabHit(const Bounds& b, const V3& o, const V3& invD, Float tmin, Float tmax) {
+  Float t0 = tmin, t1 = tmax;
+  for (int a = 0; a < 3; a++) {
+    Float tn = (b.lo[a] - o[a]) * invD[a];
+    Float tf = (b.hi[a] - o[a]) * invD[a];
+    if (tn > tf) std::swap(tn, tf);
+    tf *= 1 + 2 * 1.2e-7f * 3;  // robust slab test (Ize 2013)
+    t0 = tn > t0 ? tn : t0;
+    t1 = tf < t1 ? tf : t1;
+    if (!(t0 <= t1)) return false;
+  }
+  return true;
+}
+
+int Accel::BuildRecursive(std::vector<PrimRef>& refs, std::vector<Bounds>& pb, std::vector<V3>& pc, uint32_t begin,
+                          uint32_t end, int depth) {
+  Node node;
+  for (uint32_t i = begin; i < end; i++) node.b.Extend(pb[i]);
+  uint32_t n = end - begin;
+  int idx;
+  {
+    idx = (int)nodes.size();
+    nodes.push_back(node);
+  }
+  if (n <= 4 || depth > 60) {
+    nodes[idx].start = begin;
+    nodes[idx].count = n;
+    return idx;
+  }
+  Bounds cb;
+  for (uint32_t i = begin; i < end; i++) cb.Extend(pc[i]);
+  int bestAxis = -1, bestSplit = -1;
+  Float bestCost = Inf;
+  constexpr int B = 16;
+  for (int a = 0; a < 3; a++) {
+    Float lo = cb.lo[a], hi = cb.hi[a];
+    if (!(hi > lo)) continue;
+    Bounds bb[B];
+    int cnt[B] = {0};
+    Float scale = B / (hi - lo);
+    for (uint32_t i = begin; i < end; i++) {
+      int k = std::min(B - 1, int((pc[i][a] - lo) * scale));
+      bb[k].Extend(pb[i]);
+      cnt[k]++;
+    }
+    Bounds left[B];
+    int lc[B];
+    Bounds acc;
+    int c = 0;
+    for (int k = 0; k < B; k++) {
+      acc.Extend(bb[k]);
+      c += cnt[k];
+      left[k] = acc;
+      lc[k] = c;
+    }
+    acc = Bounds();
+    c = 0;
+    for (int k = B - 1; k > 0; k--) {
+      acc.Extend(bb[k]);
+      c += cnt[k];
+      Float cost = left[k - 1].Area() * lc[k - 1] + acc.Area() * c;
+      if (lc[k - 1] > 0 && c > 0 && cost < bestCost) {
+        bestCost = cost;
+        bestAxis = a;
+        bestSplit = k;
+      }
+    }
+  }
+  uint32_t mid;
+  if (bestAxis < 0) {
+    mid = begin + n / 2;
+  } else {
+    Float lo = cb.lo[bestAxis];
+    Float scale = B / (cb.hi[bestAxis] - lo);
+    uint32_t i = begin, j = end;
+    while (i < j) {
+      int k = std::min(B - 1, int((pc[i][bestAxis] - lo) * scale));
+      if (k < bestSplit) {
+        i++;
+      } else {
+        j--;
+        std::swap(refs[i], refs[j]);
+        std::swap(pb[i], pb[j]);
+        std::swap(pc[i], pc[j]);
+      }
+    }
+    mid = i;
+    if (mid == begin || mid == end) mid = begin + n / 2;
+  }
+  int l = BuildRecursive(refs, pb, pc, begin, mid, depth + 1);
+  int r = BuildRecursive(refs, pb, pc, mid, end, depth + 1);
+  nodes[idx].left = l;
+  nodes[idx].right = r;
+  return idx;
+}
+
+void Accel::Build(const std::vector<std::unique_ptr<Shape>>& s) {
+  auto t0 = std::chrono::steady_clock::now();
+  shapes = &s;
+  std::vector<PrimRef> refs;
+  std::vector<Bounds> pb;
+  std::vector<V3> pc;
+  for (auto& sh : s) {
+    for (uint32_t p = 0; p < sh->PrimCount(); p++) {
+      refs.push_back({sh->index, p});
+      Bounds b = sh->PrimBounds(p);
+      pb.push_back(b);
+      pc.push_back(b.Center());
+      world.Extend(b);
+    }
+  }
+  nodes.clear();
+  nodes.reserve(2 * refs.size() / 2 + 16);
+  if (!refs.empty()) BuildRecursive(refs, pb, pc, 0, (uint32_t)refs.size(), 0);
+  prims = std::move(refs);
+  buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+bool Accel::Closest(const Ray& ray, HitShapeRecord& rec) const {
+  if (nodes.empty()) return false;
+  V3 invD(1 / ray.D.x, 1 / ray.D.y, 1 / ray.D.z);
+  Float tfar = ray.MaxT;
+  bool hit = false;
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const Node& n = nodes[stack[--sp]];
+    if (!SlabHit(n.b, ray.O, invD, ray.MinT, tfar)) continue;
+    if (n.count) {
+      for (uint32_t i = n.start; i < n.start + n.count; i++) {
+        HitShapeRecord r;
+        if ((*shapes)[prims[i].shape]->IntersectPrim(prims[i].prim, ray, ray.MinT, tfar, r)) {
+          rec = r;
+          tfar = r.T;
+          hit = true;
+        }
+      }
+    } else {
+      // visit the nearer child first
+      const Node& L = nodes[n.left];
+      const Node& R = nodes[n.right];
+      Float dl = Dot(L.b.Center() - ray.O, ray.D), dr = Dot(R.b.Center() - ray.O, ray.D);
+      if (dl < dr) {
+        stack[sp++] = n.right;
+        stack[sp++] = n.left;
+      } else {
+        stack[sp++] = n.left;
+        stack[sp++] = n.right;
+      }
+    }
+  }
+  return hit;
+}
+
+bool Accel::Occluded(const Ray& ray) const {
+  if (nodes.empty()) return false;
+  V3 invD(1 / ray.D.x, 1 / ray.D.y, 1 / ray.D.z);
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const Node& n = nodes[stack[--sp]];
+    if (!SlabHit(n.b, ray.O, invD, ray.MinT, ray.MaxT)) continue;
+    if (n.count) {
+      for (uint32_t i = n.start; i < n.start + n.count; i++) {
+        HitShapeRecord r;
+        if ((*shapes)[prims[i].shape]->IntersectPrim(prims[i].prim, ray, ray.MinT, ray.MaxT, r)) return true;
+      }
+    } else {
+      stack[sp++] = n.left;
+      stack[sp++] = n.right;
+    }
+  }
+  return false;
+}
+
+bool Accel::ClosestBrute(const Ray& ray, HitShapeRecord& rec) const {
+  Float tfar = ray.MaxT;
+  bool hit = false;
+  for (const auto& p : prims) {
+    HitShapeRecord r;
+    if ((*shapes)[p.shape]->IntersectPrim(p.prim, ray, ray.MinT, tfar, r)) {
+      rec = r;
+      tfar = r.T;
+      hit = true;
+    }
+  }
+  return hit;
+}
+
+bool Accel::OccludedBrute(const Ray& ray) const {
+  for (const auto& p : prims) {
+    HitShapeRecord r;
+    if ((*shapes)[p.shape]->IntersectPrim(p.prim, ray, ray.MinT, ray.MaxT, r)) return true;
+  }
+  return false;
+}
+
+}  // namespace ref

@@ -1,0 +1,43 @@
+// ORACLE / TEST INFRASTRUCTURE ONLY (see ref_math.h).
+// extern "C" surface of the CPU restatement.  Consumed through ctypes by
+// tests/ and by bench.py's cpu_baseline leg; never by the shipped library.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/rad_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ref_scene ref_scene;
+
+typedef struct ref_render_params {
+  int32_t integrator;  /* RAD_INTEGRATOR_PATH / RAD_INTEGRATOR_VOL_PATH */
+  int32_t max_depth;
+  int32_t rr_depth;
+  uint32_t spp;
+  uint32_t seed;
+  int32_t threads;     /* 0 = all */
+  int32_t tile_size;   /* 0 = 16 */
+  int32_t x_begin, x_end, y_begin, y_end; /* sub-window (0 = full) */
+} ref_render_params;
+
+const char* ref_last_error(void);
+ref_scene* ref_scene_create(const rad_scene_desc* desc, int quirks);
+void ref_scene_destroy(ref_scene* s);
+double ref_scene_build_ms(const ref_scene* s);
+void ref_scene_bounds(const ref_scene* s, float* lo, float* hi);
+int ref_intersect_closest(const ref_scene* s, const rad_ray* rays, uint32_t n, int brute, rad_hit* hits);
+int ref_intersect_any(const ref_scene* s, const rad_ray* rays, uint32_t n, int brute, uint8_t* occluded);
+float ref_intersect_prim(const ref_scene* s, const rad_ray* ray, uint32_t geom, uint32_t prim);
+void ref_camera_ray(const ref_scene* s, float sx, float sy, rad_ray* out);
+int ref_render(const ref_scene* s, const ref_render_params* p, float* fb, float* sumsq, double* seconds);
+int ref_bsdf_eval(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out);
+int ref_bsdf_sample(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out);
+int ref_grid_eval(const rad_medium_desc* d, const float* in, uint32_t n, float* out);
+float ref_medium_transmittance(const ref_scene* s, int medium, const rad_ray* ray, uint32_t channel, uint32_t seed);
+
+#ifdef __cplusplus
+}
+#endif

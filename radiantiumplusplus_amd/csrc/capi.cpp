@@ -1,0 +1,324 @@
+// extern "C" entry points of include/rad_hip.h: device context, scene upload,
+// BVH build, ray-batch intersection, asynchronous render jobs and the
+// per-function probes.  No exception crosses this boundary.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+
+#include "../../include/rad_hip.h"
+#include "render_job.h"
+#include "scene_host.h"
+#include "wavefront.h"
+
+using namespace rad;
+
+namespace {
+thread_local std::string g_last_error;
+
+rad_status Fail(rad_status code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+rad_status Guard(F&& f) {
+  try {
+    f();
+    return RAD_OK;
+  } catch (const StatusError& e) {
+    return Fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return Fail(RAD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    std::string m = e.what();
+    rad_status code = m.find("hip") != std::string::npos ? RAD_ERR_HIP : RAD_ERR_INVALID_ARG;
+    return Fail(code, m);
+  }
+}
+}  // namespace
+
+namespace rad {
+
+void SetLastError(const std::string& msg) { g_last_error = msg; }
+
+void UploadScene(rad_scene* s) {
+  hipStream_t st = s->ctx->stream;
+  HostScene& h = s->host;
+  s->d_shapes.upload(h.shapes.data(), h.shapes.size() * sizeof(ShapeRec), st);
+  s->d_pos.upload(h.pos.data(), h.pos.size() * sizeof(float), st);
+  s->d_nrm.upload(h.nrm.data(), h.nrm.size() * sizeof(float), st);
+  s->d_uv.upload(h.uv.data(), h.uv.size() * sizeof(float), st);
+  s->d_idx.upload(h.idx.data(), h.idx.size() * sizeof(uint32_t), st);
+  s->d_bsdfs.upload(h.bsdfs.data(), h.bsdfs.size() * sizeof(BsdfRec), st);
+  s->d_lights.upload(h.lights.data(), h.lights.size() * sizeof(LightRec), st);
+  s->d_cdf.upload(h.light_cdf.data(), h.light_cdf.size() * sizeof(float), st);
+  s->d_media.upload(h.media.data(), h.media.size() * sizeof(MediumRec), st);
+  s->d_grid.upload(h.grid.data(), h.grid.size() * sizeof(float), st);
+  DevScene& d = s->dev;
+  d = DevScene{};
+  d.shapes = s->d_shapes.as<ShapeRec>();
+  d.shape_count = uint32_t(h.shapes.size());
+  d.pos = s->d_pos.as<float>();
+  d.nrm = s->d_nrm.as<float>();
+  d.uv = s->d_uv.as<float>();
+  d.idx = s->d_idx.as<uint32_t>();
+  d.bsdfs = s->d_bsdfs.as<BsdfRec>();
+  d.lights = s->d_lights.as<LightRec>();
+  d.light_count = uint32_t(h.lights.size());
+  d.env_light = h.env_light;
+  d.light_pdf = 1.0f / float(h.lights.size());
+  d.light_cdf = s->d_cdf.as<float>();
+  d.media = s->d_media.as<MediumRec>();
+  d.global_medium = h.global_medium;
+  d.grid_data = s->d_grid.as<float>();
+  d.cam = h.cam;
+  // skybox world sphere (skybox.cpp:136-141, bounding_sphere.cpp:15-18)
+  f3 lo = mk(h.world_lo[0], h.world_lo[1], h.world_lo[2]), hi = mk(h.world_hi[0], h.world_hi[1], h.world_hi[2]);
+  f3 c = (lo + hi) * 0.5f;
+  d.world_center = c;
+  d.world_radius = std::max(kRayEps, norm(c - hi) * (1.f + kShadowEps));
+  d.quirks = 1;
+  HipCheck(hipStreamSynchronize(st), "scene upload");
+}
+
+}  // namespace rad
+
+extern "C" {
+
+const char* rad_hip_last_error(void) { return g_last_error.c_str(); }
+
+size_t rad_abi_sizeof(const char* name) {
+  if (!name) return 0;
+  std::string n(name);
+#define RAD_SZ(T) \
+  if (n == #T) return sizeof(T);
+  RAD_SZ(rad_shape_desc)
+  RAD_SZ(rad_bsdf_desc)
+  RAD_SZ(rad_light_desc)
+  RAD_SZ(rad_medium_desc)
+  RAD_SZ(rad_camera_desc)
+  RAD_SZ(rad_scene_desc)
+  RAD_SZ(rad_ray)
+  RAD_SZ(rad_hit)
+  RAD_SZ(rad_render_desc)
+  RAD_SZ(rad_job_stats)
+  RAD_SZ(rad_bvh_opts)
+  RAD_SZ(rad_scene_info)
+#undef RAD_SZ
+  return 0;
+}
+
+rad_status rad_hip_init(int32_t device_ordinal, rad_ctx** out) {
+  if (!out) return Fail(RAD_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  return Guard([&] {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) throw StatusError(RAD_ERR_HIP, "no HIP device available: " + HipErr(e));
+    if (device_ordinal < 0 || device_ordinal >= count) throw StatusError(RAD_ERR_INVALID_ARG, "device ordinal out of range");
+    HipCheck(hipSetDevice(device_ordinal), "hipSetDevice");
+    auto* c = new rad_ctx();
+    c->device = device_ordinal;
+    hipDeviceProp_t prop;
+    HipCheck(hipGetDeviceProperties(&prop, device_ordinal), "hipGetDeviceProperties");
+    c->arch = prop.gcnArchName;
+    c->cu_count = prop.multiProcessorCount;
+    HipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    *out = c;
+  });
+}
+
+void rad_hip_shutdown(rad_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+rad_status rad_hip_scene_create(rad_ctx* ctx, const rad_scene_desc* desc, rad_scene** out) {
+  if (!ctx || !desc || !out) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  *out = nullptr;
+  return Guard([&] {
+    HipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+    auto s = std::make_unique<rad_scene>();
+    s->ctx = ctx;
+    AssembleScene(*desc, s->host);
+    UploadScene(s.get());
+    *out = s.release();
+  });
+}
+
+rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
+  if (!scene) return Fail(RAD_ERR_INVALID_ARG, "scene is NULL");
+  return Guard([&] {
+    std::lock_guard<std::mutex> lk(scene->mu);
+    HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
+    int maxLeaf = opts && opts->max_leaf_size > 0 ? opts->max_leaf_size : 4;
+    int builder = opts ? opts->builder : 0;
+    BvhBuildResult r;
+    HostScene& h = scene->host;
+    uint32_t n = uint32_t(h.prim_bounds.size() / 6);
+    auto t0 = std::chrono::steady_clock::now();
+    (void)builder;  // the device builder is selected in BuildBvh4 once available
+    {
+      BuildBvh4Host(h.prim_bounds.data(), n, maxLeaf, r);
+      std::vector<float4> ordered(3 * size_t(n));
+      for (uint32_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) ordered[3 * size_t(i) + k] = h.prim_records[3 * size_t(r.order[i]) + k];
+      scene->d_prims.upload(ordered.data(), ordered.size() * sizeof(float4), scene->ctx->stream);
+      scene->d_nodes.upload(r.nodes.data(), r.nodes.size() * sizeof(BvhNode4), scene->ctx->stream);
+      HipCheck(hipStreamSynchronize(scene->ctx->stream), "BVH upload");
+    }
+    scene->bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    scene->dev.nodes = scene->d_nodes.as<BvhNode4>();
+    scene->dev.prims = scene->d_prims.as<float4>();
+    scene->dev.prim_count = n;
+    scene->dev.node_count = uint32_t(r.nodes.size());
+    scene->bvh_depth = r.depth;
+    scene->sah_cost = r.sah_cost;
+    scene->bvh_ready = true;
+  });
+}
+
+rad_status rad_hip_scene_info(const rad_scene* scene, rad_scene_info* info) {
+  if (!scene || !info) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  std::memset(info, 0, sizeof(*info));
+  info->primitive_count = scene->host.prim_bounds.size() / 6;
+  info->node_count = scene->dev.node_count;
+  info->device_bytes = scene->device_bytes();
+  info->bvh_build_ms = scene->bvh_ms;
+  for (int a = 0; a < 3; a++) {
+    info->world_min[a] = scene->host.world_lo[a];
+    info->world_max[a] = scene->host.world_hi[a];
+  }
+  info->bvh_depth = scene->bvh_depth;
+  info->sah_cost = scene->sah_cost;
+  return RAD_OK;
+}
+
+void rad_hip_scene_destroy(rad_scene* scene) {
+  if (!scene) return;
+  (void)hipSetDevice(scene->ctx->device);
+  delete scene;
+}
+
+static void RequireBvh(rad_scene* s) {
+  if (!s->bvh_ready) throw StatusError(RAD_ERR_STATE, "rad_hip_bvh_build has not been called on this scene");
+}
+
+rad_status rad_hip_intersect_closest(rad_scene* scene, const rad_ray* rays, uint32_t n, rad_hit* hits) {
+  if (!scene || (n && (!rays || !hits))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    std::lock_guard<std::mutex> lk(scene->mu);
+    RequireBvh(scene);
+    HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
+    hipStream_t st = scene->ctx->stream;
+    scene->d_rays.alloc(std::max<size_t>(16, sizeof(rad_ray) * size_t(n)));
+    scene->d_hits.alloc(std::max<size_t>(16, sizeof(rad_hit) * size_t(n)));
+    if (n) HipCheck(hipMemcpyAsync(scene->d_rays.ptr, rays, sizeof(rad_ray) * size_t(n), hipMemcpyHostToDevice, st), "copy rays");
+    LaunchClosest(scene->dev, scene->d_rays.ptr, n, scene->d_hits.ptr, st);
+    HipCheck(hipGetLastError(), "closest-hit launch");
+    if (n) HipCheck(hipMemcpyAsync(hits, scene->d_hits.ptr, sizeof(rad_hit) * size_t(n), hipMemcpyDeviceToHost, st), "copy hits");
+    HipCheck(hipStreamSynchronize(st), "closest-hit");
+  });
+}
+
+rad_status rad_hip_intersect_any(rad_scene* scene, const rad_ray* rays, uint32_t n, uint8_t* occluded) {
+  if (!scene || (n && (!rays || !occluded))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    std::lock_guard<std::mutex> lk(scene->mu);
+    RequireBvh(scene);
+    HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
+    hipStream_t st = scene->ctx->stream;
+    scene->d_rays.alloc(std::max<size_t>(16, sizeof(rad_ray) * size_t(n)));
+    scene->d_occ.alloc(std::max<size_t>(16, size_t(n)));
+    if (n) HipCheck(hipMemcpyAsync(scene->d_rays.ptr, rays, sizeof(rad_ray) * size_t(n), hipMemcpyHostToDevice, st), "copy rays");
+    LaunchAny(scene->dev, scene->d_rays.ptr, n, scene->d_occ.as<uint8_t>(), st);
+    HipCheck(hipGetLastError(), "occlusion launch");
+    if (n) HipCheck(hipMemcpyAsync(occluded, scene->d_occ.ptr, size_t(n), hipMemcpyDeviceToHost, st), "copy results");
+    HipCheck(hipStreamSynchronize(st), "occlusion");
+  });
+}
+
+rad_status rad_hip_intersect_closest_device(rad_scene* scene, const rad_ray* d_rays, uint32_t n, rad_hit* d_hits,
+                                            void* stream) {
+  if (!scene || (n && (!d_rays || !d_hits))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    RequireBvh(scene);
+    LaunchClosest(scene->dev, d_rays, n, d_hits, (hipStream_t)stream);
+    HipCheck(hipGetLastError(), "closest-hit launch");
+  });
+}
+
+rad_status rad_hip_intersect_any_device(rad_scene* scene, const rad_ray* d_rays, uint32_t n, uint8_t* d_occ,
+                                        void* stream) {
+  if (!scene || (n && (!d_rays || !d_occ))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    RequireBvh(scene);
+    LaunchAny(scene->dev, d_rays, n, d_occ, (hipStream_t)stream);
+    HipCheck(hipGetLastError(), "occlusion launch");
+  });
+}
+
+rad_status rad_hip_render_start(rad_scene* scene, const rad_render_desc* desc, rad_job** out) {
+  if (!scene || !desc || !out) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  *out = nullptr;
+  return Guard([&] {
+    RequireBvh(scene);
+    *out = StartJob(scene, *desc);
+  });
+}
+
+rad_status rad_hip_job_poll(rad_job* job, uint64_t* done, uint64_t* total, int32_t* complete) {
+  if (!job) return Fail(RAD_ERR_INVALID_ARG, "job is NULL");
+  PollJob(job, done, total, complete);
+  return RAD_OK;
+}
+
+rad_status rad_hip_job_cancel(rad_job* job) {
+  if (!job) return Fail(RAD_ERR_INVALID_ARG, "job is NULL");
+  CancelJob(job);
+  return RAD_OK;
+}
+
+rad_status rad_hip_job_wait(rad_job* job, float* fb_rgb, float* fb_sumsq, rad_job_stats* stats) {
+  if (!job) return Fail(RAD_ERR_INVALID_ARG, "job is NULL");
+  return Guard([&] { WaitJob(job, fb_rgb, fb_sumsq, stats); });
+}
+
+void rad_hip_job_destroy(rad_job* job) { DestroyJob(job); }
+
+static rad_status BsdfProbe(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out,
+                            bool sample) {
+  if (!bsdf || (n && (!in || !out))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw StatusError(RAD_ERR_HIP, "no HIP device available");
+    BsdfRec b = MakeBsdfRec(*bsdf);
+    size_t inBytes = size_t(n) * 6 * sizeof(float), outBytes = size_t(n) * (sample ? 8 : 4) * sizeof(float);
+    DeviceBuffer din, dout;
+    din.alloc(std::max<size_t>(inBytes, 16));
+    dout.alloc(std::max<size_t>(outBytes, 16));
+    HipCheck(hipMemcpy(din.ptr, in, inBytes, hipMemcpyHostToDevice), "copy probe input");
+    if (sample)
+      LaunchBsdfSample(b, quirks != 0, din.as<float>(), n, dout.as<float>(), nullptr);
+    else
+      LaunchBsdfEval(b, quirks != 0, din.as<float>(), n, dout.as<float>(), nullptr);
+    HipCheck(hipGetLastError(), "probe launch");
+    HipCheck(hipMemcpy(out, dout.ptr, outBytes, hipMemcpyDeviceToHost), "copy probe output");
+  });
+}
+
+rad_status rad_hip_bsdf_eval(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out) {
+  return BsdfProbe(bsdf, quirks, in, n, out, false);
+}
+
+rad_status rad_hip_bsdf_sample(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out) {
+  return BsdfProbe(bsdf, quirks, in, n, out, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Asynchronous render jobs: one host thread and one HIP stream per job drive
+// the wavefront loop (the GPU analogue of SampleRenderer::Start's render
+// thread, sample_renderer.cpp:60-122).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/rad_hip.h"
+#include "scene_host.h"
+
+namespace rad {
+
+struct StatusError : std::runtime_error {
+  rad_status code;
+  StatusError(rad_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc);
+void PollJob(rad_job* job, uint64_t* done, uint64_t* total, int32_t* complete);
+void CancelJob(rad_job* job);
+void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats);
+void DestroyJob(rad_job* job);
+
+// Device bytes a job of this shape needs (admission control).
+uint64_t JobFootprint(const rad_scene* scene, const rad_render_desc& desc, uint32_t* slots_per_pixel);
+
+void UploadScene(rad_scene* s);
+
+}  // namespace rad

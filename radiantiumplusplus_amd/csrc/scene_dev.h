@@ -1,0 +1,131 @@
+// Device-resident scene layout.
+//
+// Geometry for traversal lives in two HBM arrays:
+//   nodes  - 4-wide BVH nodes, 128 B each (one cache line): six 4-float slab
+//            planes, four child links, four leaf counts;
+//   prims  - primitives in leaf order, three float4 (48 B) each:
+//            triangle: (v0, geomID) (v1, primID) (v2, PRIM_TRI)
+//            sphere:   (center, radius) (-, geomID) (-, PRIM_SPHERE)
+//            rectangle:(-) (-, geomID) (-, PRIM_RECT); matrix in its ShapeRec.
+// Shading data (vertex attributes, per-shape records, BSDF / light / medium
+// records) is separate, so traversal touches only nodes + prims.
+#pragma once
+
+#include "../../include/rad_hip.h"
+#include "rad_math.h"
+
+namespace rad {
+
+enum : uint32_t { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
+enum : uint32_t { SHAPE_HAS_NORMALS = 1, SHAPE_HAS_UVS = 2 };
+enum : int32_t { SHAPE_MESH = 0, SHAPE_SPHERE = 1, SHAPE_RECTANGLE = 2 };
+
+struct alignas(128) BvhNode4 {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t child[4];   // inner: node index; leaf: first primitive; empty: -1
+  uint32_t count[4];  // 0: inner node; > 0: leaf primitive count
+};
+static_assert(sizeof(BvhNode4) == 128, "BVH node must be one 128-byte line");
+
+struct ShapeRec {
+  int32_t type;       // SHAPE_*
+  int32_t bsdf;
+  int32_t light;
+  int32_t med_in, med_out;
+  uint32_t flags;     // SHAPE_HAS_*
+  uint32_t vtx_offset;
+  uint32_t tri_offset;
+  uint32_t tri_count;
+  float area;
+  Affine to_world;    // rectangle: ToWorld; sphere: _toWorld (UV parameterisation)
+  Affine to_local;    // rectangle: ToObject; sphere: inverse of _toWorld
+  f3 frame_s, frame_t, frame_n;  // rectangle frame (dpdu, dpdv, N)
+  f3 center;
+  float radius;
+  uint32_t cdf_offset;  // mesh emitters: triangle-area CDF in DevScene::light_cdf
+  float cdf_sum, cdf_norm;
+};
+
+enum : int32_t { BSDF_DIFFUSE = 0, BSDF_ROUGH_METAL = 1, BSDF_ROUGH_GLASS = 2, BSDF_DISNEY = 3 };
+enum : uint32_t {
+  LOBE_DIFFUSE = 1,
+  LOBE_GLOSSY = 2,
+  LOBE_DELTA = 4,
+  LOBE_REFLECTION = 8,
+  LOBE_TRANSMISSION = 16,
+  LOBE_NO_DELTA = LOBE_DIFFUSE | LOBE_GLOSSY
+};
+
+struct BsdfRec {
+  int32_t type;
+  uint32_t lobes;         // BsdfType flags of the instance (render/bsdf.h:10-40)
+  int32_t beckmann;
+  int32_t sample_visible;
+  uint32_t disney_flags;
+  f3 reflectance, transmittance, eta3, k3;
+  float alpha_u, alpha_v;
+  float eta, inv_eta;     // rough_glass int/ext; disney eta
+  f3 base_color;
+  float roughness, anisotropic, spec_trans, sheen, sheen_tint, flatness, spec_tint, metallic, clearcoat,
+      clearcoat_gloss;
+  float spec_rate, cc_rate, diff_rate;
+};
+
+enum : int32_t { LIGHT_AREA = 0, LIGHT_SKYBOX = 1 };
+struct LightRec {
+  int32_t type;
+  int32_t shape;
+  f3 radiance;
+  Affine to_world;  // skybox orientation
+};
+
+enum : int32_t { MEDIUM_HOMOGENEOUS = 0, MEDIUM_HETEROGENEOUS = 1 };
+struct MediumRec {
+  int32_t type;
+  int32_t phase;      // 0 isotropic, 1 HG
+  float g;
+  f3 sigma_t;         // homogeneous sigma_t * scale (or constant heterogeneous)
+  f3 albedo;
+  float scale;
+  float max_density;  // heterogeneous majorant (grid max * scale)
+  Affine to_local;    // world -> [0,1]^3 grid space
+  f3 bbox_lo, bbox_hi;
+  int32_t sx, sy, sz, channels, wrap;
+  int32_t has_grid;
+  uint64_t grid_offset;  // float offset into DevScene::grid_data
+};
+
+struct CameraRec {
+  Mat4 clip_to_camera;   // inverse of scale * translate * perspective (perspective.cpp:33-52)
+  Affine camera_to_world;
+  float near_clip, far_clip;
+  int32_t width, height;
+  float rcp_w, rcp_h;
+};
+
+struct DevScene {
+  const BvhNode4* nodes;
+  const float4* prims;
+  uint32_t prim_count, node_count;
+  const ShapeRec* shapes;
+  uint32_t shape_count;
+  const float* pos;        // world-space positions, 3 floats per vertex
+  const float* nrm;        // world-space (unnormalised) normals, 3 floats per vertex
+  const float* uv;         // 2 floats per vertex
+  const uint32_t* idx;     // 3 indices per triangle, local to the shape's vertex range
+  const BsdfRec* bsdfs;
+  const LightRec* lights;
+  uint32_t light_count;
+  int32_t env_light;
+  float light_pdf;
+  const float* light_cdf;
+  const MediumRec* media;
+  int32_t global_medium;
+  const float* grid_data;
+  CameraRec cam;
+  f3 world_center;
+  float world_radius;
+  uint32_t quirks;
+};
+
+}  // namespace rad

@@ -1,0 +1,457 @@
+#include "scene_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace rad {
+
+std::string HipErr(hipError_t e) { return std::string(hipGetErrorName(e)) + ": " + hipGetErrorString(e); }
+
+void HipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + " failed: " + HipErr(e));
+}
+
+void DeviceBuffer::reset() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+void DeviceBuffer::alloc(size_t n) {
+  if (n <= bytes && ptr) return;
+  reset();
+  if (n == 0) return;
+  HipCheck(hipMalloc(&ptr, n), "hipMalloc");
+  bytes = n;
+}
+
+void DeviceBuffer::upload(const void* src, size_t n, hipStream_t s) {
+  alloc(std::max<size_t>(n, 16));
+  if (n) HipCheck(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+
+namespace {
+
+// Row-major 4x4 with a float cofactor inverse (Transform, transform.cpp:5-16).
+struct M4 {
+  float m[16];
+  static M4 Identity() {
+    M4 r{};
+    r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.f;
+    return r;
+  }
+  static M4 From(const float* a) {
+    M4 r;
+    std::memcpy(r.m, a, sizeof(r.m));
+    return r;
+  }
+  M4 operator*(const M4& o) const {
+    M4 r{};
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        float s = 0;
+        for (int k = 0; k < 4; k++) s += m[i * 4 + k] * o.m[k * 4 + j];
+        r.m[i * 4 + j] = s;
+      }
+    return r;
+  }
+  bool IsIdentity() const {
+    for (int i = 0; i < 16; i++)
+      if (m[i] != ((i % 5) == 0 ? 1.f : 0.f)) return false;
+    return true;
+  }
+  M4 Inverse() const {
+    // Gauss-Jordan in double, rounded to float
+    double a[4][8];
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        a[i][j] = m[i * 4 + j];
+        a[i][j + 4] = i == j ? 1.0 : 0.0;
+      }
+    for (int c = 0; c < 4; c++) {
+      int p = c;
+      for (int r = c + 1; r < 4; r++)
+        if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+      if (std::fabs(a[p][c]) < 1e-300) throw std::runtime_error("transform matrix is not invertible");
+      if (p != c)
+        for (int j = 0; j < 8; j++) std::swap(a[p][j], a[c][j]);
+      double inv = 1.0 / a[c][c];
+      for (int j = 0; j < 8; j++) a[c][j] *= inv;
+      for (int r = 0; r < 4; r++) {
+        if (r == c) continue;
+        double f = a[r][c];
+        if (f == 0) continue;
+        for (int j = 0; j < 8; j++) a[r][j] -= f * a[c][j];
+      }
+    }
+    M4 r;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) r.m[i * 4 + j] = float(a[i][j + 4]);
+    return r;
+  }
+  f3 Point(f3 v) const {  // with homogeneous divide
+    float r0 = m[0] * v.x + m[1] * v.y + m[2] * v.z + m[3];
+    float r1 = m[4] * v.x + m[5] * v.y + m[6] * v.z + m[7];
+    float r2 = m[8] * v.x + m[9] * v.y + m[10] * v.z + m[11];
+    float r3 = m[12] * v.x + m[13] * v.y + m[14] * v.z + m[15];
+    return mk(r0 / r3, r1 / r3, r2 / r3);
+  }
+  f3 Vec(f3 v) const {
+    return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+              m[8] * v.x + m[9] * v.y + m[10] * v.z);
+  }
+  f3 TVec(f3 v) const {  // transpose of the linear part
+    return mk(m[0] * v.x + m[4] * v.y + m[8] * v.z, m[1] * v.x + m[5] * v.y + m[9] * v.z,
+              m[2] * v.x + m[6] * v.y + m[10] * v.z);
+  }
+  Affine ToAffine() const {
+    Affine a;
+    for (int i = 0; i < 12; i++) a.m[i] = m[i];
+    return a;
+  }
+};
+
+const float kCubeV[24][3] = {{1, -1, -1}, {1, -1, 1},   {-1, -1, 1}, {-1, -1, -1}, {1, 1, -1},  {-1, 1, -1},
+                             {-1, 1, 1},  {1, 1, 1},    {1, -1, -1}, {1, 1, -1},   {1, 1, 1},   {1, -1, 1},
+                             {1, -1, 1},  {1, 1, 1},    {-1, 1, 1},  {-1, -1, 1},  {-1, -1, 1}, {-1, 1, 1},
+                             {-1, 1, -1}, {-1, -1, -1}, {1, 1, -1},  {1, -1, -1},  {-1, -1, -1}, {-1, 1, -1}};
+const float kCubeN[6][3] = {{0, -1, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {-1, 0, 0}, {0, 0, -1}};
+const uint32_t kCubeT[12][3] = {{0, 1, 2},    {3, 0, 2},    {4, 5, 6},    {7, 4, 6},    {8, 9, 10},   {11, 8, 10},
+                                {12, 13, 14}, {15, 12, 14}, {16, 17, 18}, {19, 16, 18}, {20, 21, 22}, {23, 20, 22}};
+
+float4 F4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+float AsF(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+void GrowBounds(float* b, f3 p) {
+  b[0] = std::min(b[0], p.x);
+  b[1] = std::min(b[1], p.y);
+  b[2] = std::min(b[2], p.z);
+  b[3] = std::max(b[3], p.x);
+  b[4] = std::max(b[4], p.y);
+  b[5] = std::max(b[5], p.z);
+}
+
+}  // namespace
+
+BsdfRec MakeBsdfRec(const rad_bsdf_desc& d) {
+  BsdfRec b{};
+  auto v3 = [](const float* a) { return mk(a[0], a[1], a[2]); };
+  b.type = d.type;
+  b.beckmann = d.distribution == RAD_MICROFACET_BECKMANN;
+  b.sample_visible = d.sample_visible;
+  b.disney_flags = d.disney_flags;
+  b.reflectance = v3(d.reflectance);
+  b.transmittance = v3(d.transmittance);
+  b.eta3 = v3(d.eta);
+  b.k3 = v3(d.k);
+  b.alpha_u = d.alpha_u;
+  b.alpha_v = d.alpha_v;
+  b.base_color = v3(d.base_color);
+  b.roughness = d.roughness;
+  b.anisotropic = d.anisotropic;
+  b.spec_trans = d.spec_trans;
+  b.sheen = d.sheen;
+  b.sheen_tint = d.sheen_tint;
+  b.flatness = d.flatness;
+  b.spec_tint = d.spec_tint;
+  b.metallic = d.metallic;
+  b.clearcoat = d.clearcoat;
+  b.clearcoat_gloss = d.clearcoat_gloss;
+  b.spec_rate = d.specular_sampling_rate;
+  b.cc_rate = d.clearcoat_sampling_rate;
+  b.diff_rate = d.diffuse_sampling_rate;
+  switch (d.type) {
+    case RAD_BSDF_DIFFUSE:
+      b.lobes = LOBE_DIFFUSE | LOBE_REFLECTION;  // diffuse.cpp:42
+      break;
+    case RAD_BSDF_ROUGH_METAL:
+      b.lobes = LOBE_GLOSSY | LOBE_REFLECTION;  // rough_metal.cpp:23
+      break;
+    case RAD_BSDF_ROUGH_GLASS:
+      b.lobes = LOBE_GLOSSY | LOBE_REFLECTION | LOBE_TRANSMISSION;  // rough_glass.cpp:224
+      b.eta = d.int_ior / d.ext_ior;
+      b.inv_eta = d.ext_ior / d.int_ior;
+      break;
+    case RAD_BSDF_DISNEY: {  // disney.cpp:86-109
+      bool hasSpecTrans = d.disney_flags & RAD_DISNEY_HAS_SPEC_TRANS;
+      if (d.disney_flags & RAD_DISNEY_HAS_ETA) {
+        b.eta = d.disney_eta;
+        if (hasSpecTrans && b.eta == 1.f) b.eta = 1.001f;
+      } else {
+        float specular = d.specular;
+        if (hasSpecTrans && specular == 0.f) specular = 0.001f;
+        b.eta = 2.0f * (1.0f / (1.0f - std::sqrt(0.08f * specular))) - 1.0f;
+      }
+      b.inv_eta = 1.0f / b.eta;
+      b.lobes = LOBE_DIFFUSE | LOBE_REFLECTION;
+      if (d.disney_flags & RAD_DISNEY_HAS_CLEARCOAT) b.lobes |= LOBE_GLOSSY;
+      if (hasSpecTrans) b.lobes |= LOBE_TRANSMISSION;
+      break;
+    }
+    default:
+      throw std::runtime_error("unsupported BSDF type " + std::to_string(d.type));
+  }
+  return b;
+}
+
+void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
+  out = HostScene();
+  for (uint32_t i = 0; i < desc.bsdf_count; i++) out.bsdfs.push_back(MakeBsdfRec(desc.bsdfs[i]));
+  if (desc.light_count == 0) throw std::runtime_error("scene has no light (Scene::SampleLight needs one)");
+  // media
+  for (uint32_t i = 0; i < desc.medium_count; i++) {
+    const rad_medium_desc& m = desc.media[i];
+    MediumRec r{};
+    r.type = m.type;
+    r.phase = m.phase;
+    r.g = m.g;
+    if (m.phase == RAD_PHASE_HG && !(m.g > -1.f && m.g < 1.f)) throw std::runtime_error("param g must lie in (-1, 1)");
+    r.albedo = mk(m.albedo[0], m.albedo[1], m.albedo[2]);
+    r.scale = m.scale;
+    if (m.type == RAD_MEDIUM_HOMOGENEOUS) {
+      r.sigma_t = mk(m.sigma_t[0] * m.scale, m.sigma_t[1] * m.scale, m.sigma_t[2] * m.scale);
+    } else {
+      // heterogeneous.cpp:15-29: data lives in [-1,1]^3 of the entity; sampled in [0,1]^3
+      M4 A = M4::Identity();
+      A.m[0] = A.m[5] = A.m[10] = 2.f;
+      A.m[3] = A.m[7] = A.m[11] = -1.f;
+      M4 toWorld = M4::From(m.to_world) * A;
+      M4 toLocal = toWorld.Inverse();
+      r.to_local = toLocal.ToAffine();
+      float bb[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
+      for (int c = 0; c < 8; c++) GrowBounds(bb, toWorld.Point(mk(c & 1 ? 1.f : 0.f, c & 2 ? 1.f : 0.f, c & 4 ? 1.f : 0.f)));
+      r.bbox_lo = mk(bb[0], bb[1], bb[2]);
+      r.bbox_hi = mk(bb[3], bb[4], bb[5]);
+      float maxValue = -kInf;
+      if (m.grid && m.grid_size[0] > 0) {
+        if (m.grid_channels != 1 && m.grid_channels != 3) throw std::runtime_error("volume grid needs 1 or 3 channels");
+        r.has_grid = 1;
+        r.sx = m.grid_size[0];
+        r.sy = m.grid_size[1];
+        r.sz = m.grid_size[2];
+        r.channels = m.grid_channels;
+        r.wrap = m.grid_wrap;
+        size_t n = size_t(r.sx) * r.sy * r.sz * r.channels;
+        r.grid_offset = out.grid.size();
+        out.grid.insert(out.grid.end(), m.grid, m.grid + n);
+        for (size_t k = 0; k < n; k++) maxValue = std::max(maxValue, m.grid[k]);
+      } else {
+        r.sigma_t = mk(m.sigma_t[0], m.sigma_t[1], m.sigma_t[2]);
+        maxValue = std::max(m.sigma_t[0], std::max(m.sigma_t[1], m.sigma_t[2]));
+      }
+      r.max_density = maxValue * m.scale;
+    }
+    out.media.push_back(r);
+  }
+  out.global_medium = desc.global_medium;
+  // lights
+  for (uint32_t i = 0; i < desc.light_count; i++) {
+    const rad_light_desc& l = desc.lights[i];
+    LightRec r{};
+    r.type = l.type;
+    r.shape = -1;
+    r.radiance = mk(l.radiance[0], l.radiance[1], l.radiance[2]);
+    r.to_world = M4::From(l.to_world).ToAffine();
+    out.lights.push_back(r);
+    if (l.type == RAD_LIGHT_SKYBOX) out.env_light = int32_t(i);  // the last env light wins (scene.cpp:25-33)
+  }
+  float wb[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
+  // shapes, in geomID order
+  for (uint32_t si = 0; si < desc.shape_count; si++) {
+    const rad_shape_desc& d = desc.shapes[si];
+    ShapeRec s{};
+    s.bsdf = d.bsdf;
+    s.light = d.light;
+    s.med_in = d.medium_inside;
+    s.med_out = d.medium_outside;
+    if (d.bsdf >= int32_t(desc.bsdf_count) || d.light >= int32_t(desc.light_count) ||
+        d.medium_inside >= int32_t(desc.medium_count) || d.medium_outside >= int32_t(desc.medium_count))
+      throw std::runtime_error("shape " + std::to_string(si) + " references an out-of-range record");
+    if (d.light >= 0) {
+      if (desc.lights[d.light].type != RAD_LIGHT_AREA) throw std::runtime_error("only area lights attach to shapes");
+      out.lights[d.light].shape = int32_t(si);
+    }
+    M4 tw = M4::From(d.to_world);
+    if (d.type == RAD_SHAPE_MESH || d.type == RAD_SHAPE_CUBE) {
+      s.type = SHAPE_MESH;
+      bool ident = tw.IsIdentity();
+      M4 tl = tw.Inverse();
+      s.vtx_offset = uint32_t(out.pos.size() / 3);
+      s.tri_offset = uint32_t(out.idx.size() / 3);
+      uint32_t nv, nt;
+      if (d.type == RAD_SHAPE_MESH) {
+        if (!d.positions || !d.indices) throw std::runtime_error("mesh without positions / indices");
+        nv = d.vertex_count;
+        nt = d.triangle_count;
+        for (uint32_t v = 0; v < nv; v++) {
+          f3 p = mk(d.positions[3 * v], d.positions[3 * v + 1], d.positions[3 * v + 2]);
+          if (!ident) p = tw.Point(p);  // mesh.cpp:28-35
+          out.pos.insert(out.pos.end(), {p.x, p.y, p.z});
+        }
+        if (d.normals) {
+          s.flags |= SHAPE_HAS_NORMALS;
+          for (uint32_t v = 0; v < nv; v++) {
+            f3 n = mk(d.normals[3 * v], d.normals[3 * v + 1], d.normals[3 * v + 2]);
+            if (!ident) n = tl.TVec(n);  // ApplyNormalToWorld, not renormalised (mesh.cpp:36-42)
+            out.nrm.insert(out.nrm.end(), {n.x, n.y, n.z});
+          }
+        }
+        if (d.uvs) {
+          s.flags |= SHAPE_HAS_UVS;
+          out.uv.insert(out.uv.end(), d.uvs, d.uvs + 2 * size_t(nv));
+        }
+        for (uint32_t t = 0; t < 3 * nt; t++) {
+          if (d.indices[t] >= nv) throw std::runtime_error("mesh index out of range");
+          out.idx.push_back(d.indices[t]);
+        }
+      } else {  // cube.cpp:15-123 (its UVs are never written there; zero here)
+        nv = 24;
+        nt = 12;
+        s.flags |= SHAPE_HAS_NORMALS | SHAPE_HAS_UVS;
+        for (int v = 0; v < 24; v++) {
+          f3 p = tw.Point(mk(kCubeV[v][0], kCubeV[v][1], kCubeV[v][2]));
+          out.pos.insert(out.pos.end(), {p.x, p.y, p.z});
+          const float* n = kCubeN[v / 4];
+          f3 nn = tl.TVec(mk(n[0], n[1], n[2]));
+          out.nrm.insert(out.nrm.end(), {nn.x, nn.y, nn.z});
+          out.uv.insert(out.uv.end(), {0.f, 0.f});
+        }
+        for (int t = 0; t < 12; t++)
+          for (int k = 0; k < 3; k++) out.idx.push_back(kCubeT[t][k]);
+      }
+      // keep position / normal / uv arrays index-aligned with pos for every mesh
+      if (!(s.flags & SHAPE_HAS_NORMALS)) out.nrm.resize(out.pos.size(), 0.f);
+      if (!(s.flags & SHAPE_HAS_UVS)) out.uv.resize(out.pos.size() / 3 * 2, 0.f);
+      s.tri_count = nt;
+      // triangle-area CDF accumulated in double (distribution.cpp:19-30, mesh_base.cpp:120-131)
+      s.cdf_offset = uint32_t(out.light_cdf.size() / 2);  // (cumulative, area) pairs
+      double sum = 0;
+      const float* P = out.pos.data() + 3 * size_t(s.vtx_offset);
+      const uint32_t* I = out.idx.data() + 3 * size_t(s.tri_offset);
+      for (uint32_t t = 0; t < nt; t++) {
+        f3 p0 = mk(P[3 * I[3 * t]], P[3 * I[3 * t] + 1], P[3 * I[3 * t] + 2]);
+        f3 p1 = mk(P[3 * I[3 * t + 1]], P[3 * I[3 * t + 1] + 1], P[3 * I[3 * t + 1] + 2]);
+        f3 p2 = mk(P[3 * I[3 * t + 2]], P[3 * I[3 * t + 2] + 1], P[3 * I[3 * t + 2] + 2]);
+        float area = norm(cross(p1 - p0, p2 - p0)) * 0.5f;
+        sum += double(area);
+        if (d.light >= 0) {
+          out.light_cdf.push_back(float(sum));
+          out.light_cdf.push_back(area);
+        }
+        float b[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
+        GrowBounds(b, p0);
+        GrowBounds(b, p1);
+        GrowBounds(b, p2);
+        out.prim_bounds.insert(out.prim_bounds.end(), b, b + 6);
+        out.prim_records.push_back(F4(p0, AsF(si)));
+        out.prim_records.push_back(F4(p1, AsF(t)));
+        out.prim_records.push_back(F4(p2, AsF(PRIM_TRI)));
+        GrowBounds(wb, p0);
+        GrowBounds(wb, p1);
+        GrowBounds(wb, p2);
+      }
+      s.cdf_sum = float(sum);
+      s.cdf_norm = float(1.0 / sum);
+      s.area = float(sum);
+    } else if (d.type == RAD_SHAPE_SPHERE) {  // sphere.cpp:173-190
+      s.type = SHAPE_SPHERE;
+      float la2 = sqnorm(tw.Vec(mk(1, 0, 0))), lb2 = sqnorm(tw.Vec(mk(0, 1, 0))), lc2 = sqnorm(tw.Vec(mk(0, 0, 1)));
+      if (std::fabs(la2 - lb2) > 0.0001f || std::fabs(la2 - lc2) > 0.0001f)
+        throw std::runtime_error("Sphere does not support non uniform transformations");
+      s.center = tw.Point(mk(d.center[0], d.center[1], d.center[2]));
+      s.radius = norm(tw.Vec(mk(0, 0, d.radius)));
+      M4 ts = M4::Identity();
+      ts.m[0] = ts.m[5] = ts.m[10] = s.radius;
+      ts.m[3] = s.center.x;
+      ts.m[7] = s.center.y;
+      ts.m[11] = s.center.z;
+      M4 sw = ts * tw;
+      s.to_world = sw.ToAffine();
+      s.to_local = sw.Inverse().ToAffine();
+      s.area = 4.0f * kPi * sqr(s.radius);
+      float b[6];
+      b[0] = s.center.x - s.radius;
+      b[1] = s.center.y - s.radius;
+      b[2] = s.center.z - s.radius;
+      b[3] = s.center.x + s.radius;
+      b[4] = s.center.y + s.radius;
+      b[5] = s.center.z + s.radius;
+      out.prim_bounds.insert(out.prim_bounds.end(), b, b + 6);
+      out.prim_records.push_back(F4(s.center, s.radius));
+      out.prim_records.push_back(make_float4(0, 0, 0, AsF(si)));
+      out.prim_records.push_back(make_float4(0, 0, 0, AsF(PRIM_SPHERE)));
+      GrowBounds(wb, mk(b[0], b[1], b[2]));
+      GrowBounds(wb, mk(b[3], b[4], b[5]));
+    } else if (d.type == RAD_SHAPE_RECTANGLE) {  // rectangle.cpp:164-179
+      s.type = SHAPE_RECTANGLE;
+      M4 t = tw;
+      if (d.flip_normal) {
+        M4 f = M4::Identity();
+        f.m[10] = -1.f;
+        t = tw * f;
+      }
+      M4 tl = t.Inverse();
+      s.to_world = t.ToAffine();
+      s.to_local = tl.ToAffine();
+      f3 dpdu = t.Vec(mk(2, 0, 0)), dpdv = t.Vec(mk(0, 2, 0));
+      s.frame_s = dpdu;
+      s.frame_t = dpdv;
+      s.frame_n = normalize(tl.TVec(mk(0, 0, 1)));
+      s.area = norm(cross(dpdu, dpdv));
+      float b[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
+      GrowBounds(b, t.Point(mk(-1, -1, 0)));
+      GrowBounds(b, t.Point(mk(1, -1, 0)));
+      GrowBounds(b, t.Point(mk(1, 1, 0)));
+      GrowBounds(b, t.Point(mk(-1, 1, 0)));
+      out.prim_bounds.insert(out.prim_bounds.end(), b, b + 6);
+      out.prim_records.push_back(make_float4(0, 0, 0, 0));
+      out.prim_records.push_back(make_float4(0, 0, 0, AsF(si)));
+      out.prim_records.push_back(make_float4(0, 0, 0, AsF(PRIM_RECT)));
+      GrowBounds(wb, mk(b[0], b[1], b[2]));
+      GrowBounds(wb, mk(b[3], b[4], b[5]));
+    } else {
+      throw std::runtime_error("unknown shape type " + std::to_string(d.type));
+    }
+    out.shapes.push_back(s);
+  }
+  for (auto& l : out.lights)
+    if (l.type == LIGHT_AREA && l.shape < 0) throw std::runtime_error("area light without a shape");
+  for (int a = 0; a < 3; a++) {
+    out.world_lo[a] = wb[a];
+    out.world_hi[a] = wb[3 + a];
+  }
+  // camera (perspective.cpp:15-70)
+  const rad_camera_desc& c = desc.camera;
+  if (c.width <= 0 || c.height <= 0) throw std::runtime_error("camera resolution must be positive");
+  CameraRec cam{};
+  cam.near_clip = c.near_clip;
+  cam.far_clip = c.far_clip;
+  cam.width = c.width;
+  cam.height = c.height;
+  cam.rcp_w = 1.0f / c.width;
+  cam.rcp_h = 1.0f / c.height;
+  float aspect = c.width / float(c.height);
+  float recip = 1.0f / (c.far_clip - c.near_clip);
+  float cot = 1.0f / std::tan((c.fov / 2.0f) * (kPi / 180.0f));
+  float P[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, c.far_clip * recip, -c.near_clip * c.far_clip * recip, 0, 0, 1, 0};
+  float S[16] = {-0.5f, 0, 0, 0, 0, -0.5f * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  float T[16] = {1, 0, 0, -1, 0, 1, 0, -1.0f / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
+  M4 toClip = M4::From(S) * M4::From(T) * M4::From(P);
+  M4 inv = toClip.Inverse();
+  std::memcpy(cam.clip_to_camera.m, inv.m, sizeof(inv.m));
+  cam.camera_to_world = M4::From(c.to_world).ToAffine();
+  out.cam = cam;
+}
+
+}  // namespace rad
+
+uint64_t rad_scene::device_bytes() const {
+  return d_nodes.bytes + d_prims.bytes + d_shapes.bytes + d_pos.bytes + d_nrm.bytes + d_uv.bytes + d_idx.bytes +
+         d_bsdfs.bytes + d_lights.bytes + d_cdf.bytes + d_media.bytes + d_grid.bytes;
+}

@@ -1,0 +1,81 @@
+// Host-side scene assembly: turns a rad_scene_desc into the device records of
+// scene_dev.h, builds the BVH and uploads everything to HBM.  Mirrors the
+// object construction of BuildContext::Build (build_context.cpp:88-321) and
+// the shape / light / medium constructors it calls.
+#pragma once
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rad_hip.h"
+#include "bvh_build.h"
+#include "scene_dev.h"
+
+struct rad_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string arch;
+  int cu_count = 0;
+};
+
+namespace rad {
+
+struct HostScene {
+  std::vector<ShapeRec> shapes;
+  std::vector<float> pos, nrm, uv;
+  std::vector<uint32_t> idx;
+  std::vector<BsdfRec> bsdfs;
+  std::vector<LightRec> lights;
+  std::vector<float> light_cdf;
+  std::vector<MediumRec> media;
+  std::vector<float> grid;
+  int32_t env_light = -1;
+  int32_t global_medium = -1;
+  CameraRec cam{};
+  // primitive list before BVH ordering
+  std::vector<float> prim_bounds;       // 6 per primitive
+  std::vector<float4> prim_records;     // 3 per primitive
+  float world_lo[3], world_hi[3];
+  bool quirks_default = true;
+};
+
+// Throws std::runtime_error on invalid input.
+void AssembleScene(const rad_scene_desc& desc, HostScene& out);
+
+// Reference-default BSDF record from the desc (eta derivations of disney.cpp:86-101 etc.).
+BsdfRec MakeBsdfRec(const rad_bsdf_desc& d);
+
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { reset(); }
+  void reset();
+  void alloc(size_t n);
+  void upload(const void* src, size_t n, hipStream_t s);
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
+std::string HipErr(hipError_t e);
+void HipCheck(hipError_t e, const char* what);
+
+}  // namespace rad
+
+struct rad_scene {
+  rad_ctx* ctx = nullptr;
+  rad::HostScene host;
+  rad::DevScene dev{};
+  rad::DeviceBuffer d_nodes, d_prims, d_shapes, d_pos, d_nrm, d_uv, d_idx, d_bsdfs, d_lights, d_cdf, d_media, d_grid;
+  rad::DeviceBuffer d_rays, d_hits, d_occ;  // staging for the host-pointer intersect entry points
+  std::mutex mu;
+  bool bvh_ready = false;
+  double bvh_ms = 0;
+  uint32_t bvh_depth = 0;
+  float sah_cost = 0;
+  uint64_t device_bytes() const;
+};

@@ -1,0 +1,282 @@
+// Per-vertex device helpers of the integrators: counter-based RNG, camera ray
+// generation, surface-interaction reconstruction, light sampling / pdfs.
+#pragma once
+
+#include "bsdf_dev.h"
+#include "traverse.h"
+
+namespace rad {
+
+// ---------------------------------------------------------------- RNG
+// Independent uniform samples keyed by (seed, request, pixel, sample, dim):
+// the stream of a sample never depends on which other work shares the GPU.
+RAD_HD uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+RAD_HD uint32_t sample_key(uint32_t seed, uint32_t request, uint32_t pixel, uint32_t sample) {
+  uint32_t k = mix32(seed ^ mix32(request * 0x9e3779b9u + 0x85ebca6bu));
+  k = mix32(k ^ pixel);
+  k = mix32(k ^ (sample * 0xc2b2ae35u + 0x27d4eb2fu));
+  return k;
+}
+RAD_HD float rng_u(uint32_t key, uint32_t dim) {
+  uint32_t h = mix32(key ^ mix32(dim * 0x9e3779b9u + 0x632be5abu));
+  return float(h >> 8) * 0x1p-24f;  // [0, 1)
+}
+
+// ---------------------------------------------------------------- camera (perspective.cpp:73-82)
+__device__ inline void camera_ray(const CameraRec& c, float sx, float sy, f3& o, f3& d, float& tmin, float& tmax) {
+  f3 ndc = mk(sx * c.rcp_w, sy * c.rcp_h, 0.f);
+  f3 nearP = c.clip_to_camera.point_h(ndc);
+  f3 dir = normalize(nearP);
+  float invZ = 1.0f / dir.z;
+  o = c.camera_to_world.point(mk(0.f, 0.f, 0.f));
+  d = normalize(c.camera_to_world.vec(dir));
+  tmin = c.near_clip * invZ;
+  tmax = c.far_clip * invZ;
+}
+
+// ---------------------------------------------------------------- surface interaction
+struct SurfaceHit {
+  f3 p;      // position
+  f3 n;      // geometric normal
+  Frame sh;  // shading frame
+  f3 wi;     // -ray direction in the shading frame
+  int32_t shape;
+};
+
+RAD_HD f3 load3(const float* a, size_t i) { return mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+
+// HitShapeRecord::ComputeSurfaceInteraction (interaction.cpp:121-127) with the
+// per-shape ComputeInteraction and ComputeShadingFrame (interaction.cpp:45-53).
+__device__ inline SurfaceHit surface_hit(const DevScene& s, f3 ro, f3 rd, float t, float u, float v, uint32_t prim,
+                                         uint32_t geom) {
+  const ShapeRec& sr = s.shapes[geom];
+  SurfaceHit h;
+  h.shape = int32_t(geom);
+  f3 dpdu;
+  if (sr.type == SHAPE_MESH) {  // mesh_base.cpp:38-82
+    const uint32_t* I = s.idx + 3 * size_t(sr.tri_offset + prim);
+    size_t f0 = sr.vtx_offset + I[0], f1 = sr.vtx_offset + I[1], f2 = sr.vtx_offset + I[2];
+    f3 p0 = load3(s.pos, f0), p1 = load3(s.pos, f1), p2 = load3(s.pos, f2);
+    float b0 = 1.f - u - v;
+    f3 dp0 = p1 - p0, dp1 = p2 - p0;
+    h.p = p0 * b0 + (p1 * u + (p2 * v));
+    h.n = normalize(cross(dp0, dp1));
+    if (sr.flags & SHAPE_HAS_UVS) {
+      float u0 = s.uv[2 * f0], v0 = s.uv[2 * f0 + 1];
+      float u1 = s.uv[2 * f1], v1 = s.uv[2 * f1 + 1];
+      float u2 = s.uv[2 * f2], v2 = s.uv[2 * f2 + 1];
+      float du0 = u1 - u0, dv0 = v1 - v0, du1 = u2 - u0, dv1 = v2 - v0;
+      float det = du0 * dv1 - (dv0 * du1);
+      float invDet = det == 0.f ? 0.f : 1.f / det;
+      dpdu = invDet == 0.f ? splat(0.f) : (dv1 * dp0 - (dv0 * dp1)) * invDet;
+    } else {
+      f3 y;
+      coordinate_system(h.n, dpdu, y);
+    }
+    if (sr.flags & SHAPE_HAS_NORMALS) {
+      f3 n0 = load3(s.nrm, f0), n1 = load3(s.nrm, f1), n2 = load3(s.nrm, f2);
+      f3 sn = n0 * b0 + (n1 * u + (n2 * v));
+      h.sh.n = sn * (1.f / sqrtf(sqnorm(sn)));
+    } else {
+      h.sh.n = h.n;
+    }
+  } else if (sr.type == SHAPE_SPHERE) {  // sphere.cpp:97 (normal at the hit) and :215-249
+    f3 ng = normalize((rd * t + ro) - sr.center);
+    h.sh.n = ng;
+    h.p = fmadd(ng, splat(sr.radius), sr.center);
+    f3 local = sr.to_local.point(h.p);
+    dpdu = sr.to_world.vec(mk(-local.y, local.x, 0.f) * (2.f * kPi));
+    h.n = ng;
+  } else {  // rectangle.cpp:204-219
+    f3 p = rd * t + ro;
+    f3 tr = mk(sr.to_world.m[3], sr.to_world.m[7], sr.to_world.m[11]);
+    h.p = sr.frame_n * dot(tr - p, sr.frame_n) + p;
+    h.n = sr.frame_n;
+    h.sh.n = sr.frame_n;
+    dpdu = sr.frame_s;
+  }
+  if (is_zero(dpdu)) {
+    f3 y;
+    coordinate_system(h.sh.n, h.sh.s, y);
+  } else {
+    h.sh.s = normalize(h.sh.n * (-dot(h.sh.n, dpdu)) + dpdu);
+  }
+  h.sh.t = cross(h.sh.n, h.sh.s);
+  h.wi = h.sh.to_local(-rd);
+  return h;
+}
+
+// Interaction::OffsetP (interaction.cpp:10-14)
+RAD_HD f3 offset_p(f3 p, f3 n, f3 d) {
+  float mag = (1.f + maxc(absv(p))) * kRayEps;
+  mag = mulsign(mag, dot(n, d));
+  return fmadd(splat(mag), n, p);
+}
+
+// ---------------------------------------------------------------- lights (scene.cpp:65-118, light/*.cpp)
+struct LightSample {
+  f3 p, n, dir;
+  float dist, pdf;
+  f3 li;
+  bool delta;
+};
+
+// DiscreteDistribution1D::SampleReuse over a mesh emitter's area CDF
+// (distribution.cpp:35-53); cdf holds (cumulative, area) pairs.
+__device__ inline uint32_t sample_reuse(const float* cdf, uint32_t n, float sum, float norm, float xi, float& rem) {
+  float value = xi * sum;
+  uint32_t lo = 0, hi = n;  // lower_bound
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (cdf[2 * mid] < value)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  uint32_t index = min(lo, n - 1);
+  float pmf = cdf[2 * index + 1] * norm;
+  float c = index == 0 ? 0.f : cdf[2 * (index - 1)] * norm;
+  rem = (xi - c) / pmf;
+  return index;
+}
+
+__device__ inline LightSample sample_area_shape(const DevScene& s, const ShapeRec& sr, f3 refp, float u1, float u2) {
+  LightSample ls;
+  ls.delta = false;
+  if (sr.type == SHAPE_SPHERE) {  // sphere.cpp:265-320
+    f3 dcv = sr.center - refp;
+    float dc2 = sqnorm(dcv);
+    float radAdj = sr.radius * (1.f - kRayEps);
+    if (dc2 > sqr(radAdj)) {
+      float invDc = 1.f / sqrtf(dc2);
+      float sinMax = sr.radius * invDc;
+      float sinMax2 = sqr(sinMax);
+      float invSinMax = 1.f / sinMax;
+      float cosMax = safe_sqrt(1.f - sinMax2);
+      float sin2 = sinMax2 > 0.00068523262f ? 1.f - sqr(fmaf(cosMax - 1.f, u1, 1.f)) : sinMax2 * u1;
+      float cosT = safe_sqrt(1.f - sin2);
+      float cosA = sin2 * invSinMax + cosT * safe_sqrt(fmaf(-sin2, sqr(invSinMax), 1.f));
+      float sinA = safe_sqrt(fmaf(-cosA, cosA, 1.f));
+      float ang = u2 * (2.f * kPi);
+      Frame fr = frame_from_n(dcv * -invDc);
+      f3 d = fr.to_world(mk(cosf(ang) * sinA, sinf(ang) * sinA, cosA));
+      ls.p = fmadd(d, splat(sr.radius), sr.center);
+      ls.n = d;
+      ls.dir = ls.p - refp;
+      ls.dist = sqrtf(sqnorm(ls.dir));
+      ls.dir = ls.dir / ls.dist;
+      ls.pdf = 1.f / (2.f * kPi) / (1.f - cosMax);
+    } else {
+      float z = fmaf(-2.f, u2, 1.f), r = safe_sqrt(fmaf(-z, z, 1.f)), a = 2.f * kPi * u1;
+      f3 d = mk(r * cosf(a), r * sinf(a), z);
+      ls.p = fmadd(d, splat(sr.radius), sr.center);
+      ls.n = d;
+      ls.dir = ls.p - refp;
+      float dist2 = sqnorm(ls.dir);
+      ls.dist = sqrtf(dist2);
+      ls.dir = ls.dir / ls.dist;
+      ls.pdf = (1.f / sr.area) * dist2 / fabsf(dot(ls.dir, ls.n));
+    }
+    ls.delta = sr.radius == 0.f;
+    return ls;
+  }
+  float pdfA;
+  if (sr.type == SHAPE_RECTANGLE) {  // rectangle.cpp:221-230
+    ls.p = sr.to_world.point(mk(u1 * 2.f - 1.f, u2 * 2.f - 1.f, 0.f));
+    ls.n = sr.frame_n;
+    pdfA = 1.f / sr.area;
+  } else {  // mesh_base.cpp:84-112
+    float rem;
+    uint32_t tri = sample_reuse(s.light_cdf + 2 * size_t(sr.cdf_offset), sr.tri_count, sr.cdf_sum, sr.cdf_norm, u2, rem);
+    const uint32_t* I = s.idx + 3 * size_t(sr.tri_offset + tri);
+    size_t f0 = sr.vtx_offset + I[0], f1 = sr.vtx_offset + I[1], f2 = sr.vtx_offset + I[2];
+    f3 p0 = load3(s.pos, f0), p1 = load3(s.pos, f1), p2 = load3(s.pos, f2);
+    f3 e0 = p1 - p0, e1 = p2 - p0;
+    float t = safe_sqrt(1.f - u1);
+    float bx = 1.f - t, by = t * rem;
+    ls.p = e0 * bx + (e1 * by + p0);
+    pdfA = sr.cdf_norm;
+    if (sr.flags & SHAPE_HAS_NORMALS) {
+      f3 n0 = load3(s.nrm, f0), n1 = load3(s.nrm, f1), n2 = load3(s.nrm, f2);
+      ls.n = normalize(n0 * (1.f - bx - by) + (n1 * bx + (n2 * by)));
+    } else {
+      ls.n = normalize(cross(e0, e1));
+    }
+  }
+  // Shape::SampleDirection (shape.cpp:7-18)
+  ls.dir = ls.p - refp;
+  float d2 = sqnorm(ls.dir);
+  ls.dist = sqrtf(d2);
+  ls.dir = ls.dir / ls.dist;
+  float dp = fabsf(dot(ls.dir, ls.n));
+  float x = d2 / dp;
+  ls.pdf = pdfA * (isfinite(x) ? x : 0.f);
+  return ls;
+}
+
+// Scene::SampleLightDirection (scene.cpp:87-96)
+__device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul, float u1, float u2) {
+  uint32_t index = 0;
+  float lpdf = 1.f;
+  if (s.light_count >= 2) {
+    index = min(uint32_t(ul * float(s.light_count)), s.light_count - 1);
+    lpdf = s.light_pdf;
+  }
+  const LightRec& L = s.lights[index];
+  LightSample ls;
+  if (L.type == LIGHT_SKYBOX) {  // skybox.cpp:65-88 (constant map)
+    float radius = fmaxf(s.world_radius, norm(refp - s.world_center));
+    float dist = 2.f * radius;
+    float z = fmaf(-2.f, u2, 1.f), r = safe_sqrt(fmaf(-z, z, 1.f)), a = 2.f * kPi * u1;
+    f3 dir = normalize(mk(r * cosf(a), r * sinf(a), z));
+    f3 wd = L.to_world.vec(dir);
+    ls.p = wd * dist + refp;
+    ls.n = -wd;
+    ls.dir = wd;
+    ls.dist = dist;
+    ls.pdf = 1.f / (4.f * kPi);
+    ls.delta = false;
+    ls.li = L.radiance;
+  } else {  // light/area.cpp:50-75
+    ls = sample_area_shape(s, s.shapes[L.shape], refp, u1, u2);
+    ls.li = dot(-ls.dir, ls.n) >= 0.f ? L.radiance : splat(0.f);
+  }
+  ls.pdf *= lpdf;
+  return ls;
+}
+
+// Light::PdfDirection * 1/N for a light hit from `refp` (scene.cpp:98-103).
+__device__ inline float light_pdf_dir(const DevScene& s, int32_t light, f3 refp, f3 p, f3 n_shading, f3 dir,
+                                      float dist) {
+  const LightRec& L = s.lights[light];
+  if (L.type == LIGHT_SKYBOX) return (1.f / (4.f * kPi)) * s.light_pdf;
+  if (!(dot(-dir, n_shading) >= 0.f)) return 0.f;  // area.cpp:77-80
+  const ShapeRec& sr = s.shapes[L.shape];
+  float pdf;
+  if (sr.type == SHAPE_SPHERE) {  // sphere.cpp:322-328
+    float sinA = sr.radius * (1.f / norm(sr.center - refp));
+    float cosA = safe_sqrt(1.f - sinA * sinA);
+    pdf = sinA < kOneMinusEps ? 1.f / (2.f * kPi) / (1.f - cosA) : (1.f / sr.area) * sqr(dist) / fabsf(dot(dir, n_shading));
+  } else {  // shape.cpp:20-25
+    float pa = sr.type == SHAPE_RECTANGLE ? 1.f / sr.area : sr.cdf_norm;
+    float dp = fabsf(dot(dir, n_shading));
+    pdf = pa * (dp == 0.f ? 0.f : sqr(dist) / dp);
+  }
+  return pdf * s.light_pdf;
+}
+
+RAD_HD float mis_weight(float a, float b) {  // power heuristic (path.cpp:231-236)
+  a *= a;
+  b *= b;
+  float w = a / (a + b);
+  return isfinite(w) ? w : 0.f;
+}
+
+}  // namespace rad

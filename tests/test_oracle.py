@@ -1,0 +1,299 @@
+"""Pins for the CPU restatement (the oracle) before it judges the GPU.
+
+The reference ships no tests or golden vectors (its scenes are untracked), so
+each oracle function is pinned by closed forms, invariants, special cases and
+brute force, chosen so that a plausible mistake fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from radiantiumplusplus_amd import _ffi as F
+from radiantiumplusplus_amd import scenes
+
+RNG = np.random.default_rng(1234)
+
+
+def unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def tiny_scene():
+    """One triangle (geom 0), one sphere (geom 1), one rectangle light (geom 2)."""
+    b = scenes.SceneBuilder()
+    m = b.diffuse(0.5)
+    b.mesh(np.array([[0, 0, 5], [1, 0, 5], [0, 1, 5]], np.float32), np.array([0, 1, 2], np.uint32), bsdf=m)
+    b.sphere(center=(3, 0, 4), radius=1.0, bsdf=m)
+    light = b.rectangle(to_world=scenes.trs(translate=(0, 0, 10), scale=(2, 2, 2)))
+    b.area_light(light, 1.0)
+    b.set_camera(8, 8, 45, origin=(0, 0, -1), target=(0, 0, 0), far=1000)
+    return b
+
+
+def ray(o, d, tmin=0.0, tmax=1e30):
+    r = np.zeros(1, O.RAY_DTYPE)
+    r["o"], r["d"], r["tmin"], r["tmax"] = o, d, tmin, tmax
+    return r
+
+
+# ------------------------------------------------------------------ intersection
+def test_triangle_closed_form():
+    s = O.OracleScene(tiny_scene())
+    h = s.closest(ray((0.25, 0.25, 0), (0, 0, 1)), brute=True)[0]
+    assert h["geom"] == 0 and h["prim"] == 0
+    assert h["t"] == pytest.approx(5.0)
+    # Embree convention: p = (1-u-v) v0 + u v1 + v v2
+    assert h["u"] == pytest.approx(0.25) and h["v"] == pytest.approx(0.25)
+    h = s.closest(ray((0.6, 0.1, 0), (0, 0, 1)), brute=True)[0]
+    assert h["u"] == pytest.approx(0.6) and h["v"] == pytest.approx(0.1)
+    # outside the triangle: passes through to the light at z = 10
+    h = s.closest(ray((0.8, 0.8, 0), (0, 0, 1)), brute=True)[0]
+    assert h["geom"] == 2 and h["t"] == pytest.approx(10.0)
+
+
+def test_sphere_closed_form():
+    s = O.OracleScene(tiny_scene())
+    h = s.closest(ray((3, 0, 0), (0, 0, 1)), brute=True)[0]
+    assert h["geom"] == 1 and h["t"] == pytest.approx(3.0)
+    # ray starting inside the sphere reports the far root
+    h = s.closest(ray((3, 0, 4), (0, 1, 0)), brute=True)[0]
+    assert h["geom"] == 1 and h["t"] == pytest.approx(1.0)
+    # tmax shorter than the hit: miss
+    h = s.closest(ray((3, 0, 0), (0, 0, 1), tmax=2.5), brute=True)[0]
+    assert h["geom"] == 0xFFFFFFFF
+
+
+def test_rectangle_closed_form_and_bounds():
+    s = O.OracleScene(tiny_scene())
+    h = s.closest(ray((1.9, -1.9, 0), (0, 0, 1)), brute=True)[0]
+    assert h["geom"] == 2 and h["t"] == pytest.approx(10.0)
+    assert h["u"] == pytest.approx(0.95) and h["v"] == pytest.approx(-0.95)
+    h = s.closest(ray((-2.1, 0, 0), (0, 0, 1)), brute=True)[0]
+    assert h["geom"] == 0xFFFFFFFF
+
+
+def random_rays(lo, hi, n, rng=RNG):
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = unit(rng.normal(size=(n, 3))).astype(np.float32)
+    r = np.zeros(n, O.RAY_DTYPE)
+    r["o"], r["d"], r["tmin"], r["tmax"] = o, d, 0.0, 1e30
+    return r
+
+
+def test_bvh_matches_brute_force():
+    b = scenes.many_balls(n_spheres=24, slices=16, width=8, height=8)
+    s = O.OracleScene(b)
+    lo, hi = s.bounds()
+    rays = random_rays(lo, hi, 4000)
+    hb = s.closest(rays, brute=True)
+    hv = s.closest(rays, brute=False)
+    same = (hb["geom"] == hv["geom"]) & (hb["prim"] == hv["prim"])
+    # identical primitive test: any disagreement must be an exact-t tie
+    assert np.all(same | (hb["t"] == hv["t"]))
+    assert np.array_equal(s.occluded(rays, brute=True), s.occluded(rays, brute=False))
+    assert (hb["geom"] != 0xFFFFFFFF).mean() > 0.2
+
+
+# ------------------------------------------------------------------ camera
+def test_camera_horizontal_fov_and_orientation():
+    """fov is horizontal; screen x = 0 looks along +left, y = 0 along +up
+    (perspective.cpp:33-52)."""
+    W, H, fov = 40, 20, 60.0
+    b = tiny_scene()
+    b.set_camera(W, H, fov, to_world=np.eye(4, dtype=np.float32), far=100)
+    s = O.OracleScene(b)
+    t = math.tan(math.radians(fov / 2))
+    aspect = W / H
+    r = s.camera_ray(0.0, 0.0)
+    assert np.allclose(r["d"], unit(np.array([t, t / aspect, 1.0])), atol=1e-6)
+    r = s.camera_ray(W, H)
+    assert np.allclose(r["d"], unit(np.array([-t, -t / aspect, 1.0])), atol=1e-6)
+    r = s.camera_ray(W / 2, H / 2)
+    assert np.allclose(r["d"], [0, 0, 1], atol=1e-6)
+    # near / far clip expressed along the ray (MinT = near/dz, MaxT = far/dz)
+    r = s.camera_ray(0.0, 0.0)
+    dz = r["d"][2]
+    assert r["tmin"] == pytest.approx(0.001 / dz) and r["tmax"] == pytest.approx(100 / dz)
+
+
+# ------------------------------------------------------------------ BSDFs
+def bsdf(kind, **kw):
+    b = scenes.SceneBuilder()
+    i = getattr(b, kind)(**kw)
+    return b.bsdf_desc(b.bsdfs[i])
+
+
+def hemi(n, rng=RNG, lower=False):
+    d = unit(rng.normal(size=(n, 3)))
+    d[:, 2] = np.abs(d[:, 2]) * (-1 if lower else 1)
+    return d.astype(np.float32)
+
+
+def test_diffuse_eval_closed_form():
+    d = bsdf("diffuse", reflectance=(0.2, 0.4, 0.8))
+    wi, wo = hemi(64), hemi(64)
+    out = O.bsdf_eval(d, wi, wo)
+    assert np.allclose(out[:, :3], np.array([0.2, 0.4, 0.8]) / np.pi * wo[:, 2:3], rtol=1e-6)
+    assert np.allclose(out[:, 3], wo[:, 2] / np.pi, rtol=1e-6)
+    out = O.bsdf_eval(d, wi, hemi(64, lower=True))
+    assert np.all(out == 0)
+
+
+def normal_incidence(desc):
+    z = np.tile(np.array([[0, 0, 1]], np.float32), (1, 1))
+    return O.bsdf_eval(desc, z, z)[0]
+
+
+def test_conductor_fresnel_at_normal_incidence():
+    """D(z) = 1/(pi a^2), G = 1, so f(z, z) = F(1) / (4 pi a^2) with
+    F(1) = ((eta-1)^2 + k^2) / ((eta+1)^2 + k^2)."""
+    eta, k, a = np.array([0.2, 0.9, 1.1]), np.array([3.9, 2.4, 2.1]), 0.3
+    d = bsdf("rough_metal", alpha=a, eta=eta, k=k)
+    f = normal_incidence(d)[:3]
+    F1 = ((eta - 1) ** 2 + k ** 2) / ((eta + 1) ** 2 + k ** 2)
+    assert np.allclose(f * 4 * np.pi * a * a, F1, rtol=1e-4)
+
+
+def test_dielectric_fresnel_at_normal_incidence():
+    a, eta = 0.25, 1.5
+    d = bsdf("rough_glass", alpha=a, int_ior=eta, ext_ior=1.0)
+    f = normal_incidence(d)[:3]
+    F1 = ((eta - 1) / (eta + 1)) ** 2
+    assert np.allclose(f * 4 * np.pi * a * a, F1, rtol=1e-4)
+
+
+@pytest.mark.parametrize("kind,kw", [
+    ("diffuse", {}),
+    ("rough_metal", {"alpha": 0.3}),
+    ("rough_glass", {"alpha": 0.3}),
+    ("disney", {"base_color": (0.6, 0.4, 0.3), "roughness": 0.4, "metallic": 0.3}),
+    ("disney", {"base_color": (0.6, 0.4, 0.3), "roughness": 0.3, "spec_trans": 0.7, "clearcoat": 0.5,
+                "clearcoat_gloss": 0.5, "sheen": 0.4, "sheen_tint": 0.5, "spec_tint": 0.2, "flatness": 0.3}),
+])
+def test_sample_is_consistent_with_eval(kind, kw):
+    """Sample() returns the same f and pdf as Eval()/Pdf() at the sampled wo."""
+    d = bsdf(kind, **kw)
+    n = 2000
+    wi = hemi(n)
+    lobe, xi = RNG.random(n, dtype=np.float32), RNG.random((n, 2), dtype=np.float32)
+    s = O.bsdf_sample(d, wi, lobe, xi)
+    ok = s[:, 3] > 0
+    assert ok.mean() > 0.5
+    e = O.bsdf_eval(d, wi[ok], s[ok, :3])
+    assert np.allclose(e[:, 3], s[ok, 3], rtol=2e-3, atol=1e-6)
+    assert np.allclose(e[:, :3], s[ok, 4:7], rtol=2e-3, atol=1e-6)
+
+
+@pytest.mark.parametrize("kind,kw,total", [
+    ("diffuse", {}, 1.0),
+    ("rough_metal", {"alpha": 0.4}, None),
+    ("rough_glass", {"alpha": 0.4}, None),
+])
+def test_pdf_integrates_to_at_most_one(kind, kw, total):
+    d = bsdf(kind, **kw)
+    n = 400000
+    wi = np.tile(unit(np.array([[0.4, 0.1, 0.8]], np.float32)), (n, 1)).astype(np.float32)
+    wo = unit(RNG.normal(size=(n, 3))).astype(np.float32)
+    pdf = O.bsdf_eval(d, wi, wo)[:, 3]
+    integral = pdf.mean() * 4 * np.pi
+    if total is not None:
+        assert integral == pytest.approx(total, abs=0.01)
+    else:
+        assert 0.8 < integral < 1.01
+
+
+def albedo_by_sampling(d, wi, n, quirks):
+    wis = np.tile(wi, (n, 1)).astype(np.float32)
+    s = O.bsdf_sample(d, wis, RNG.random(n, dtype=np.float32), RNG.random((n, 2), dtype=np.float32), quirks=quirks)
+    w = np.where(s[:, 3:4] > 0, s[:, 4:7] / np.maximum(s[:, 3:4], 1e-30), 0)
+    return w.mean(0)
+
+
+def albedo_by_quadrature(d, wi, n):
+    wis = np.tile(wi, (n, 1)).astype(np.float32)
+    wo = unit(RNG.normal(size=(n, 3))).astype(np.float32)
+    f = O.bsdf_eval(d, wis, wo)[:, :3]
+    return f.mean(0) * 4 * np.pi
+
+
+def test_visible_normal_sampling_quirk():
+    """Frame::SinCosPhi (frame.cpp:48-56) clamps with cwiseMin(-1).cwiseMax(1),
+    so the azimuth of every visible-normal sample is computed with
+    (sin, cos) = (1, 1).  With quirks off the sampler is unbiased (E[f/pdf]
+    equals the quadrature of f); with quirks on it is measurably biased."""
+    d = bsdf("rough_metal", alpha=0.5)
+    wi = unit(np.array([[0.7, -0.3, 0.5]], np.float32))
+    ref = albedo_by_quadrature(d, wi, 1_000_000)
+    fixed = albedo_by_sampling(d, wi, 400_000, quirks=False)
+    quirky = albedo_by_sampling(d, wi, 400_000, quirks=True)
+    assert np.allclose(fixed, ref, rtol=0.02)
+    assert np.abs(quirky - ref).max() / ref.max() > 0.05
+
+
+# ------------------------------------------------------------------ heterogeneous grid
+def grid_desc(g):
+    m = F.MediumDesc()
+    g = np.ascontiguousarray(g, np.float32)
+    m.grid_size[:] = [g.shape[2], g.shape[1], g.shape[0]]
+    m.grid_channels = 1
+    m.grid_wrap = F.RAD_WRAP_CLAMP
+    m.grid = g.ctypes.data_as(F.FP)
+    m._keep = g
+    return m
+
+
+def test_grid_lookup_special_cases():
+    """volume/grid.cpp:41-75 is not trilinear: scale by (size-1), neighbour
+    chosen by the half-voxel offset, weights |frac - 0.5|."""
+    g2 = np.arange(8, dtype=np.float32).reshape(2, 2, 2)  # v[z, y, x] = 4z + 2y + x
+    m = grid_desc(g2)
+    out = O.grid_eval(m, np.array([[0, 0, 0], [1, 1, 1]], np.float32))[:, 0]
+    assert out[0] == pytest.approx(0.0)  # the corner voxel itself
+    # at the far corner every axis blends 50/50 with the previous voxel: the mean of all 8
+    assert out[1] == pytest.approx(g2.mean())
+    g3 = RNG.random((3, 3, 3)).astype(np.float32)
+    m3 = grid_desc(g3)
+    # 0.75 * (3-1) = 1.5 exactly: frac 0.5, zero weight on the neighbour -> the centre voxel
+    assert O.grid_eval(m3, np.array([[0.75, 0.75, 0.75]], np.float32))[0, 0] == pytest.approx(g3[1, 1, 1])
+    # constant grids reproduce the constant anywhere (weights sum to one)
+    mc = grid_desc(np.full((4, 5, 6), 2.5, np.float32))
+    assert np.allclose(O.grid_eval(mc, RNG.random((100, 3)).astype(np.float32)), 2.5)
+
+
+# ------------------------------------------------------------------ integrators
+def furnace_scene(rho, res=24):
+    b = scenes.SceneBuilder()
+    b.sphere(center=(0, 0, 0), radius=1.0, bsdf=b.diffuse(rho))
+    b.skybox(1.0)
+    b.set_camera(res, res, 40, origin=(0, 0, -4), target=(0, 0, 0), far=100)
+    b.set_render("path", spp=64, max_depth=-1, rr_depth=100, seed=7)
+    return b
+
+
+def test_path_furnace_closed_form():
+    """A convex diffuse object in a uniform environment L reflects rho * L;
+    the background shows L exactly."""
+    rho = 0.6
+    b = furnace_scene(rho)
+    s = O.OracleScene(b)
+    fb, sq, _ = s.render(threads=4)
+    c = fb[12, 12]
+    assert np.allclose(fb[0, 0], 1.0)
+    # centre pixel: the sphere
+    n = b.sampler["sample_count"]
+    var = sq[12, 12] / n - c ** 2
+    assert np.all(np.abs(c - rho) < 5 * np.sqrt(var / n) + 1e-3)
+    assert np.allclose(c, rho, atol=0.03)
+
+
+def test_sample_filter_counts_dropped_samples():
+    """Dropped samples still divide by spp (sample_renderer.cpp:101-114): a
+    pixel whose every sample is valid averages to the mean, so scaling spp
+    leaves a constant-radiance background unchanged."""
+    b = furnace_scene(0.5, res=8)
+    s = O.OracleScene(b)
+    fb1, _, _ = s.render(spp=4, threads=2)
+    fb2, _, _ = s.render(spp=16, threads=2)
+    assert np.allclose(fb1[0, 0], 1.0) and np.allclose(fb2[0, 0], 1.0)
