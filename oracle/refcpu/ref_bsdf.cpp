@@ -51,23 +51,37 @@ static Spectrum FresnelConductor(Float cosThetaI, const Spectrum& eta, const Spe
 }
 
 // ---------------------------------------------------------------- microfacet (microfacet.cpp)
-// erf^-1 for the Beckmann visible-normal sampler: Newton iterations on
-// std::erf from a logarithmic starting guess (an own derivation, not the
-// polynomial the reference uses; both are accurate to float precision).
+// erf^-1 for the Beckmann visible-normal sampler, from the mathematics rather
+// than the reference's polynomial: the argument enters through 1 - a^2
+// clamped at FLT_MIN as in math_ext.h:114-140 (so |a| >= 1 saturates), and
+// y = erfc^-1(c) with c = (1 - a^2) / (1 + sqrt(a^2)) is found by Newton on
+// log(erfc(y)) = log(c) in double precision.
 static Float ErfInv(Float a) {
-  double x = a;
-  if (x <= -1) return -Inf;
-  if (x >= 1) return Inf;
-  double w = -std::log((1.0 - x) * (1.0 + x));
-  double y = std::copysign(std::sqrt(std::max(0.0, w - std::log(std::max(w, 1e-300)) * 0.5) ), x) * 0.8;
-  for (int i = 0; i < 60; i++) {
-    double err = std::erf(y) - x;
-    double d = 2.0 / std::sqrt(3.14159265358979323846) * std::exp(-y * y);
-    double step = err / d;
-    y -= step;
-    if (std::abs(step) < 1e-15 * std::max(1.0, std::abs(y))) break;
+  if (std::abs(a) < Float(0.5)) {  // well-conditioned core: Newton on erf(y) = |a|
+    double A = std::abs(a), y = A * 0.886226925452758;
+    for (int i = 0; i < 30; i++) {
+      double step = (std::erf(y) - A) / (1.1283791670955126 * std::exp(-y * y));
+      y -= step;
+      if (std::abs(step) <= 1e-16 * std::max(1e-300, y)) break;
+    }
+    return std::copysign(Float(y), a);
   }
-  return (Float)y;
+  Float q = std::max(std::fma(a, -a, Float(1)), std::numeric_limits<Float>::min());
+  double Q = q;
+  double c = Q / (1.0 + std::sqrt(1.0 - Q));
+  if (c >= 1.0) return Float(0) * a;
+  double L = std::log(c);
+  double y = c > 0.5 ? (1.0 - c) * 0.886226925452758 : std::sqrt(std::max(-L - 0.5 * std::log(-L), 1e-6));
+  for (int i = 0; i < 60; i++) {
+    double e = std::erfc(y);
+    double f = std::log(e) - L;
+    double fp = -1.1283791670955126 * std::exp(-y * y) / e;
+    double step = f / fp;
+    y -= step;
+    if (y < 0.0) y = 0.0;
+    if (std::abs(step) <= 1e-15 * std::max(1.0, y)) break;
+  }
+  return std::copysign(Float(y), a);
 }
 
 struct Microfacet {
@@ -627,6 +641,8 @@ struct Disney final : Bsdf {
     return pdf;
   }
 };
+
+float RefErfInv(float a) { return ErfInv(a); }
 
 std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks) {
   auto spec = [](const float* v) { return Spectrum(v[0], v[1], v[2]); };

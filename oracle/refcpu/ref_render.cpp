@@ -633,6 +633,8 @@ int ref_bsdf_sample(const rad_bsdf_desc* d, int quirks, const float* in, uint32_
   }
 }
 
+float ref_erfinv(float a) { return ref::RefErfInv(a); }
+
 int ref_grid_eval(const rad_medium_desc* d, const float* in, uint32_t n, float* out) {
   GridVolume g;
   g.sx = d->grid_size[0];

@@ -252,6 +252,7 @@ struct Bsdf {
   virtual Float Pdf(const SurfaceInteraction& si, const V3& wo) const = 0;
 };
 std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks);
+float RefErfInv(float a);
 
 // ---- lights (render/light.h) ----
 struct Scene;

@@ -9,20 +9,28 @@
 namespace ref {
 
 // ---------------------------------------------------------------- triangles
+// Products of the primitive tests with every rounding spelled out (fused
+// multiply-adds in a fixed association): the device kernels evaluate the same
+// expressions, so hits agree bit for bit.
+static inline Float DotX(const V3& a, const V3& b) { return std::fma(a.x, b.x, std::fma(a.y, b.y, a.z * b.z)); }
+static inline V3 CrossX(const V3& a, const V3& b) {
+  return {std::fma(a.y, b.z, -(a.z * b.y)), std::fma(a.z, b.x, -(a.x * b.z)), std::fma(a.x, b.y, -(a.y * b.x))};
+}
+
 bool IntersectTriangleMT(const V3& v0, const V3& v1, const V3& v2, const Ray& ray, Float tnear, Float tfar,
                          Float& t, Float& u, Float& v) {
   V3 e1 = v0 - v1;
   V3 e2 = v2 - v0;
-  V3 Ng = Cross(e2, e1);
+  V3 Ng = CrossX(e2, e1);
   V3 C = v0 - ray.O;
-  V3 R = Cross(C, ray.D);
-  Float den = Dot(Ng, ray.D);
+  V3 R = CrossX(C, ray.D);
+  Float den = DotX(Ng, ray.D);
   Float absDen = std::abs(den);
-  Float U = Dot(R, e2);
-  Float V = Dot(R, e1);
+  Float U = DotX(R, e2);
+  Float V = DotX(R, e1);
   if (den < 0) { U = -U; V = -V; }
   if (!(den != 0 && U >= 0 && V >= 0 && U + V <= absDen)) return false;
-  Float T = Dot(Ng, C);
+  Float T = DotX(Ng, C);
   if (den < 0) T = -T;
   if (!(absDen * tnear < T && T <= absDen * tfar)) return false;
   t = T / absDen;
@@ -164,7 +172,7 @@ static std::tuple<bool, Float, Float> SolveQuadratic(Float a, Float b, Float c) 
   bool linearCase = a == 0.0f;
   bool validLinear = linearCase && b != 0.0f;
   Float x0 = -c / b, x1 = -c / b;
-  Float discrim = b * b - Float(4) * a * c;
+  Float discrim = std::fma(b, b, -((Float(4) * a) * c));
   bool validQuadratic = !linearCase && (discrim >= 0.0f);
   if (validQuadratic) {
     Float rootDiscrim = std::sqrt(discrim);
@@ -189,9 +197,9 @@ Bounds SphereShape::PrimBounds(uint32_t) const {
 // sphere.cpp:40-54, 71-120
 bool SphereShape::IntersectPrim(uint32_t, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const {
   V3 o = ray.O - center;
-  Float a = SquaredNorm(ray.D);
-  Float b = 2 * Dot(o, ray.D);
-  Float c = SquaredNorm(o) - Sqr(radius);
+  Float a = DotX(ray.D, ray.D);
+  Float b = 2 * DotX(o, ray.D);
+  Float c = std::fma(-radius, radius, DotX(o, o));
   auto [found, nearT, farT] = SolveQuadratic(a, b, c);
   bool outBounds = !(nearT <= far && farT >= near);
   bool inBounds = nearT < near && farT > far;
@@ -310,14 +318,14 @@ Bounds RectangleShape::PrimBounds(uint32_t) const {  // rectangle.cpp:48-64
 // rectangle.cpp:34-46: ray taken to object space with an unnormalised direction.
 bool RectangleShape::IntersectPrim(uint32_t, const Ray& ray, Float near, Float far, HitShapeRecord& rec) const {
   const auto& M = toObject;
-  V3 o(M[0][0] * ray.O.x + M[0][1] * ray.O.y + M[0][2] * ray.O.z + M[0][3],
-       M[1][0] * ray.O.x + M[1][1] * ray.O.y + M[1][2] * ray.O.z + M[1][3],
-       M[2][0] * ray.O.x + M[2][1] * ray.O.y + M[2][2] * ray.O.z + M[2][3]);
-  V3 d(M[0][0] * ray.D.x + M[0][1] * ray.D.y + M[0][2] * ray.D.z,
-       M[1][0] * ray.D.x + M[1][1] * ray.D.y + M[1][2] * ray.D.z,
-       M[2][0] * ray.D.x + M[2][1] * ray.D.y + M[2][2] * ray.D.z);
+  auto P = [&](int r) {
+    return std::fma(M[r][0], ray.O.x, std::fma(M[r][1], ray.O.y, std::fma(M[r][2], ray.O.z, M[r][3])));
+  };
+  auto L = [&](int r) { return std::fma(M[r][0], ray.D.x, std::fma(M[r][1], ray.D.y, M[r][2] * ray.D.z)); };
+  V3 o(P(0), P(1), P(2));
+  V3 d(L(0), L(1), L(2));
   Float t = -o.z / d.z;
-  V3 local = d * t + o;
+  V3 local(std::fma(d.x, t, o.x), std::fma(d.y, t, o.y), std::fma(d.z, t, o.z));
   bool isHit = t >= near && t <= far && std::abs(local.x) <= 1 && std::abs(local.y) <= 1;
   if (!isHit) return false;
   rec.T = t;

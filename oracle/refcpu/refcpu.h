@@ -36,6 +36,7 @@ int ref_render(const ref_scene* s, const ref_render_params* p, float* fb, float*
 int ref_bsdf_eval(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out);
 int ref_bsdf_sample(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out);
 int ref_grid_eval(const rad_medium_desc* d, const float* in, uint32_t n, float* out);
+float ref_erfinv(float a);
 float ref_medium_transmittance(const ref_scene* s, int medium, const rad_ray* ray, uint32_t channel, uint32_t seed);
 
 #ifdef __cplusplus

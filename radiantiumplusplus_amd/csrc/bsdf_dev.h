@@ -54,18 +54,38 @@ RAD_HD f3 fresnel_conductor(float cosThetaI, f3 eta, f3 k) {
 }
 
 // ---------------------------------------------------------------- microfacet (microfacet.cpp:12-277)
-// erf^-1 by Newton iterations on erff (Beckmann visible-normal sampling).
+// erf^-1 for the Beckmann visible-normal sampler.  Like the reference's
+// ErfInv (math_ext.h:114-140) the argument enters through 1 - a^2 clamped at
+// FLT_MIN, so |a| >= 1 saturates instead of diverging.  Solved in double:
+// with c = (1 - a^2) / (1 + sqrt(a^2)) = erfc(|y|), Newton on
+// log(erfc(y)) = log(c), which is concave and decreasing, so the iteration
+// converges monotonically after at most one overshoot.
 __device__ inline float erfinv_newton(float a) {
-  if (a <= -1.f) return -kInf;
-  if (a >= 1.f) return kInf;
-  float w = -logf((1.0f - a) * (1.0f + a));
-  float y = copysignf(sqrtf(fmaxf(0.f, w - 0.5f * logf(fmaxf(w, 1e-30f)))), a) * 0.8f;
-  for (int i = 0; i < 12; i++) {
-    float err = erff(y) - a;
-    float d = 1.1283791670955126f * expf(-y * y);
-    y -= err / d;
+  if (fabsf(a) < 0.5f) {  // well-conditioned core: Newton on erf(y) = |a|
+    double A = fabs(double(a)), y = A * 0.886226925452758;
+    for (int i = 0; i < 30; i++) {
+      double step = (erf(y) - A) / (1.1283791670955126 * exp(-y * y));
+      y -= step;
+      if (fabs(step) <= 1e-16 * fmax(1e-300, y)) break;
+    }
+    return copysignf(float(y), a);
   }
-  return y;
+  float q = fmaxf(fmaf(a, -a, 1.0f), 1.17549435e-38f);
+  double Q = q;
+  double c = Q / (1.0 + sqrt(1.0 - Q));
+  if (c >= 1.0) return 0.0f * a;
+  double L = log(c);
+  double y = c > 0.5 ? (1.0 - c) * 0.886226925452758 : sqrt(fmax(-L - 0.5 * log(-L), 1e-6));
+  for (int i = 0; i < 60; i++) {
+    double e = erfc(y);
+    double f = log(e) - L;
+    double fp = -1.1283791670955126 * exp(-y * y) / e;
+    double step = f / fp;
+    y -= step;
+    if (y < 0.0) y = 0.0;
+    if (fabs(step) <= 1e-15 * fmax(1.0, y)) break;
+  }
+  return copysignf(float(y), a);
 }
 
 struct Microfacet {

@@ -13,28 +13,35 @@ struct Hit {
   uint32_t geom;  // 0xffffffff = miss
 };
 
+// Products of the primitive tests with every rounding spelled out (explicit
+// fused multiply-adds in a fixed association), so the device and the host
+// oracle produce bit-identical t, u, v whatever contraction the compiler
+// applies elsewhere.
+__device__ __forceinline__ float dotx(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ f3 crossx(f3 a, f3 b) {
+  return mk(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
+}
+
 // Moller-Trumbore in the form Embree 3 applies to its TriangleM primitives:
 // e1 = v0 - v1, e2 = v2 - v0, Ng = e2 x e1, C = v0 - O, R = C x D,
 // accept den != 0, U, V >= 0, U + V <= |den|, |den|*tnear < T <= |den|*tfar.
-// Written without contraction so it is bit-identical to the host oracle.
 __device__ __forceinline__ bool isect_tri(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float tnear, float tfar, float& t,
                                           float& u, float& v) {
-#pragma clang fp contract(off)
   f3 e1 = v0 - v1;
   f3 e2 = v2 - v0;
-  f3 ng = cross(e2, e1);
+  f3 ng = crossx(e2, e1);
   f3 c = v0 - o;
-  f3 r = cross(c, d);
-  float den = dot(ng, d);
+  f3 r = crossx(c, d);
+  float den = dotx(ng, d);
   float absDen = fabsf(den);
-  float U = dot(r, e2);
-  float V = dot(r, e1);
+  float U = dotx(r, e2);
+  float V = dotx(r, e1);
   if (den < 0.f) {
     U = -U;
     V = -V;
   }
   if (!(den != 0.f && U >= 0.f && V >= 0.f && U + V <= absDen)) return false;
-  float T = dot(ng, c);
+  float T = dotx(ng, c);
   if (den < 0.f) T = -T;
   if (!(absDen * tnear < T && T <= absDen * tfar)) return false;
   t = T / absDen;
@@ -45,15 +52,14 @@ __device__ __forceinline__ bool isect_tri(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float
 
 // Sphere user geometry (sphere.cpp:40-54 with SolveQuadratic, math_ext.h:79-97).
 __device__ __forceinline__ bool isect_sphere(f3 center, float radius, f3 ro, f3 rd, float near, float far, float& t) {
-#pragma clang fp contract(off)
   f3 o = ro - center;
-  float a = sqnorm(rd);
-  float b = 2.0f * dot(o, rd);
-  float c = sqnorm(o) - sqr(radius);
+  float a = dotx(rd, rd);
+  float b = 2.0f * dotx(o, rd);
+  float c = fmaf(-radius, radius, dotx(o, o));
   bool linearCase = a == 0.0f;
   bool validLinear = linearCase && b != 0.0f;
   float x0 = -c / b, x1 = -c / b;
-  float discrim = b * b - 4.0f * a * c;
+  float discrim = fmaf(b, b, -((4.0f * a) * c));
   bool validQuadratic = !linearCase && (discrim >= 0.0f);
   if (validQuadratic) {
     float rootDiscrim = sqrtf(discrim);
@@ -77,11 +83,14 @@ __device__ __forceinline__ bool isect_sphere(f3 center, float radius, f3 ro, f3 
 // |x|, |y| <= 1, with an unnormalised object-space direction.
 __device__ __forceinline__ bool isect_rect(const Affine& toObj, f3 ro, f3 rd, float near, float far, float& t,
                                            float& u, float& v) {
-#pragma clang fp contract(off)
-  f3 o = toObj.point(ro);
-  f3 d = toObj.vec(rd);
+  const float* m = toObj.m;
+  f3 o = mk(fmaf(m[0], ro.x, fmaf(m[1], ro.y, fmaf(m[2], ro.z, m[3]))),
+            fmaf(m[4], ro.x, fmaf(m[5], ro.y, fmaf(m[6], ro.z, m[7]))),
+            fmaf(m[8], ro.x, fmaf(m[9], ro.y, fmaf(m[10], ro.z, m[11]))));
+  f3 d = mk(fmaf(m[0], rd.x, fmaf(m[1], rd.y, m[2] * rd.z)), fmaf(m[4], rd.x, fmaf(m[5], rd.y, m[6] * rd.z)),
+            fmaf(m[8], rd.x, fmaf(m[9], rd.y, m[10] * rd.z)));
   float tt = -o.z / d.z;
-  f3 local = d * tt + o;
+  f3 local = mk(fmaf(d.x, tt, o.x), fmaf(d.y, tt, o.y), fmaf(d.z, tt, o.z));
   if (!(tt >= near && tt <= far && fabsf(local.x) <= 1.f && fabsf(local.y) <= 1.f)) return false;
   t = tt;
   u = local.x;

@@ -54,6 +54,8 @@ def lib():
         L.ref_bsdf_eval.argtypes = [C.POINTER(F.BsdfDesc), C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_bsdf_sample.argtypes = [C.POINTER(F.BsdfDesc), C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_grid_eval.argtypes = [C.POINTER(F.MediumDesc), C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ref_erfinv.restype = C.c_float
+        L.ref_erfinv.argtypes = [C.c_float]
         L.ref_medium_transmittance.restype = C.c_float
         L.ref_medium_transmittance.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32, C.c_uint32]
         _lib = L

@@ -54,6 +54,27 @@ def test_eval_matches_oracle(kind, kw, quirks):
     assert close(g, o)
 
 
+@pytest.mark.parametrize("kind,kw", [("rough_metal", {"alpha": 0.4, "distribution": "beckmann"}),
+                                     ("rough_metal", {"alpha": 0.3}),
+                                     ("disney", {"base_color": (0.5, 0.5, 0.5), "roughness": 0.6, "metallic": 0.5})])
+def test_sample_matches_oracle_at_grazing_incidence(kind, kw):
+    """Silhouettes: cos(theta_i) in [1e-3, 0.1] drives the visible-normal
+    samplers to their extreme slopes; every sample must agree."""
+    rng = np.random.default_rng(17)
+    d = desc(kind, kw)
+    n = 20000
+    phi = rng.uniform(0, 2 * np.pi, n)
+    cz = rng.uniform(1e-3, 0.1, n)
+    sz = np.sqrt(1 - cz * cz)
+    wi = np.stack([sz * np.cos(phi), sz * np.sin(phi), cz], 1).astype(np.float32)
+    lobe, xi = rng.random(n, dtype=np.float32), rng.random((n, 2), dtype=np.float32)
+    for quirks in (True, False):
+        g = hip.bsdf_sample(d, wi, lobe, xi, quirks=quirks)
+        o = O.bsdf_sample(d, wi, lobe, xi, quirks=quirks)
+        valid = (o[:, 3] > 0) | (g[:, 3] > 0)
+        assert close(g[valid], o[valid], frac=0.9995)
+
+
 @pytest.mark.parametrize("quirks", [True, False])
 @pytest.mark.parametrize("kind,kw", CASES)
 def test_sample_matches_oracle(kind, kw, quirks):

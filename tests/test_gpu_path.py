@@ -91,10 +91,13 @@ def test_furnace_closed_form(hip_ctx):
 def test_job_progress_and_cancel(hip_ctx):
     b = scenes.many_balls(seed=0, width=128, height=128, spp=16)
     s = hip.Scene.from_builder(hip_ctx, b)
-    job = s.start(b.render_desc(spp=4096, seed=1))
+    # Tens of seconds of work, so the cancel always lands while it runs.
+    spp = 1 << 20
+    job = s.start(b.render_desc(spp=spp, seed=1))
     time.sleep(0.2)
     done, total, complete = job.poll()
-    assert total == 128 * 128 * 4096
+    assert total == 128 * 128 * spp
+    assert not complete and done < total
     job.cancel()
     with pytest.raises(hip.RadError) as e:
         job.wait()

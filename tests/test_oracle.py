@@ -232,6 +232,20 @@ def test_visible_normal_sampling_quirk():
     assert np.abs(quirky - ref).max() / ref.max() > 0.05
 
 
+def test_erfinv_round_trip_and_saturation():
+    """erf(erfinv(x)) = x; at |x| >= 1 the reference clamps 1 - x^2 at FLT_MIN
+    (math_ext.h:114-140), so the result saturates where 1 - erf(y)^2 = FLT_MIN."""
+    L = O.lib()
+    for x in [0.0, 1e-6, 0.1, -0.5, 0.9, 0.999, -0.99999, 0.9999999]:
+        y = L.ref_erfinv(x)
+        assert math.erf(y) == pytest.approx(x, rel=2e-6, abs=1e-7)
+    sat = L.ref_erfinv(1.0)
+    assert 9.0 < sat < 9.5
+    assert L.ref_erfinv(2.0) == pytest.approx(sat) and L.ref_erfinv(-3.0) == pytest.approx(-sat)
+    fmin = np.finfo(np.float32).tiny
+    assert 1 - math.erf(sat) ** 2 == pytest.approx(fmin, rel=1e-4) or math.erfc(sat) == pytest.approx(fmin / 2, rel=1e-3)
+
+
 # ------------------------------------------------------------------ heterogeneous grid
 def grid_desc(g):
     m = F.MediumDesc()
