@@ -5,11 +5,15 @@ Workload (default): 1920x1080, 64 spp, max depth 16, a 2048x2048-vertex fBm
 terrain (8.38M triangles) plus 256 UV spheres of 2,116 triangles with GGX
 conductor / rough dielectric / Disney materials, a constant environment and
 a small distant spherical "sun" emitter.  One step = one full render of that
-frame.  Prints one JSON line (see README of the measurement contract).
+frame.  Prints one JSON line.
 
 The CPU baseline is the oracle (the CPU restatement of the reference's
-integrator, its own binned-SAH BVH, OpenMP over tiles) timed on a bounded
-window of the same frame at full spp.
+integrator, its own binned-SAH BVH, OpenMP over tiles) timed on an even
+subsample of the same frame's tiles at full spp.
+
+`--copy-probe MIB` instead runs a plain device-to-device copy of MIB MiB a
+few times; its FETCH_SIZE / WRITE_SIZE counter readings calibrate the
+counters that give the traversal kernel's HBM traffic.
 """
 import argparse
 import json
@@ -22,6 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "samples/s at equal relMSE, guided vs plain GPU path; p50/p99 render latency"
 HBM_PEAK_GBS = 8000.0
 TRACE_BYTES_PER_RAY = 52  # ray read 32 B + hit write 20 B per closest-hit query
 
@@ -36,7 +41,80 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--grid", type=int, default=2048)
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    p.add_argument("--summarize", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                   help="print a text summary of rocprofv3 csv output directories")
     return p.parse_args()
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(d, "*" + suffix))):
+        rows += list(csv.DictReader(open(f)))
+    return rows
+
+
+def _short(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def summarize(dirs, copy_mib=1024):
+    """Kernel-trace statistics plus FETCH_SIZE / WRITE_SIZE per dispatch,
+    scaled by the readings of the plain copy of `copy_mib` MiB."""
+    trace, fetch, write, cfetch, cwrite = dirs
+    out = ["kernel                 calls   total_ms    avg_us    min_us    max_us   vgpr  scratch"]
+    stats = {}
+    for r in _csv_rows(trace, "kernel_stats.csv"):
+        stats[_short(r["Name"])] = r
+    res = {}
+    for r in _csv_rows(trace, "kernel_trace.csv"):
+        res.setdefault(_short(r["Kernel_Name"]), (r["VGPR_Count"], r["Scratch_Size"]))
+    for k, r in sorted(stats.items(), key=lambda kv: -float(kv[1]["TotalDurationNs"])):
+        v, sc = res.get(k, ("?", "?"))
+        out.append(f"{k[:22]:22s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:10.2f} "
+                   f"{float(r['AverageNs']) / 1e3:9.1f} {float(r['MinNs']) / 1e3:9.1f} {float(r['MaxNs']) / 1e3:9.1f} "
+                   f"{v:>6s} {sc:>8s}")
+
+    def per_kernel(d):
+        acc = {}
+        for r in _csv_rows(d, "counter_collection.csv"):
+            acc.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        return acc
+
+    cf, cw = per_kernel(cfetch), per_kernel(cwrite)
+    copy_kib = copy_mib * 1024.0
+    kf = [v for k, v in cf.items() if "copyBuffer" in k][0]
+    kw = [v for k, v in cw.items() if "copyBuffer" in k][0]
+    f_scale, w_scale = copy_kib / (sum(kf) / len(kf)), copy_kib / (sum(kw) / len(kw))
+    out.append("")
+    out.append(f"plain copy of {copy_mib} MiB: FETCH_SIZE {sum(kf) / len(kf):.0f} KiB, WRITE_SIZE "
+               f"{sum(kw) / len(kw):.0f} KiB per dispatch -> scale factors fetch x{f_scale:.3f}, write x{w_scale:.3f}")
+    out.append("")
+    out.append("kernel                 dispatches  read_MB/launch  write_MB/launch  (scaled)")
+    f, w = per_kernel(fetch), per_kernel(write)
+    for k in sorted(f, key=lambda k: -sum(f[k])):
+        if k not in w:
+            continue
+        rd = sum(f[k]) / len(f[k]) * f_scale * 1024 / 1e6
+        wr = sum(w[k]) / len(w[k]) * w_scale * 1024 / 1e6
+        out.append(f"{k[:22]:22s} {len(f[k]):10d} {rd:15.1f} {wr:16.1f}")
+    return "\n".join(out)
+
+
+def copy_probe(mib):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 5}), flush=True)
 
 
 def make_scene(cfg, args):
@@ -56,35 +134,57 @@ def make_scene(cfg, args):
     return b, name
 
 
+TILE = 16
+
+
+def _tile_pixels(W, H, step):
+    """Pixels in tiles 0, step, 2*step, ... of the row-major 16x16 tile list."""
+    tx, ty = (W + TILE - 1) // TILE, (H + TILE - 1) // TILE
+    n = 0
+    for t in range(0, tx * ty, step):
+        bx, by = (t % tx) * TILE, (t // tx) * TILE
+        n += (min(bx + TILE, W) - bx) * (min(by + TILE, H) - by)
+    return n
+
+
+def _coprime_step(target, tiles_x):
+    s = max(1, int(round(target)))
+    while np.gcd(s, tiles_x) != 1:
+        s += 1
+    return s
+
+
 def cpu_baseline(b, seconds):
-    """Time the oracle on a centred window of the frame at full spp."""
+    """Time the oracle, all host threads, on an even subsample of the frame's
+    16x16 tiles at full spp.  The tile stride is coprime to the tiles per
+    row, so the subsample covers every row and column band of the image; it
+    is calibrated so that the measured run takes about `seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
-    threads = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     t0 = time.time()
     o = O.OracleScene(b)
     build_s = time.time() - t0
     W, H = b.camera["width"], b.camera["height"]
     spp = b.sampler["sample_count"]
-    # calibrate with a small window, then size the measured window to ~seconds
-    side = 16
-    win = (W // 2 - side // 2, W // 2 + side // 2, H // 2 - side // 2, H // 2 + side // 2)
-    _, _, t = o.render(window=win, threads=threads, want_sumsq=False)
-    rate = side * side * spp / max(t, 1e-6)
-    npx = max(64, int(rate * seconds / spp))
-    side = max(8, int(np.sqrt(npx)))
-    side = min(side, min(W, H))
-    win = (W // 2 - side // 2, W // 2 - side // 2 + side, H // 2 - side // 2, H // 2 - side // 2 + side)
-    _, _, t = o.render(window=win, threads=threads, want_sumsq=False)
-    samples = side * side * spp
+    tiles_x = (W + TILE - 1) // TILE
+    ntiles = tiles_x * ((H + TILE - 1) // TILE)
+    # calibration: enough tiles to keep every thread busy
+    step = _coprime_step(max(1, ntiles // (8 * threads)), tiles_x)
+    for _ in range(4):
+        _, _, t = o.render(threads=threads, want_sumsq=False, tile_step=step)
+        if t >= 0.5 * seconds or step == 1:
+            break
+        step = _coprime_step(max(1.0, step * t / seconds), tiles_x)
+    samples = _tile_pixels(W, H, step) * spp
     return {
         "value": samples / t,
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle path integrator, centred {side}x{side}-pixel window at {spp} spp "
-                  f"({samples} samples, {t:.1f} s; BVH build {build_s:.1f} s excluded)",
+        "sample": f"oracle {b.renderer['type']} integrator, 1 in {step} of the {ntiles} 16x16 tiles of the "
+                  f"{W}x{H} frame at {spp} spp ({samples} samples, {t:.1f} s; BVH build {build_s:.1f} s excluded)",
     }
 
 
@@ -94,6 +194,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
+    if args.summarize:
+        print(summarize(args.summarize))
+        return
     from radiantiumplusplus_amd import _ffi as F
     from radiantiumplusplus_amd import hip
 
@@ -132,7 +237,7 @@ def main():
     segs = sum(s.segments for s in stats)
     achieved = TRACE_BYTES_PER_RAY * segs / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
     out = {
-        "metric": "samples/s (GPU wavefront path tracer)",
+        "metric": METRIC,
         "value": samples / total,
         "unit": "samples/s",
         "n_gpus": 1,

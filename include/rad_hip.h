@@ -219,7 +219,7 @@ typedef struct rad_render_desc {
   uint32_t spp;            /* sampler.sample_count */
   uint32_t seed;           /* sampler.seed */
   uint32_t flags;          /* RAD_RENDER_* */
-  uint32_t slots_per_pixel;/* wavefront slots per pixel, 0 = autotuned */
+  uint32_t path_slots;     /* paths in flight (wavefront slots), 0 = chosen by the library */
   int32_t guiding;         /* learned path guiding: 0 off, 1 on */
   uint32_t request_id;     /* RNG stream key for serving (0 for single renders) */
   uint32_t reserved;

@@ -547,8 +547,11 @@ int ref_render(const ref_scene* s, const ref_render_params* p, float* fb, float*
   int threads = p->threads > 0 ? p->threads : omp_get_max_threads();
   auto t0 = std::chrono::steady_clock::now();
   std::atomic<int> err{0};
+  const int step = p->tile_step > 1 ? p->tile_step : 1;
+  const int ntiles = (tilesX * tilesY + step - 1) / step;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
-  for (int t = 0; t < tilesX * tilesY; t++) {
+  for (int ti = 0; ti < ntiles; ti++) {
+    const int t = ti * step;
     int bx = x0 + (t % tilesX) * tile, by = y0 + (t / tilesX) * tile;
     int ex = std::min(bx + tile, x1), ey = std::min(by + tile, y1);
     uint32_t seed = uint32_t(bx) * uint32_t(W) + uint32_t(by);

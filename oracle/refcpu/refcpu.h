@@ -21,6 +21,7 @@ typedef struct ref_render_params {
   int32_t threads;     /* 0 = all */
   int32_t tile_size;   /* 0 = 16 */
   int32_t x_begin, x_end, y_begin, y_end; /* sub-window (0 = full) */
+  int32_t tile_step;   /* render tiles t = 0, step, 2*step, ... of the row-major tile list (0 = all) */
 } ref_render_params;
 
 const char* ref_last_error(void);

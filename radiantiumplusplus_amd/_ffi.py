@@ -157,7 +157,7 @@ class RenderDesc(C.Structure):
         ("spp", C.c_uint32),
         ("seed", C.c_uint32),
         ("flags", C.c_uint32),
-        ("slots_per_pixel", C.c_uint32),
+        ("path_slots", C.c_uint32),
         ("guiding", C.c_int32),
         ("request_id", C.c_uint32),
         ("reserved", C.c_uint32),

@@ -44,4 +44,105 @@ __device__ inline f3 grid_eval(const float* g, int sx, int sy, int sz, int ch, i
   return fmadd(y0, splat(1.f - dw), y1 * dw);
 }
 
+// ---------------------------------------------------------------- media
+// A sampled medium interaction (interaction.h:102-111): T = kInf when the
+// sampled distance lies beyond the ray segment inside the medium.
+struct MediumSample {
+  float T, minT;
+  f3 p;
+  f3 sig_s, sig_n, sig_t, comb;
+};
+
+__device__ inline f3 medium_sigma_t(const DevScene& s, const MediumRec& m, f3 p) {
+  if (m.type == RAD_MEDIUM_HOMOGENEOUS) return m.sigma_t;
+  if (!m.has_grid) return m.sigma_t * m.scale;  // heterogeneous.cpp:41-44
+  f3 q = m.to_local.point(p);
+  return grid_eval(s.grid_data + m.grid_offset, m.sx, m.sy, m.sz, m.channels, m.wrap, q) * m.scale;
+}
+
+// Medium::SampleInteraction (medium.cpp:19-43) with the homogeneous /
+// heterogeneous IntersectAABB, GetMajorant and GetScatteringCoefficients.
+__device__ inline MediumSample medium_sample(const DevScene& s, const MediumRec& m, f3 o, f3 d, float rayMaxT, float u,
+                                             uint32_t ch) {
+  MediumSample mi;
+  bool active;
+  float mint, maxt;
+  if (m.type == RAD_MEDIUM_HOMOGENEOUS) {  // homogeneous.cpp:23-25
+    active = true;
+    mint = 0.f;
+    maxt = kInf;
+  } else {  // heterogeneous.cpp:66-79
+    active = !is_zero(d) || (o.x >= m.bbox_lo.x && o.y >= m.bbox_lo.y && o.z >= m.bbox_lo.z && o.x <= m.bbox_hi.x &&
+                             o.y <= m.bbox_hi.y && o.z <= m.bbox_hi.z);
+    f3 r = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    f3 t1 = mul(m.bbox_lo - o, r), t2 = mul(m.bbox_hi - o, r);
+    f3 a = mk(fminf(t1.x, t2.x), fminf(t1.y, t2.y), fminf(t1.z, t2.z));
+    f3 b = mk(fmaxf(t1.x, t2.x), fmaxf(t1.y, t2.y), fmaxf(t1.z, t2.z));
+    mint = fmaxf(fmaxf(a.x, a.y), a.z);
+    maxt = fminf(fminf(b.x, b.y), b.z);
+    active = active && maxt >= mint;
+  }
+  active = active && (isfinite(mint) || isfinite(maxt));
+  if (!active) {
+    mint = 0.f;
+    maxt = kInf;
+  }
+  mint = fmaxf(0.f, mint);
+  maxt = fminf(rayMaxT, maxt);
+  mi.comb = m.type == RAD_MEDIUM_HOMOGENEOUS ? m.sigma_t : splat(m.max_density);
+  float mc = ch == 0 ? mi.comb.x : (ch == 1 ? mi.comb.y : mi.comb.z);
+  float t = mint + (-logf(1.f - u) / mc);
+  mi.T = (active && t <= maxt) ? t : kInf;
+  mi.p = d * t + o;
+  mi.minT = mint;
+  mi.sig_t = medium_sigma_t(s, m, mi.p);
+  mi.sig_s = mul(mi.sig_t, m.albedo);
+  mi.sig_n = m.type == RAD_MEDIUM_HOMOGENEOUS ? splat(0.f) : splat(m.max_density) - mi.sig_t;
+  return mi;
+}
+
+RAD_HD float chan(f3 v, uint32_t c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
+
+// Medium::EvalTrAndPdf (medium.cpp:45-59); siT = kInf on a miss.
+__device__ inline void medium_tr_pdf(const MediumSample& mi, float siT, f3& tr, f3& pdf) {
+  float t = fminf(mi.T, siT) - mi.minT;
+  if (t <= 0.f) {
+    tr = pdf = splat(1.f);
+    return;
+  }
+  tr = mk(expf(mi.comb.x * -t), expf(mi.comb.y * -t), expf(mi.comb.z * -t));
+  pdf = siT < mi.T ? tr : mul(tr, mi.comb);
+  if (tr.x == pdf.x && tr.y == pdf.y && tr.z == pdf.z) tr = pdf = splat(1.f);
+}
+
+RAD_HD float eval_hg(float g, float cosTheta) {  // henyey_greenstein.cpp:11-16
+  float temp = 1.f + g * g + 2.f * g * cosTheta;
+  return (1.f / (4.f * kPi)) * (1.f - g * g) / (temp * sqrtf(temp));
+}
+
+// Phase sampling in the frame of wi = -D (henyey_greenstein.cpp:18-34,
+// isotropic.cpp:16-21).
+__device__ inline f3 phase_sample(const MediumRec& m, f3 wi, float u1, float u2, float& pdf) {
+  if (m.phase != RAD_PHASE_HG) {
+    float z = fmaf(-2.f, u2, 1.f), r = safe_sqrt(fmaf(-z, z, 1.f)), a = 2.f * kPi * u1;
+    pdf = 1.f / (4.f * kPi);
+    return mk(r * cosf(a), r * sinf(a), z);
+  }
+  float g = m.g;
+  float sqrTerm = (1.f - g * g) / (1.f - g + 2.f * g * u1);
+  float cosTheta = (1.f + g * g - sqrTerm * sqrTerm) / (2.f * g);
+  if (fabsf(g) < kFloatEps) cosTheta = 1.f - 2.f * u1;
+  float sinTheta = safe_sqrt(1.f - cosTheta * cosTheta);
+  float a = 2.f * kPi * u2;
+  Frame fr = frame_from_n(wi);
+  f3 wo = fr.to_world(mk(sinTheta * cosf(a), sinTheta * sinf(a), -cosTheta));
+  pdf = eval_hg(g, -cosTheta);
+  return wo;
+}
+
+__device__ inline float phase_eval(const MediumRec& m, f3 wi, f3 wo) {
+  if (m.phase != RAD_PHASE_HG) return 1.f / (4.f * kPi);
+  return eval_hg(m.g, dot(wo, wi));
+}
+
 }  // namespace rad

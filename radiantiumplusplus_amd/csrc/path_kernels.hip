@@ -1,96 +1,83 @@
 // Wavefront kernels of the `path` integrator (renderer/path.cpp:131-227) and
 // the ray-batch intersection kernels (accel/embree.cpp:79-111).
+#include <algorithm>
+
 #include "shading_dev.h"
 #include "wavefront.h"
 
 namespace rad {
 
-// Append `pred` lanes of this wavefront to a queue with one atomic per wave.
-// Must be reached by every lane of the wave (uniform control flow).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-  unsigned long long mask = __ballot(pred);
-  if (mask == 0ull) return 0;
-  uint32_t lane = __lane_id();
-  uint32_t leader = __ffsll((long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-}
-
-__device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
-  unsigned long long mask = __ballot(pred);
-  if (mask != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)mask) - 1))
-    atomicAdd(counter, (unsigned long long)__popcll(mask));
-}
-
-__global__ void k_init(Wavefront w) {
+// Start of a pass: every slot idle and on the dead list, so the first regen
+// hands each one a sample.
+__global__ void k_init_pass(Wavefront w, int first) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= w.Q) return;
-  w.meta[s] = make_uint4(s % w.npix, SAMPLE_NONE, 0u, 0u);
-  w.acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-  w.acc2[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-  w.dead[s] = s;
+  if (first && s < w.npix) {
+    w.pacc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    w.pacc2[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (s < w.Q) {
+    w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);
+    w.dead[s] = s;
+  }
   if (s == 0) {
     w.cnt[CNT_Q0] = 0;
     w.cnt[CNT_Q1] = 0;
     w.cnt[CNT_DEAD] = w.Q;
     w.cnt[CNT_SHADOW] = 0;
-    for (int i = 0; i < ST_WORDS; i++) w.stats[i] = 0ull;
+    w.cnt[CNT_NEXT] = 0;
+    w.cnt[CNT_FETCH_EXT] = 0;
+    w.cnt[CNT_FETCH_SH] = 0;
+    if (first)
+      for (int i = 0; i < ST_WORDS; i++) w.stats[i] = 0ull;
   }
 }
 
-// Finalise the sample a dead slot carried (sample_renderer.cpp:100-105) and
-// start its next camera sample (sample_renderer.cpp:97-99).
+// Store the sample a dead slot finished into the result planes and start the
+// next camera sample of the pass on it (sample_renderer.cpp:97-99).
 __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int in) {
   const uint32_t n = w.cnt[CNT_DEAD];
   const uint32_t stride = gridDim.x * blockDim.x;
-  const bool quirks = (w.flags & WF_QUIRKS) != 0;
+  const uint32_t pass = w.rows * w.npix;
+  const size_t plane = size_t(pass);
+  const bool more = w.cnt[CNT_NEXT] < pass;  // stale is fine: only skips atomics once exhausted
+  const uint32_t W = uint32_t(scene.cam.width);
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
-    bool start = false, finished = false;
+    bool finished = false;
     uint32_t s = 0;
     if (valid) {
       s = w.dead[i];
       uint4 m = w.meta[s];
-      if (m.y < w.spp) {
+      if (m.y != SAMPLE_NONE) {
         float4 l = w.lsum[s];
-        bool bad = isnan(l.x) || isnan(l.y) || isnan(l.z);
-        // the reference's HasInfinity / HasNegative look at the first channel only (spectrum.h:30-31)
-        if (quirks)
-          bad = bad || isinf(l.x) || l.x < 0.f;
-        else
-          bad = bad || isinf(l.x) || isinf(l.y) || isinf(l.z) || l.x < 0.f || l.y < 0.f || l.z < 0.f;
-        if (!bad) {
-          float4 a = w.acc[s], a2 = w.acc2[s];
-          w.acc[s] = make_float4(a.x + l.x, a.y + l.y, a.z + l.z, a.w + 1.f);
-          w.acc2[s] = make_float4(a2.x + l.x * l.x, a2.y + l.y * l.y, a2.z + l.z * l.z, a2.w);
-        }
+        size_t r = size_t(m.y - w.row0) * w.npix + m.x;
+        w.res[r] = l.x;
+        w.res[r + plane] = l.y;
+        w.res[r + 2 * plane] = l.z;
         finished = true;
       }
-      uint32_t lane = s / w.npix;
-      uint32_t next = m.y == SAMPLE_NONE ? lane : (m.y >= w.spp ? SAMPLE_DONE : m.y + w.K);
-      if (next < w.spp) {
-        uint32_t pixel = m.x;
-        uint32_t key = sample_key(w.seed, w.request, pixel, next);
-        uint32_t px = pixel % uint32_t(scene.cam.width), py = pixel / uint32_t(scene.cam.width);
-        float jx = rng_u(key, 0), jy = rng_u(key, 1);
-        f3 o, d;
-        float tmin, tmax;
-        camera_ray(scene.cam, float(px) + jx, float(py) + jy, o, d, tmin, tmax);
-        w.ray_o[s] = make_float4(o.x, o.y, o.z, tmin);
-        w.ray_d[s] = make_float4(d.x, d.y, d.z, tmax);
-        w.thr[s] = make_float4(1.f, 1.f, 1.f, 1.f);
-        w.lsum[s] = make_float4(0.f, 0.f, 0.f, 1.f);
-        w.prev[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
-        w.meta[s] = make_uint4(pixel, next, 0u, 2u);
-        w.key[s] = key;
-        if (w.medium) w.medium[s] = 0xffffffffu;  // vol_path: medium / channel chosen at shade time
-        start = true;
-      } else {
-        w.meta[s] = make_uint4(m.x, SAMPLE_DONE, 0u, 0u);
-      }
+    }
+    uint32_t id = more ? wave_append(&w.cnt[CNT_NEXT], valid) : pass;
+    bool start = valid && id < pass;
+    if (start) {
+      uint32_t pixel = id % w.npix, sample = w.row0 + id / w.npix;
+      uint32_t key = sample_key(w.seed, w.request, pixel, sample);
+      uint32_t px = pixel % W, py = pixel / W;
+      float jx = rng_u(key, 0), jy = rng_u(key, 1);
+      f3 o, d;
+      float tmin, tmax;
+      camera_ray(scene.cam, float(px) + jx, float(py) + jy, o, d, tmin, tmax);
+      w.ray_o[s] = make_float4(o.x, o.y, o.z, tmin);
+      w.ray_d[s] = make_float4(d.x, d.y, d.z, tmax);
+      w.thr[s] = make_float4(1.f, 1.f, 1.f, 1.f);
+      w.lsum[s] = make_float4(0.f, 0.f, 0.f, 1.f);
+      w.prev[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
+      w.meta[s] = make_uint4(pixel, sample, 0u, 2u);
+      w.key[s] = key;
+      if (w.medium) w.medium[s] = 0xffffffffu;  // vol_path: medium / channel chosen at shade time
+    } else if (valid) {
+      w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);  // idle for the rest of the pass
     }
     uint32_t pos = wave_append(&w.cnt[in], start);
     if (start) w.queue[in][pos] = s;
@@ -103,25 +90,35 @@ __global__ void k_reset(uint32_t* cnt, int out) {
     cnt[out] = 0;
     cnt[CNT_DEAD] = 0;
     cnt[CNT_SHADOW] = 0;
+    cnt[CNT_FETCH_EXT] = 0;
+    cnt[CNT_FETCH_SH] = 0;
   }
 }
 
-__global__ void __launch_bounds__(256) k_extend(Wavefront w, DevScene scene, int in) {
-  const uint32_t n = w.cnt[in];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
-    if (i < n) {
-      uint32_t s = w.queue[in][i];
-      float4 ro = w.ray_o[s], rd = w.ray_d[s];
-      Hit h;
-      h.geom = 0xffffffffu;
-      bool hit = trace<false>(scene, mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z), ro.w, rd.w, h);
-      w.hit[s] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.prim : 0u));
-      w.hit_geom[s] = hit ? h.geom : 0xffffffffu;
-    }
-    wave_count(&w.stats[ST_SEGMENTS], i < n);
+struct ExtendJob {
+  const uint32_t* queue;
+  const float4 *ray_o, *ray_d;
+  float4* hit;
+  uint32_t* hit_geom;
+  __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
+    uint32_t s = queue[i];
+    float4 a = ray_o[s], b = ray_d[s];
+    o = mk(a.x, a.y, a.z);
+    d = mk(b.x, b.y, b.z);
+    tmin = a.w;
+    tmax = b.w;
+    tag = s;
+    return true;
   }
+  __device__ __forceinline__ void finish(uint32_t, uint32_t s, const TraceRay& r) {
+    hit[s] = make_float4(r.h.t, r.h.u, r.h.v, __uint_as_float(r.hit ? r.h.prim : 0u));
+    hit_geom[s] = r.hit ? r.h.geom : 0xffffffffu;
+  }
+};
+
+__global__ void __launch_bounds__(kTraceBlock) k_extend(Wavefront w, DevScene scene, int in) {
+  ExtendJob job{w.queue[in], w.ray_o, w.ray_d, w.hit, w.hit_geom};
+  trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
 }
 
 // One bounce of Path::Li (path.cpp:141-225) for every queued path.
@@ -250,71 +247,120 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   }
 }
 
-__global__ void __launch_bounds__(256) k_shadow(Wavefront w, DevScene scene) {
-  const uint32_t n = w.cnt[CNT_SHADOW];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
-    if (i < n) {
-      float4 o = w.sh_o[i], d = w.sh_d[i];
-      Hit h;
-      bool occ = trace<true>(scene, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.f, o.w, h);
-      if (!occ) {
-        uint32_t s = __float_as_uint(d.w);
-        float4 c = w.sh_c[i];
-        float4 l = w.lsum[s];
-        w.lsum[s] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
-      }
-    }
-    wave_count(&w.stats[ST_SHADOW], i < n);
+struct ShadowJob {
+  const float4 *sh_o, *sh_d, *sh_c;
+  float4* lsum;
+  __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
+    float4 a = sh_o[i], b = sh_d[i];
+    o = mk(a.x, a.y, a.z);
+    d = mk(b.x, b.y, b.z);
+    tmin = 0.f;
+    tmax = a.w;
+    tag = __float_as_uint(b.w);
+    return true;
   }
+  __device__ __forceinline__ void finish(uint32_t i, uint32_t s, const TraceRay& r) {
+    if (r.hit) return;  // occluded
+    float4 c = sh_c[i];
+    float4 l = lsum[s];
+    lsum[s] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+  }
+};
+
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(Wavefront w, DevScene scene) {
+  ShadowJob job{w.sh_o, w.sh_d, w.sh_c, w.lsum};
+  trace_stream<true>(scene, w.cnt[CNT_SHADOW], &w.cnt[CNT_FETCH_SH], job);
 }
 
-// fb = (sum over the K slots of a pixel) / spp (sample_renderer.cpp:109-114).
-__global__ void k_resolve(Wavefront w, float* fb, float* sq) {
+// Add the pass's samples of each pixel in sample order, dropping the ones
+// the reference drops (sample_renderer.cpp:100-105: NaN anywhere; infinite
+// or negative values, which the reference's HasInfinity / HasNegative look
+// for in the first channel only, spectrum.h:30-31).
+__global__ void k_accum(Wavefront w) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= w.npix) return;
-  float r = 0.f, g = 0.f, b = 0.f, r2 = 0.f, g2 = 0.f, b2 = 0.f;
-  for (uint32_t k = 0; k < w.K; k++) {
-    float4 a = w.acc[p + k * w.npix], a2 = w.acc2[p + k * w.npix];
-    r += a.x;
-    g += a.y;
-    b += a.z;
-    r2 += a2.x;
-    g2 += a2.y;
-    b2 += a2.z;
+  const bool quirks = (w.flags & WF_QUIRKS) != 0;
+  const size_t plane = size_t(w.rows) * w.npix;
+  float4 a = w.pacc[p], a2 = w.pacc2[p];
+  for (uint32_t r = 0; r < w.rows; r++) {
+    size_t k = size_t(r) * w.npix + p;
+    float x = w.res[k], y = w.res[k + plane], z = w.res[k + 2 * plane];
+    bool bad = isnan(x) || isnan(y) || isnan(z);
+    if (quirks)
+      bad = bad || isinf(x) || x < 0.f;
+    else
+      bad = bad || isinf(x) || isinf(y) || isinf(z) || x < 0.f || y < 0.f || z < 0.f;
+    if (!bad) {
+      a = make_float4(a.x + x, a.y + y, a.z + z, a.w + 1.f);
+      a2 = make_float4(a2.x + x * x, a2.y + y * y, a2.z + z * z, a2.w);
+    }
   }
+  w.pacc[p] = a;
+  w.pacc2[p] = a2;
+}
+
+// fb = sum / spp (sample_renderer.cpp:109-114).
+__global__ void k_final(Wavefront w, float* fb, float* sq) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= w.npix) return;
+  float4 a = w.pacc[p];
   float coeff = 1.0f / float(w.spp);
-  fb[3 * p + 0] = r * coeff;
-  fb[3 * p + 1] = g * coeff;
-  fb[3 * p + 2] = b * coeff;
+  fb[3 * p + 0] = a.x * coeff;
+  fb[3 * p + 1] = a.y * coeff;
+  fb[3 * p + 2] = a.z * coeff;
   if (sq) {
-    sq[3 * p + 0] = r2;
-    sq[3 * p + 1] = g2;
-    sq[3 * p + 2] = b2;
+    float4 a2 = w.pacc2[p];
+    sq[3 * p + 0] = a2.x;
+    sq[3 * p + 1] = a2.y;
+    sq[3 * p + 2] = a2.z;
   }
 }
 
-__global__ void __launch_bounds__(256) k_closest(DevScene scene, const float4* rays, uint32_t n, float* hits) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float4 a = rays[2 * i], b = rays[2 * i + 1];
-  Hit h;
-  bool hit = trace<false>(scene, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, h);
-  float* o = hits + 5 * size_t(i);
-  o[0] = hit ? h.t : b.w;
-  o[1] = hit ? h.u : 0.f;
-  o[2] = hit ? h.v : 0.f;
-  o[3] = __uint_as_float(hit ? h.prim : 0xffffffffu);
-  o[4] = __uint_as_float(hit ? h.geom : 0xffffffffu);
+struct ClosestJob {
+  const float4* rays;
+  float* hits;
+  __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
+    float4 a = rays[2 * i], b = rays[2 * i + 1];
+    o = mk(a.x, a.y, a.z);
+    d = mk(b.x, b.y, b.z);
+    tmin = a.w;
+    tmax = b.w;
+    tag = __float_as_uint(b.w);
+    return true;
+  }
+  __device__ __forceinline__ void finish(uint32_t i, uint32_t tmax_bits, const TraceRay& r) {
+    float* o = hits + 5 * size_t(i);
+    o[0] = r.hit ? r.h.t : __uint_as_float(tmax_bits);
+    o[1] = r.hit ? r.h.u : 0.f;
+    o[2] = r.hit ? r.h.v : 0.f;
+    o[3] = __uint_as_float(r.hit ? r.h.prim : 0xffffffffu);
+    o[4] = __uint_as_float(r.hit ? r.h.geom : 0xffffffffu);
+  }
+};
+
+struct AnyJob {
+  const float4* rays;
+  uint8_t* occ;
+  __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
+    float4 a = rays[2 * i], b = rays[2 * i + 1];
+    o = mk(a.x, a.y, a.z);
+    d = mk(b.x, b.y, b.z);
+    tmin = a.w;
+    tmax = b.w;
+    tag = 0;
+    return true;
+  }
+  __device__ __forceinline__ void finish(uint32_t i, uint32_t, const TraceRay& r) { occ[i] = r.hit ? 1 : 0; }
+};
+
+__global__ void __launch_bounds__(kTraceBlock) k_closest(DevScene scene, const float4* rays, uint32_t n, float* hits) {
+  ClosestJob job{rays, hits};
+  trace_stream<false>(scene, n, nullptr, job);
 }
 
-__global__ void __launch_bounds__(256) k_any(DevScene scene, const float4* rays, uint32_t n, uint8_t* occ) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float4 a = rays[2 * i], b = rays[2 * i + 1];
-  Hit h;
-  occ[i] = trace<true>(scene, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, h) ? 1 : 0;
+__global__ void __launch_bounds__(kTraceBlock) k_any(DevScene scene, const float4* rays, uint32_t n, uint8_t* occ) {
+  AnyJob job{rays, occ};
+  trace_stream<true>(scene, n, nullptr, job);
 }
 
 __global__ void k_bsdf_eval(BsdfRec b, bool quirks, const float* in, uint32_t n, float* out) {
@@ -351,30 +397,37 @@ __global__ void k_bsdf_sample(BsdfRec b, bool quirks, const float* in, uint32_t 
 
 static inline unsigned Blocks(uint64_t n, int t) { return unsigned((n + t - 1) / t); }
 
-void LaunchInit(const Wavefront& w, hipStream_t s) {
-  hipLaunchKernelGGL(k_init, dim3(Blocks(w.Q, 256)), dim3(256), 0, s, w);
+void LaunchInitPass(const Wavefront& w, bool first, hipStream_t s) {
+  uint32_t n = first ? std::max(w.Q, w.npix) : w.Q;
+  hipLaunchKernelGGL(k_init_pass, dim3(Blocks(n, 256)), dim3(256), 0, s, w, first ? 1 : 0);
 }
 void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_regen, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 void LaunchReset(const Wavefront& w, int out, hipStream_t s) { hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w.cnt, out); }
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_extend, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  hipLaunchKernelGGL(k_extend, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene, in);
 }
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_shade_path, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow, dim3(c.blocks), dim3(256), 0, s, w, scene);
+  hipLaunchKernelGGL(k_shadow, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene);
 }
-void LaunchResolve(const Wavefront& w, float* fb, float* sq, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve, dim3(Blocks(w.npix, 256)), dim3(256), 0, s, w, fb, sq);
+void LaunchAccum(const Wavefront& w, hipStream_t s) {
+  hipLaunchKernelGGL(k_accum, dim3(Blocks(w.npix, 256)), dim3(256), 0, s, w);
+}
+void LaunchFinal(const Wavefront& w, float* fb, float* sq, hipStream_t s) {
+  hipLaunchKernelGGL(k_final, dim3(Blocks(w.npix, 256)), dim3(256), 0, s, w, fb, sq);
 }
 void LaunchClosest(const DevScene& scene, const void* rays, uint32_t n, void* hits, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_closest, dim3(Blocks(n, 256)), dim3(256), 0, s, scene, (const float4*)rays, n, (float*)hits);
+  if (n)
+    hipLaunchKernelGGL(k_closest, dim3(Blocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s, scene, (const float4*)rays, n,
+                       (float*)hits);
 }
 void LaunchAny(const DevScene& scene, const void* rays, uint32_t n, uint8_t* occ, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_any, dim3(Blocks(n, 256)), dim3(256), 0, s, scene, (const float4*)rays, n, occ);
+  if (n)
+    hipLaunchKernelGGL(k_any, dim3(Blocks(n, kTraceBlock)), dim3(kTraceBlock), 0, s, scene, (const float4*)rays, n, occ);
 }
 void LaunchBsdfEval(const BsdfRec& b, bool quirks, const float* in, uint32_t n, float* out, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_bsdf_eval, dim3(Blocks(n, 128)), dim3(128), 0, s, b, quirks, in, n, out);

@@ -128,103 +128,236 @@ __device__ __forceinline__ bool isect_prim(const DevScene& s, uint32_t p, f3 o, 
   return true;
 }
 
+// ---------------------------------------------------------------- traversal
+// Each lane keeps its traversal stack as (node, entry distance) pairs: the
+// newest kLdsStack entries in LDS (a ring, lane-interleaved so a wave's
+// accesses hit distinct banks), older ones spilled to a private overflow
+// array that only deep traversals touch.
 constexpr int kStackDepth = 64;
+constexpr int kLdsStack = 16;
+constexpr int kTraceBlock = 256;  // threads per block of every traversal kernel
 
-// ANY = true: return at the first accepted primitive (occlusion).
-template <bool ANY>
-__device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin, float tmax, Hit& h,
-                                      uint32_t* visits = nullptr) {
-  f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  f3 oinv = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
-  uint32_t stack[kStackDepth];
-  int sp = 0;
-  uint32_t node = 0;
-  float tfar = tmax;
-  bool hit = false;
-  uint32_t nvisit = 0;
-  const float4* base = reinterpret_cast<const float4*>(s.nodes);
-  for (;;) {
-    const float4* q = base + 8 * size_t(node);
-    float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
-    int4 ch = *reinterpret_cast<const int4*>(q + 6);
-    uint4 cn = *reinterpret_cast<const uint4*>(q + 7);
-    nvisit++;
-    float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
-    float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
-    float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
-    int cc[4] = {ch.x, ch.y, ch.z, ch.w};
-    uint32_t cnt[4] = {cn.x, cn.y, cn.z, cn.w};
-    float tn[4];
-    bool m[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      float t0x = fmaf(lo_x[i], inv.x, -oinv.x), t1x = fmaf(hi_x[i], inv.x, -oinv.x);
-      float t0y = fmaf(lo_y[i], inv.y, -oinv.y), t1y = fmaf(hi_y[i], inv.y, -oinv.y);
-      float t0z = fmaf(lo_z[i], inv.z, -oinv.z), t1z = fmaf(hi_z[i], inv.z, -oinv.z);
-      float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-      float tfb = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tfar));
-      // slack for the rounding of the fused slab arithmetic
-      tnear = tnear - fabsf(tnear) * 4.8e-7f;
-      tfb = tfb + fabsf(tfb) * 4.8e-7f;
-      m[i] = cc[i] >= 0 && tnear <= tfb;
-      tn[i] = tnear;
+struct TraceStack {
+  uint32_t* node;  // LDS, entry k of this lane at node[k * kTraceBlock]
+  float* dist;
+  uint32_t ov_node[kStackDepth - kLdsStack];
+  float ov_dist[kStackDepth - kLdsStack];
+  int sp, lo;  // entries [0, lo) live in the overflow array, [lo, sp) in LDS
+
+  __device__ __forceinline__ void push(uint32_t n, float d) {
+    if (sp >= kStackDepth) return;  // unreachable for trees of depth <= kStackDepth / 3
+    if (sp - lo == kLdsStack) {
+      int k = (lo & (kLdsStack - 1)) * kTraceBlock;
+      ov_node[lo] = node[k];
+      ov_dist[lo] = dist[k];
+      lo++;
     }
-    // leaves first (nearest-first among them does not matter for correctness)
+    int k = (sp & (kLdsStack - 1)) * kTraceBlock;
+    node[k] = n;
+    dist[k] = d;
+    sp++;
+  }
+  __device__ __forceinline__ void pop(uint32_t& n, float& d) {
+    sp--;
+    if (sp < lo) {
+      lo = sp;
+      n = ov_node[sp];
+      d = ov_dist[sp];
+    } else {
+      int k = (sp & (kLdsStack - 1)) * kTraceBlock;
+      n = node[k];
+      d = dist[k];
+    }
+  }
+};
+
+struct TraceRay {
+  f3 o, d, inv, oinv;
+  float tmin, tfar;
+  uint32_t node;
+  bool hit;
+  Hit h;
+};
+
+__device__ __forceinline__ void trace_begin(TraceRay& r, TraceStack& st, f3 o, f3 d, float tmin, float tmax) {
+  r.o = o;
+  r.d = d;
+  r.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  r.oinv = mk(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
+  r.tmin = tmin;
+  r.tfar = tmax;
+  r.node = 0;
+  r.hit = false;
+  r.h.geom = 0xffffffffu;
+  st.sp = 0;
+  st.lo = 0;
+}
+
+__device__ __forceinline__ void cswap(float& da, uint32_t& na, float& db, uint32_t& nb) {
+  bool sw = db < da;
+  float td = sw ? db : da;
+  uint32_t tn = sw ? nb : na;
+  db = sw ? da : db;
+  nb = sw ? na : nb;
+  da = td;
+  na = tn;
+}
+
+// Visit one node: test its four child boxes, intersect the primitives of hit
+// leaf children, descend into the nearest hit inner child and push the
+// others (far first).  Returns true when the ray is finished.
+template <bool ANY>
+__device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, TraceStack& st) {
+  const float4* q = reinterpret_cast<const float4*>(s.nodes) + 8 * size_t(r.node);
+  float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+  int4 ch = *reinterpret_cast<const int4*>(q + 6);
+  uint4 cn = *reinterpret_cast<const uint4*>(q + 7);
+  const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
+  const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
+  const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+  const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
+  const uint32_t cnt[4] = {cn.x, cn.y, cn.z, cn.w};
+  float tn[4];
+  bool m[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (m[i] && cnt[i] > 0) {
-        uint32_t first = uint32_t(cc[i]);
-        for (uint32_t k = 0; k < cnt[i]; k++) {
-          if (isect_prim(s, first + k, o, d, tmin, tfar, h)) {
-            hit = true;
-            if (ANY) {
-              if (visits) *visits = nvisit;
-              return true;
+  for (int i = 0; i < 4; i++) {
+    float t0x = fmaf(lo_x[i], r.inv.x, -r.oinv.x), t1x = fmaf(hi_x[i], r.inv.x, -r.oinv.x);
+    float t0y = fmaf(lo_y[i], r.inv.y, -r.oinv.y), t1y = fmaf(hi_y[i], r.inv.y, -r.oinv.y);
+    float t0z = fmaf(lo_z[i], r.inv.z, -r.oinv.z), t1z = fmaf(hi_z[i], r.inv.z, -r.oinv.z);
+    float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+    float tfb = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), r.tfar));
+    // slack for the rounding of the fused slab arithmetic
+    tnear = tnear - fabsf(tnear) * 4.8e-7f;
+    tfb = tfb + fabsf(tfb) * 4.8e-7f;
+    m[i] = cc[i] >= 0 && tnear <= tfb;
+    tn[i] = tnear;
+  }
+  // leaf children first: they can only shorten tfar
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (m[i] && cnt[i] > 0) {
+      uint32_t first = uint32_t(cc[i]);
+      for (uint32_t k = 0; k < cnt[i]; k++) {
+        if (isect_prim(s, first + k, r.o, r.d, r.tmin, r.tfar, r.h)) {
+          r.hit = true;
+          if (ANY) return true;
+        }
+      }
+    }
+  }
+  // inner children still in front of the closest hit, sorted by a fixed network
+  float d0 = kInf, d1 = kInf, d2 = kInf, d3 = kInf;
+  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  if (m[0] && cnt[0] == 0 && tn[0] <= r.tfar) { d0 = tn[0]; n0 = uint32_t(cc[0]); }
+  if (m[1] && cnt[1] == 0 && tn[1] <= r.tfar) { d1 = tn[1]; n1 = uint32_t(cc[1]); }
+  if (m[2] && cnt[2] == 0 && tn[2] <= r.tfar) { d2 = tn[2]; n2 = uint32_t(cc[2]); }
+  if (m[3] && cnt[3] == 0 && tn[3] <= r.tfar) { d3 = tn[3]; n3 = uint32_t(cc[3]); }
+  cswap(d0, n0, d1, n1);
+  cswap(d2, n2, d3, n3);
+  cswap(d0, n0, d2, n2);
+  cswap(d1, n1, d3, n3);
+  cswap(d1, n1, d2, n2);
+  if (d0 != kInf) {
+    if (d3 != kInf) st.push(n3, d3);
+    if (d2 != kInf) st.push(n2, d2);
+    if (d1 != kInf) st.push(n1, d1);
+    r.node = n0;
+    return false;
+  }
+  // pop, skipping subtrees entered beyond the closest hit found since the push
+  while (st.sp > 0) {
+    uint32_t n;
+    float d;
+    st.pop(n, d);
+    if (d <= r.tfar) {
+      r.node = n;
+      return false;
+    }
+  }
+  return true;
+}
+
+// Stream traversal for kernels that own a batch of rays.  A lane whose ray
+// finishes is refilled from the batch while the other lanes of its wave keep
+// traversing; refills take one atomic per wave on `fetch` (or, with
+// fetch == nullptr, each wave takes the 64 items at its own position once).
+// Job: bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag)
+//      void finish(uint32_t i, uint32_t tag, const TraceRay& r)
+// Launch with kTraceBlock threads per block; every lane must call this.
+template <bool ANY, class Job>
+__device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint32_t* fetch, Job& job) {
+  __shared__ uint32_t lds_node[kLdsStack * kTraceBlock];
+  __shared__ float lds_dist[kLdsStack * kTraceBlock];
+  constexpr int kRefillBelow = 48;  // refill once fewer lanes than this are busy
+  const uint32_t lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  TraceStack st;
+  st.node = lds_node + threadIdx.x;
+  st.dist = lds_dist + threadIdx.x;
+  TraceRay r;
+  bool active = false, exhausted = false;
+  uint32_t idx = 0, tag = 0;
+  for (;;) {
+    if (!exhausted) {
+      unsigned long long need = __ballot(!active);
+      if (need) {
+        uint32_t want = (uint32_t)__popcll(need);
+        uint32_t base;
+        if (fetch) {
+          uint32_t leader = (uint32_t)(__ffsll((long long)need) - 1);
+          uint32_t b = 0;
+          if (lane == leader) b = atomicAdd(fetch, want);
+          base = __shfl(b, (int)leader);
+          if (base + want >= n) exhausted = true;
+        } else {
+          base = (blockIdx.x * kTraceBlock + (threadIdx.x & ~63u));
+          exhausted = true;
+        }
+        if (!active) {
+          uint32_t i = base + (fetch ? (uint32_t)__popcll(need & below) : lane);
+          if (i < n) {
+            f3 o, d;
+            float tmin, tmax;
+            if (job.load(i, o, d, tmin, tmax, tag)) {
+              idx = i;
+              trace_begin(r, st, o, d, tmin, tmax);
+              active = true;
             }
           }
         }
       }
     }
-    // inner children, nearest first
-    float bd = kInf;
-    int bi = -1;
-    int ni = 0;
-    uint32_t inner[4];
-    float dist[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (m[i] && cnt[i] == 0 && tn[i] <= tfar) {
-        inner[ni] = uint32_t(cc[i]);
-        dist[ni] = tn[i];
-        ni++;
-      }
-    }
-    if (ni > 0) {
-      // order by distance: push the far ones, descend into the nearest
-      for (int a = 1; a < ni; a++) {
-        for (int b = a; b > 0 && dist[b] > dist[b - 1]; b--) {
-          float tdist = dist[b];
-          dist[b] = dist[b - 1];
-          dist[b - 1] = tdist;
-          uint32_t tn2 = inner[b];
-          inner[b] = inner[b - 1];
-          inner[b - 1] = tn2;
-        }
-      }
-      // dist is now descending; the last entry is the nearest
-      for (int a = 0; a < ni - 1; a++) {
-        if (sp < kStackDepth) stack[sp++] = inner[a];
-      }
-      node = inner[ni - 1];
-      (void)bd;
-      (void)bi;
+    if (__ballot(active) == 0ull) {
+      if (exhausted) break;
       continue;
     }
-    if (sp == 0) break;
-    node = stack[--sp];
+    for (int k = 0; k < 256; k++) {
+      if (active && trace_step<ANY>(s, r, st)) {
+        job.finish(idx, tag, r);
+        active = false;
+      }
+      unsigned long long busy = __ballot(active);
+      if (busy == 0ull) break;
+      if (!exhausted && __popcll(busy) < kRefillBelow) break;
+    }
   }
-  if (visits) *visits = nvisit;
-  return hit;
+}
+
+// Single-ray traversal for kernels that trace a data-dependent number of rays
+// per lane (launched with at most kTraceBlock threads per block).
+template <bool ANY>
+__device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin, float tmax, Hit& h) {
+  __shared__ uint32_t lds_node1[kLdsStack * kTraceBlock];
+  __shared__ float lds_dist1[kLdsStack * kTraceBlock];
+  TraceRay r;
+  TraceStack st;
+  st.node = lds_node1 + threadIdx.x;
+  st.dist = lds_dist1 + threadIdx.x;
+  trace_begin(r, st, o, d, tmin, tmax);
+  while (!trace_step<ANY>(s, r, st)) {
+  }
+  h = r.h;
+  return r.hit;
 }
 
 }  // namespace rad

@@ -19,6 +19,336 @@ __global__ void k_grid_eval(const float* grid, int sx, int sy, int sz, int ch, i
   out[3 * i + 2] = v.z;
 }
 
+__device__ __forceinline__ void wave_sum(unsigned long long* counter, uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0 && v) atomicAdd(counter, (unsigned long long)v);
+}
+
+// Shadow work items of vol_path carry the current medium (+1, 8 bits), the
+// spectral channel (2 bits) and the path's rng dimension at emission (22
+// bits); the transmittance estimator draws from its own stream keyed by both.
+__device__ __forceinline__ uint32_t pack_vol(int32_t medium, uint32_t ch, uint32_t dim) {
+  return uint32_t(medium + 1) | (ch << 8) | (dim << 10);
+}
+__device__ __forceinline__ uint32_t shadow_key(uint32_t key, uint32_t dim) { return mix32(key + 0x9e3779b9u * (dim + 1u)); }
+
+// One pass of VolPath::Li's loop (vol_path.cpp:27-217) for every queued path:
+// Russian roulette and the depth limit at the top of the loop, delta tracking
+// on one spectral channel, null collisions (counted as bounces), real
+// scattering with phase-function NEE and sampling, and the surface branch
+// (emission with MIS, pass-through medium boundaries that are not counted,
+// BSDF NEE and sampling).  NEE contributions are finished by k_vol_shadow.
+__global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, int in) {
+  const uint32_t n = w.cnt[in];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const bool quirks = (w.flags & WF_QUIRKS) != 0;
+  const int out = 1 - in;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    bool alive = false, emit = false, msample = false;
+    uint32_t s = 0;
+    float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
+    if (i < n) {
+      s = w.queue[in][i];
+      float4 ro4 = w.ray_o[s], rd4 = w.ray_d[s];
+      f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
+      float tmin = ro4.w, tmax = rd4.w;
+      float4 hh = w.hit[s];
+      uint32_t geom = w.hit_geom[s];
+      float4 t4 = w.thr[s];
+      f3 T = mk(t4.x, t4.y, t4.z);
+      float eta = t4.w;
+      float4 l4 = w.lsum[s];
+      f3 L = mk(l4.x, l4.y, l4.z);
+      float prevPdf = l4.w;
+      float4 p4 = w.prev[s];
+      f3 prevP = mk(p4.x, p4.y, p4.z);
+      bool spec = (__float_as_uint(p4.w) & PATH_SPECULAR) != 0;
+      uint4 m = w.meta[s];
+      uint32_t depth = m.z, dim = m.w;
+      const uint32_t key = w.key[s];
+      uint32_t medw = w.medium[s];
+      int32_t medium;
+      uint32_t ch;
+      if (medw == MED_UNSET) {  // SampleChannel (vol_path.cpp:219-222) and the global medium
+        ch = min(uint32_t(rng_u(key, dim) * 3.f), 2u);
+        dim++;
+        medium = scene.global_medium;
+      } else {
+        ch = medw >> 24;
+        medium = int32_t(medw & 0xffffffu) - 1;
+      }
+      bool go = true;
+      if (int32_t(depth) >= w.rr_depth) {
+        float rr = fminf(maxc(T) * sqr(eta), 0.95f);
+        if (rng_u(key, dim) > rr)
+          go = false;
+        else
+          T = T * (1.f / rr);
+        dim++;
+      }
+      if (go && w.max_depth >= 0 && int32_t(depth) + 1 >= w.max_depth) go = false;
+      uint32_t next_depth = depth + 1;
+      if (go) {
+        bool isMedium = medium >= 0, isSurface = !isMedium;
+        bool isNull = false, isScatter = false, escaped = false;
+        const float siT = geom != 0xffffffffu ? hh.x : kInf;
+        MediumSample mei;
+        mei.T = kInf;
+        if (isMedium) {
+          const MediumRec& M = scene.media[medium];
+          // the reference shortens a homogeneous ray to mei.T before tracing;
+          // a hit beyond mei.T is then ignored below, so the full-length hit
+          // of the extend kernel gives the same result
+          mei = medium_sample(scene, M, ro, rd, tmax, rng_u(key, dim), ch);
+          dim++;
+          msample = true;
+          if (siT < mei.T) mei.T = kInf;
+          f3 tr, pdf;
+          medium_tr_pdf(mei, siT, tr, pdf);
+          float pc = chan(pdf, ch);
+          f3 after = mul(T, tr) / pc;
+          T = pc <= 0.f ? splat(0.f) : after;
+          escaped = mei.T == kInf;
+          isMedium = mei.T != kInf;
+          bool nullScatter = rng_u(key, dim) > chan(mei.sig_t, ch) / chan(mei.comb, ch);
+          dim++;
+          isNull = nullScatter && isMedium;
+          isScatter = !isNull && isMedium;
+          if (isNull) T = mul(T, mei.sig_n) * chan(mei.comb, ch) / chan(mei.sig_n, ch);
+          if (isScatter) prevP = mei.p;
+        }
+        if (isNull) {  // continue along the ray from the collision
+          ro = mei.p;
+          alive = true;
+        }
+        if (isScatter) {
+          const MediumRec& M = scene.media[medium];
+          T = mul(T, mei.sig_s) * chan(mei.comb, ch) / chan(mei.sig_t, ch);
+          f3 wi = -rd;
+          float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
+          uint32_t sdim = dim;
+          dim += 3;
+          LightSample ls = sample_light(scene, mei.p, ul, u1, u2);
+          if (ls.pdf > 0.f) {
+            float ph = phase_eval(M, wi, ls.dir);
+            float wgt = ls.delta ? 1.f : mis_weight(ls.pdf, ph);
+            f3 c = mul(T, ls.li) * wgt * ph / ls.pdf;
+            if (!is_zero(c)) {  // MediumInteraction::SpawnRay: no offset (its normal is zero)
+              emit = true;
+              so = make_float4(mei.p.x, mei.p.y, mei.p.z, ls.dist);
+              sdir = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, __uint_as_float(s));
+              sc = make_float4(c.x, c.y, c.z, __uint_as_float(pack_vol(medium, ch, sdim)));
+            }
+          }
+          float pp;
+          f3 wo = phase_sample(M, wi, rng_u(key, dim), rng_u(key, dim + 1), pp);
+          dim += 2;
+          ro = mei.p;
+          rd = wo;
+          tmin = 0.f;
+          tmax = kFloatMax;
+          spec = false;
+          prevPdf = pp;
+          alive = true;
+        }
+        isSurface = isSurface || escaped;
+        if (isSurface) {
+          if (geom == 0xffffffffu) {  // escaped: environment emission, then stop
+            if (scene.env_light >= 0) {
+              float wgt = 1.f;
+              if (!spec) wgt = mis_weight(prevPdf, light_pdf_dir(scene, scene.env_light, prevP, rd, rd, rd, 0.f));
+              L += mul(T, scene.lights[scene.env_light].radiance) * wgt;
+            }
+          } else {
+            SurfaceHit si = surface_hit(scene, ro, rd, hh.x, hh.y, hh.z, __float_as_uint(hh.w), geom);
+            const ShapeRec& sr = scene.shapes[geom];
+            if (sr.light >= 0) {
+              float wgt = 1.f;
+              if (!spec) {
+                f3 rel = si.p - prevP;
+                float dist = norm(rel);
+                f3 dir = rel / dist;
+                wgt = mis_weight(prevPdf, light_pdf_dir(scene, sr.light, prevP, si.p, si.sh.n, dir, dist));
+              }
+              f3 le = dot(si.sh.to_world(si.wi), si.n) >= 0.f ? scene.lights[sr.light].radiance : splat(0.f);
+              L += mul(T, le) * wgt;
+            } else if (sr.bsdf < 0) {  // medium boundary without a BSDF: pass through (vol_path.cpp:171-176)
+              ro = offset_p(si.p, si.n, rd);
+              tmin = 0.f;
+              tmax = kFloatMax;
+              medium = dot(rd, si.n) > 0.f ? sr.med_out : sr.med_in;
+              next_depth = depth;
+              alive = true;
+            } else {
+              const BsdfRec& b = scene.bsdfs[sr.bsdf];
+              if (b.lobes & LOBE_NO_DELTA) {  // NEE through media (vol_path.cpp:178-190)
+                float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
+                uint32_t sdim = dim;
+                dim += 3;
+                LightSample ls = sample_light(scene, si.p, ul, u1, u2);
+                if (ls.pdf > 0.f) {
+                  f3 wo = si.sh.to_local(ls.dir);
+                  float bp;
+                  f3 f = bsdf_eval(b, quirks, si.wi, wo, bp);
+                  float wgt = ls.delta ? 1.f : mis_weight(ls.pdf, bp);
+                  f3 c = mul(mul(T, f), ls.li) * wgt / ls.pdf;
+                  if (!is_zero(c)) {
+                    f3 o = offset_p(si.p, si.n, ls.dir);  // SpawnRay(dsr.Dir)
+                    emit = true;
+                    so = make_float4(o.x, o.y, o.z, ls.dist);
+                    sdir = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, __uint_as_float(s));
+                    sc = make_float4(c.x, c.y, c.z, __uint_as_float(pack_vol(medium, ch, sdim)));
+                  }
+                }
+              }
+              float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
+              dim += 3;
+              BsdfSample bs;
+              f3 f = bsdf_sample(b, quirks, si.wi, ul, u1, u2, bs);
+              if (bs.pdf > 0.f) {
+                T = mul(T, f) / bs.pdf;
+                eta *= bs.eta;
+                f3 wo = si.sh.to_world(bs.wo);
+                ro = offset_p(si.p, si.n, wo);
+                rd = wo;
+                tmin = 0.f;
+                tmax = kFloatMax;
+                prevPdf = bs.pdf;
+                prevP = si.p;
+                spec = (bs.lobes & LOBE_DELTA) != 0;
+                medium = dot(wo, si.n) > 0.f ? sr.med_out : sr.med_in;
+                alive = true;
+              }
+            }
+          }
+        }
+      }
+      if (alive) {
+        w.ray_o[s] = make_float4(ro.x, ro.y, ro.z, tmin);
+        w.ray_d[s] = make_float4(rd.x, rd.y, rd.z, tmax);
+        w.thr[s] = make_float4(T.x, T.y, T.z, eta);
+        w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
+        w.meta[s] = make_uint4(m.x, m.y, next_depth, dim);
+        w.medium[s] = uint32_t(medium + 1) | (ch << 24);
+      }
+      w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
+    }
+    bool valid = i < n;
+    uint32_t qpos = wave_append(&w.cnt[out], valid && alive);
+    if (valid && alive) w.queue[out][qpos] = s;
+    uint32_t dpos = wave_append(&w.cnt[CNT_DEAD], valid && !alive);
+    if (valid && !alive) w.dead[dpos] = s;
+    uint32_t spos = wave_append(&w.cnt[CNT_SHADOW], emit);
+    if (emit) {
+      w.sh_o[spos] = so;
+      w.sh_d[spos] = sdir;
+      w.sh_c[spos] = sc;
+    }
+    wave_count(&w.stats[ST_MEDIUM], msample);
+  }
+}
+
+// The transmittance loop of VolPath::SampleLight (vol_path.cpp:224-342):
+// ratio tracking on the path's channel through media, passing through
+// surfaces without a BSDF, blocked by any surface with one.
+__global__ void __launch_bounds__(kTraceBlock) k_vol_shadow(Wavefront w, DevScene scene) {
+  const uint32_t n = w.cnt[CNT_SHADOW];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t steps = 0;
+    if (i < n) {
+      float4 o4 = w.sh_o[i], d4 = w.sh_d[i], c4 = w.sh_c[i];
+      const uint32_t s = __float_as_uint(d4.w);
+      const uint32_t pk = __float_as_uint(c4.w);
+      int32_t medium = int32_t(pk & 0xffu) - 1;
+      const uint32_t ch = (pk >> 8) & 3u;
+      const uint32_t skey = shadow_key(w.key[s], pk >> 10);
+      uint32_t sdim = 0;
+      f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+      const float dist = o4.w;
+      float totalDist = 0.f;
+      f3 tr = splat(1.f);
+      bool active = true, needsIntersection = true;
+      Hit h;
+      bool hitValid = false;
+      float siT = kInf;
+      for (int guard = 0; active && guard < 100000; guard++) {
+        float remaining = dist * (1.f - kShadowEps) - totalDist;
+        active = active && remaining > 0.f;
+        if (!active) break;
+        bool escaped = false;
+        bool activeMedium = active && medium >= 0;
+        bool activeSurface = active && !activeMedium;
+        if (activeMedium) {
+          const MediumRec& M = scene.media[medium];
+          MediumSample mei = medium_sample(scene, M, o, d, remaining, rng_u(skey, sdim++), ch);
+          steps++;
+          float traceMax = remaining;
+          if (M.type == RAD_MEDIUM_HOMOGENEOUS && mei.T != kInf) traceMax = fminf(mei.T, remaining);
+          if (needsIntersection) {
+            hitValid = trace<false>(scene, o, d, 0.f, traceMax, h);
+            siT = hitValid ? h.t : kInf;
+          }
+          if (siT < mei.T) mei.T = kInf;
+          float t = fminf(remaining, fminf(mei.T, siT)) - mei.minT;
+          if (t >= 0.f) {
+            f3 e = mk(expf(mei.comb.x * -t), expf(mei.comb.y * -t), expf(mei.comb.z * -t));
+            f3 ffp = (siT < mei.T || mei.T > remaining) ? e : mul(e, mei.comb);
+            float trPdf = chan(ffp, ch);
+            f3 mTr = trPdf > 0.f ? e / trPdf : splat(0.f);
+            tr = mul(tr, mTr);
+          }
+          if (mei.T > remaining && mei.T != kInf) totalDist = dist;
+          if (mei.T > remaining) mei.T = kInf;
+          escaped = mei.T == kInf;
+          activeMedium = mei.T != kInf;
+          if (activeMedium) {
+            totalDist += mei.T;
+            o = mei.p;
+            siT = siT - mei.T;
+            tr = mul(tr, mei.sig_n);
+          }
+        }
+        bool intersect = activeSurface && needsIntersection;
+        if (intersect) {
+          hitValid = trace<false>(scene, o, d, 0.f, remaining, h);
+          siT = hitValid ? h.t : kInf;
+        }
+        needsIntersection = needsIntersection && !intersect;
+        activeSurface = activeSurface || escaped;
+        if (activeSurface) totalDist += siT;
+        activeSurface = activeSurface && hitValid && siT != kInf && active && !activeMedium;
+        int32_t nextMedium = medium;
+        if (activeSurface) {
+          const ShapeRec& sr = scene.shapes[h.geom];
+          if (sr.bsdf >= 0) tr = splat(0.f);
+          // the hit distance was measured from the current origin along d
+          SurfaceHit si = surface_hit(scene, o, d, h.t, h.u, h.v, h.prim, h.geom);
+          o = offset_p(si.p, si.n, d);
+          if (sr.med_in >= 0 || sr.med_out >= 0) nextMedium = dot(d, si.n) > 0.f ? sr.med_out : sr.med_in;
+        }
+        needsIntersection = needsIntersection || activeSurface;
+        active = active && (activeMedium || activeSurface) && !is_zero(tr);
+        medium = nextMedium;
+      }
+      if (!is_zero(tr)) {
+        float4 l = w.lsum[s];
+        w.lsum[s] = make_float4(l.x + c4.x * tr.x, l.y + c4.y * tr.y, l.z + c4.z * tr.z, l.w);
+      }
+    }
+    wave_sum(&w.stats[ST_MEDIUM], steps);
+  }
+}
+
+void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_shade_vol, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+}
+void LaunchVolShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_vol_shadow, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene);
+}
+
 void SetLastError(const std::string& msg);
 
 }  // namespace rad

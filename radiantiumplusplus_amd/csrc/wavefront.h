@@ -1,11 +1,15 @@
 // Wavefront state of a render job: structure-of-arrays path records in HBM,
-// compacted work queues and per-slot accumulators.
+// compacted work queues and the per-sample result planes.
 //
-// A job owns Q = pixels * K path slots; slot s always serves pixel s % pixels
-// and its samples lane, lane + K, lane + 2K, ... (lane = s / pixels), so the
-// per-slot accumulators can be reduced per pixel in a fixed order without
-// atomics.  Each iteration runs
-//   regen   (dead slots -> finalise the finished sample, start the next one)
+// A job owns Q path slots that are not tied to pixels.  Camera samples are
+// handed out from a global counter in sample-major order (id -> pixel
+// id % pixels, sample row0 + id / pixels), so every slot stays busy until
+// the pass runs out of samples and the primary rays of a wave are adjacent
+// pixels.  A finished sample's radiance goes to its own entry of the result
+// planes; after the pass, k_accum adds the entries of each pixel in sample
+// order (the order of sample_renderer.cpp:97-105), so the image does not
+// depend on which slot ran which sample.  Each iteration runs
+//   regen   (finished samples -> result planes, dead slots -> next sample)
 //   extend  (closest hit for every queued path)
 //   shade   (emission + MIS, NEE shadow-ray emission, BSDF sampling, RR)
 //   shadow  (occlusion test, add unoccluded NEE contributions)
@@ -18,11 +22,21 @@
 
 namespace rad {
 
-enum : uint32_t { SAMPLE_NONE = 0xffffffffu, SAMPLE_DONE = 0xfffffffeu };
+enum : uint32_t { SAMPLE_NONE = 0xffffffffu };
 enum : uint32_t { PATH_SPECULAR = 1u };
-enum : uint32_t { CNT_Q0 = 0, CNT_Q1 = 1, CNT_DEAD = 2, CNT_SHADOW = 3, CNT_WORDS = 8 };
-enum : uint32_t { ST_SEGMENTS = 0, ST_SHADOW = 1, ST_SAMPLES = 2, ST_MEDIUM = 3, ST_WORDS = 8 };
+enum : uint32_t {
+  CNT_Q0 = 0,
+  CNT_Q1 = 1,
+  CNT_DEAD = 2,
+  CNT_SHADOW = 3,
+  CNT_NEXT = 4,       // next sample id of the pass
+  CNT_FETCH_EXT = 5,  // extend-kernel work counter
+  CNT_FETCH_SH = 6,   // shadow-kernel work counter
+  CNT_WORDS = 8
+};
+enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_WORDS = 4 };
 enum : uint32_t { WF_QUIRKS = 1u };
+enum : uint32_t { MED_UNSET = 0xffffffffu };
 
 struct Wavefront {
   float4* ray_o;   // origin, tmin
@@ -30,11 +44,9 @@ struct Wavefront {
   float4* thr;     // throughput, eta
   float4* lsum;    // radiance of the sample in flight, previous BSDF pdf
   float4* prev;    // previous vertex position, path flags (bits)
-  uint4* meta;     // pixel, sample index, depth, rng dimension
+  uint4* meta;     // pixel, sample index (SAMPLE_NONE = idle), depth, rng dimension
   uint32_t* key;   // rng key of the sample in flight
-  uint32_t* medium;  // vol_path: current medium (+1, 0 = none) and channel in the high byte
-  float4* acc;     // per-slot radiance sum
-  float4* acc2;    // per-slot squared-radiance sum
+  uint32_t* medium;  // vol_path: MED_UNSET, or current medium + 1 (0 = none) | channel << 24
   float4* hit;     // t, u, v, prim (bits)
   uint32_t* hit_geom;
   uint32_t* queue[2];
@@ -42,28 +54,55 @@ struct Wavefront {
   float4* sh_o;    // shadow origin, tmax
   float4* sh_d;    // shadow direction, slot (bits)
   float4* sh_c;    // shadow contribution
+  float* res;      // result planes: R, G, B of rows * npix samples each
+  float4* pacc;    // per-pixel radiance sum
+  float4* pacc2;   // per-pixel squared-radiance sum
   uint32_t* cnt;   // CNT_* words
   unsigned long long* stats;  // ST_* words
-  uint32_t npix, K, Q, spp;
+  uint32_t npix, Q, spp;
+  uint32_t row0, rows;   // samples [row0, row0 + rows) of every pixel form the current pass
   uint32_t seed, request;
   int32_t max_depth, rr_depth;
   uint32_t flags;
   int32_t integrator;
 };
 
+// Append `pred` lanes of this wavefront to a queue with one atomic per wave.
+// Must be reached by every lane of the wave (uniform control flow).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return 0;
+  uint32_t lane = __lane_id();
+  uint32_t leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  if (mask != 0ull && __lane_id() == (uint32_t)(__ffsll((long long)mask) - 1))
+    atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
 struct LaunchCfg {
-  int blocks;
+  int blocks;   // persistent grid of the per-path kernels
   int threads;
 };
 
 // Host-side launchers (defined in path_kernels.hip).
-void LaunchInit(const Wavefront& w, hipStream_t s);
+void LaunchInitPass(const Wavefront& w, bool first, hipStream_t s);
 void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchReset(const Wavefront& w, int out, hipStream_t s);
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
-void LaunchResolve(const Wavefront& w, float* fb, float* fb_sumsq, hipStream_t s);
+void LaunchAccum(const Wavefront& w, hipStream_t s);
+// vol_path (defined in volpath_kernels.hip).
+void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
+void LaunchVolShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
+void LaunchFinal(const Wavefront& w, float* fb, float* fb_sumsq, hipStream_t s);
 
 // Ray-batch entry points.
 void LaunchClosest(const DevScene& scene, const void* rays, uint32_t n, void* hits, hipStream_t s);

@@ -453,7 +453,7 @@ class SceneBuilder:
         self._keep += [shapes, bsdfs, lights, media]
         return desc
 
-    def render_desc(self, spp=None, seed=None, flags=0, slots_per_pixel=0, guiding=0, request_id=0):
+    def render_desc(self, spp=None, seed=None, flags=0, path_slots=0, guiding=0, request_id=0):
         r = F.RenderDesc()
         r.integrator = F.RAD_INTEGRATOR_VOL_PATH if self.renderer["type"] == "vol_path" else F.RAD_INTEGRATOR_PATH
         r.max_depth = self.renderer["max_depth"]
@@ -461,7 +461,7 @@ class SceneBuilder:
         r.spp = int(spp if spp is not None else self.sampler["sample_count"])
         r.seed = int(seed if seed is not None else self.sampler["seed"])
         r.flags = flags
-        r.slots_per_pixel = slots_per_pixel
+        r.path_slots = path_slots
         r.guiding = guiding
         r.request_id = request_id
         return r

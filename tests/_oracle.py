@@ -26,6 +26,7 @@ class RenderParams(C.Structure):
         ("x_end", C.c_int32),
         ("y_begin", C.c_int32),
         ("y_end", C.c_int32),
+        ("tile_step", C.c_int32),
     ]
 
 
@@ -106,7 +107,8 @@ class OracleScene:
         lib().ref_camera_ray(self.h, sx, sy, r.ctypes.data)
         return r[0]
 
-    def render(self, spp=None, seed=None, threads=0, window=None, want_sumsq=True, max_depth=None, rr_depth=None):
+    def render(self, spp=None, seed=None, threads=0, window=None, want_sumsq=True, max_depth=None, rr_depth=None,
+               tile_step=0):
         b = self.builder
         W, H = b.camera["width"], b.camera["height"]
         p = RenderParams()
@@ -118,6 +120,7 @@ class OracleScene:
         p.threads = threads
         if window:
             p.x_begin, p.x_end, p.y_begin, p.y_end = window
+        p.tile_step = tile_step
         fb = np.zeros((H, W, 3), np.float32)
         sq = np.zeros((H, W, 3), np.float32) if want_sumsq else None
         secs = C.c_double()

@@ -2,6 +2,9 @@
 
 Per-tile z-test: the difference of two independent Monte Carlo tile means
 divided by the standard error estimated from the per-pixel second moments.
+The standard error has a floor of 2e-6 of the tile mean: both renderers sum
+float32 samples, so tiles whose samples are all equal (a constant sky seen
+directly) differ by summation-order rounding while their variance is zero.
 relMSE over tile means: mean((a - b)^2 / (b^2 + eps)).
 """
 import numpy as np
@@ -24,6 +27,7 @@ def tile_mean_and_var(mean, sumsq, spp, t):
 def compare(a_mean, a_sq, a_spp, b_mean, b_sq, b_spp, tile=8, eps=1e-4):
     ma, va = tile_mean_and_var(a_mean.astype(np.float64), a_sq.astype(np.float64), a_spp, tile)
     mb, vb = tile_mean_and_var(b_mean.astype(np.float64), b_sq.astype(np.float64), b_spp, tile)
-    z = (ma - mb) / np.sqrt(va + vb + 1e-20)
+    floor = (2e-6 * np.maximum(np.abs(ma), np.abs(mb))) ** 2
+    z = (ma - mb) / np.sqrt(va + vb + floor + 1e-20)
     rel = ((ma - mb) ** 2 / (mb ** 2 + eps)).mean()
     return z, rel
