@@ -1,0 +1,130 @@
+// Device side of path guiding: the one-blob input encoding, the decoded
+// von Mises-Fisher mixture (pdf and sampling), and the rule for which
+// vertices are guided.
+#pragma once
+
+#include "guide.h"
+#include "shading_dev.h"
+
+namespace rad {
+namespace guide {
+
+// Scalars in [0, 1] that the one-blob encoding spreads over bins:
+// position (3), normal (polar, azimuth), outgoing direction (polar, azimuth).
+struct FeatScalars {
+  float s[7];
+};
+
+RAD_HD float dir_polar(f3 d) { return fminf(fmaxf(0.5f * (d.y + 1.f), 0.f), 1.f); }
+__device__ inline float dir_azimuth(f3 d) { return fminf(fmaxf(atan2f(d.z, d.x) * (0.5f / kPi) + 0.5f, 0.f), 1.f); }
+
+__device__ inline FeatScalars feat_scalars(const float* f9, const float lo[3], const float inv_ext[3]) {
+  FeatScalars r;
+  for (int i = 0; i < 3; i++) r.s[i] = fminf(fmaxf((f9[i] - lo[i]) * inv_ext[i], 0.f), 1.f);
+  f3 n = mk(f9[3], f9[4], f9[5]), wi = mk(f9[6], f9[7], f9[8]);
+  r.s[3] = dir_polar(n);
+  r.s[4] = dir_azimuth(n);
+  r.s[5] = dir_polar(wi);
+  r.s[6] = dir_azimuth(wi);
+  return r;
+}
+
+// One-blob value of input feature f: a unit-width triangle kernel centred on
+// the scalar's position among the bins.
+__device__ __forceinline__ float one_blob(const FeatScalars& fs, int f) {
+  int dim, bin, bins;
+  if (f < 3 * kPosBins) {
+    dim = f / kPosBins;
+    bin = f % kPosBins;
+    bins = kPosBins;
+  } else {
+    int g = f - 3 * kPosBins;
+    dim = 3 + g / kDirBins;
+    bin = g % kDirBins;
+    bins = kDirBins;
+  }
+  float x = fs.s[dim] * float(bins) - (float(bin) + 0.5f);
+  return fmaxf(0.f, 1.f - fabsf(x));
+}
+
+struct Mixture {
+  float pi[kLobes];
+  f3 mu[kLobes];
+  float kappa[kLobes];
+};
+
+// raw: 40 network outputs (logit, mean xyz, log kappa per lobe).
+template <class T>
+__device__ inline Mixture decode(const T* raw) {
+  Mixture m;
+  float mx = -kInf;
+  for (int l = 0; l < kLobes; l++) mx = fmaxf(mx, float(raw[kRawPerLobe * l]));
+  float sum = 0.f;
+  for (int l = 0; l < kLobes; l++) {
+    m.pi[l] = expf(float(raw[kRawPerLobe * l]) - mx);
+    sum += m.pi[l];
+  }
+  for (int l = 0; l < kLobes; l++) {
+    m.pi[l] /= sum;
+    f3 v = mk(float(raw[kRawPerLobe * l + 1]), float(raw[kRawPerLobe * l + 2]), float(raw[kRawPerLobe * l + 3]));
+    float len = sqrtf(sqnorm(v));
+    m.mu[l] = len > 1e-6f ? v / len : mk(0.f, 1.f, 0.f);
+    m.kappa[l] = expf(fminf(fmaxf(float(raw[kRawPerLobe * l + 4]), -3.f), 6.f));
+  }
+  return m;
+}
+
+// vMF density on the sphere: kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}.
+__device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
+  float norm = kappa / ((2.f * kPi) * (-expm1f(-2.f * kappa)));
+  return norm * expf(kappa * (dot(mu, w) - 1.f));
+}
+
+__device__ inline float mixture_pdf(const Mixture& m, f3 w) {
+  float p = 0.f;
+  for (int l = 0; l < kLobes; l++) p += m.pi[l] * vmf_pdf(m.mu[l], m.kappa[l], w);
+  return p;
+}
+
+// Pick a lobe with u0, then sample it (inverse CDF of the polar cosine).
+__device__ inline f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
+  int l = 0;
+  float c = m.pi[0];
+  while (l < kLobes - 1 && u0 >= c) {
+    l++;
+    c += m.pi[l];
+  }
+  float k = m.kappa[l];
+  float w = 1.f + logf(fmaxf(u1 + (1.f - u1) * expf(-2.f * k), 1e-30f)) / k;
+  w = fminf(fmaxf(w, -1.f), 1.f);
+  float r = safe_sqrt(1.f - w * w);
+  float a = 2.f * kPi * u2;
+  Frame fr = frame_from_n(m.mu[l]);
+  return normalize(fr.to_world(mk(r * cosf(a), r * sinf(a), w)));
+}
+
+// A vertex is guided when its BSDF has no delta lobe and its sampler's
+// density equals its pdf: with the reference's visible-normal azimuth quirk
+// on, only BSDFs that do not sample visible normals qualify.
+__device__ __forceinline__ bool guidable(const BsdfRec& b, bool quirks) {
+  if (b.lobes & LOBE_DELTA) return false;
+  if (b.type == BSDF_DIFFUSE) return true;
+  if (!quirks) return true;
+  return (b.type == BSDF_ROUGH_METAL || b.type == BSDF_ROUGH_GLASS) && !b.sample_visible;
+}
+
+// Geometric normal of a hit, for the network inputs.
+__device__ inline f3 geometric_normal(const DevScene& s, f3 p, uint32_t prim, uint32_t geom) {
+  const ShapeRec& sr = s.shapes[geom];
+  if (sr.type == SHAPE_MESH) {
+    const uint32_t* I = s.idx + 3 * size_t(sr.tri_offset + prim);
+    f3 p0 = load3(s.pos, sr.vtx_offset + I[0]), p1 = load3(s.pos, sr.vtx_offset + I[1]),
+       p2 = load3(s.pos, sr.vtx_offset + I[2]);
+    return normalize(cross(p1 - p0, p2 - p0));
+  }
+  if (sr.type == SHAPE_SPHERE) return normalize(p - sr.center);
+  return sr.frame_n;
+}
+
+}  // namespace guide
+}  // namespace rad

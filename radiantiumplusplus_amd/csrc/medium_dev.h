@@ -131,7 +131,7 @@ __device__ inline f3 phase_sample(const MediumRec& m, f3 wi, float u1, float u2,
   float g = m.g;
   float sqrTerm = (1.f - g * g) / (1.f - g + 2.f * g * u1);
   float cosTheta = (1.f + g * g - sqrTerm * sqrTerm) / (2.f * g);
-  if (fabsf(g) < kFloatEps) cosTheta = 1.f - 2.f * u1;
+  if (fabsf(g) < kFloat32Eps) cosTheta = 1.f - 2.f * u1;
   float sinTheta = safe_sqrt(1.f - cosTheta * cosTheta);
   float a = 2.f * kPi * u2;
   Frame fr = frame_from_n(wi);

@@ -21,7 +21,6 @@ constexpr float kShadowEps = kRayEps * 10.0f;        // math_ext.h:16
 constexpr float kOneMinusEps = 0x1.fffffep-1f;
 constexpr float kInf = __builtin_huge_valf();
 constexpr float kFloatMax = 3.402823466e+38f;
-constexpr float kFloatEps = 0x1p-24f;  // math_ext.h:9
 
 struct f3 {
   float x, y, z;
