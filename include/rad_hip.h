@@ -209,6 +209,8 @@ enum { /* rad_job_stats.kernel_ms slots */
   RAD_KERNEL_EXTEND = 1,
   RAD_KERNEL_SHADE = 2,
   RAD_KERNEL_SHADOW = 3,
+  RAD_KERNEL_GUIDE = 4,    /* guided-vertex gathering + network inference */
+  RAD_KERNEL_TRAIN = 5,    /* guide training steps between passes */
   RAD_KERNEL_COUNT = 8
 };
 
@@ -220,7 +222,7 @@ typedef struct rad_render_desc {
   uint32_t seed;           /* sampler.seed */
   uint32_t flags;          /* RAD_RENDER_* */
   uint32_t path_slots;     /* paths in flight (wavefront slots), 0 = chosen by the library */
-  int32_t guiding;         /* learned path guiding: 0 off, 1 on */
+  int32_t guiding;         /* learned path guiding: 0 off, 1 train online and guide, 2 guide with the scene's current weights */
   uint32_t request_id;     /* RNG stream key for serving (0 for single renders) */
   uint32_t reserved;
 } rad_render_desc;
@@ -236,6 +238,11 @@ typedef struct rad_job_stats {
   uint32_t slots;          /* wavefront capacity */
   double kernel_ms[RAD_KERNEL_COUNT]; /* summed kernel time per stage (RAD_RENDER_PROFILE) */
   uint32_t kernel_launches[RAD_KERNEL_COUNT];
+  uint32_t passes;         /* sample passes (result-plane batches) */
+  uint32_t guide_steps;    /* training steps the scene's guide has taken in total */
+  uint64_t guide_records;  /* training records collected by this render */
+  float guide_loss;        /* mean loss of the last training step */
+  float guide_alpha;       /* probability of sampling the guide in the final pass */
 } rad_job_stats;
 
 typedef struct rad_bvh_opts {
@@ -325,6 +332,33 @@ rad_status rad_hip_bsdf_eval(const rad_bsdf_desc* bsdf, uint32_t quirks, const f
 rad_status rad_hip_bsdf_sample(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out);
 /* Heterogeneous grid lookup (volume/grid.cpp:41-75): in = n x {p[3]} in [0,1]^3, out = n x 3. */
 rad_status rad_hip_grid_eval(const rad_medium_desc* medium, const float* in, uint32_t n, float* out);
+
+/* ---- learned path guiding ------------------------------------------------------
+ * No counterpart in the reference (its path integrator samples the BSDF only,
+ * path.cpp:192-210).  A 5-layer MLP (80-64-64-64-64-48, FP16 MFMA, FP32
+ * accumulation) maps a surface vertex to an 8-lobe vMF mixture that is
+ * combined with BSDF sampling by one-sample MIS when rad_render_desc.guiding
+ * is set.  The scene keeps the trained weights between renders.  The probes
+ * below run the same kernels the renderer runs. */
+enum { RAD_GUIDE_PARAMS = 20784, RAD_GUIDE_RAW = 48, RAD_GUIDE_FEATS = 9 };
+typedef struct rad_guide_record {
+  float p[3], n[3], wi[3]; /* vertex position, geometric normal, outgoing (-ray) direction */
+  float dir[3];            /* sampled incident direction */
+  float thr[3], lbase[3];  /* training bookkeeping (unused by the probes) */
+  float w;                 /* sample weight F / p */
+  uint32_t prev;
+} rad_guide_record;
+/* raw = n x 48 network outputs for feats = n x 9 {p, n, wi}; bounds = {lo[3], hi[3]}. */
+rad_status rad_hip_guide_forward(const float* params, const float* bounds, const float* feats, uint32_t n, float* raw);
+/* Mean loss (1/n) sum_j min(w_j * wnorm, 64) * -log q(dir_j) and its gradient (RAD_GUIDE_PARAMS floats). */
+rad_status rad_hip_guide_grad(const float* params, const float* bounds, const rad_guide_record* recs, uint32_t n,
+                              float wnorm, float* grad, float* loss);
+/* Mixture of one raw row (40 used values): pdf of n directions / n samples {dir[3], pdf} from xi = n x 3. */
+rad_status rad_hip_guide_pdf(const float* raw, const float* dirs, uint32_t n, float* pdf);
+rad_status rad_hip_guide_sample(const float* raw, const float* xi, uint32_t n, float* out);
+/* The scene's guide weights: read (with the number of training steps taken) or replace (NULL = re-initialise). */
+rad_status rad_hip_scene_guide_get(rad_scene* scene, float* params, uint32_t* steps);
+rad_status rad_hip_scene_guide_set(rad_scene* scene, const float* params);
 
 #ifdef __cplusplus
 }

@@ -176,11 +176,31 @@ class JobStats(C.Structure):
         ("slots", C.c_uint32),
         ("kernel_ms", C.c_double * 8),
         ("kernel_launches", C.c_uint32 * 8),
+        ("passes", C.c_uint32),
+        ("guide_steps", C.c_uint32),
+        ("guide_records", C.c_uint64),
+        ("guide_loss", C.c_float),
+        ("guide_alpha", C.c_float),
     ]
 
 
 RAD_RENDER_PROFILE = 1 << 2
 RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_SHADE, RAD_KERNEL_SHADOW = 0, 1, 2, 3
+RAD_KERNEL_GUIDE, RAD_KERNEL_TRAIN = 4, 5
+RAD_GUIDE_PARAMS, RAD_GUIDE_RAW, RAD_GUIDE_FEATS = 20784, 48, 9
+
+
+class GuideRecord(C.Structure):
+    _fields_ = [
+        ("p", C.c_float * 3),
+        ("n", C.c_float * 3),
+        ("wi", C.c_float * 3),
+        ("dir", C.c_float * 3),
+        ("thr", C.c_float * 3),
+        ("lbase", C.c_float * 3),
+        ("w", C.c_float),
+        ("prev", C.c_uint32),
+    ]
 
 
 
@@ -217,4 +237,5 @@ ABI_STRUCTS = {
     "rad_bvh_opts": BvhOpts,
     "rad_scene_info": SceneInfo,
     "rad_job_stats": JobStats,
+    "rad_guide_record": GuideRecord,
 }

@@ -43,6 +43,7 @@ SOURCES = [
     "capi.cpp",
     "path_kernels.hip",
     "volpath_kernels.hip",
+    "guide_kernels.hip",
     "host/json.cpp",
     "host/loader.cpp",
     "host/exr.cpp",

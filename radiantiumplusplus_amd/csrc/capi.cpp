@@ -106,6 +106,7 @@ size_t rad_abi_sizeof(const char* name) {
   RAD_SZ(rad_hit)
   RAD_SZ(rad_render_desc)
   RAD_SZ(rad_job_stats)
+  RAD_SZ(rad_guide_record)
   RAD_SZ(rad_bvh_opts)
   RAD_SZ(rad_scene_info)
 #undef RAD_SZ

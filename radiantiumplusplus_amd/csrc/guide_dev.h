@@ -55,15 +55,18 @@ struct Mixture {
 
 // raw: 40 network outputs (logit, mean xyz, log kappa per lobe).
 template <class T>
-__device__ inline Mixture decode(const T* raw) {
+__device__ __forceinline__ Mixture decode(const T* raw) {
   Mixture m;
   float mx = -kInf;
+#pragma unroll
   for (int l = 0; l < kLobes; l++) mx = fmaxf(mx, float(raw[kRawPerLobe * l]));
   float sum = 0.f;
+#pragma unroll
   for (int l = 0; l < kLobes; l++) {
     m.pi[l] = expf(float(raw[kRawPerLobe * l]) - mx);
     sum += m.pi[l];
   }
+#pragma unroll
   for (int l = 0; l < kLobes; l++) {
     m.pi[l] /= sum;
     f3 v = mk(float(raw[kRawPerLobe * l + 1]), float(raw[kRawPerLobe * l + 2]), float(raw[kRawPerLobe * l + 3]));
@@ -80,34 +83,44 @@ __device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
   return norm * expf(kappa * (dot(mu, w) - 1.f));
 }
 
-__device__ inline float mixture_pdf(const Mixture& m, f3 w) {
+__device__ __forceinline__ float mixture_pdf(const Mixture& m, f3 w) {
   float p = 0.f;
+#pragma unroll
   for (int l = 0; l < kLobes; l++) p += m.pi[l] * vmf_pdf(m.mu[l], m.kappa[l], w);
   return p;
 }
 
-// Pick a lobe with u0, then sample it (inverse CDF of the polar cosine).
-__device__ inline f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
-  int l = 0;
-  float c = m.pi[0];
-  while (l < kLobes - 1 && u0 >= c) {
-    l++;
+// Pick a lobe with u0 (selects only, no dynamic indexing), then sample it
+// by inverting the CDF of the cosine to the lobe axis.
+__device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
+  f3 mu = m.mu[kLobes - 1];
+  float k = m.kappa[kLobes - 1];
+  float c = 0.f;
+  bool found = false;
+#pragma unroll
+  for (int l = 0; l < kLobes - 1; l++) {
     c += m.pi[l];
+    bool pick = !found && u0 < c;
+    // exact 0/1 blend: a select chain here is turned into a scratch lookup
+    float pf = pick ? 1.f : 0.f, qf = 1.f - pf;
+    mu = mk(fmaf(m.mu[l].x, pf, mu.x * qf), fmaf(m.mu[l].y, pf, mu.y * qf), fmaf(m.mu[l].z, pf, mu.z * qf));
+    k = fmaf(m.kappa[l], pf, k * qf);
+    found = found || pick;
   }
-  float k = m.kappa[l];
   float w = 1.f + logf(fmaxf(u1 + (1.f - u1) * expf(-2.f * k), 1e-30f)) / k;
   w = fminf(fmaxf(w, -1.f), 1.f);
   float r = safe_sqrt(1.f - w * w);
   float a = 2.f * kPi * u2;
-  Frame fr = frame_from_n(m.mu[l]);
+  Frame fr = frame_from_n(mu);
   return normalize(fr.to_world(mk(r * cosf(a), r * sinf(a), w)));
 }
 
-// A vertex is guided when its BSDF has no delta lobe and its sampler's
+// A vertex is guided when its BSDF is reflective only (a guided refraction
+// would need the BSDF's relative IOR), has no delta lobe, and its sampler's
 // density equals its pdf: with the reference's visible-normal azimuth quirk
 // on, only BSDFs that do not sample visible normals qualify.
 __device__ __forceinline__ bool guidable(const BsdfRec& b, bool quirks) {
-  if (b.lobes & LOBE_DELTA) return false;
+  if (b.lobes & (LOBE_DELTA | LOBE_TRANSMISSION)) return false;
   if (b.type == BSDF_DIFFUSE) return true;
   if (!quirks) return true;
   return (b.type == BSDF_ROUGH_METAL || b.type == BSDF_ROUGH_GLASS) && !b.sample_visible;

@@ -2,6 +2,7 @@
 // the ray-batch intersection kernels (accel/embree.cpp:79-111).
 #include <algorithm>
 
+#include "guide_dev.h"
 #include "shading_dev.h"
 #include "wavefront.h"
 
@@ -18,6 +19,7 @@ __global__ void k_init_pass(Wavefront w, int first) {
   if (s < w.Q) {
     w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);
     w.dead[s] = s;
+    if (w.glast) w.glast[s] = w.gpend[s] = 0xffffffffu;
   }
   if (s == 0) {
     w.cnt[CNT_Q0] = 0;
@@ -30,6 +32,31 @@ __global__ void k_init_pass(Wavefront w, int first) {
     if (first)
       for (int i = 0; i < ST_WORDS; i++) w.stats[i] = 0ull;
   }
+}
+
+// Training targets of a finished path: each record's weight is the
+// luminance of (L_final - L_before_bounce) / throughput_before_bounce, the
+// one-sample estimate of F / p for the direction sampled there (F: incident
+// radiance times BSDF cosine).  A record still pending (the path ended right
+// after its bounce) gets no contribution.
+__device__ __forceinline__ void finish_records(const Wavefront& w, uint32_t s, f3 L) {
+  uint32_t r = w.glast[s];
+  const uint32_t pend = w.gpend[s];
+  float wsum = 0.f;
+  for (int guard = 0; r != 0xffffffffu && guard < 4096; guard++) {
+    guide::TrainRecord& rc = w.recs[r];
+    f3 lb = r == pend ? L : mk(rc.lbase[0], rc.lbase[1], rc.lbase[2]);
+    f3 d = mk(fmaxf(L.x - lb.x, 0.f), fmaxf(L.y - lb.y, 0.f), fmaxf(L.z - lb.z, 0.f));
+    f3 q = mk(rc.thr[0] > 0.f ? d.x / rc.thr[0] : 0.f, rc.thr[1] > 0.f ? d.y / rc.thr[1] : 0.f,
+              rc.thr[2] > 0.f ? d.z / rc.thr[2] : 0.f);
+    float wv = 0.212671f * q.x + 0.715160f * q.y + 0.072169f * q.z;
+    wv = isfinite(wv) ? wv : 0.f;
+    rc.w = wv;
+    wsum += wv;
+    r = rc.prev;
+  }
+  if (wsum > 0.f) atomicAdd(reinterpret_cast<float*>(&w.gcnt[G_WSUM]), wsum);
+  w.glast[s] = w.gpend[s] = 0xffffffffu;
 }
 
 // Store the sample a dead slot finished into the result planes and start the
@@ -56,6 +83,7 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
         w.res[r + plane] = l.y;
         w.res[r + 2 * plane] = l.z;
         finished = true;
+        if (w.flags & WF_RECORD) finish_records(w, s, mk(l.x, l.y, l.z));
       }
     }
     uint32_t id = more ? wave_append(&w.cnt[CNT_NEXT], valid) : pass;
@@ -75,6 +103,7 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
       w.prev[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
       w.meta[s] = make_uint4(pixel, sample, 0u, 2u);
       w.key[s] = key;
+      if (w.flags & WF_RECORD) w.glast[s] = w.gpend[s] = 0xffffffffu;
       if (w.medium) w.medium[s] = 0xffffffffu;  // vol_path: medium / channel chosen at shade time
     } else if (valid) {
       w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);  // idle for the rest of the pass
@@ -121,17 +150,26 @@ __global__ void __launch_bounds__(kTraceBlock) k_extend(Wavefront w, DevScene sc
   trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
 }
 
-// One bounce of Path::Li (path.cpp:141-225) for every queued path.
+// One bounce of Path::Li (path.cpp:141-225) for every queued path.  GUIDE:
+// at guided vertices the direction is drawn from the learned vMF mixture
+// (probability alpha) or the BSDF, and weighted by the one-sample MIS
+// density alpha q + (1 - alpha) p_bsdf; NEE uses the same density in its MIS
+// weight; training records of the vertices are kept for 1 in rec_stride slots.
+template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene, int in) {
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
+  const bool guideOn = GUIDE && (w.flags & WF_GUIDE) != 0;
+  const bool record = GUIDE && (w.flags & WF_RECORD) != 0;
+  const float alpha = w.net.alpha;
   const int out = 1 - in;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
-    bool alive = false, emit = false;
+    bool alive = false, emit = false, want_rec = false;
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
+    guide::TrainRecord rec;
     if (i < n) {
       s = w.queue[in][i];
       float4 ro4 = w.ray_o[s], rd4 = w.ray_d[s];
@@ -151,6 +189,15 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       uint4 m = w.meta[s];
       uint32_t depth = m.z, dim = m.w;
       uint32_t key = w.key[s];
+      if (record) {  // the previous vertex's record: radiance gathered before its bounce
+        uint32_t pr = w.gpend[s];
+        if (pr != 0xffffffffu) {
+          w.recs[pr].lbase[0] = L.x;
+          w.recs[pr].lbase[1] = L.y;
+          w.recs[pr].lbase[2] = L.z;
+          w.gpend[s] = 0xffffffffu;
+        }
+      }
       if (geom == 0xffffffffu) {
         // miss: environment emission (scene.cpp:105-113, skybox.cpp:53-63)
         if (scene.env_light >= 0) {
@@ -175,6 +222,10 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
         bool stop = (w.max_depth >= 0 && int32_t(depth) + 1 >= w.max_depth) || sr.light >= 0;
         if (!stop) {
           const BsdfRec& b = scene.bsdfs[sr.bsdf];
+          const bool guided = GUIDE && guide::guidable(b, quirks);
+          guide::Mixture mx;
+          if (guided && guideOn) mx = guide::decode(w.graw + size_t(s) * guide::kOut);
+          const float a = (guided && guideOn) ? alpha : 0.f;
           if (b.lobes & LOBE_NO_DELTA) {  // NEE (path.cpp:173-191)
             float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
             dim += 3;
@@ -183,6 +234,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
               f3 wo = si.sh.to_local(ls.dir);
               float bp;
               f3 f = bsdf_eval(b, quirks, si.wi, wo, bp);
+              if (a > 0.f) bp = a * guide::mixture_pdf(mx, ls.dir) + (1.f - a) * bp;
               // Scene::IsOcclude setup (scene.cpp:53-63)
               f3 o = offset_p(si.p, si.n, ls.p - si.p);
               f3 dd = ls.p - o;
@@ -200,19 +252,52 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
               }
             }
           }
-          // BSDF sampling (path.cpp:192-210)
+          // BSDF sampling (path.cpp:192-210), or the guide with probability a
           float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
           dim += 3;
           BsdfSample bs;
-          f3 f = bsdf_sample(b, quirks, si.wi, ul, u1, u2, bs);
-          if (bs.pdf > 0.f) {
-            T = mul(T, f) / bs.pdf;
+          f3 f;
+          f3 wo;
+          float pdf;
+          bool useGuide = false;
+          if (a > 0.f) {
+            useGuide = rng_u(key, dim) < a;
+            dim++;
+          }
+          if (useGuide) {
+            f3 wg = guide::mixture_sample(mx, ul, u1, u2);
+            float bp;
+            f = bsdf_eval(b, quirks, si.wi, si.sh.to_local(wg), bp);
+            wo = wg;
+            pdf = a * guide::mixture_pdf(mx, wg) + (1.f - a) * bp;
+            bs.eta = 1.f;
+            bs.lobes = LOBE_GLOSSY;
+            if (is_zero(f)) pdf = 0.f;  // zero-valued paths end here without bias
+          } else {
+            f = bsdf_sample(b, quirks, si.wi, ul, u1, u2, bs);
+            wo = si.sh.to_world(bs.wo);
+            pdf = bs.pdf;
+            if (a > 0.f && pdf > 0.f) pdf = a * guide::mixture_pdf(mx, wo) + (1.f - a) * bs.pdf;
+          }
+          if (pdf > 0.f) {
+            if (record && guided && (s % w.rec_stride) == 0) {
+              want_rec = true;
+              f3 gn = guide::geometric_normal(scene, si.p, __float_as_uint(hh.w), geom);
+              rec.p[0] = si.p.x, rec.p[1] = si.p.y, rec.p[2] = si.p.z;
+              rec.n[0] = gn.x, rec.n[1] = gn.y, rec.n[2] = gn.z;
+              rec.wi[0] = -rd.x, rec.wi[1] = -rd.y, rec.wi[2] = -rd.z;
+              rec.dir[0] = wo.x, rec.dir[1] = wo.y, rec.dir[2] = wo.z;
+              rec.thr[0] = T.x, rec.thr[1] = T.y, rec.thr[2] = T.z;
+              rec.lbase[0] = rec.lbase[1] = rec.lbase[2] = 0.f;
+              rec.w = 0.f;
+              rec.prev = w.glast[s];
+            }
+            T = mul(T, f) / pdf;
             eta *= bs.eta;
-            f3 wo = si.sh.to_world(bs.wo);
             f3 no = offset_p(si.p, si.n, wo);
             w.ray_o[s] = make_float4(no.x, no.y, no.z, 0.f);
             w.ray_d[s] = make_float4(wo.x, wo.y, wo.z, kFloatMax);
-            prevPdf = bs.pdf;
+            prevPdf = pdf;
             prevP = si.p;
             spec = (bs.lobes & LOBE_DELTA) != 0;
             alive = true;
@@ -234,6 +319,14 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       w.meta[s] = make_uint4(m.x, m.y, depth + 1, dim);
     }
     bool valid = i < n;
+    if (record) {
+      uint32_t r = wave_append(&w.gcnt[G_RECORDS], want_rec);
+      if (want_rec && r < w.rec_cap) {
+        w.recs[r] = rec;
+        w.glast[s] = r;
+        w.gpend[s] = r;  // baseline filled by the next bounce (or the path's end)
+      }
+    }
     uint32_t qpos = wave_append(&w.cnt[out], valid && alive);
     if (valid && alive) w.queue[out][qpos] = s;
     uint32_t dpos = wave_append(&w.cnt[CNT_DEAD], valid && !alive);
@@ -245,6 +338,48 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       w.sh_c[spos] = sc;
     }
   }
+}
+
+// Guided vertices of this iteration: surface hits whose BSDF the guide may
+// sample; their network inputs go to the guide queue for k_guide_infer.
+__global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene, int in) {
+  const uint32_t n = w.cnt[in];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const bool quirks = (w.flags & WF_QUIRKS) != 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    bool g = false;
+    uint32_t s = 0;
+    f3 p = splat(0.f), gn = splat(0.f), wi = splat(0.f);
+    if (i < n) {
+      s = w.queue[in][i];
+      uint32_t geom = w.hit_geom[s];
+      if (geom != 0xffffffffu) {
+        const ShapeRec& sr = scene.shapes[geom];
+        if (sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks)) {
+          float4 ro = w.ray_o[s], rd = w.ray_d[s];
+          float4 hh = w.hit[s];
+          f3 d = mk(rd.x, rd.y, rd.z);
+          p = d * hh.x + mk(ro.x, ro.y, ro.z);
+          gn = guide::geometric_normal(scene, p, __float_as_uint(hh.w), geom);
+          wi = -d;
+          g = true;
+        }
+      }
+    }
+    uint32_t pos = wave_append(&w.gcnt[G_QUEUE], g);
+    if (g) {
+      w.gq[pos] = s;
+      float* f = w.gfeat + size_t(pos) * guide::kFeat;
+      f[0] = p.x, f[1] = p.y, f[2] = p.z;
+      f[3] = gn.x, f[4] = gn.y, f[5] = gn.z;
+      f[6] = wi.x, f[7] = wi.y, f[8] = wi.z;
+    }
+  }
+}
+
+__global__ void k_guide_reset(uint32_t* gcnt) {
+  if (threadIdx.x == 0) gcnt[G_QUEUE] = 0;
 }
 
 struct ShadowJob {
@@ -409,7 +544,14 @@ void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const Launc
   hipLaunchKernelGGL(k_extend, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene, in);
 }
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade_path, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  if (w.flags & (WF_GUIDE | WF_RECORD))
+    hipLaunchKernelGGL(k_shade_path<true>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  else
+    hipLaunchKernelGGL(k_shade_path<false>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+}
+void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_guide_reset, dim3(1), dim3(64), 0, s, w.gcnt);
+  hipLaunchKernelGGL(k_guide_prep, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_shadow, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene);

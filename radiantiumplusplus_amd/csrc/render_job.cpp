@@ -6,7 +6,9 @@
 #include <cstring>
 #include <mutex>
 #include <thread>
+#include <vector>
 
+#include "guide_state.h"
 #include "wavefront.h"
 
 struct rad_job {
@@ -15,8 +17,10 @@ struct rad_job {
   rad::Wavefront w{};
   rad::DeviceBuffer arena;
   rad::DeviceBuffer d_fb, d_sq;
+  rad::DeviceBuffer d_gnet;  // job-private copy of the guide weights (guiding = 2)
+  std::shared_ptr<rad::guide::GuideState> guide;
   hipStream_t stream = nullptr;
-  uint32_t* h_cnt = nullptr;  // pinned readback: counters + stats
+  uint32_t* h_cnt = nullptr;  // pinned readback: counters + stats + guide words
   std::thread th;
   std::atomic<bool> cancel{false};
   std::atomic<bool> complete{false};
@@ -28,9 +32,14 @@ struct rad_job {
   uint32_t iterations = 0;
   uint32_t passes = 0;
   uint64_t segments = 0, shadow_rays = 0;
+  uint64_t guide_records = 0;
+  float guide_loss = 0.f, guide_alpha = 0.f;
   uint64_t stats[rad::ST_WORDS] = {};
   double kernel_ms[RAD_KERNEL_COUNT] = {};
   uint32_t kernel_launches[RAD_KERNEL_COUNT] = {};
+  std::vector<uint32_t> pass_rows;
+  std::vector<uint8_t> pass_train;
+  uint32_t* batch = nullptr;  // device minibatch indices (inside the arena)
   bool joined = false;
 };
 
@@ -40,10 +49,17 @@ namespace {
 
 constexpr uint64_t kDefaultSlots = 4u << 20;     // paths in flight
 constexpr uint64_t kResultBudget = 2ull << 30;   // bytes of result planes per pass
+constexpr uint32_t kRecordCap = 2u << 20;        // guide training records per pass
+constexpr uint32_t kRecordSlots = 512u << 10;    // slots that record training data
+constexpr uint32_t kTrainBatch = 8192;           // records per training step
+constexpr uint32_t kTrainSteps = 96;             // training steps after each training pass
+constexpr float kTrainLr = 4e-3f;
+constexpr float kGuideAlpha = 0.5f;
 size_t Align(size_t v) { return (v + 255) & ~size_t(255); }
 
 struct JobShape {
   uint32_t npix, rows, Q;
+  bool guided;
   uint64_t bytes;
 };
 
@@ -60,28 +76,106 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   uint64_t Q = desc.path_slots > 0 ? desc.path_slots : kDefaultSlots;
   Q = std::max<uint64_t>(1, std::min(Q, pass));
   j.Q = uint32_t(Q);
+  j.guided = desc.guiding != 0;
   j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * 6 + Align(pass * 4) * 3 + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
+  if (j.guided)
+    j.bytes += Align(Q * 4 * guide::kFeat) + Align(Q * 4) * 3 + Align(Q * 2 * guide::kOut) +
+               Align(uint64_t(kRecordCap) * sizeof(guide::TrainRecord)) + Align(kTrainBatch * 4) + 256 +
+               Align(guide::kWeights * 2) + Align(guide::kBiases * 4);
   return j;
 }
 
+// Sample passes: without training, as many rows as the result planes hold;
+// with online training, passes of 1, 1, 2, 4, ... rows (each followed by
+// training on the records it produced) until a quarter of the samples are
+// spent, then the rest.
+void PlanPasses(rad_job* job, uint32_t maxRows) {
+  const uint32_t spp = job->desc.spp;
+  uint32_t done = 0;
+  if (job->desc.guiding == 1) {
+    uint32_t r = 1;
+    bool first = true;
+    while (done + r <= std::max(1u, spp / 4) && r <= maxRows) {
+      job->pass_rows.push_back(r);
+      job->pass_train.push_back(1);
+      done += r;
+      if (!first) r *= 2;
+      first = false;
+    }
+  }
+  while (done < spp) {
+    uint32_t r = std::min(maxRows, spp - done);
+    job->pass_rows.push_back(r);
+    job->pass_train.push_back(0);
+    done += r;
+  }
+}
+
+void Train(rad_job* job, hipStream_t st) {
+  Wavefront& w = job->w;
+  const uint32_t* g = job->h_cnt + CNT_WORDS + 2 * ST_WORDS;
+  uint32_t nrec = std::min(g[G_RECORDS], w.rec_cap);
+  float wsum;
+  std::memcpy(&wsum, &g[G_WSUM], 4);
+  job->guide_records += nrec;
+  if (nrec >= 256 && wsum > 0.f) {
+    guide::GuideState& gs = *job->guide;
+    const float wnorm = float(nrec) / wsum;
+    const guide::GuideNet net = w.net;
+    for (uint32_t k = 0; k < kTrainSteps; k++) {
+      HipCheck(hipMemsetAsync(gs.scal.ptr, 0, 4, st), "clear loss");
+      guide::LaunchBatch(job->batch, kTrainBatch, nrec, gs.steps * 7919u + k + 1u, st);
+      guide::LaunchTrain(net, w.recs, job->batch, kTrainBatch, wnorm, gs.grad.as<float>(), gs.scal.as<float>(), st);
+      gs.Step(kTrainLr, 1.f / float(kTrainBatch), st);
+    }
+    HipCheck(hipGetLastError(), "guide training launch");
+    float loss = 0.f;
+    HipCheck(hipMemcpyAsync(&job->h_cnt[CNT_WORDS + 2 * ST_WORDS + G_WORDS], gs.scal.ptr, 4, hipMemcpyDeviceToHost, st),
+             "loss readback");
+    HipCheck(hipStreamSynchronize(st), "guide training");
+    std::memcpy(&loss, &job->h_cnt[CNT_WORDS + 2 * ST_WORDS + G_WORDS], 4);
+    job->guide_loss = loss / float(kTrainBatch);
+  }
+  HipCheck(hipMemsetAsync(w.gcnt, 0, G_WORDS * 4, st), "clear guide counters");
+}
+
 void RunJob(rad_job* job) {
+  std::unique_lock<std::mutex> trainLock;
   try {
     HipCheck(hipSetDevice(job->scene->ctx->device), "hipSetDevice");
+    if (job->desc.guiding == 1) trainLock = std::unique_lock<std::mutex>(job->scene->guide_mu);
     const DevScene scene = job->scene->dev;
     Wavefront& w = job->w;
     hipStream_t st = job->stream;
     LaunchCfg cfg{std::max(64, job->scene->ctx->cu_count * 8), 256};
+    const int inferBlocks = std::max(64, job->scene->ctx->cu_count * 3);
     const bool prof = (job->desc.flags & RAD_RENDER_PROFILE) != 0;
-    hipEvent_t ev[RAD_KERNEL_COUNT + 1] = {};
+    const size_t readback = CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t) + G_WORDS * 4;
+    hipEvent_t ev[6] = {};
     if (prof)
       for (auto& e : ev) HipCheck(hipEventCreate(&e), "hipEventCreate");
     auto t0 = std::chrono::steady_clock::now();
-    const uint32_t rows = w.rows;
-    for (uint32_t row0 = 0; row0 < w.spp && job->status == RAD_OK; row0 += rows) {
+    const uint32_t baseFlags = w.flags & WF_QUIRKS;
+    const float* lo = job->scene->host.world_lo;
+    const float* hi = job->scene->host.world_hi;
+    uint32_t row0 = 0;
+    for (size_t pi = 0; pi < job->pass_rows.size() && job->status == RAD_OK; pi++) {
       w.row0 = row0;
-      w.rows = std::min(rows, w.spp - row0);
-      LaunchInitPass(w, row0 == 0, st);
+      w.rows = job->pass_rows[pi];
+      row0 += w.rows;
+      w.flags = baseFlags;
+      if (job->desc.guiding == 2) {
+        w.flags |= WF_GUIDE;
+        w.net.alpha = kGuideAlpha;
+      } else if (job->desc.guiding == 1) {
+        const bool trained = job->guide->steps > 0;
+        w.net = job->guide->Net(lo, hi, trained ? kGuideAlpha : 0.f);
+        if (trained) w.flags |= WF_GUIDE;
+        if (job->pass_train[pi]) w.flags |= WF_RECORD;
+      }
+      job->guide_alpha = (w.flags & WF_GUIDE) ? w.net.alpha : 0.f;
+      LaunchInitPass(w, pi == 0, st);
       int in = 0;
       for (;;) {
         if (prof) HipCheck(hipEventRecord(ev[0], st), "event");
@@ -90,29 +184,40 @@ void RunJob(rad_job* job) {
         if (prof) HipCheck(hipEventRecord(ev[1], st), "event");
         LaunchExtend(w, scene, in, cfg, st);
         if (prof) HipCheck(hipEventRecord(ev[2], st), "event");
+        if (w.flags & WF_GUIDE) {
+          LaunchGuidePrep(w, scene, in, cfg, st);
+          guide::LaunchInfer(w.net, w.gfeat, &w.gcnt[G_QUEUE], 0, w.gq, w.graw, inferBlocks, st);
+        }
+        if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
         const bool vol = w.integrator == RAD_INTEGRATOR_VOL_PATH;
         if (vol)
           LaunchShadeVol(w, scene, in, cfg, st);
         else
           LaunchShadePath(w, scene, in, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
+        if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
         if (vol)
           LaunchVolShadow(w, scene, cfg, st);
         else
           LaunchShadow(w, scene, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
+        if (prof) HipCheck(hipEventRecord(ev[5], st), "event");
         HipCheck(hipGetLastError(), "wavefront launch");
         HipCheck(hipMemcpyAsync(job->h_cnt, w.cnt, CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t),
                                 hipMemcpyDeviceToHost, st),
                  "counter readback");
+        if (w.gcnt)
+          HipCheck(hipMemcpyAsync(job->h_cnt + CNT_WORDS + 2 * ST_WORDS, w.gcnt, G_WORDS * 4, hipMemcpyDeviceToHost, st),
+                   "guide counter readback");
         HipCheck(hipStreamSynchronize(st), "wavefront iteration");
         job->iterations++;
         if (prof) {
-          for (int k = 0; k < 4; k++) {
+          // event pairs -> RAD_KERNEL_REGEN, EXTEND, GUIDE, SHADE, SHADOW
+          const int slot[5] = {RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_GUIDE, RAD_KERNEL_SHADE, RAD_KERNEL_SHADOW};
+          for (int k = 0; k < 5; k++) {
+            if (slot[k] == RAD_KERNEL_GUIDE && !(w.flags & WF_GUIDE)) continue;
             float ms = 0.f;
             HipCheck(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "event time");
-            job->kernel_ms[k] += ms;
-            job->kernel_launches[k]++;
+            job->kernel_ms[slot[k]] += ms;
+            job->kernel_launches[slot[k]]++;
           }
         }
         const uint64_t* hs = reinterpret_cast<const uint64_t*>(job->h_cnt + CNT_WORDS);
@@ -132,8 +237,16 @@ void RunJob(rad_job* job) {
       if (job->status != RAD_OK) break;
       LaunchAccum(w, st);
       job->passes++;
+      if (w.flags & WF_RECORD) {
+        auto tt = std::chrono::steady_clock::now();
+        Train(job, st);
+        if (prof) {
+          job->kernel_ms[RAD_KERNEL_TRAIN] +=
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
+          job->kernel_launches[RAD_KERNEL_TRAIN]++;
+        }
+      }
     }
-    w.rows = rows;
     LaunchFinal(w, job->d_fb.as<float>(), job->d_sq.as<float>(), st);
     HipCheck(hipGetLastError(), "resolve launch");
     HipCheck(hipStreamSynchronize(st), "resolve");
@@ -141,6 +254,7 @@ void RunJob(rad_job* job) {
     HipCheck(hipMemcpy(job->h_cnt, w.cnt, CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t),
                        hipMemcpyDeviceToHost),
              "stats readback");
+    (void)readback;
     const uint64_t* hs = reinterpret_cast<const uint64_t*>(job->h_cnt + CNT_WORDS);
     for (int i = 0; i < ST_WORDS; i++) job->stats[i] = hs[i];
     if (prof)
@@ -167,6 +281,9 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc) {
   if (desc.spp == 0) throw StatusError(RAD_ERR_INVALID_ARG, "spp must be positive");
   if (desc.integrator != RAD_INTEGRATOR_PATH && desc.integrator != RAD_INTEGRATOR_VOL_PATH)
     throw StatusError(RAD_ERR_INVALID_ARG, "unknown integrator");
+  if (desc.guiding < 0 || desc.guiding > 2) throw StatusError(RAD_ERR_INVALID_ARG, "guiding must be 0, 1 or 2");
+  if (desc.guiding && desc.integrator != RAD_INTEGRATOR_PATH)
+    throw StatusError(RAD_ERR_UNSUPPORTED, "learned guiding is available for the path integrator");
   HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
   // the reference dereferences the BSDF of every hit surface in `path` (path.cpp:172)
   if (desc.integrator == RAD_INTEGRATOR_PATH) {
@@ -183,6 +300,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc) {
   const JobShape js = Shape(scene, desc);
   const uint32_t npix = js.npix, Q = js.Q;
   job->total = uint64_t(npix) * desc.spp;
+  PlanPasses(job.get(), js.rows);
   // one arena for every per-slot, per-sample and per-pixel array
   job->arena.alloc(js.bytes);
   char* p = static_cast<char*>(job->arena.ptr);
@@ -214,6 +332,33 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc) {
   w.pacc2 = (float4*)take(size_t(npix) * 16);
   w.cnt = (uint32_t*)take(CNT_WORDS * 4 + ST_WORDS * 8);
   w.stats = reinterpret_cast<unsigned long long*>(w.cnt + CNT_WORDS);
+  if (js.guided) {
+    w.gfeat = (float*)take(u4 * guide::kFeat);
+    w.gq = (uint32_t*)take(u4);
+    w.gpend = (uint32_t*)take(u4);
+    w.glast = (uint32_t*)take(u4);
+    w.graw = (_Float16*)take(size_t(Q) * 2 * guide::kOut);
+    w.recs = (guide::TrainRecord*)take(size_t(kRecordCap) * sizeof(guide::TrainRecord));
+    job->batch = (uint32_t*)take(kTrainBatch * 4);
+    w.gcnt = (uint32_t*)take(256);
+    w.rec_cap = kRecordCap;
+    w.rec_stride = std::max(1u, Q / kRecordSlots);
+    _Float16* wh = (_Float16*)take(guide::kWeights * 2);
+    float* bias = (float*)take(guide::kBiases * 4);
+    {
+      std::lock_guard<std::mutex> lk(scene->mu);
+      if (!scene->guide) scene->guide = std::make_shared<guide::GuideState>();
+      job->guide = scene->guide;
+    }
+    if (desc.guiding == 2) {  // snapshot: concurrent training of the scene's guide cannot touch this render
+      HipCheck(hipMemcpy(wh, job->guide->wh.ptr, guide::kWeights * 2, hipMemcpyDeviceToDevice), "guide snapshot");
+      HipCheck(hipMemcpy(bias, job->guide->bias.ptr, guide::kBiases * 4, hipMemcpyDeviceToDevice), "guide snapshot");
+      w.net = job->guide->Net(scene->host.world_lo, scene->host.world_hi, kGuideAlpha);
+      w.net.w = wh;
+      w.net.b = bias;
+    }
+    HipCheck(hipMemset(w.gcnt, 0, 256), "guide counters");
+  }
   job->d_sq.alloc(size_t(npix) * 3 * sizeof(float));
   job->d_fb.alloc(size_t(npix) * 3 * sizeof(float));
   w.npix = npix;
@@ -227,7 +372,8 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc) {
   w.rr_depth = desc.rr_depth;
   w.flags = (desc.flags & RAD_RENDER_FIX_QUIRKS) ? 0u : WF_QUIRKS;
   w.integrator = desc.integrator;
-  HipCheck(hipHostMalloc((void**)&job->h_cnt, CNT_WORDS * 4 + ST_WORDS * 8, hipHostMallocDefault), "hipHostMalloc");
+  HipCheck(hipHostMalloc((void**)&job->h_cnt, CNT_WORDS * 4 + ST_WORDS * 8 + 2 * G_WORDS * 4, hipHostMallocDefault),
+           "hipHostMalloc");
   HipCheck(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking), "hipStreamCreate");
   rad_job* raw = job.release();
   raw->th = std::thread(RunJob, raw);
@@ -266,6 +412,11 @@ void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats) {
       stats->kernel_ms[k] = job->kernel_ms[k];
       stats->kernel_launches[k] = job->kernel_launches[k];
     }
+    stats->passes = job->passes;
+    stats->guide_steps = job->guide ? job->guide->steps : 0;
+    stats->guide_records = job->guide_records;
+    stats->guide_loss = job->guide_loss;
+    stats->guide_alpha = job->guide_alpha;
   }
   if (job->status == RAD_ERR_CANCELLED) throw StatusError(RAD_ERR_CANCELLED, "render cancelled");
 }

@@ -66,6 +66,12 @@ void HipCheck(hipError_t e, const char* what);
 
 }  // namespace rad
 
+namespace rad {
+namespace guide {
+struct GuideState;
+}
+}  // namespace rad
+
 struct rad_scene {
   rad_ctx* ctx = nullptr;
   rad::HostScene host;
@@ -77,5 +83,7 @@ struct rad_scene {
   double bvh_ms = 0;
   uint32_t bvh_depth = 0;
   float sah_cost = 0;
+  std::shared_ptr<rad::guide::GuideState> guide;  // created on first guided render
+  std::mutex guide_mu;  // held by a render that trains the guide
   uint64_t device_bytes() const;
 };

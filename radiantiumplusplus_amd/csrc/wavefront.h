@@ -18,6 +18,7 @@
 
 #include <cstdint>
 
+#include "guide.h"
 #include "scene_dev.h"
 
 namespace rad {
@@ -35,7 +36,8 @@ enum : uint32_t {
   CNT_WORDS = 8
 };
 enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_WORDS = 4 };
-enum : uint32_t { WF_QUIRKS = 1u };
+enum : uint32_t { WF_QUIRKS = 1u, WF_GUIDE = 2u, WF_RECORD = 4u };
+enum : uint32_t { G_QUEUE = 0, G_RECORDS = 1, G_WSUM = 2, G_WORDS = 4 };
 enum : uint32_t { MED_UNSET = 0xffffffffu };
 
 struct Wavefront {
@@ -59,6 +61,16 @@ struct Wavefront {
   float4* pacc2;   // per-pixel squared-radiance sum
   uint32_t* cnt;   // CNT_* words
   unsigned long long* stats;  // ST_* words
+  // learned guiding (null when off)
+  float* gfeat;      // per guide-queue entry: position, geometric normal, outgoing direction
+  uint32_t* gq;      // guide queue: slots whose vertex is guided this iteration
+  _Float16* graw;    // per slot: 48 raw network outputs
+  uint32_t* gpend;   // per slot: training record still waiting for its radiance baseline
+  uint32_t* glast;   // per slot: newest training record of the path in flight
+  guide::TrainRecord* recs;
+  uint32_t* gcnt;    // G_* words (G_WSUM holds float bits)
+  uint32_t rec_cap, rec_stride;
+  guide::GuideNet net;
   uint32_t npix, Q, spp;
   uint32_t row0, rows;   // samples [row0, row0 + rows) of every pixel form the current pass
   uint32_t seed, request;
@@ -98,6 +110,7 @@ void LaunchReset(const Wavefront& w, int out, hipStream_t s);
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
+void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchAccum(const Wavefront& w, hipStream_t s);
 // vol_path (defined in volpath_kernels.hip).
 void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);

@@ -66,6 +66,12 @@ def lib():
                 "rad_loaded_scene_device": (C.c_char_p, [vp]),
                 "rad_loaded_scene_free": (None, [vp]),
                 "rad_write_exr": (i32, [C.c_char_p, vp, i32, i32]),
+                "rad_hip_guide_forward": (i32, [vp, vp, vp, u32, vp]),
+                "rad_hip_guide_grad": (i32, [vp, vp, vp, u32, C.c_float, vp, C.POINTER(C.c_float)]),
+                "rad_hip_guide_pdf": (i32, [vp, vp, u32, vp]),
+                "rad_hip_guide_sample": (i32, [vp, vp, u32, vp]),
+                "rad_hip_scene_guide_get": (i32, [vp, vp, C.POINTER(u32)]),
+                "rad_hip_scene_guide_set": (i32, [vp, vp]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -138,6 +144,21 @@ class Scene:
         out = np.zeros(len(rays), np.uint8)
         check(lib().rad_hip_intersect_any(self.h, rays.ctypes.data, len(rays), out.ctypes.data))
         return out
+
+    def guide_params(self):
+        """The scene's guide weights (flat FP32) and training steps taken."""
+        p = np.zeros(F.RAD_GUIDE_PARAMS, np.float32)
+        steps = C.c_uint32()
+        check(lib().rad_hip_scene_guide_get(self.h, p.ctypes.data, C.byref(steps)))
+        return p, steps.value
+
+    def set_guide_params(self, params=None):
+        """Replace the guide weights (None = re-initialise)."""
+        if params is None:
+            check(lib().rad_hip_scene_guide_set(self.h, None))
+        else:
+            p = np.ascontiguousarray(params, np.float32)
+            check(lib().rad_hip_scene_guide_set(self.h, p.ctypes.data))
 
     def start(self, render_desc: F.RenderDesc):
         return Job(self, render_desc)

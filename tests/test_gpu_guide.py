@@ -1,0 +1,172 @@
+"""Learned path guiding on the GPU: the FP16-MFMA network against the float64
+oracle (forward outputs and loss gradients, FP16 tolerance 1e-2), the vMF
+mixture probes, unbiasedness of guided renders against the unguided tracer,
+and the gain of the trained network over sampling without it."""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import stats
+from radiantiumplusplus_amd import _ffi as F
+from radiantiumplusplus_amd import hip, scenes
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import guide_ref as G  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = np.array([-1.0, -1.0, -1.0], np.float32), np.array([1.0, 1.0, 1.0], np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def random_params(seed):
+    p = G.init_params(seed)
+    rng = np.random.default_rng(seed)
+    for _, bo in G.OFFS:
+        p[bo:bo + 48] += rng.normal(0, 0.1, 48).astype(np.float32)
+    wo, _ = G.OFFS[-1]
+    p[wo:wo + 64 * 48] *= 8.0  # informative mixtures, not the near-uniform initial one
+    return p
+
+
+def random_feats(rng, n):
+    f = np.concatenate([rng.uniform(-1, 1, (n, 3)), rng.normal(size=(n, 3)), rng.normal(size=(n, 3))], 1)
+    f[:, 3:6] /= np.linalg.norm(f[:, 3:6], axis=1, keepdims=True)
+    f[:, 6:9] /= np.linalg.norm(f[:, 6:9], axis=1, keepdims=True)
+    return np.ascontiguousarray(f, np.float32)
+
+
+def gpu_forward(p, feats):
+    lib = hip.lib()
+    raw = np.zeros((len(feats), 48), np.float32)
+    bounds = np.concatenate([LO, HI]).astype(np.float32)
+    hip.check(lib.rad_hip_guide_forward(_fp(p), _fp(bounds), _fp(feats), len(feats), _fp(raw)))
+    return raw
+
+
+def test_forward_matches_oracle(hip_ctx):
+    rng = np.random.default_rng(0)
+    p = random_params(1)
+    feats = random_feats(rng, 1000)
+    raw = gpu_forward(p, feats)
+    _, Z = G.forward(p, feats, LO, HI)
+    ref = Z[-1]
+    err = np.linalg.norm(raw - ref) / np.linalg.norm(ref)
+    print(f"forward rel err {err:.2e}")
+    assert err < 1e-2
+    # every vertex's row, not just the aggregate
+    row = np.linalg.norm(raw - ref, axis=1) / (np.linalg.norm(ref, axis=1) + 1e-3)
+    assert np.quantile(row, 0.99) < 3e-2
+
+
+def test_gradient_matches_oracle(hip_ctx):
+    rng = np.random.default_rng(2)
+    p = random_params(3)
+    n = 1024
+    feats = random_feats(rng, n)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    w = rng.exponential(1.0, n)
+    w[rng.random(n) < 0.2] = 0.0
+    wnorm = 1.0 / w.mean()
+    recs = np.zeros(n, np.dtype([("p", "<f4", 3), ("n", "<f4", 3), ("wi", "<f4", 3), ("dir", "<f4", 3),
+                                 ("thr", "<f4", 3), ("lbase", "<f4", 3), ("w", "<f4"), ("prev", "<u4")]))
+    recs["p"], recs["n"], recs["wi"] = feats[:, 0:3], feats[:, 3:6], feats[:, 6:9]
+    recs["dir"], recs["w"] = dirs, w
+    grad = np.zeros(F.RAD_GUIDE_PARAMS, np.float32)
+    loss = C.c_float()
+    bounds = np.concatenate([LO, HI]).astype(np.float32)
+    hip.check(hip.lib().rad_hip_guide_grad(_fp(p), _fp(bounds), recs.ctypes.data_as(C.c_void_p), n, C.c_float(wnorm),
+                                           _fp(grad), C.byref(loss)))
+    ref_loss, ref = G.loss_and_grad(p, feats, dirs, w.astype(np.float32), wnorm, LO, HI)
+    print(f"loss gpu {loss.value:.6f} oracle {ref_loss:.6f}")
+    assert abs(loss.value - ref_loss) <= 1e-2 * abs(ref_loss)
+    for l, ((wo, bo), (nin, nout)) in enumerate(zip(G.OFFS, G.DIMS)):
+        for lo_, hi_ in ((wo, bo), (bo, bo + nout)):
+            e = np.linalg.norm(grad[lo_:hi_] - ref[lo_:hi_]) / (np.linalg.norm(ref[lo_:hi_]) + 1e-12)
+            print(f"layer {l} [{lo_}:{hi_}] rel err {e:.2e}")
+            assert e < 1e-2
+
+
+def test_mixture_probes(hip_ctx):
+    rng = np.random.default_rng(4)
+    raw = rng.normal(size=48).astype(np.float32)
+    raw[4:40:5] = rng.uniform(-1, 3, 8)
+    dirs = rng.normal(size=(4000, 3))
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    pdf = np.zeros(4000, np.float32)
+    hip.check(hip.lib().rad_hip_guide_pdf(_fp(raw), _fp(dirs), 4000, _fp(pdf)))
+    assert np.allclose(pdf, G.pdf(raw, dirs), rtol=1e-4, atol=1e-7)
+    n = 200000
+    xi = rng.random((n, 3)).astype(np.float32)
+    out = np.zeros((n, 4), np.float32)
+    hip.check(hip.lib().rad_hip_guide_sample(_fp(raw), _fp(xi), n, _fp(out)))
+    d = out[:, :3].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)
+    assert np.allclose(out[:, 3], G.pdf(raw, d), rtol=1e-3, atol=1e-6)
+    # the samples follow the density: E[d] and E[d_y^2] against quadrature of the pdf
+    t = (np.arange(400) + 0.5) * np.pi / 400
+    f = (np.arange(800) + 0.5) * 2 * np.pi / 800
+    T, P = np.meshgrid(t, f, indexing="ij")
+    q = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
+    dw = (np.sin(T) * (np.pi / 400) * (2 * np.pi / 800)).reshape(-1)
+    pq = G.pdf(raw, q) * dw
+    for k, fn in enumerate([lambda v: v[:, 0], lambda v: v[:, 1], lambda v: v[:, 2], lambda v: v[:, 1] ** 2]):
+        expect = (fn(q) * pq).sum()
+        vals = fn(d)
+        se = vals.std() / np.sqrt(n)
+        assert abs(vals.mean() - expect) < 5 * se + 1e-4, (k, vals.mean(), expect, se)
+
+
+def small_room(width=48, height=27):
+    b = scenes.guiding_room(width=width, height=height, spp=256)
+    return b
+
+
+def render(ctx, scene, b, spp, seed, guiding=0, flags=0):
+    rd = b.render_desc(spp=spp, seed=seed, flags=flags | F.RAD_RENDER_WANT_SUMSQ)
+    rd.guiding = guiding
+    return scene.render(rd, want_sumsq=True)
+
+
+def test_guided_render_is_unbiased(hip_ctx):
+    """Guided (online-trained) and unguided renders of the door-gap room
+    converge to the same image: per-tile z-test at 4 sigma."""
+    b = small_room()
+    s = hip.Scene.from_builder(hip_ctx, b)
+    fb_p, sq_p, _ = render(hip_ctx, s, b, 4096, seed=1, guiding=0)
+    fb_g, sq_g, st = render(hip_ctx, s, b, 4096, seed=2, guiding=1)
+    print(f"guide steps {st.guide_steps} records {st.guide_records} loss {st.guide_loss:.4f} alpha {st.guide_alpha}")
+    assert st.guide_steps > 0 and st.guide_alpha > 0
+    assert np.isfinite(fb_g).all()
+    z, rel = stats.compare(fb_g, sq_g, 4096, fb_p, sq_p, 4096, tile=8)
+    print(f"max |z| {np.abs(z).max():.2f}  relMSE {rel:.2e}")
+    assert np.abs(z).max() < 4.0
+    assert rel < 2e-3
+
+
+def test_trained_guide_reduces_error(hip_ctx):
+    """At equal spp the trained guide's error against a converged unguided
+    reference is below that of BSDF sampling alone (quirks off, so the GGX
+    boxes are guided too)."""
+    b = small_room(64, 36)
+    fl = F.RAD_RENDER_FIX_QUIRKS
+    s = hip.Scene.from_builder(hip_ctx, b)
+    ref, _, _ = render(hip_ctx, s, b, 16384, seed=10, flags=fl)
+    # train once (guiding = 1), then compare fixed-weight guiding (2) with plain sampling at 64 spp
+    render(hip_ctx, s, b, 256, seed=11, guiding=1, flags=fl)
+    errs_p, errs_g = [], []
+    for k in range(4):
+        fp, _, _ = render(hip_ctx, s, b, 64, seed=100 + k, guiding=0, flags=fl)
+        fg, _, _ = render(hip_ctx, s, b, 64, seed=200 + k, guiding=2, flags=fl)
+        errs_p.append(np.mean((fp - ref) ** 2 / (ref ** 2 + 1e-2)))
+        errs_g.append(np.mean((fg - ref) ** 2 / (ref ** 2 + 1e-2)))
+    mp, mg = np.mean(errs_p), np.mean(errs_g)
+    print(f"relMSE plain {mp:.4e} guided {mg:.4e} ratio {mg / mp:.3f}")
+    assert mg < mp
