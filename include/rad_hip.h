@@ -224,7 +224,7 @@ typedef struct rad_render_desc {
   uint32_t path_slots;     /* paths in flight (wavefront slots), 0 = chosen by the library */
   int32_t guiding;         /* learned path guiding: 0 off, 1 train online and guide, 2 guide with the scene's current weights */
   uint32_t request_id;     /* RNG stream key for serving (0 for single renders) */
-  uint32_t reserved;
+  int32_t width, height;   /* image size override (0 = the scene camera's); fov and pose are the scene's */
 } rad_render_desc;
 
 typedef struct rad_job_stats {
@@ -332,6 +332,42 @@ rad_status rad_hip_bsdf_eval(const rad_bsdf_desc* bsdf, uint32_t quirks, const f
 rad_status rad_hip_bsdf_sample(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out);
 /* Heterogeneous grid lookup (volume/grid.cpp:41-75): in = n x {p[3]} in [0,1]^3, out = n x 3. */
 rad_status rad_hip_grid_eval(const rad_medium_desc* medium, const float* in, uint32_t n, float* out);
+
+
+/* ---- render serving ---------------------------------------------------------------
+ * No counterpart in the reference (its CLI renders one scene per process,
+ * rad.offline.cli/main.cpp:8-71).  A server admits requests (a resident scene +
+ * a render description) in arrival order while their HBM footprint fits the
+ * budget, runs admitted requests concurrently on their own streams, reuses the
+ * arenas of finished requests, and reports p50 / p99 latency and throughput.
+ * A request's image depends only on its scene and description. */
+typedef struct rad_server rad_server;
+enum { RAD_REQ_QUEUED = 0, RAD_REQ_RUNNING = 1, RAD_REQ_DONE = 2, RAD_REQ_FAILED = 3 };
+typedef struct rad_server_opts {
+  uint64_t hbm_budget;     /* bytes of job arenas in flight or pooled (0 = 32 GiB) */
+  uint32_t max_inflight;   /* concurrent requests (0 = 8) */
+  uint32_t reserved;
+} rad_server_opts;
+typedef struct rad_request_times {
+  double queue_ms, run_ms, latency_ms; /* submit -> admit -> done */
+} rad_request_times;
+typedef struct rad_server_stats {
+  uint64_t submitted, completed, failed, queued, running;
+  uint64_t samples;        /* camera samples of completed requests */
+  double wall_ms;          /* first submit to last completion */
+  double samples_per_s;
+  double p50_ms, p99_ms, mean_ms; /* request latency (submit -> done) */
+  uint64_t held_bytes, peak_held_bytes, budget_bytes;
+} rad_server_stats;
+rad_status rad_hip_server_create(rad_ctx* ctx, const rad_server_opts* opts, rad_server** out);
+/* Queue a render; request_id 0 in desc is replaced by the server's id (the RNG stream key). */
+rad_status rad_hip_server_submit(rad_server* server, rad_scene* scene, const rad_render_desc* desc, uint64_t* request);
+rad_status rad_hip_server_poll(rad_server* server, uint64_t request, int32_t* state);
+/* Block until the request finishes; copy its image (W x H x 3) and release it. */
+rad_status rad_hip_server_wait(rad_server* server, uint64_t request, float* fb_rgb, float* fb_sumsq,
+                               rad_job_stats* stats, rad_request_times* times);
+rad_status rad_hip_server_stats(rad_server* server, rad_server_stats* out);
+void rad_hip_server_destroy(rad_server* server);
 
 /* ---- learned path guiding ------------------------------------------------------
  * No counterpart in the reference (its path integrator samples the BSDF only,

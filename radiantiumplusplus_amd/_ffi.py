@@ -160,7 +160,8 @@ class RenderDesc(C.Structure):
         ("path_slots", C.c_uint32),
         ("guiding", C.c_int32),
         ("request_id", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
     ]
 
 
@@ -224,6 +225,36 @@ class SceneInfo(C.Structure):
 RAY_DTYPE_FIELDS = ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")
 
 
+RAD_REQ_QUEUED, RAD_REQ_RUNNING, RAD_REQ_DONE, RAD_REQ_FAILED = 0, 1, 2, 3
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RequestTimes(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("queued", C.c_uint64),
+        ("running", C.c_uint64),
+        ("samples", C.c_uint64),
+        ("wall_ms", C.c_double),
+        ("samples_per_s", C.c_double),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("mean_ms", C.c_double),
+        ("held_bytes", C.c_uint64),
+        ("peak_held_bytes", C.c_uint64),
+        ("budget_bytes", C.c_uint64),
+    ]
+
+
 ABI_STRUCTS = {
     "rad_shape_desc": ShapeDesc,
     "rad_bsdf_desc": BsdfDesc,
@@ -238,4 +269,7 @@ ABI_STRUCTS = {
     "rad_scene_info": SceneInfo,
     "rad_job_stats": JobStats,
     "rad_guide_record": GuideRecord,
+    "rad_server_opts": ServerOpts,
+    "rad_request_times": RequestTimes,
+    "rad_server_stats": ServerStats,
 }

@@ -44,6 +44,7 @@ SOURCES = [
     "path_kernels.hip",
     "volpath_kernels.hip",
     "guide_kernels.hip",
+    "server.cpp",
     "host/json.cpp",
     "host/loader.cpp",
     "host/exr.cpp",

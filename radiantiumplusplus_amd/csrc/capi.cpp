@@ -107,6 +107,9 @@ size_t rad_abi_sizeof(const char* name) {
   RAD_SZ(rad_render_desc)
   RAD_SZ(rad_job_stats)
   RAD_SZ(rad_guide_record)
+  RAD_SZ(rad_server_opts)
+  RAD_SZ(rad_request_times)
+  RAD_SZ(rad_server_stats)
   RAD_SZ(rad_bvh_opts)
   RAD_SZ(rad_scene_info)
 #undef RAD_SZ

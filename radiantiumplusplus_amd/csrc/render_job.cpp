@@ -15,7 +15,8 @@ struct rad_job {
   rad_scene* scene = nullptr;
   rad_render_desc desc{};
   rad::Wavefront w{};
-  rad::DeviceBuffer arena;
+  rad::DevScene dev{};  // the scene's device view with this job's camera
+  std::unique_ptr<rad::DeviceBuffer> arena;
   rad::DeviceBuffer d_fb, d_sq;
   rad::DeviceBuffer d_gnet;  // job-private copy of the guide weights (guiding = 2)
   std::shared_ptr<rad::guide::GuideState> guide;
@@ -63,9 +64,18 @@ struct JobShape {
   uint64_t bytes;
 };
 
+void ImageSize(const rad_scene* scene, const rad_render_desc& desc, int32_t& W, int32_t& H) {
+  if (desc.width < 0 || desc.height < 0 || (desc.width > 0) != (desc.height > 0))
+    throw StatusError(RAD_ERR_INVALID_ARG, "image size override needs both width and height");
+  W = desc.width > 0 ? desc.width : scene->host.cam.width;
+  H = desc.height > 0 ? desc.height : scene->host.cam.height;
+}
+
 JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   JobShape j{};
-  j.npix = uint32_t(scene->host.cam.width) * uint32_t(scene->host.cam.height);
+  int32_t W, H;
+  ImageSize(scene, desc, W, H);
+  j.npix = uint32_t(W) * uint32_t(H);
   const uint64_t spp = std::max(1u, desc.spp);
   uint64_t rows = kResultBudget / (uint64_t(j.npix) * 12);
   rows = std::min<uint64_t>(std::max<uint64_t>(rows, 1), spp);
@@ -145,7 +155,7 @@ void RunJob(rad_job* job) {
   try {
     HipCheck(hipSetDevice(job->scene->ctx->device), "hipSetDevice");
     if (job->desc.guiding == 1) trainLock = std::unique_lock<std::mutex>(job->scene->guide_mu);
-    const DevScene scene = job->scene->dev;
+    const DevScene scene = job->dev;
     Wavefront& w = job->w;
     hipStream_t st = job->stream;
     LaunchCfg cfg{std::max(64, job->scene->ctx->cu_count * 8), 256};
@@ -277,7 +287,7 @@ uint64_t JobFootprint(const rad_scene* scene, const rad_render_desc& desc, uint3
   return j.bytes;
 }
 
-rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc) {
+rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr<DeviceBuffer> reuse) {
   if (desc.spp == 0) throw StatusError(RAD_ERR_INVALID_ARG, "spp must be positive");
   if (desc.integrator != RAD_INTEGRATOR_PATH && desc.integrator != RAD_INTEGRATOR_VOL_PATH)
     throw StatusError(RAD_ERR_INVALID_ARG, "unknown integrator");
@@ -297,13 +307,29 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc) {
   auto job = std::make_unique<rad_job>();
   job->scene = scene;
   job->desc = desc;
+  job->dev = scene->dev;
+  {
+    int32_t W, H;
+    ImageSize(scene, desc, W, H);
+    if (W != scene->host.cam.width || H != scene->host.cam.height) {
+      rad_camera_desc c = scene->host.camera_desc;
+      c.width = W;
+      c.height = H;
+      job->dev.cam = MakeCameraRec(c);
+    }
+  }
   const JobShape js = Shape(scene, desc);
   const uint32_t npix = js.npix, Q = js.Q;
   job->total = uint64_t(npix) * desc.spp;
   PlanPasses(job.get(), js.rows);
   // one arena for every per-slot, per-sample and per-pixel array
-  job->arena.alloc(js.bytes);
-  char* p = static_cast<char*>(job->arena.ptr);
+  if (reuse && reuse->bytes >= js.bytes) {
+    job->arena = std::move(reuse);
+  } else {
+    job->arena = std::make_unique<DeviceBuffer>();
+    job->arena->alloc(js.bytes);
+  }
+  char* p = static_cast<char*>(job->arena->ptr);
   auto take = [&](size_t n) {
     char* r = p;
     p += Align(n);
@@ -420,6 +446,17 @@ void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats) {
   }
   if (job->status == RAD_ERR_CANCELLED) throw StatusError(RAD_ERR_CANCELLED, "render cancelled");
 }
+
+std::unique_ptr<DeviceBuffer> TakeArena(rad_job* job) {
+  if (!job->joined && job->th.joinable()) {
+    job->cancel.store(true);
+    job->th.join();
+    job->joined = true;
+  }
+  return std::move(job->arena);
+}
+
+bool JobComplete(const rad_job* job) { return job->complete.load(); }
 
 void DestroyJob(rad_job* job) {
   if (!job) return;

@@ -3,6 +3,7 @@
 // thread, sample_renderer.cpp:60-122).
 #pragma once
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -16,7 +17,11 @@ struct StatusError : std::runtime_error {
   StatusError(rad_status c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
-rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc);
+// `reuse`: an arena from an earlier job, taken when large enough.
+rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr<DeviceBuffer> reuse = nullptr);
+bool JobComplete(const rad_job* job);
+// Detach the job's arena (joins its thread first) so a later job can reuse it.
+std::unique_ptr<DeviceBuffer> TakeArena(rad_job* job);
 void PollJob(rad_job* job, uint64_t* done, uint64_t* total, int32_t* complete);
 void CancelJob(rad_job* job);
 void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats);

@@ -426,8 +426,13 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
     out.world_lo[a] = wb[a];
     out.world_hi[a] = wb[3 + a];
   }
-  // camera (perspective.cpp:15-70)
-  const rad_camera_desc& c = desc.camera;
+  out.camera_desc = desc.camera;
+  out.cam = MakeCameraRec(desc.camera);
+}
+
+// Perspective camera record (perspective.cpp:15-70): the inverse of
+// scale * translate * perspective with the horizontal field of view.
+CameraRec MakeCameraRec(const rad_camera_desc& c) {
   if (c.width <= 0 || c.height <= 0) throw std::runtime_error("camera resolution must be positive");
   CameraRec cam{};
   cam.near_clip = c.near_clip;
@@ -446,7 +451,7 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
   M4 inv = toClip.Inverse();
   std::memcpy(cam.clip_to_camera.m, inv.m, sizeof(inv.m));
   cam.camera_to_world = M4::From(c.to_world).ToAffine();
-  out.cam = cam;
+  return cam;
 }
 
 }  // namespace rad

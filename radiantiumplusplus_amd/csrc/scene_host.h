@@ -34,6 +34,7 @@ struct HostScene {
   int32_t env_light = -1;
   int32_t global_medium = -1;
   CameraRec cam{};
+  rad_camera_desc camera_desc{};
   // primitive list before BVH ordering
   std::vector<float> prim_bounds;       // 6 per primitive
   std::vector<float4> prim_records;     // 3 per primitive
@@ -43,6 +44,8 @@ struct HostScene {
 
 // Throws std::runtime_error on invalid input.
 void AssembleScene(const rad_scene_desc& desc, HostScene& out);
+
+CameraRec MakeCameraRec(const rad_camera_desc& c);
 
 // Reference-default BSDF record from the desc (eta derivations of disney.cpp:86-101 etc.).
 BsdfRec MakeBsdfRec(const rad_bsdf_desc& d);

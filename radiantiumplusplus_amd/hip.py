@@ -192,7 +192,8 @@ class Job:
         check(lib().rad_hip_render_start(scene.h, C.byref(desc), C.byref(h)))
         self.h = h
         cam = scene.desc.camera
-        self.width, self.height = cam.width, cam.height
+        self.width = desc.width if desc.width > 0 else cam.width
+        self.height = desc.height if desc.height > 0 else cam.height
 
     def poll(self):
         d, t, c = C.c_uint64(), C.c_uint64(), C.c_int32()
