@@ -49,7 +49,7 @@ SOURCES = [
     "host/loader.cpp",
     "host/exr.cpp",
 ]
-CLI_SOURCES = ["host/cli.cpp"]
+CLI_SOURCES = ["host/cli.cpp", "host/renderer.cpp"]
 
 
 def _sources():
