@@ -245,11 +245,23 @@ typedef struct rad_job_stats {
   float guide_alpha;       /* probability of sampling the guide in the final pass */
 } rad_job_stats;
 
+enum { RAD_BVH_DEVICE = 0, RAD_BVH_HOST = 1 };
 typedef struct rad_bvh_opts {
-  int32_t builder;         /* 0 = device (GPU binned SAH), 1 = host (CPU binned SAH) */
-  int32_t max_leaf_size;   /* 0 = default */
-} rad_bvh_opts;
+  int32_t builder;         /* RAD_BVH_DEVICE: GPU (Morton-ordered binned SAH); RAD_BVH_HOST: CPU binned SAH */
+  int32_t max_leaf_size;   /* 0 = default (4) */
+} rad_bvh_opts;         /* opts = NULL selects the host builder */
 
+/* Structural check of the built BVH (node / primitive buffers read back). */
+typedef struct rad_bvh_report {
+  uint64_t node_count, primitive_count;
+  uint32_t depth, max_leaf;
+  uint32_t structure_errors;         /* out-of-range or repeated node references, unreachable nodes */
+  uint32_t unreferenced_or_repeated; /* primitives not referenced exactly once by the leaves */
+  uint32_t child_box_violations;     /* child boxes not inside the parent's slot box */
+  uint32_t leaf_box_violations;      /* triangles / spheres not inside their leaf box */
+  float sah_cost;                    /* traversal 1, intersection 1, relative to the root box area */
+  int32_t builder;
+} rad_bvh_report;
 typedef struct rad_scene_info {
   uint64_t primitive_count;
   uint64_t node_count;
@@ -283,6 +295,7 @@ rad_status rad_hip_scene_create(rad_ctx* ctx, const rad_scene_desc* desc, rad_sc
 /* Build the acceleration structure (replaces the Embree ctor, embree.cpp:18-43). */
 rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts);
 rad_status rad_hip_scene_info(const rad_scene* scene, rad_scene_info* info);
+rad_status rad_hip_bvh_check(rad_scene* scene, rad_bvh_report* report);
 void rad_hip_scene_destroy(rad_scene* scene);
 
 /* Closest hit for n rays (Embree::RayIntersectPreliminary, embree.cpp:87-111).

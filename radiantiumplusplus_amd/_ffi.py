@@ -188,6 +188,7 @@ class JobStats(C.Structure):
 RAD_RENDER_PROFILE = 1 << 2
 RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_SHADE, RAD_KERNEL_SHADOW = 0, 1, 2, 3
 RAD_KERNEL_GUIDE, RAD_KERNEL_TRAIN = 4, 5
+RAD_BVH_DEVICE, RAD_BVH_HOST = 0, 1
 RAD_GUIDE_PARAMS, RAD_GUIDE_RAW, RAD_GUIDE_FEATS = 20784, 48, 9
 
 
@@ -255,6 +256,21 @@ class ServerStats(C.Structure):
     ]
 
 
+class BvhReport(C.Structure):
+    _fields_ = [
+        ("node_count", C.c_uint64),
+        ("primitive_count", C.c_uint64),
+        ("depth", C.c_uint32),
+        ("max_leaf", C.c_uint32),
+        ("structure_errors", C.c_uint32),
+        ("unreferenced_or_repeated", C.c_uint32),
+        ("child_box_violations", C.c_uint32),
+        ("leaf_box_violations", C.c_uint32),
+        ("sah_cost", C.c_float),
+        ("builder", C.c_int32),
+    ]
+
+
 ABI_STRUCTS = {
     "rad_shape_desc": ShapeDesc,
     "rad_bsdf_desc": BsdfDesc,
@@ -272,4 +288,5 @@ ABI_STRUCTS = {
     "rad_server_opts": ServerOpts,
     "rad_request_times": RequestTimes,
     "rad_server_stats": ServerStats,
+    "rad_bvh_report": BvhReport,
 }

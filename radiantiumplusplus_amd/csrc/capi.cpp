@@ -10,6 +10,7 @@
 #include <thread>
 
 #include "../../include/rad_hip.h"
+#include "bvh_gpu.h"
 #include "render_job.h"
 #include "scene_host.h"
 #include "wavefront.h"
@@ -40,6 +41,10 @@ rad_status Guard(F&& f) {
   }
 }
 }  // namespace
+
+namespace rad {
+constexpr int kDefaultBuilder = RAD_BVH_HOST;
+}  // namespace rad
 
 namespace rad {
 
@@ -110,6 +115,7 @@ size_t rad_abi_sizeof(const char* name) {
   RAD_SZ(rad_server_opts)
   RAD_SZ(rad_request_times)
   RAD_SZ(rad_server_stats)
+  RAD_SZ(rad_bvh_report)
   RAD_SZ(rad_bvh_opts)
   RAD_SZ(rad_scene_info)
 #undef RAD_SZ
@@ -162,13 +168,22 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
     std::lock_guard<std::mutex> lk(scene->mu);
     HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
     int maxLeaf = opts && opts->max_leaf_size > 0 ? opts->max_leaf_size : 4;
-    int builder = opts ? opts->builder : 0;
-    BvhBuildResult r;
+    int builder = opts ? opts->builder : kDefaultBuilder;
+    if (builder != RAD_BVH_DEVICE && builder != RAD_BVH_HOST) throw StatusError(RAD_ERR_INVALID_ARG, "unknown BVH builder");
     HostScene& h = scene->host;
     uint32_t n = uint32_t(h.prim_bounds.size() / 6);
     auto t0 = std::chrono::steady_clock::now();
-    (void)builder;  // the device builder is selected in BuildBvh4 once available
-    {
+    uint32_t nodeCount = 0, depth = 0;
+    float sah = 0.f;
+    if (builder == RAD_BVH_DEVICE) {
+      DeviceBvhStats ds;
+      BuildBvh4Device(h.prim_bounds.data(), h.prim_records.data(), n, maxLeaf, scene->ctx->stream, scene->d_nodes,
+                      scene->d_prims, ds);
+      nodeCount = ds.node_count;
+      depth = ds.depth;
+      sah = ds.sah_cost;
+    } else {
+      BvhBuildResult r;
       BuildBvh4Host(h.prim_bounds.data(), n, maxLeaf, r);
       std::vector<float4> ordered(3 * size_t(n));
       for (uint32_t i = 0; i < n; i++)
@@ -176,15 +191,102 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
       scene->d_prims.upload(ordered.data(), ordered.size() * sizeof(float4), scene->ctx->stream);
       scene->d_nodes.upload(r.nodes.data(), r.nodes.size() * sizeof(BvhNode4), scene->ctx->stream);
       HipCheck(hipStreamSynchronize(scene->ctx->stream), "BVH upload");
+      nodeCount = uint32_t(r.nodes.size());
+      depth = r.depth;
+      sah = r.sah_cost;
     }
+    scene->builder = builder;
     scene->bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     scene->dev.nodes = scene->d_nodes.as<BvhNode4>();
     scene->dev.prims = scene->d_prims.as<float4>();
     scene->dev.prim_count = n;
-    scene->dev.node_count = uint32_t(r.nodes.size());
-    scene->bvh_depth = r.depth;
-    scene->sah_cost = r.sah_cost;
+    scene->dev.node_count = nodeCount;
+    scene->bvh_depth = depth;
+    scene->sah_cost = sah;
     scene->bvh_ready = true;
+  });
+}
+
+rad_status rad_hip_bvh_check(rad_scene* scene, rad_bvh_report* rep) {
+  if (!scene || !rep) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    std::lock_guard<std::mutex> lk(scene->mu);
+    if (!scene->bvh_ready) throw StatusError(RAD_ERR_STATE, "scene has no BVH");
+    HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
+    const uint32_t m = scene->dev.node_count, n = scene->dev.prim_count;
+    std::vector<BvhNode4> nodes(m);
+    std::vector<float4> prims(3 * size_t(n));
+    HipCheck(hipMemcpy(nodes.data(), scene->d_nodes.ptr, size_t(m) * sizeof(BvhNode4), hipMemcpyDeviceToHost), "nodes");
+    HipCheck(hipMemcpy(prims.data(), scene->d_prims.ptr, prims.size() * sizeof(float4), hipMemcpyDeviceToHost), "prims");
+    std::memset(rep, 0, sizeof(*rep));
+    std::vector<uint8_t> seen(n, 0), visited(m, 0);
+    std::vector<uint32_t> stack{0}, depthOf{1};
+    auto inside = [](float a0, float a1, float b0, float b1) { return b0 <= a0 && a1 <= b1; };
+    while (!stack.empty()) {
+      uint32_t x = stack.back(), d = depthOf.back();
+      stack.pop_back();
+      depthOf.pop_back();
+      if (x >= m || visited[x]) {
+        rep->structure_errors++;
+        continue;
+      }
+      visited[x] = 1;
+      rep->depth = std::max(rep->depth, d);
+      const BvhNode4& nd = nodes[x];
+      for (int s = 0; s < 4; s++) {
+        if (nd.child[s] < 0) continue;
+        if (nd.count[s] > 0) {
+          rep->max_leaf = std::max(rep->max_leaf, nd.count[s]);
+          for (uint32_t k = 0; k < nd.count[s]; k++) {
+            uint32_t p = uint32_t(nd.child[s]) + k;
+            if (p >= n) {
+              rep->structure_errors++;
+              continue;
+            }
+            if (seen[p]++) rep->structure_errors++;
+            const float4 a = prims[3 * size_t(p)], b = prims[3 * size_t(p) + 1], c = prims[3 * size_t(p) + 2];
+            uint32_t kind;
+            std::memcpy(&kind, &c.w, 4);
+            float lo[3], hi[3];
+            if (kind == PRIM_TRI) {
+              lo[0] = std::min({a.x, b.x, c.x}), hi[0] = std::max({a.x, b.x, c.x});
+              lo[1] = std::min({a.y, b.y, c.y}), hi[1] = std::max({a.y, b.y, c.y});
+              lo[2] = std::min({a.z, b.z, c.z}), hi[2] = std::max({a.z, b.z, c.z});
+            } else if (kind == PRIM_SPHERE) {
+              lo[0] = a.x - a.w, hi[0] = a.x + a.w, lo[1] = a.y - a.w, hi[1] = a.y + a.w, lo[2] = a.z - a.w, hi[2] = a.z + a.w;
+            } else {
+              continue;  // rectangles: bounded through their shape transform
+            }
+            if (!(inside(lo[0], hi[0], nd.lox[s], nd.hix[s]) && inside(lo[1], hi[1], nd.loy[s], nd.hiy[s]) &&
+                  inside(lo[2], hi[2], nd.loz[s], nd.hiz[s])))
+              rep->leaf_box_violations++;
+          }
+        } else {
+          uint32_t c = uint32_t(nd.child[s]);
+          if (c >= m) {
+            rep->structure_errors++;
+            continue;
+          }
+          const BvhNode4& ch = nodes[c];
+          for (int t = 0; t < 4; t++) {
+            if (ch.child[t] < 0) continue;
+            if (!(inside(ch.lox[t], ch.hix[t], nd.lox[s], nd.hix[s]) && inside(ch.loy[t], ch.hiy[t], nd.loy[s], nd.hiy[s]) &&
+                  inside(ch.loz[t], ch.hiz[t], nd.loz[s], nd.hiz[s])))
+              rep->child_box_violations++;
+          }
+          stack.push_back(c);
+          depthOf.push_back(d + 1);
+        }
+      }
+    }
+    for (uint32_t p = 0; p < n; p++)
+      if (seen[p] != 1) rep->unreferenced_or_repeated++;
+    for (uint32_t x = 0; x < m; x++)
+      if (!visited[x]) rep->structure_errors++;
+    rep->node_count = m;
+    rep->primitive_count = n;
+    rep->sah_cost = BvhSahCost(nodes);
+    rep->builder = scene->builder;
   });
 }
 

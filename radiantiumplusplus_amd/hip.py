@@ -72,6 +72,7 @@ def lib():
                 "rad_hip_guide_sample": (i32, [vp, vp, u32, vp]),
                 "rad_hip_scene_guide_get": (i32, [vp, vp, C.POINTER(u32)]),
                 "rad_hip_scene_guide_set": (i32, [vp, vp]),
+                "rad_hip_bvh_check": (i32, [vp, C.POINTER(F.BvhReport)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -109,7 +110,8 @@ class Context:
 class Scene:
     """A scene resident in HBM plus its acceleration structure."""
 
-    def __init__(self, ctx: Context, desc: F.SceneDesc, keepalive=None, builder=0, max_leaf=0, build_bvh=True):
+    def __init__(self, ctx: Context, desc: F.SceneDesc, keepalive=None, builder=F.RAD_BVH_HOST, max_leaf=0,
+                 build_bvh=True):
         self.ctx = ctx
         self.desc = desc
         self._keep = keepalive
@@ -124,7 +126,7 @@ class Scene:
         desc = builder_obj.to_desc()
         return cls(ctx, desc, keepalive=builder_obj, **kw)
 
-    def build_bvh(self, builder=0, max_leaf=0):
+    def build_bvh(self, builder=F.RAD_BVH_HOST, max_leaf=0):
         opts = F.BvhOpts(builder, max_leaf)
         check(lib().rad_hip_bvh_build(self.h, C.byref(opts)))
 
@@ -144,6 +146,12 @@ class Scene:
         out = np.zeros(len(rays), np.uint8)
         check(lib().rad_hip_intersect_any(self.h, rays.ctypes.data, len(rays), out.ctypes.data))
         return out
+
+    def bvh_check(self):
+        """Structural report of the built BVH (rad_hip_bvh_check)."""
+        r = F.BvhReport()
+        check(lib().rad_hip_bvh_check(self.h, C.byref(r)))
+        return r
 
     def guide_params(self):
         """The scene's guide weights (flat FP32) and training steps taken."""
