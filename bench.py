@@ -42,6 +42,12 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--grid", type=int, default=2048)
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    p.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
+    p.add_argument("--unsorted", action="store_true", help="shade with one kernel (no material sort)")
+    p.add_argument("--builder", choices=["host", "device"], default="host", help="BVH builder")
+    p.add_argument("--no-serving", action="store_true", help="skip the serving-latency measurement")
+    p.add_argument("--guide-scale", type=float, default=1.0, help="image scale of the guiding workload")
+    p.add_argument("--serve-requests", type=int, default=40)
     p.add_argument("--summarize", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                    help="print a text summary of rocprofv3 csv output directories")
     return p.parse_args()
@@ -188,6 +194,78 @@ def cpu_baseline(b, seconds):
     }
 
 
+def rel_mse(x, ref):
+    return float(np.mean((x.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)))
+
+
+def guiding_measure(ctx, scale=1.0, budget=256, ref_spp=2048):
+    """Door-gap room, 1280x720, 256 spp budget: wall time for learned guiding
+    (online training inside the budget) to reach the plain GPU tracer's
+    relMSE against a converged plain render, assuming relMSE ~ 1 / spp.
+    Quirks off, so the glossy GGX boxes are guided too."""
+    from radiantiumplusplus_amd import _ffi as F
+    from radiantiumplusplus_amd import hip, scenes
+
+    W, H = int(1280 * scale), int(720 * scale)
+    b = scenes.guiding_room(width=W, height=H, spp=budget)
+    s = hip.Scene.from_builder(ctx, b)
+    fl = F.RAD_RENDER_FIX_QUIRKS
+    ref, _, st_ref = s.render(b.render_desc(spp=ref_spp, seed=7, flags=fl))
+    t = time.time()
+    plain, _, st_p = s.render(b.render_desc(spp=budget, seed=8, flags=fl))
+    t_plain = time.time() - t
+    s.set_guide_params(None)
+    d = b.render_desc(spp=budget, seed=9, flags=fl)
+    d.guiding = 1
+    t = time.time()
+    guided, _, st_g = s.render(d)
+    t_guided = time.time() - t
+    e_p, e_g = rel_mse(plain, ref), rel_mse(guided, ref)
+    t_equal = t_guided * e_g / e_p
+    return {
+        "workload": f"guiding_room_{W}x{H}_{budget}spp",
+        "plain_relmse": e_p,
+        "guided_relmse": e_g,
+        "plain_s": t_plain,
+        "guided_s": t_guided,
+        "guided_time_to_plain_relmse_s": t_equal,
+        "time_ratio_guided_over_plain": t_equal / t_plain,
+        "reference": f"plain GPU render at {ref_spp} spp",
+        "guide_steps": st_g.guide_steps,
+        "guide_records": st_g.guide_records,
+        "guide_final_loss": st_g.guide_loss,
+        "guided_samples_per_s": W * H * budget / t_guided,
+    }
+
+
+def serving_measure(ctx, n_requests=40, rate=20.0):
+    """Seeded Poisson stream (mean 20 req/s) over resident scenes of the
+    many-balls, large-mesh and volume workloads; resolution, spp and guiding
+    drawn per request."""
+    from radiantiumplusplus_amd import hip, scenes, serve
+
+    builders = {"many_balls": scenes.many_balls(), "large_mesh": scenes.large_mesh(),
+                "volume_box": scenes.volume_box()}
+    res = {k: hip.Scene.from_builder(ctx, b) for k, b in builders.items()}
+    specs = serve.poisson_requests(n_requests, rate=rate, seed=4)
+    srv = serve.Server(ctx, hbm_budget=64 << 30, max_inflight=8)
+    out = serve.run_stream(srv, res, builders, specs)
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r.times.latency_ms for r in out])
+    return {
+        "workload": f"poisson_{rate:g}rps_{n_requests}req_seed4",
+        "requests": n_requests,
+        "completed": int(st.completed),
+        "samples_per_s": st.samples_per_s,
+        "p50_ms": st.p50_ms,
+        "p99_ms": st.p99_ms,
+        "mean_ms": st.mean_ms,
+        "max_ms": float(lat.max()),
+        "peak_hbm_bytes": int(st.peak_held_bytes),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -207,12 +285,12 @@ def main():
     gen_s = time.time() - t0
     ctx = hip.Context(0)
     t0 = time.time()
-    scene = hip.Scene.from_builder(ctx, b)
+    scene = hip.Scene.from_builder(ctx, b, builder=F.RAD_BVH_DEVICE if args.builder == "device" else F.RAD_BVH_HOST)
     setup_s = time.time() - t0
     info = scene.info()
     W, H = b.camera["width"], b.camera["height"]
     spp = b.sampler["sample_count"]
-    rd = b.render_desc(spp=spp, seed=1, flags=F.RAD_RENDER_PROFILE)
+    rd = b.render_desc(spp=spp, seed=1, flags=F.RAD_RENDER_PROFILE | (F.RAD_RENDER_UNSORTED if args.unsorted else 0))
     print(f"# scene {name}: {info.primitive_count} primitives, {info.node_count} BVH nodes, "
           f"build {info.bvh_build_ms:.0f} ms, generation {gen_s:.1f} s, upload+build {setup_s:.1f} s",
           file=sys.stderr, flush=True)
@@ -275,10 +353,18 @@ def main():
             "iterations_per_step": sum(s.iterations for s in stats) / args.steps,
             "slots": stats[0].slots,
             "kernel_ms_per_step": {k: sum(s.kernel_ms[i] for s in stats) / args.steps
-                                   for i, k in enumerate(["regen", "extend", "shade", "shadow"])},
+                                   for i, k in enumerate(["regen", "extend", "shade", "shadow", "guide", "train"])},
             "bvh_build_ms": info.bvh_build_ms,
+            "bvh_builder": args.builder,
+            "material_sorted": not args.unsorted,
         },
     }
+    if args.config == "large_mesh" and not args.no_guiding:
+        print("# guided vs plain (door-gap room)", file=sys.stderr, flush=True)
+        out["guiding"] = guiding_measure(ctx, args.guide_scale)
+    if args.config == "large_mesh" and not args.no_serving:
+        print("# serving stream", file=sys.stderr, flush=True)
+        out["serving"] = serving_measure(ctx, args.serve_requests)
     if not args.no_cpu:
         cb = cpu_baseline(b, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -201,7 +201,10 @@ enum {
   RAD_RENDER_WANT_SUMSQ = 1u << 1,
   /* Time every wavefront kernel with HIP events on the job's stream
    * (rad_job_stats.kernel_ms). */
-  RAD_RENDER_PROFILE = 1u << 2
+  RAD_RENDER_PROFILE = 1u << 2,
+  /* Shade every path with one kernel instead of sorting the queue by
+   * material into specialised kernels (for comparison). */
+  RAD_RENDER_UNSORTED = 1u << 3
 };
 
 enum { /* rad_job_stats.kernel_ms slots */

@@ -186,6 +186,7 @@ class JobStats(C.Structure):
 
 
 RAD_RENDER_PROFILE = 1 << 2
+RAD_RENDER_UNSORTED = 1 << 3
 RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_SHADE, RAD_KERNEL_SHADOW = 0, 1, 2, 3
 RAD_KERNEL_GUIDE, RAD_KERNEL_TRAIN = 4, 5
 RAD_BVH_DEVICE, RAD_BVH_HOST = 0, 1

@@ -653,4 +653,38 @@ __device__ inline f3 bsdf_sample(const BsdfRec& b, bool quirks, f3 wi, float lob
   }
 }
 
+// Material-specialised dispatch for kernels that shade one BSDF type
+// (MAT = BSDF_*; any other value keeps the run-time switch).
+template <int MAT>
+__device__ __forceinline__ f3 bsdf_eval_t(const BsdfRec& b, bool quirks, f3 wi, f3 wo, float& pdf) {
+  if constexpr (MAT == BSDF_DIFFUSE) {
+    return diffuse_eval(b, wi, wo, pdf);
+  } else if constexpr (MAT == BSDF_ROUGH_METAL) {
+    return metal_eval(b, quirks, wi, wo, pdf);
+  } else if constexpr (MAT == BSDF_ROUGH_GLASS) {
+    return glass_eval(b, quirks, wi, wo, pdf);
+  } else if constexpr (MAT == BSDF_DISNEY) {
+    pdf = disney_pdf(b, quirks, wi, wo);
+    return disney_f(b, quirks, wi, wo);
+  } else {
+    return bsdf_eval(b, quirks, wi, wo, pdf);
+  }
+}
+
+template <int MAT>
+__device__ __forceinline__ f3 bsdf_sample_t(const BsdfRec& b, bool quirks, f3 wi, float lobe, float u1, float u2,
+                                            BsdfSample& s) {
+  if constexpr (MAT == BSDF_DIFFUSE) {
+    return diffuse_sample(b, wi, u1, u2, s);
+  } else if constexpr (MAT == BSDF_ROUGH_METAL) {
+    return metal_sample(b, quirks, wi, u1, u2, s);
+  } else if constexpr (MAT == BSDF_ROUGH_GLASS) {
+    return glass_sample(b, quirks, wi, lobe, u1, u2, s);
+  } else if constexpr (MAT == BSDF_DISNEY) {
+    return disney_sample(b, quirks, wi, lobe, u1, u2, s);
+  } else {
+    return bsdf_sample(b, quirks, wi, lobe, u1, u2, s);
+  }
+}
+
 }  // namespace rad

@@ -121,6 +121,7 @@ __global__ void k_reset(uint32_t* cnt, int out) {
     cnt[CNT_SHADOW] = 0;
     cnt[CNT_FETCH_EXT] = 0;
     cnt[CNT_FETCH_SH] = 0;
+    for (int k = 0; k < 4; k++) cnt[CNT_MAT + k] = 0;
   }
 }
 
@@ -155,9 +156,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_extend(Wavefront w, DevScene sc
 // (probability alpha) or the BSDF, and weighted by the one-sample MIS
 // density alpha q + (1 - alpha) p_bsdf; NEE uses the same density in its MIS
 // weight; training records of the vertices are kept for 1 in rec_stride slots.
-template <bool GUIDE>
+template <bool GUIDE, int MAT = -1>
 __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene, int in) {
-  const uint32_t n = w.cnt[in];
+  const uint32_t n = MAT >= 0 ? w.cnt[CNT_MAT + MAT] : w.cnt[in];
+  const uint32_t* const queue = MAT >= 0 ? w.mq[MAT] : w.queue[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
   const bool guideOn = GUIDE && (w.flags & WF_GUIDE) != 0;
@@ -171,7 +173,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     guide::TrainRecord rec;
     if (i < n) {
-      s = w.queue[in][i];
+      s = queue[i];
       float4 ro4 = w.ray_o[s], rd4 = w.ray_d[s];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float4 hh = w.hit[s];
@@ -233,7 +235,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             if (ls.pdf > 0.f) {
               f3 wo = si.sh.to_local(ls.dir);
               float bp;
-              f3 f = bsdf_eval(b, quirks, si.wi, wo, bp);
+              f3 f = bsdf_eval_t<MAT>(b, quirks, si.wi, wo, bp);
               if (a > 0.f) bp = a * guide::mixture_pdf(mx, ls.dir) + (1.f - a) * bp;
               // Scene::IsOcclude setup (scene.cpp:53-63)
               f3 o = offset_p(si.p, si.n, ls.p - si.p);
@@ -267,14 +269,14 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           if (useGuide) {
             f3 wg = guide::mixture_sample(mx, ul, u1, u2);
             float bp;
-            f = bsdf_eval(b, quirks, si.wi, si.sh.to_local(wg), bp);
+            f = bsdf_eval_t<MAT>(b, quirks, si.wi, si.sh.to_local(wg), bp);
             wo = wg;
             pdf = a * guide::mixture_pdf(mx, wg) + (1.f - a) * bp;
             bs.eta = 1.f;
             bs.lobes = LOBE_GLOSSY;
             if (is_zero(f)) pdf = 0.f;  // zero-valued paths end here without bias
           } else {
-            f = bsdf_sample(b, quirks, si.wi, ul, u1, u2, bs);
+            f = bsdf_sample_t<MAT>(b, quirks, si.wi, ul, u1, u2, bs);
             wo = si.sh.to_world(bs.wo);
             pdf = bs.pdf;
             if (a > 0.f && pdf > 0.f) pdf = a * guide::mixture_pdf(mx, wo) + (1.f - a) * bs.pdf;
@@ -374,6 +376,28 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
       f[0] = p.x, f[1] = p.y, f[2] = p.z;
       f[3] = gn.x, f[4] = gn.y, f[5] = gn.z;
       f[6] = wi.x, f[7] = wi.y, f[8] = wi.z;
+    }
+  }
+}
+
+// Partition the extend queue by material (misses go with the diffuse bucket)
+// so each shade kernel runs one BSDF's code.
+__global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
+  const uint32_t n = w.cnt[in];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint32_t i = base + threadIdx.x;
+    int key = -1;
+    uint32_t s = 0;
+    if (i < n) {
+      s = w.queue[in][i];
+      uint32_t geom = w.hit_geom[s];
+      key = geom == 0xffffffffu ? BSDF_DIFFUSE : scene.bsdfs[scene.shapes[geom].bsdf].type;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      uint32_t pos = wave_append(&w.cnt[CNT_MAT + k], key == k);
+      if (key == k) w.mq[k][pos] = s;
     }
   }
 }
@@ -549,6 +573,28 @@ void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const La
   else
     hipLaunchKernelGGL(k_shade_path<false>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
+template <bool G>
+static void LaunchSortedImpl(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_classify, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  // the diffuse kernel also takes the misses, so it always runs
+  hipLaunchKernelGGL((k_shade_path<G, BSDF_DIFFUSE>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  if (mask & (1u << BSDF_ROUGH_METAL))
+    hipLaunchKernelGGL((k_shade_path<G, BSDF_ROUGH_METAL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  if (mask & (1u << BSDF_ROUGH_GLASS))
+    hipLaunchKernelGGL((k_shade_path<G, BSDF_ROUGH_GLASS>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  if (mask & (1u << BSDF_DISNEY))
+    hipLaunchKernelGGL((k_shade_path<G, BSDF_DISNEY>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+}
+
+void LaunchShadeSorted(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
+                       hipStream_t s) {
+  if (w.flags & (WF_GUIDE | WF_RECORD))
+    LaunchSortedImpl<true>(w, scene, in, mask, c, s);
+  else
+    LaunchSortedImpl<false>(w, scene, in, mask, c, s);
+}
+
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_guide_reset, dim3(1), dim3(64), 0, s, w.gcnt);
   hipLaunchKernelGGL(k_guide_prep, dim3(c.blocks), dim3(256), 0, s, w, scene, in);

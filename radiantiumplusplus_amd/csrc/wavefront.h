@@ -33,7 +33,8 @@ enum : uint32_t {
   CNT_NEXT = 4,       // next sample id of the pass
   CNT_FETCH_EXT = 5,  // extend-kernel work counter
   CNT_FETCH_SH = 6,   // shadow-kernel work counter
-  CNT_WORDS = 8
+  CNT_MAT = 8,        // 4 words: per-material queue counts (diffuse + misses, metal, glass, Disney)
+  CNT_WORDS = 12
 };
 enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_WORDS = 4 };
 enum : uint32_t { WF_QUIRKS = 1u, WF_GUIDE = 2u, WF_RECORD = 4u };
@@ -52,6 +53,7 @@ struct Wavefront {
   float4* hit;     // t, u, v, prim (bits)
   uint32_t* hit_geom;
   uint32_t* queue[2];
+  uint32_t* mq[4];   // the extend queue partitioned by material (k_classify)
   uint32_t* dead;
   float4* sh_o;    // shadow origin, tmax
   float4* sh_d;    // shadow direction, slot (bits)
@@ -109,6 +111,10 @@ void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const Launch
 void LaunchReset(const Wavefront& w, int out, hipStream_t s);
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
+// Material-sorted shading: bucket the extend queue by BSDF type, then one
+// specialised shade kernel per type present (mat_mask bit = BSDF type).
+void LaunchShadeSorted(const Wavefront& w, const DevScene& scene, int in, uint32_t mat_mask, const LaunchCfg& c,
+                       hipStream_t s);
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchAccum(const Wavefront& w, hipStream_t s);
