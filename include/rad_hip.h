@@ -228,6 +228,8 @@ typedef struct rad_render_desc {
   int32_t guiding;         /* learned path guiding: 0 off, 1 train online and guide, 2 guide with the scene's current weights */
   uint32_t request_id;     /* RNG stream key for serving (0 for single renders) */
   int32_t width, height;   /* image size override (0 = the scene camera's); fov and pose are the scene's */
+  uint32_t blocks_per_cu;        /* persistent grid of the per-path kernels (0 = tuned / default) */
+  uint32_t trace_blocks_per_cu;  /* persistent grid of the traversal kernels (0 = tuned / default) */
 } rad_render_desc;
 
 typedef struct rad_job_stats {
@@ -246,7 +248,24 @@ typedef struct rad_job_stats {
   uint64_t guide_records;  /* training records collected by this render */
   float guide_loss;        /* mean loss of the last training step */
   float guide_alpha;       /* probability of sampling the guide in the final pass */
+  uint32_t blocks_per_cu, trace_blocks_per_cu; /* launch configuration used */
 } rad_job_stats;
+
+/* Launch-configuration autotuner.  Times short renders of `probe` (its spp is
+ * the per-trial sample count) over path-slot counts and persistent grid sizes
+ * by coordinate descent, stores the fastest configuration in the scene (used
+ * by later renders whose desc leaves those fields 0) and reports every trial.
+ * Images do not depend on these settings. */
+typedef struct rad_tune_trial {
+  uint32_t path_slots, blocks_per_cu, trace_blocks_per_cu;
+  float ms;
+} rad_tune_trial;
+typedef struct rad_tune_report {
+  uint32_t trials;
+  rad_tune_trial trial[24];
+  uint32_t path_slots, blocks_per_cu, trace_blocks_per_cu; /* chosen */
+  float best_ms;
+} rad_tune_report;
 
 enum { RAD_BVH_DEVICE = 0, RAD_BVH_HOST = 1 };
 typedef struct rad_bvh_opts {
@@ -298,6 +317,7 @@ rad_status rad_hip_scene_create(rad_ctx* ctx, const rad_scene_desc* desc, rad_sc
 /* Build the acceleration structure (replaces the Embree ctor, embree.cpp:18-43). */
 rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts);
 rad_status rad_hip_scene_info(const rad_scene* scene, rad_scene_info* info);
+rad_status rad_hip_autotune(rad_scene* scene, const rad_render_desc* probe, rad_tune_report* report);
 rad_status rad_hip_bvh_check(rad_scene* scene, rad_bvh_report* report);
 void rad_hip_scene_destroy(rad_scene* scene);
 

@@ -162,6 +162,8 @@ class RenderDesc(C.Structure):
         ("request_id", C.c_uint32),
         ("width", C.c_int32),
         ("height", C.c_int32),
+        ("blocks_per_cu", C.c_uint32),
+        ("trace_blocks_per_cu", C.c_uint32),
     ]
 
 
@@ -182,6 +184,8 @@ class JobStats(C.Structure):
         ("guide_records", C.c_uint64),
         ("guide_loss", C.c_float),
         ("guide_alpha", C.c_float),
+        ("blocks_per_cu", C.c_uint32),
+        ("trace_blocks_per_cu", C.c_uint32),
     ]
 
 
@@ -272,6 +276,22 @@ class BvhReport(C.Structure):
     ]
 
 
+class TuneTrial(C.Structure):
+    _fields_ = [("path_slots", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("ms", C.c_float)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [
+        ("trials", C.c_uint32),
+        ("trial", TuneTrial * 24),
+        ("path_slots", C.c_uint32),
+        ("blocks_per_cu", C.c_uint32),
+        ("trace_blocks_per_cu", C.c_uint32),
+        ("best_ms", C.c_float),
+    ]
+
+
 ABI_STRUCTS = {
     "rad_shape_desc": ShapeDesc,
     "rad_bsdf_desc": BsdfDesc,
@@ -290,4 +310,5 @@ ABI_STRUCTS = {
     "rad_request_times": RequestTimes,
     "rad_server_stats": ServerStats,
     "rad_bvh_report": BvhReport,
+    "rad_tune_report": TuneReport,
 }

@@ -116,6 +116,7 @@ size_t rad_abi_sizeof(const char* name) {
   RAD_SZ(rad_request_times)
   RAD_SZ(rad_server_stats)
   RAD_SZ(rad_bvh_report)
+  RAD_SZ(rad_tune_report)
   RAD_SZ(rad_bvh_opts)
   RAD_SZ(rad_scene_info)
 #undef RAD_SZ
@@ -204,6 +205,62 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
     scene->bvh_depth = depth;
     scene->sah_cost = sah;
     scene->bvh_ready = true;
+  });
+}
+
+static void RequireBvh(rad_scene* s) {
+  if (!s->bvh_ready) throw StatusError(RAD_ERR_STATE, "rad_hip_bvh_build has not been called on this scene");
+}
+
+rad_status rad_hip_autotune(rad_scene* scene, const rad_render_desc* probe, rad_tune_report* rep) {
+  if (!scene || !probe || !rep) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    RequireBvh(scene);
+    std::memset(rep, 0, sizeof(*rep));
+    const uint64_t samples = uint64_t(probe->width > 0 ? probe->width : scene->host.cam.width) *
+                             uint64_t(probe->height > 0 ? probe->height : scene->host.cam.height) * probe->spp;
+    auto trial = [&](uint32_t slots, uint32_t bpc, uint32_t tpc) {
+      rad_render_desc d = *probe;
+      d.path_slots = slots;
+      d.blocks_per_cu = bpc;
+      d.trace_blocks_per_cu = tpc;
+      d.flags &= ~uint32_t(RAD_RENDER_PROFILE);
+      rad_job* j = StartJob(scene, d);
+      rad_job_stats st{};
+      try {
+        WaitJob(j, nullptr, nullptr, &st);
+      } catch (...) {
+        DestroyJob(j);
+        throw;
+      }
+      DestroyJob(j);
+      float ms = float(st.render_ms);
+      if (rep->trials < 24) rep->trial[rep->trials++] = rad_tune_trial{slots, bpc, tpc, ms};
+      return ms;
+    };
+    trial(4u << 20, 8, 8);  // warm-up (first-use costs)
+    uint32_t bestSlots = 4u << 20, bestB = 8, bestT = 8;
+    float best = 3.4e38f;
+    for (uint32_t slots : {1u << 20, 2u << 20, 4u << 20, 8u << 20}) {
+      if (slots > 2 * samples && slots != (1u << 20)) continue;
+      float ms = trial(slots, bestB, bestT);
+      if (ms < best) best = ms, bestSlots = slots;
+    }
+    for (uint32_t b : {4u, 16u}) {
+      float ms = trial(bestSlots, b, bestT);
+      if (ms < best) best = ms, bestB = b;
+    }
+    for (uint32_t t : {2u, 4u, 16u}) {
+      float ms = trial(bestSlots, bestB, t);
+      if (ms < best) best = ms, bestT = t;
+    }
+    scene->tuned_slots = bestSlots;
+    scene->tuned_blocks_per_cu = bestB;
+    scene->tuned_trace_blocks_per_cu = bestT;
+    rep->path_slots = bestSlots;
+    rep->blocks_per_cu = bestB;
+    rep->trace_blocks_per_cu = bestT;
+    rep->best_ms = best;
   });
 }
 
@@ -312,9 +369,6 @@ void rad_hip_scene_destroy(rad_scene* scene) {
   delete scene;
 }
 
-static void RequireBvh(rad_scene* s) {
-  if (!s->bvh_ready) throw StatusError(RAD_ERR_STATE, "rad_hip_bvh_build has not been called on this scene");
-}
 
 rad_status rad_hip_intersect_closest(rad_scene* scene, const rad_ray* rays, uint32_t n, rad_hit* hits) {
   if (!scene || (n && (!rays || !hits))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");

@@ -565,7 +565,7 @@ void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const Launch
 }
 void LaunchReset(const Wavefront& w, int out, hipStream_t s) { hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w.cnt, out); }
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_extend, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene, in);
+  hipLaunchKernelGGL(k_extend, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene, in);
 }
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   if (w.flags & (WF_GUIDE | WF_RECORD))
@@ -600,7 +600,7 @@ void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const La
   hipLaunchKernelGGL(k_guide_prep, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene);
+  hipLaunchKernelGGL(k_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);
 }
 void LaunchAccum(const Wavefront& w, hipStream_t s) {
   hipLaunchKernelGGL(k_accum, dim3(Blocks(w.npix, 256)), dim3(256), 0, s, w);

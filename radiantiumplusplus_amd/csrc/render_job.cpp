@@ -42,6 +42,7 @@ struct rad_job {
   std::vector<uint8_t> pass_train;
   uint32_t* batch = nullptr;  // device minibatch indices (inside the arena)
   bool sorted = false;        // material-sorted shading
+  uint32_t blocks_per_cu = 0, trace_blocks_per_cu = 0;
   uint32_t mat_mask = 0;      // BSDF types present in the scene
   bool joined = false;
 };
@@ -85,7 +86,7 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   if (rows == 0) throw StatusError(RAD_ERR_INVALID_ARG, "image too large");
   j.rows = uint32_t(rows);
   const uint64_t pass = rows * j.npix;
-  uint64_t Q = desc.path_slots > 0 ? desc.path_slots : kDefaultSlots;
+  uint64_t Q = desc.path_slots > 0 ? desc.path_slots : scene->tuned_slots ? scene->tuned_slots : kDefaultSlots;
   Q = std::max<uint64_t>(1, std::min(Q, pass));
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
@@ -160,7 +161,14 @@ void RunJob(rad_job* job) {
     const DevScene scene = job->dev;
     Wavefront& w = job->w;
     hipStream_t st = job->stream;
-    LaunchCfg cfg{std::max(64, job->scene->ctx->cu_count * 8), 256};
+    const int cus = job->scene->ctx->cu_count;
+    const int bpc = job->desc.blocks_per_cu > 0 ? int(job->desc.blocks_per_cu)
+                    : job->scene->tuned_blocks_per_cu ? int(job->scene->tuned_blocks_per_cu) : 8;
+    const int tpc = job->desc.trace_blocks_per_cu > 0 ? int(job->desc.trace_blocks_per_cu)
+                    : job->scene->tuned_trace_blocks_per_cu ? int(job->scene->tuned_trace_blocks_per_cu) : 8;
+    LaunchCfg cfg{std::max(64, cus * bpc), std::max(64, cus * tpc)};
+    job->blocks_per_cu = uint32_t(bpc);
+    job->trace_blocks_per_cu = uint32_t(tpc);
     const int inferBlocks = std::max(64, job->scene->ctx->cu_count * 3);
     const bool prof = (job->desc.flags & RAD_RENDER_PROFILE) != 0;
     const size_t readback = CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t) + G_WORDS * 4;
@@ -450,6 +458,8 @@ void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats) {
     stats->guide_records = job->guide_records;
     stats->guide_loss = job->guide_loss;
     stats->guide_alpha = job->guide_alpha;
+    stats->blocks_per_cu = job->blocks_per_cu;
+    stats->trace_blocks_per_cu = job->trace_blocks_per_cu;
   }
   if (job->status == RAD_ERR_CANCELLED) throw StatusError(RAD_ERR_CANCELLED, "render cancelled");
 }

@@ -89,5 +89,7 @@ struct rad_scene {
   float sah_cost = 0;
   std::shared_ptr<rad::guide::GuideState> guide;  // created on first guided render
   std::mutex guide_mu;  // held by a render that trains the guide
+  // launch configuration chosen by rad_hip_autotune (0 = library default)
+  uint32_t tuned_slots = 0, tuned_blocks_per_cu = 0, tuned_trace_blocks_per_cu = 0;
   uint64_t device_bytes() const;
 };

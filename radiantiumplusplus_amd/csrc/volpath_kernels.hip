@@ -346,7 +346,7 @@ void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const Lau
   hipLaunchKernelGGL(k_shade_vol, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 void LaunchVolShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_vol_shadow, dim3(c.blocks), dim3(kTraceBlock), 0, s, w, scene);
+  hipLaunchKernelGGL(k_vol_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);
 }
 
 void SetLastError(const std::string& msg);

@@ -101,8 +101,8 @@ __device__ __forceinline__ void wave_count(unsigned long long* counter, bool pre
 }
 
 struct LaunchCfg {
-  int blocks;   // persistent grid of the per-path kernels
-  int threads;
+  int blocks;        // persistent grid of the per-path kernels (regen, shade, classify)
+  int trace_blocks;  // persistent grid of the traversal kernels (extend, shadow)
 };
 
 // Host-side launchers (defined in path_kernels.hip).

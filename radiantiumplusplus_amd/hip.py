@@ -73,6 +73,7 @@ def lib():
                 "rad_hip_scene_guide_get": (i32, [vp, vp, C.POINTER(u32)]),
                 "rad_hip_scene_guide_set": (i32, [vp, vp]),
                 "rad_hip_bvh_check": (i32, [vp, C.POINTER(F.BvhReport)]),
+                "rad_hip_autotune": (i32, [vp, C.POINTER(F.RenderDesc), C.POINTER(F.TuneReport)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -146,6 +147,12 @@ class Scene:
         out = np.zeros(len(rays), np.uint8)
         check(lib().rad_hip_intersect_any(self.h, rays.ctypes.data, len(rays), out.ctypes.data))
         return out
+
+    def autotune(self, probe: F.RenderDesc):
+        """Time short renders over launch configurations; the scene keeps the fastest."""
+        r = F.TuneReport()
+        check(lib().rad_hip_autotune(self.h, C.byref(probe), C.byref(r)))
+        return r
 
     def bvh_check(self):
         """Structural report of the built BVH (rad_hip_bvh_check)."""

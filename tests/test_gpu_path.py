@@ -136,3 +136,25 @@ def test_json_scene_end_to_end(hip_ctx, tmp_path):
         assert np.array_equal(fb1, fb2)
     finally:
         hip.lib().rad_loaded_scene_free(h)
+
+
+def test_autotune_and_launch_independence(hip_ctx):
+    """The autotuner picks a launch configuration the scene keeps; images do
+    not depend on slots, grid sizes or material sorting."""
+    b = scenes.many_balls(seed=0, width=128, height=128, spp=16, n_spheres=24)
+    s = hip.Scene.from_builder(hip_ctx, b)
+    rep = s.autotune(b.render_desc(spp=16, seed=1))
+    print("trials", [(t.path_slots, t.blocks_per_cu, t.trace_blocks_per_cu, round(t.ms, 2))
+                     for t in rep.trial[:rep.trials]], "chosen", rep.path_slots, rep.blocks_per_cu,
+          rep.trace_blocks_per_cu)
+    assert rep.trials >= 8
+    assert rep.path_slots in (1 << 20, 2 << 20, 4 << 20, 8 << 20)
+    a, _, st = s.render(b.render_desc(spp=16, seed=5))
+    assert (st.blocks_per_cu, st.trace_blocks_per_cu) == (rep.blocks_per_cu, rep.trace_blocks_per_cu)
+    d = b.render_desc(spp=16, seed=5)
+    d.path_slots, d.blocks_per_cu, d.trace_blocks_per_cu = 4096, 2, 2
+    c, _, st2 = s.render(d)
+    assert st2.slots == 4096
+    assert np.array_equal(a, c)
+    u = s.render(b.render_desc(spp=16, seed=5, flags=F.RAD_RENDER_UNSORTED))[0]
+    assert np.array_equal(a, u)
