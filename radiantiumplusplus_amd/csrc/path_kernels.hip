@@ -62,6 +62,7 @@ __device__ __forceinline__ void finish_records(const Wavefront& w, uint32_t s, f
 // Store the sample a dead slot finished into the result planes and start the
 // next camera sample of the pass on it (sample_renderer.cpp:97-99).
 __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int in) {
+  __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[CNT_DEAD];
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t pass = w.rows * w.npix;
@@ -86,7 +87,14 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
         if (w.flags & WF_RECORD) finish_records(w, s, mk(l.x, l.y, l.z));
       }
     }
-    uint32_t id = more ? wave_append(&w.cnt[CNT_NEXT], valid) : pass;
+    uint32_t id = pass;
+    if (more) {
+      uint32_t* c1[1] = {&w.cnt[CNT_NEXT]};
+      bool p1[1] = {valid};
+      uint32_t r1[1];
+      block_append<1>(c1, p1, r1, lds);
+      id = r1[0];
+    }
     bool start = valid && id < pass;
     if (start) {
       uint32_t pixel = id % w.npix, sample = w.row0 + id / w.npix;
@@ -108,9 +116,14 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
     } else if (valid) {
       w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);  // idle for the rest of the pass
     }
-    uint32_t pos = wave_append(&w.cnt[in], start);
-    if (start) w.queue[in][pos] = s;
-    wave_count(&w.stats[ST_SAMPLES], finished);
+    {
+      uint32_t* c1[1] = {&w.cnt[in]};
+      bool p1[1] = {start};
+      uint32_t r1[1];
+      block_append<1>(c1, p1, r1, lds);
+      if (start) w.queue[in][r1[0]] = s;
+    }
+    block_count(&w.stats[ST_SAMPLES], finished ? 1u : 0u, lds);
   }
 }
 
@@ -158,6 +171,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_extend(Wavefront w, DevScene sc
 // weight; training records of the vertices are kept for 1 in rec_stride slots.
 template <bool GUIDE, int MAT = -1>
 __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene, int in) {
+  __shared__ uint32_t lds[32];
   const uint32_t n = MAT >= 0 ? w.cnt[CNT_MAT + MAT] : w.cnt[in];
   const uint32_t* const queue = MAT >= 0 ? w.mq[MAT] : w.queue[in];
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -322,22 +336,27 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     }
     bool valid = i < n;
     if (record) {
-      uint32_t r = wave_append(&w.gcnt[G_RECORDS], want_rec);
+      uint32_t* c1[1] = {&w.gcnt[G_RECORDS]};
+      bool p1[1] = {want_rec};
+      uint32_t r1[1];
+      block_append<1>(c1, p1, r1, lds);
+      const uint32_t r = r1[0];
       if (want_rec && r < w.rec_cap) {
         w.recs[r] = rec;
         w.glast[s] = r;
         w.gpend[s] = r;  // baseline filled by the next bounce (or the path's end)
       }
     }
-    uint32_t qpos = wave_append(&w.cnt[out], valid && alive);
-    if (valid && alive) w.queue[out][qpos] = s;
-    uint32_t dpos = wave_append(&w.cnt[CNT_DEAD], valid && !alive);
-    if (valid && !alive) w.dead[dpos] = s;
-    uint32_t spos = wave_append(&w.cnt[CNT_SHADOW], emit);
+    uint32_t* c3[3] = {&w.cnt[out], &w.cnt[CNT_DEAD], &w.cnt[CNT_SHADOW]};
+    bool p3[3] = {valid && alive, valid && !alive, emit};
+    uint32_t r3[3];
+    block_append<3>(c3, p3, r3, lds);
+    if (valid && alive) w.queue[out][r3[0]] = s;
+    if (valid && !alive) w.dead[r3[1]] = s;
     if (emit) {
-      w.sh_o[spos] = so;
-      w.sh_d[spos] = sdir;
-      w.sh_c[spos] = sc;
+      w.sh_o[r3[2]] = so;
+      w.sh_d[r3[2]] = sdir;
+      w.sh_c[r3[2]] = sc;
     }
   }
 }
@@ -345,6 +364,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
 // Guided vertices of this iteration: surface hits whose BSDF the guide may
 // sample; their network inputs go to the guide queue for k_guide_infer.
 __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene, int in) {
+  __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
@@ -369,7 +389,11 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
         }
       }
     }
-    uint32_t pos = wave_append(&w.gcnt[G_QUEUE], g);
+    uint32_t* c1[1] = {&w.gcnt[G_QUEUE]};
+    bool p1[1] = {g};
+    uint32_t r1[1];
+    block_append<1>(c1, p1, r1, lds);
+    const uint32_t pos = r1[0];
     if (g) {
       w.gq[pos] = s;
       float* f = w.gfeat + size_t(pos) * guide::kFeat;
@@ -383,6 +407,7 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
 // Partition the extend queue by material (misses go with the diffuse bucket)
 // so each shade kernel runs one BSDF's code.
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
+  __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -394,10 +419,14 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
       uint32_t geom = w.hit_geom[s];
       key = geom == 0xffffffffu ? BSDF_DIFFUSE : scene.bsdfs[scene.shapes[geom].bsdf].type;
     }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      uint32_t pos = wave_append(&w.cnt[CNT_MAT + k], key == k);
-      if (key == k) w.mq[k][pos] = s;
+    uint32_t* c4[4] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3]};
+    bool p4[4] = {key == 0, key == 1, key == 2, key == 3};
+    uint32_t r4[4];
+    block_append<4>(c4, p4, r4, lds);
+    if (key >= 0) {
+      uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : w.mq[3];
+      uint32_t r = key == 0 ? r4[0] : key == 1 ? r4[1] : key == 2 ? r4[2] : r4[3];
+      q[r] = s;
     }
   }
 }

@@ -279,8 +279,10 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
 
 // Stream traversal for kernels that own a batch of rays.  A lane whose ray
 // finishes is refilled from the batch while the other lanes of its wave keep
-// traversing; refills take one atomic per wave on `fetch` (or, with
-// fetch == nullptr, each wave takes the 64 items at its own position once).
+// traversing.  Each wave takes rays from a private chunk of kChunk
+// consecutive items and grabs the next chunk with one atomic on `fetch`, so
+// the shared counter sees one atomic per kChunk rays (with fetch == nullptr
+// each wave takes the 64 items at its own position once).
 // Job: bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag)
 //      void finish(uint32_t i, uint32_t tag, const TraceRay& r)
 // Launch with kTraceBlock threads per block; every lane must call this.
@@ -289,6 +291,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
   __shared__ uint32_t lds_node[kLdsStack * kTraceBlock];
   __shared__ float lds_dist[kLdsStack * kTraceBlock];
   constexpr int kRefillBelow = 48;  // refill once fewer lanes than this are busy
+  constexpr uint32_t kChunk = 256;
   const uint32_t lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   TraceStack st;
@@ -297,38 +300,57 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
   TraceRay r;
   bool active = false, exhausted = false;
   uint32_t idx = 0, tag = 0;
+  uint32_t cbase = 0, cleft = 0;  // the wave's chunk (wave-uniform)
   for (;;) {
-    if (!exhausted) {
+    if (!exhausted || cleft > 0) {
       unsigned long long need = __ballot(!active);
       if (need) {
-        uint32_t want = (uint32_t)__popcll(need);
-        uint32_t base;
+        const uint32_t want = (uint32_t)__popcll(need);
+        const uint32_t rank = (uint32_t)__popcll(need & below);
+        uint32_t i = 0xffffffffu;
         if (fetch) {
-          uint32_t leader = (uint32_t)(__ffsll((long long)need) - 1);
-          uint32_t b = 0;
-          if (lane == leader) b = atomicAdd(fetch, want);
-          base = __shfl(b, (int)leader);
-          if (base + want >= n) exhausted = true;
+          uint32_t nbase = 0, nleft = 0;
+          if (cleft < want && !exhausted) {
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(fetch, kChunk);
+            b = __shfl(b, 0);
+            if (b >= n) {
+              exhausted = true;
+            } else {
+              nbase = b;
+              nleft = min(kChunk, n - b);
+              if (b + kChunk >= n) exhausted = true;
+            }
+          }
+          if (rank < cleft)
+            i = cbase + rank;
+          else if (rank - cleft < nleft)
+            i = nbase + (rank - cleft);
+          const uint32_t fromOld = min(want, cleft);
+          cbase += fromOld;
+          cleft -= fromOld;
+          if (nleft) {
+            const uint32_t fromNew = min(want - fromOld, nleft);
+            cbase = nbase + fromNew;
+            cleft = nleft - fromNew;
+          }
         } else {
-          base = (blockIdx.x * kTraceBlock + (threadIdx.x & ~63u));
+          i = blockIdx.x * kTraceBlock + (threadIdx.x & ~63u) + lane;
           exhausted = true;
         }
-        if (!active) {
-          uint32_t i = base + (fetch ? (uint32_t)__popcll(need & below) : lane);
-          if (i < n) {
-            f3 o, d;
-            float tmin, tmax;
-            if (job.load(i, o, d, tmin, tmax, tag)) {
-              idx = i;
-              trace_begin(r, st, o, d, tmin, tmax);
-              active = true;
-            }
+        if (!active && i < n) {
+          f3 o, d;
+          float tmin, tmax;
+          if (job.load(i, o, d, tmin, tmax, tag)) {
+            idx = i;
+            trace_begin(r, st, o, d, tmin, tmax);
+            active = true;
           }
         }
       }
     }
     if (__ballot(active) == 0ull) {
-      if (exhausted) break;
+      if (exhausted && cleft == 0) break;
       continue;
     }
     for (int k = 0; k < 256; k++) {
@@ -338,7 +360,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
       }
       unsigned long long busy = __ballot(active);
       if (busy == 0ull) break;
-      if (!exhausted && __popcll(busy) < kRefillBelow) break;
+      if ((!exhausted || cleft > 0) && __popcll(busy) < kRefillBelow) break;
     }
   }
 }

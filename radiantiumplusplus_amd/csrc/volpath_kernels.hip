@@ -19,11 +19,6 @@ __global__ void k_grid_eval(const float* grid, int sx, int sy, int sz, int ch, i
   out[3 * i + 2] = v.z;
 }
 
-__device__ __forceinline__ void wave_sum(unsigned long long* counter, uint32_t v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if (__lane_id() == 0 && v) atomicAdd(counter, (unsigned long long)v);
-}
-
 // Shadow work items of vol_path carry the current medium (+1, 8 bits), the
 // spectral channel (2 bits) and the path's rng dimension at emission (22
 // bits); the transmittance estimator draws from its own stream keyed by both.
@@ -39,6 +34,7 @@ __device__ __forceinline__ uint32_t shadow_key(uint32_t key, uint32_t dim) { ret
 // (emission with MIS, pass-through medium boundaries that are not counted,
 // BSDF NEE and sampling).  NEE contributions are finished by k_vol_shadow.
 __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, int in) {
+  __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
@@ -235,17 +231,18 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
       w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
     }
     bool valid = i < n;
-    uint32_t qpos = wave_append(&w.cnt[out], valid && alive);
-    if (valid && alive) w.queue[out][qpos] = s;
-    uint32_t dpos = wave_append(&w.cnt[CNT_DEAD], valid && !alive);
-    if (valid && !alive) w.dead[dpos] = s;
-    uint32_t spos = wave_append(&w.cnt[CNT_SHADOW], emit);
+    uint32_t* c3[3] = {&w.cnt[out], &w.cnt[CNT_DEAD], &w.cnt[CNT_SHADOW]};
+    bool p3[3] = {valid && alive, valid && !alive, emit};
+    uint32_t r3[3];
+    block_append<3>(c3, p3, r3, lds);
+    if (valid && alive) w.queue[out][r3[0]] = s;
+    if (valid && !alive) w.dead[r3[1]] = s;
     if (emit) {
-      w.sh_o[spos] = so;
-      w.sh_d[spos] = sdir;
-      w.sh_c[spos] = sc;
+      w.sh_o[r3[2]] = so;
+      w.sh_d[r3[2]] = sdir;
+      w.sh_c[r3[2]] = sc;
     }
-    wave_count(&w.stats[ST_MEDIUM], msample);
+    block_count(&w.stats[ST_MEDIUM], msample ? 1u : 0u, lds);
   }
 }
 
@@ -253,6 +250,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
 // ratio tracking on the path's channel through media, passing through
 // surfaces without a BSDF, blocked by any surface with one.
 __global__ void __launch_bounds__(kTraceBlock) k_vol_shadow(Wavefront w, DevScene scene) {
+  __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[CNT_SHADOW];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -338,7 +336,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_vol_shadow(Wavefront w, DevScen
         w.lsum[s] = make_float4(l.x + c4.x * tr.x, l.y + c4.y * tr.y, l.z + c4.z * tr.z, l.w);
       }
     }
-    wave_sum(&w.stats[ST_MEDIUM], steps);
+    block_count(&w.stats[ST_MEDIUM], steps, lds);
   }
 }
 

@@ -100,6 +100,53 @@ __device__ __forceinline__ void wave_count(unsigned long long* counter, bool pre
     atomicAdd(counter, (unsigned long long)__popcll(mask));
 }
 
+// Block-wide queue appends: thread t appends when pred[k], to counter k, for
+// up to N counters at once; one atomic per counter per block (the waves'
+// counts are combined in LDS).  Every thread of the block must call it with
+// the same N (it synchronises the block); lds holds at least N * (waves + 1)
+// words.
+template <int N>
+__device__ __forceinline__ void block_append(uint32_t* const (&ctr)[N], const bool (&pred)[N], uint32_t (&pos)[N],
+                                             uint32_t* lds) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long mask[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    mask[k] = __ballot(pred[k]);
+    if (lane == 0) lds[k * W + wave] = (uint32_t)__popcll(mask[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < uint32_t(N)) {
+    const int k = int(threadIdx.x);
+    uint32_t total = 0;
+    for (uint32_t v = 0; v < W; v++) {
+      uint32_t c = lds[k * W + v];
+      lds[k * W + v] = total;
+      total += c;
+    }
+    lds[N * W + k] = total ? atomicAdd(ctr[k], total) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) pos[k] = lds[N * W + k] + lds[k * W + wave] + (uint32_t)__popcll(mask[k] & below);
+  __syncthreads();
+}
+
+// Block-wide sum into a 64-bit statistics counter (one atomic per block).
+__device__ __forceinline__ void block_count(unsigned long long* counter, uint32_t v, uint32_t* lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const uint32_t wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  if (__lane_id() == 0) lds[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t k = 0; k < W; k++) t += lds[k];
+    if (t) atomicAdd(counter, t);
+  }
+  __syncthreads();
+}
+
 struct LaunchCfg {
   int blocks;        // persistent grid of the per-path kernels (regen, shade, classify)
   int trace_blocks;  // persistent grid of the traversal kernels (extend, shadow)

@@ -25,13 +25,13 @@ def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
-def random_params(seed):
+def random_params(seed, out_scale=2.0):
     p = G.init_params(seed)
     rng = np.random.default_rng(seed)
     for _, bo in G.OFFS:
         p[bo:bo + 48] += rng.normal(0, 0.1, 48).astype(np.float32)
     wo, _ = G.OFFS[-1]
-    p[wo:wo + 64 * 48] *= 8.0  # informative mixtures, not the near-uniform initial one
+    p[wo:wo + 64 * 48] *= out_scale  # informative mixtures, not the near-uniform initial one
     return p
 
 
@@ -52,7 +52,7 @@ def gpu_forward(p, feats):
 
 def test_forward_matches_oracle(hip_ctx):
     rng = np.random.default_rng(0)
-    p = random_params(1)
+    p = random_params(1, out_scale=8.0)
     feats = random_feats(rng, 1000)
     raw = gpu_forward(p, feats)
     _, Z = G.forward(p, feats, LO, HI)
@@ -85,13 +85,21 @@ def test_gradient_matches_oracle(hip_ctx):
     hip.check(hip.lib().rad_hip_guide_grad(_fp(p), _fp(bounds), recs.ctypes.data_as(C.c_void_p), n, C.c_float(wnorm),
                                            _fp(grad), C.byref(loss)))
     ref_loss, ref = G.loss_and_grad(p, feats, dirs, w.astype(np.float32), wnorm, LO, HI)
+    # the same oracle on FP16-rounded weights separates weight rounding from the rest
+    p16 = p.astype(np.float16).astype(np.float64)
+    _, ref16 = G.loss_and_grad(p16, feats, dirs, w.astype(np.float32), wnorm, LO, HI)
     print(f"loss gpu {loss.value:.6f} oracle {ref_loss:.6f}")
-    assert abs(loss.value - ref_loss) <= 1e-2 * abs(ref_loss)
+    errs = []
     for l, ((wo, bo), (nin, nout)) in enumerate(zip(G.OFFS, G.DIMS)):
         for lo_, hi_ in ((wo, bo), (bo, bo + nout)):
             e = np.linalg.norm(grad[lo_:hi_] - ref[lo_:hi_]) / (np.linalg.norm(ref[lo_:hi_]) + 1e-12)
-            print(f"layer {l} [{lo_}:{hi_}] rel err {e:.2e}")
-            assert e < 1e-2
+            e16 = np.linalg.norm(grad[lo_:hi_] - ref16[lo_:hi_]) / (np.linalg.norm(ref16[lo_:hi_]) + 1e-12)
+            o16 = np.linalg.norm(ref16[lo_:hi_] - ref[lo_:hi_]) / (np.linalg.norm(ref[lo_:hi_]) + 1e-12)
+            print(f"layer {l} [{lo_}:{hi_}] rel err {e:.2e} (vs fp16-weight oracle {e16:.2e}; "
+                  f"weight rounding alone {o16:.2e})")
+            errs.append(e)
+    assert abs(loss.value - ref_loss) <= 1e-2 * abs(ref_loss)
+    assert max(errs) < 1e-2, errs
 
 
 def test_mixture_probes(hip_ctx):

@@ -141,7 +141,7 @@ def test_json_scene_end_to_end(hip_ctx, tmp_path):
 def test_autotune_and_launch_independence(hip_ctx):
     """The autotuner picks a launch configuration the scene keeps; images do
     not depend on slots, grid sizes or material sorting."""
-    b = scenes.many_balls(seed=0, width=128, height=128, spp=16, n_spheres=24)
+    b = scenes.many_balls(seed=0, width=256, height=256, spp=16, n_spheres=24)
     s = hip.Scene.from_builder(hip_ctx, b)
     rep = s.autotune(b.render_desc(spp=16, seed=1))
     print("trials", [(t.path_slots, t.blocks_per_cu, t.trace_blocks_per_cu, round(t.ms, 2))
