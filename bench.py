@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     p.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     p.add_argument("--unsorted", action="store_true", help="shade with one kernel (no material sort)")
+    p.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     p.add_argument("--builder", choices=["host", "device"], default="host", help="BVH builder")
     p.add_argument("--no-serving", action="store_true", help="skip the serving-latency measurement")
     p.add_argument("--guide-scale", type=float, default=1.0, help="image scale of the guiding workload")
@@ -290,6 +291,16 @@ def main():
     info = scene.info()
     W, H = b.camera["width"], b.camera["height"]
     spp = b.sampler["sample_count"]
+    tune = None
+    if not args.no_autotune:
+        # launch configuration picked by timing short renders (images do not depend on it)
+        rep = scene.autotune(b.render_desc(spp=max(1, min(spp, (8 << 20) // (W * H))), seed=1))
+        tune = {"path_slots": rep.path_slots, "blocks_per_cu": rep.blocks_per_cu,
+                "trace_blocks_per_cu": rep.trace_blocks_per_cu,
+                "trials": [[t.path_slots, t.blocks_per_cu, t.trace_blocks_per_cu, round(t.ms, 3)]
+                           for t in rep.trial[:rep.trials]]}
+        print(f"# autotune: slots {rep.path_slots}, blocks/CU {rep.blocks_per_cu}, "
+              f"trace blocks/CU {rep.trace_blocks_per_cu}", file=sys.stderr, flush=True)
     rd = b.render_desc(spp=spp, seed=1, flags=F.RAD_RENDER_PROFILE | (F.RAD_RENDER_UNSORTED if args.unsorted else 0))
     print(f"# scene {name}: {info.primitive_count} primitives, {info.node_count} BVH nodes, "
           f"build {info.bvh_build_ms:.0f} ms, generation {gen_s:.1f} s, upload+build {setup_s:.1f} s",
@@ -357,6 +368,7 @@ def main():
             "bvh_build_ms": info.bvh_build_ms,
             "bvh_builder": args.builder,
             "material_sorted": not args.unsorted,
+            "autotune": tune,
         },
     }
     if args.config == "large_mesh" and not args.no_guiding:

@@ -136,8 +136,10 @@ __global__ void __launch_bounds__(256) k_guide_infer(GuideNet net, const float* 
 }
 
 // Per-record loss gradient with respect to the 40 raw outputs, for
-// loss_j = wn_j * (-log q(dir_j)), wn_j = min(w_j * wnorm, kMaxWeight).
-constexpr float kMaxWeight = 64.f;
+// loss_j = wn_j * (-log q(dir_j)), wn_j = min(w_j * wnorm, wmax).  The output
+// gradients are kept in FP16 multiplied by kDzScale (undone in the weight
+// gradient), so large sample weights stay inside the FP16 range.
+constexpr float kDzScale = 1.f / 32.f;
 
 __device__ inline float record_loss_grad(const float* raw, f3 dir, float wn, float* d) {
   float a[kLobes], logv[kLobes], kap[kLobes], c[kLobes], len[kLobes];
@@ -185,7 +187,7 @@ __device__ inline float record_loss_grad(const float* raw, f3 dir, float wn, flo
 // (summed over records, not yet divided by the batch size) are added to
 // grad[kParams]; the summed loss to *loss.
 __global__ void __launch_bounds__(256) k_guide_train(GuideNet net, const TrainRecord* recs, const uint32_t* batch,
-                                                     uint32_t B, float wnorm, float* grad, float* loss) {
+                                                     uint32_t B, float wnorm, float wmax, float* grad, float* loss) {
   __shared__ __align__(16) _Float16 sW[kWeights];
   __shared__ float sB[kBiases];
   __shared__ __align__(16) _Float16 sAct[kIn * 64 + 4 * kHid * 64];   // layer inputs A_0..A_4, [feature][record]
@@ -230,10 +232,10 @@ __global__ void __launch_bounds__(256) k_guide_train(GuideNet net, const TrainRe
     for (int k = 0; k < kOut; k++) d[k] = 0.f;
     float l = 0.f;
     if (valid && rec.w > 0.f) {
-      float wn = fminf(rec.w * wnorm, kMaxWeight);
+      float wn = fminf(rec.w * wnorm, wmax);
       l = record_loss_grad(sOut + rloc * kOut, mk(rec.dir[0], rec.dir[1], rec.dir[2]), wn, d);
     }
-    for (int k = 0; k < kOut; k++) sDz[dzOff(4) + k * 64 + rloc] = (_Float16)d[k];
+    for (int k = 0; k < kOut; k++) sDz[dzOff(4) + k * 64 + rloc] = (_Float16)(d[k] * kDzScale);
     if (l != 0.f) atomicAdd(loss, l);
   }
   __syncthreads();
@@ -308,13 +310,13 @@ __global__ void __launch_bounds__(256) k_guide_train(GuideNet net, const TrainRe
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         float x = acc[j];
-        if (x != 0.f) atomicAdd(&grad[WeightOffset(l) + (16 * to + 4 * g + j) * nin + 16 * ti + v], x);
+        if (x != 0.f) atomicAdd(&grad[WeightOffset(l) + (16 * to + 4 * g + j) * nin + 16 * ti + v], x * (1.f / kDzScale));
       }
     }
     for (int o = threadIdx.x; o < nout; o += blockDim.x) {
       float sacc = 0.f;
       for (int r = 0; r < 64; r++) sacc += float(sDz[dzOff(l) + o * 64 + r]);
-      if (sacc != 0.f) atomicAdd(&grad[BiasOffset(l) + o], sacc);
+      if (sacc != 0.f) atomicAdd(&grad[BiasOffset(l) + o], sacc * (1.f / kDzScale));
     }
   }
 }
@@ -431,9 +433,10 @@ void LaunchInfer(const GuideNet& net, const float* feats, const uint32_t* n_dev,
   hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, s, net, feats, n_dev, n_host, slot_of, raw);
 }
 
-void LaunchTrain(const GuideNet& net, const TrainRecord* recs, const uint32_t* batch, uint32_t B, float wnorm, float* grad,
-                 float* loss, hipStream_t s) {
-  if (B) hipLaunchKernelGGL(k_guide_train, dim3((B + 63) / 64), dim3(256), 0, s, net, recs, batch, B, wnorm, grad, loss);
+void LaunchTrain(const GuideNet& net, const TrainRecord* recs, const uint32_t* batch, uint32_t B, float wnorm, float wmax,
+                 float* grad, float* loss, hipStream_t s) {
+  if (B)
+    hipLaunchKernelGGL(k_guide_train, dim3((B + 63) / 64), dim3(256), 0, s, net, recs, batch, B, wnorm, wmax, grad, loss);
 }
 
 void GuideState::Step(float lr, float gscale, hipStream_t s) {
@@ -529,7 +532,7 @@ rad_status rad_hip_guide_grad(const float* params, const float* bounds, const ra
     dr.alloc(std::max<size_t>(16, size_t(n) * sizeof(TrainRecord)));
     HipCheck(hipMemcpy(dr.ptr, recs, size_t(n) * sizeof(TrainRecord), hipMemcpyHostToDevice), "copy records");
     HipCheck(hipMemset(st.scal.ptr, 0, 4), "clear loss");
-    LaunchTrain(net, dr.as<TrainRecord>(), nullptr, n, wnorm, st.grad.as<float>(), st.scal.as<float>(), nullptr);
+    LaunchTrain(net, dr.as<TrainRecord>(), nullptr, n, wnorm, 64.f, st.grad.as<float>(), st.scal.as<float>(), nullptr);
     HipCheck(hipGetLastError(), "guide train launch");
     HipCheck(hipMemcpy(grad, st.grad.ptr, kParams * 4, hipMemcpyDeviceToHost), "copy gradient");
     float l = 0.f;

@@ -23,8 +23,8 @@ struct GuideState {
 void InitParams(std::vector<float>& p, uint32_t seed);
 void LaunchInfer(const GuideNet& net, const float* feats, const uint32_t* n_dev, uint32_t n_host, const uint32_t* slot_of,
                  _Float16* raw, int blocks, hipStream_t s);
-void LaunchTrain(const GuideNet& net, const TrainRecord* recs, const uint32_t* batch, uint32_t B, float wnorm, float* grad,
-                 float* loss, hipStream_t s);
+void LaunchTrain(const GuideNet& net, const TrainRecord* recs, const uint32_t* batch, uint32_t B, float wnorm, float wmax,
+                 float* grad, float* loss, hipStream_t s);
 void LaunchBatch(uint32_t* batch, uint32_t B, uint32_t n, uint32_t seed, hipStream_t s);
 void LaunchPdf(const float* raw, const float* dirs, uint32_t n, float* pdf, hipStream_t s);
 void LaunchSample(const float* raw, const float* xi, uint32_t n, float* out, hipStream_t s);

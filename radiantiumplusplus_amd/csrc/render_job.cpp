@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -59,6 +60,26 @@ constexpr uint32_t kTrainBatch = 8192;           // records per training step
 constexpr uint32_t kTrainSteps = 96;             // training steps after each training pass
 constexpr float kTrainLr = 4e-3f;
 constexpr float kGuideAlpha = 0.5f;
+constexpr float kTrainWmax = 64.f;               // cap of a record's normalised weight
+
+// Training hyper-parameters; RAD_GUIDE_STEPS / RAD_GUIDE_LR / RAD_GUIDE_ALPHA
+// in the environment override the defaults above.
+uint32_t TrainSteps() {
+  const char* e = std::getenv("RAD_GUIDE_STEPS");
+  return e ? uint32_t(std::max(1, std::atoi(e))) : kTrainSteps;
+}
+float TrainLr() {
+  const char* e = std::getenv("RAD_GUIDE_LR");
+  return e ? float(std::atof(e)) : kTrainLr;
+}
+float TrainWmax() {
+  const char* e = std::getenv("RAD_GUIDE_WMAX");
+  return e ? float(std::atof(e)) : kTrainWmax;
+}
+float GuideAlpha() {
+  const char* e = std::getenv("RAD_GUIDE_ALPHA");
+  return e ? std::min(0.95f, std::max(0.05f, float(std::atof(e)))) : kGuideAlpha;
+}
 size_t Align(size_t v) { return (v + 255) & ~size_t(255); }
 
 struct JobShape {
@@ -136,11 +157,14 @@ void Train(rad_job* job, hipStream_t st) {
     guide::GuideState& gs = *job->guide;
     const float wnorm = float(nrec) / wsum;
     const guide::GuideNet net = w.net;
-    for (uint32_t k = 0; k < kTrainSteps; k++) {
+    const uint32_t steps = TrainSteps();
+    const float lr = TrainLr();
+    const float wmax = TrainWmax();
+    for (uint32_t k = 0; k < steps; k++) {
       HipCheck(hipMemsetAsync(gs.scal.ptr, 0, 4, st), "clear loss");
       guide::LaunchBatch(job->batch, kTrainBatch, nrec, gs.steps * 7919u + k + 1u, st);
-      guide::LaunchTrain(net, w.recs, job->batch, kTrainBatch, wnorm, gs.grad.as<float>(), gs.scal.as<float>(), st);
-      gs.Step(kTrainLr, 1.f / float(kTrainBatch), st);
+      guide::LaunchTrain(net, w.recs, job->batch, kTrainBatch, wnorm, wmax, gs.grad.as<float>(), gs.scal.as<float>(), st);
+      gs.Step(lr, 1.f / float(kTrainBatch), st);
     }
     HipCheck(hipGetLastError(), "guide training launch");
     float loss = 0.f;
@@ -187,10 +211,10 @@ void RunJob(rad_job* job) {
       w.flags = baseFlags;
       if (job->desc.guiding == 2) {
         w.flags |= WF_GUIDE;
-        w.net.alpha = kGuideAlpha;
+        w.net.alpha = GuideAlpha();
       } else if (job->desc.guiding == 1) {
         const bool trained = job->guide->steps > 0;
-        w.net = job->guide->Net(lo, hi, trained ? kGuideAlpha : 0.f);
+        w.net = job->guide->Net(lo, hi, trained ? GuideAlpha() : 0.f);
         if (trained) w.flags |= WF_GUIDE;
         if (job->pass_train[pi]) w.flags |= WF_RECORD;
       }
@@ -392,7 +416,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
     if (desc.guiding == 2) {  // snapshot: concurrent training of the scene's guide cannot touch this render
       HipCheck(hipMemcpy(wh, job->guide->wh.ptr, guide::kWeights * 2, hipMemcpyDeviceToDevice), "guide snapshot");
       HipCheck(hipMemcpy(bias, job->guide->bias.ptr, guide::kBiases * 4, hipMemcpyDeviceToDevice), "guide snapshot");
-      w.net = job->guide->Net(scene->host.world_lo, scene->host.world_hi, kGuideAlpha);
+      w.net = job->guide->Net(scene->host.world_lo, scene->host.world_hi, GuideAlpha());
       w.net.w = wh;
       w.net.b = bias;
     }
