@@ -62,6 +62,7 @@ struct GuideNet {
   const float* b;     // kBiases
   float lo[3], inv_ext[3];  // scene bounds for the position encoding
   float alpha;        // probability of sampling the guide (vs the BSDF)
+  float min_rough;    // glossy BSDFs with a smaller GGX/Beckmann alpha are not guided
 };
 
 // Host-owned state of a scene's guide (weights, Adam moments, buffers).

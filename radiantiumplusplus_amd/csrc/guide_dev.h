@@ -119,9 +119,13 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u
 // would need the BSDF's relative IOR), has no delta lobe, and its sampler's
 // density equals its pdf: with the reference's visible-normal azimuth quirk
 // on, only BSDFs that do not sample visible normals qualify.
-__device__ __forceinline__ bool guidable(const BsdfRec& b, bool quirks) {
+// Glossy BSDFs below min_rough are left to BSDF sampling, which is already
+// close to their product distribution.
+__device__ __forceinline__ bool guidable(const BsdfRec& b, bool quirks, float min_rough) {
   if (b.lobes & (LOBE_DELTA | LOBE_TRANSMISSION)) return false;
   if (b.type == BSDF_DIFFUSE) return true;
+  const float rough = b.type == BSDF_DISNEY ? b.roughness * b.roughness : fminf(b.alpha_u, b.alpha_v);
+  if (rough < min_rough) return false;
   if (!quirks) return true;
   return (b.type == BSDF_ROUGH_METAL || b.type == BSDF_ROUGH_GLASS) && !b.sample_visible;
 }

@@ -33,13 +33,46 @@ __device__ __forceinline__ void load_weights_lds(_Float16* sW, float* sB, const 
   for (int i = threadIdx.x; i < kBiases; i += blockDim.x) sB[i] = b[i];
 }
 
+// Fragment-major copy for inference: the 16x16 A-operand tile (t, c) of
+// layer l is 64 lanes x 4 halves, stored contiguously in lane order, so each
+// MFMA operand is one conflict-free 8-byte LDS read per lane.  (Row-major
+// rows of 64 or 80 halves put the 16 rows a wave reads on one or four
+// banks.)
+__device__ __forceinline__ int frag_base(int l, int t, int c) {
+  return HalfOffset(l) + (t * (LayerIn(l) / 16) + c) * 256;
+}
+
+__device__ __forceinline__ void load_weights_lds_frag(_Float16* sW, float* sB, const _Float16* w, const float* b) {
+  const uint2* src = reinterpret_cast<const uint2*>(w);
+  for (int i = threadIdx.x; i < kWeights / 4; i += blockDim.x) {
+    const int h = 4 * i;
+    int l = 0;
+#pragma unroll
+    for (int k = 1; k < kLayers; k++) l += h >= HalfOffset(k) ? 1 : 0;
+    const int nin = LayerIn(l);
+    const int r = h - HalfOffset(l), o = r / nin, in = r - o * nin;
+    const int lane = (o & 15) + 16 * ((in & 15) >> 2);
+    *reinterpret_cast<uint2*>(sW + frag_base(l, o >> 4, in >> 4) + 4 * lane) = src[i];
+  }
+  for (int i = threadIdx.x; i < kBiases; i += blockDim.x) sB[i] = b[i];
+}
+
+// A operand (16 rows 16t.., 16 columns 16c..) of layer l for this lane.
+template <bool FRAG>
+__device__ __forceinline__ h4 weight_operand(const _Float16* sW, int l, int t, int c, int lane) {
+  if (FRAG) return *reinterpret_cast<const h4*>(sW + frag_base(l, t, c) + 4 * lane);
+  const int v = lane & 15, g = lane >> 4;
+  return *reinterpret_cast<const h4*>(sW + HalfOffset(l) + (16 * t + v) * LayerIn(l) + 16 * c + 4 * g);
+}
+
 __device__ __forceinline__ int bias_base(int l) { return l * kHid; }
 
 // Forward pass of one 16-vertex tile.  fs: the lane's vertex scalars.
 // act[l][t]: post-ReLU outputs of hidden layer l (l = 0..3), tile t, in
 // B-operand form; out[t]: the 48 raw outputs (3 tiles).  When xs != nullptr
 // the layer-0 B operands (the one-blob inputs) are stored there as
-// xs[chunk][lane].
+// xs[chunk][lane].  FRAG: sW holds the fragment-major copy.
+template <bool FRAG>
 __device__ __forceinline__ void forward_tile(const _Float16* sW, const float* sB, const FeatScalars& fs, h4 act[4][4],
                                              f4v out[3], h4* xs) {
   const int lane = __lane_id();
@@ -55,7 +88,7 @@ __device__ __forceinline__ void forward_tile(const _Float16* sW, const float* sB
     if (xs) xs[c * 64 + lane] = b;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      h4 a = *reinterpret_cast<const h4*>(sW + HalfOffset(0) + (16 * t + v) * kIn + 16 * c + 4 * g);
+      h4 a = weight_operand<FRAG>(sW, 0, t, c, lane);
       acc[t] = mfma16(a, b, acc[t]);
     }
   }
@@ -72,7 +105,7 @@ __device__ __forceinline__ void forward_tile(const _Float16* sW, const float* sB
     for (int c = 0; c < 4; c++) {
 #pragma unroll
       for (int t = 0; t < 4; t++) {
-        h4 a = *reinterpret_cast<const h4*>(sW + HalfOffset(l) + (16 * t + v) * kHid + 16 * c + 4 * g);
+        h4 a = weight_operand<FRAG>(sW, l, t, c, lane);
         acc[t] = mfma16(a, act[l - 1][c], acc[t]);
       }
     }
@@ -88,7 +121,7 @@ __device__ __forceinline__ void forward_tile(const _Float16* sW, const float* sB
   for (int c = 0; c < 4; c++) {
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-      h4 a = *reinterpret_cast<const h4*>(sW + HalfOffset(4) + (16 * t + v) * kHid + 16 * c + 4 * g);
+      h4 a = weight_operand<FRAG>(sW, 4, t, c, lane);
       out[t] = mfma16(a, act[3][c], out[t]);
     }
   }
@@ -104,7 +137,7 @@ __global__ void __launch_bounds__(256) k_guide_infer(GuideNet net, const float* 
                                                      const uint32_t* slot_of, _Float16* raw) {
   __shared__ __align__(16) _Float16 sW[kWeights];
   __shared__ float sB[kBiases];
-  load_weights_lds(sW, sB, net.w, net.b);
+  load_weights_lds_frag(sW, sB, net.w, net.b);
   __syncthreads();
   const uint32_t n = n_dev ? *n_dev : n_host;
   const int lane = __lane_id();
@@ -121,7 +154,7 @@ __global__ void __launch_bounds__(256) k_guide_infer(GuideNet net, const float* 
     FeatScalars fs = feat_scalars(f9, net.lo, net.inv_ext);
     h4 act[4][4];
     f4v out[3];
-    forward_tile(sW, sB, fs, act, out, nullptr);
+    forward_tile<true>(sW, sB, fs, act, out, nullptr);
     if (i < n) {
       size_t base = size_t(slot_of ? slot_of[i] : i) * kOut;
 #pragma unroll
@@ -207,7 +240,7 @@ __global__ void __launch_bounds__(256) k_guide_train(GuideNet net, const TrainRe
   FeatScalars fs = feat_scalars(f9, net.lo, net.inv_ext);
   h4 act[4][4];
   f4v out[3];
-  forward_tile(sW, sB, fs, act, out, nullptr);
+  forward_tile<false>(sW, sB, fs, act, out, nullptr);
   auto actOff = [](int l) { return l == 0 ? 0 : kIn * 64 + (l - 1) * kHid * 64; };
   auto dzOff = [](int l) { return l * kHid * 64; };
   // layer inputs to LDS: X (recomputed one-blob values) and hidden activations
@@ -415,7 +448,7 @@ void GuideState::Upload(const std::vector<float>& p) {
   steps = 0;
 }
 
-GuideNet GuideState::Net(const float lo[3], const float hi[3], float alpha) const {
+GuideNet GuideState::Net(const float lo[3], const float hi[3], float alpha, float min_rough) const {
   GuideNet n;
   n.w = wh.as<_Float16>();
   n.b = bias.as<float>();
@@ -425,6 +458,7 @@ GuideNet GuideState::Net(const float lo[3], const float hi[3], float alpha) cons
     n.inv_ext[i] = e > 0.f ? 1.f / e : 0.f;
   }
   n.alpha = alpha;
+  n.min_rough = min_rough;
   return n;
 }
 

@@ -16,7 +16,7 @@ struct GuideState {
   uint64_t records_used = 0;
   GuideState();
   void Upload(const std::vector<float>& p);  // replace the weights, reset Adam
-  GuideNet Net(const float lo[3], const float hi[3], float alpha) const;
+  GuideNet Net(const float lo[3], const float hi[3], float alpha, float min_rough = 0.f) const;
   void Step(float lr, float gscale, hipStream_t s);  // Adam on the accumulated gradient
 };
 

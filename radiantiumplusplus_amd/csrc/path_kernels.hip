@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
         bool stop = (w.max_depth >= 0 && int32_t(depth) + 1 >= w.max_depth) || sr.light >= 0;
         if (!stop) {
           const BsdfRec& b = scene.bsdfs[sr.bsdf];
-          const bool guided = GUIDE && guide::guidable(b, quirks);
+          const bool guided = GUIDE && guide::guidable(b, quirks, w.net.min_rough);
           guide::Mixture mx;
           if (guided && guideOn) mx = guide::decode(w.graw + size_t(s) * guide::kOut);
           const float a = (guided && guideOn) ? alpha : 0.f;
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
       uint32_t geom = w.hit_geom[s];
       if (geom != 0xffffffffu) {
         const ShapeRec& sr = scene.shapes[geom];
-        if (sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks)) {
+        if (sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks, w.net.min_rough)) {
           float4 ro = w.ray_o[s], rd = w.ray_d[s];
           float4 hh = w.hit[s];
           f3 d = mk(rd.x, rd.y, rd.z);

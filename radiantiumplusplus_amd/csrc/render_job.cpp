@@ -61,6 +61,7 @@ constexpr uint32_t kTrainSteps = 96;             // training steps after each tr
 constexpr float kTrainLr = 4e-3f;
 constexpr float kGuideAlpha = 0.5f;
 constexpr float kTrainWmax = 64.f;               // cap of a record's normalised weight
+constexpr float kGuideMinRough = 0.f;            // glossy BSDFs below this alpha are not guided
 
 // Training hyper-parameters; RAD_GUIDE_STEPS / RAD_GUIDE_LR / RAD_GUIDE_ALPHA
 // in the environment override the defaults above.
@@ -75,6 +76,10 @@ float TrainLr() {
 float TrainWmax() {
   const char* e = std::getenv("RAD_GUIDE_WMAX");
   return e ? float(std::atof(e)) : kTrainWmax;
+}
+float GuideMinRough() {
+  const char* e = std::getenv("RAD_GUIDE_MINROUGH");
+  return e ? float(std::atof(e)) : kGuideMinRough;
 }
 float GuideAlpha() {
   const char* e = std::getenv("RAD_GUIDE_ALPHA");
@@ -212,9 +217,10 @@ void RunJob(rad_job* job) {
       if (job->desc.guiding == 2) {
         w.flags |= WF_GUIDE;
         w.net.alpha = GuideAlpha();
+        w.net.min_rough = GuideMinRough();
       } else if (job->desc.guiding == 1) {
         const bool trained = job->guide->steps > 0;
-        w.net = job->guide->Net(lo, hi, trained ? GuideAlpha() : 0.f);
+        w.net = job->guide->Net(lo, hi, trained ? GuideAlpha() : 0.f, GuideMinRough());
         if (trained) w.flags |= WF_GUIDE;
         if (job->pass_train[pi]) w.flags |= WF_RECORD;
       }
@@ -416,7 +422,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
     if (desc.guiding == 2) {  // snapshot: concurrent training of the scene's guide cannot touch this render
       HipCheck(hipMemcpy(wh, job->guide->wh.ptr, guide::kWeights * 2, hipMemcpyDeviceToDevice), "guide snapshot");
       HipCheck(hipMemcpy(bias, job->guide->bias.ptr, guide::kBiases * 4, hipMemcpyDeviceToDevice), "guide snapshot");
-      w.net = job->guide->Net(scene->host.world_lo, scene->host.world_hi, GuideAlpha());
+      w.net = job->guide->Net(scene->host.world_lo, scene->host.world_hi, GuideAlpha(), GuideMinRough());
       w.net.w = wh;
       w.net.b = bias;
     }

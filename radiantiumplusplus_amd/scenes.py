@@ -647,9 +647,14 @@ def volume_box(seed=2, width=1024, height=1024, spp=64, res=256, max_depth=32, r
     return b
 
 
-def guiding_room(seed=3, width=1280, height=720, spp=256, n_boxes=32, max_depth=-1):
+def guiding_room(seed=3, width=1280, height=720, spp=256, n_boxes=32, max_depth=-1, light_x=9.0, light_size=1.0,
+                 light_radiance=32000.0):
     """Difficult indirect lighting: a 6x3x6 closed room lit only through a
-    0.5x0.5 gap in one wall by an exterior area light, with n glossy GGX cubes."""
+    0.5x0.5 gap in the x = 3 wall by an exterior area light (light_size
+    square, centred at x = light_x, facing the gap), with n glossy GGX cubes.
+    The default light, 1x1 at 6 m behind the gap, throws a beam about 2 m wide
+    onto the far wall: that spot lights the rest of the room, which BSDF
+    sampling finds rarely."""
     rng = np.random.default_rng(seed)
     b = SceneBuilder()
     wall = b.diffuse(0.65)
@@ -671,8 +676,9 @@ def guiding_room(seed=3, width=1280, height=720, spp=256, n_boxes=32, max_depth=
         m = b.rough_metal(alpha=rng.uniform(0.08, 0.3), reflectance=rng.uniform(0.5, 0.95, 3))
         b.cube(to_world=trs(translate=(cx, sy, cz), axis=(0, 1, 0), angle=rng.uniform(0, 90), scale=(sx, sy, sz)),
                bsdf=m)
-    light = b.rectangle(to_world=trs(translate=(4.5, 1.5, 0), axis=(0, 1, 0), angle=-90, scale=(1.0, 1.0, 1.0)))
-    b.area_light(light, 60.0)
+    h = 0.5 * light_size
+    light = b.rectangle(to_world=trs(translate=(light_x, 1.5, 0), axis=(0, 1, 0), angle=-90, scale=(h, h, 1.0)))
+    b.area_light(light, light_radiance)
     b.set_camera(width, height, 70.0, origin=(-2.6, 1.6, -2.6), target=(1.0, 1.0, 1.0), far=1000.0)
     b.set_render("path", spp=spp, max_depth=max_depth, rr_depth=3, seed=seed)
     return b

@@ -185,14 +185,19 @@ def test_guided_render_is_unbiased(hip_ctx):
 
 def test_trained_guide_reduces_error(hip_ctx):
     """At equal spp the trained guide's error against a converged unguided
-    reference is below that of BSDF sampling alone (quirks off, so the GGX
-    boxes are guided too)."""
+    reference is well below that of BSDF sampling alone (quirks off, so the
+    GGX boxes are guided too).  The guide lives in scene space, so it is
+    trained on a 640x360 render of the room (about 6M training records) and
+    evaluated on a 64x36 one; measured ratio about 0.5."""
     b = small_room(64, 36)
     fl = F.RAD_RENDER_FIX_QUIRKS
     s = hip.Scene.from_builder(hip_ctx, b)
     ref, _, _ = render(hip_ctx, s, b, 16384, seed=10, flags=fl)
     # train once (guiding = 1), then compare fixed-weight guiding (2) with plain sampling at 64 spp
-    render(hip_ctx, s, b, 256, seed=11, guiding=1, flags=fl)
+    rd = b.render_desc(spp=256, seed=11, flags=fl)
+    rd.guiding, rd.width, rd.height = 1, 640, 360
+    _, _, st = s.render(rd)
+    assert st.guide_steps > 0 and st.guide_records > 1000000
     errs_p, errs_g = [], []
     for k in range(4):
         fp, _, _ = render(hip_ctx, s, b, 64, seed=100 + k, guiding=0, flags=fl)
@@ -201,4 +206,4 @@ def test_trained_guide_reduces_error(hip_ctx):
         errs_g.append(np.mean((fg - ref) ** 2 / (ref ** 2 + 1e-2)))
     mp, mg = np.mean(errs_p), np.mean(errs_g)
     print(f"relMSE plain {mp:.4e} guided {mg:.4e} ratio {mg / mp:.3f}")
-    assert mg < mp
+    assert mg < 0.75 * mp
