@@ -29,21 +29,27 @@ __device__ inline FeatScalars feat_scalars(const float* f9, const float lo[3], c
   return r;
 }
 
-// One-blob value of input feature f: a unit-width triangle kernel centred on
-// the scalar's position among the bins.
-__device__ __forceinline__ float one_blob(const FeatScalars& fs, int f) {
-  int dim, bin, bins;
-  if (f < 3 * kPosBins) {
-    dim = f / kPosBins;
-    bin = f % kPosBins;
+// One-blob value of input feature 16 c + r (0 <= r < 16): a unit-width
+// triangle kernel centred on the scalar's position among the bins.
+// Features come in 16-wide chunks: chunks 0-2 are the 16 bins of position
+// x, y, z; chunks 3 and 4 the 8 bins of (normal polar, azimuth) and
+// (direction polar, azimuth).  Callers pass the chunk as a constant, so the
+// scalar is picked without an indexed (scratch) access.
+static_assert(kPosBins == 16 && kDirBins == 8 && kIn == 80, "one-blob chunk layout");
+__device__ __forceinline__ float one_blob(const FeatScalars& fs, int c, int r) {
+  float sv;
+  int bins, bin;
+  if (c < 3) {
+    sv = c == 0 ? fs.s[0] : c == 1 ? fs.s[1] : fs.s[2];
     bins = kPosBins;
+    bin = r;
   } else {
-    int g = f - 3 * kPosBins;
-    dim = 3 + g / kDirBins;
-    bin = g % kDirBins;
+    const bool hi = r >= kDirBins;
+    sv = c == 3 ? (hi ? fs.s[4] : fs.s[3]) : (hi ? fs.s[6] : fs.s[5]);
     bins = kDirBins;
+    bin = r & (kDirBins - 1);
   }
-  float x = fs.s[dim] * float(bins) - (float(bin) + 0.5f);
+  float x = sv * float(bins) - (float(bin) + 0.5f);
   return fmaxf(0.f, 1.f - fabsf(x));
 }
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 
 #include "guide_dev.h"
+#include "guide_mlp.h"
 #include "shading_dev.h"
 #include "wavefront.h"
 
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
   }
 }
 
-__global__ void k_reset(uint32_t* cnt, int out) {
+__global__ void k_reset(uint32_t* cnt, uint32_t* gcnt, int out) {
   if (threadIdx.x == 0) {
     cnt[out] = 0;
     cnt[CNT_DEAD] = 0;
@@ -135,6 +136,7 @@ __global__ void k_reset(uint32_t* cnt, int out) {
     cnt[CNT_FETCH_EXT] = 0;
     cnt[CNT_FETCH_SH] = 0;
     for (int k = 0; k < 4; k++) cnt[CNT_MAT + k] = 0;
+    if (gcnt) gcnt[G_QUEUE] = 0;
   }
 }
 
@@ -361,8 +363,14 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   }
 }
 
-// Guided vertices of this iteration: surface hits whose BSDF the guide may
-// sample; their network inputs go to the guide queue for k_guide_infer.
+// Guided vertices of this iteration (surface hits whose BSDF the guide may
+// sample) -> the guide queue, for the unsorted shading path.
+__device__ __forceinline__ bool guided_hit(const Wavefront& w, const DevScene& scene, uint32_t geom, bool quirks) {
+  if (geom == 0xffffffffu) return false;
+  const ShapeRec& sr = scene.shapes[geom];
+  return sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks, w.net.min_rough);
+}
+
 __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
@@ -372,67 +380,108 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
     uint32_t i = base + threadIdx.x;
     bool g = false;
     uint32_t s = 0;
-    f3 p = splat(0.f), gn = splat(0.f), wi = splat(0.f);
     if (i < n) {
       s = w.queue[in][i];
-      uint32_t geom = w.hit_geom[s];
-      if (geom != 0xffffffffu) {
-        const ShapeRec& sr = scene.shapes[geom];
-        if (sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks, w.net.min_rough)) {
-          float4 ro = w.ray_o[s], rd = w.ray_d[s];
-          float4 hh = w.hit[s];
-          f3 d = mk(rd.x, rd.y, rd.z);
-          p = d * hh.x + mk(ro.x, ro.y, ro.z);
-          gn = guide::geometric_normal(scene, p, __float_as_uint(hh.w), geom);
-          wi = -d;
-          g = true;
-        }
-      }
+      g = guided_hit(w, scene, w.hit_geom[s], quirks);
     }
     uint32_t* c1[1] = {&w.gcnt[G_QUEUE]};
     bool p1[1] = {g};
     uint32_t r1[1];
     block_append<1>(c1, p1, r1, lds);
-    const uint32_t pos = r1[0];
-    if (g) {
-      w.gq[pos] = s;
-      float* f = w.gfeat + size_t(pos) * guide::kFeat;
-      f[0] = p.x, f[1] = p.y, f[2] = p.z;
-      f[3] = gn.x, f[4] = gn.y, f[5] = gn.z;
-      f[6] = wi.x, f[7] = wi.y, f[8] = wi.z;
+    if (g) w.gq[r1[0]] = s;
+  }
+}
+
+// Network inference for the guide queue: the inputs (hit position, geometric
+// normal, outgoing direction) are formed from the slot's ray and hit, and
+// the 48 raw outputs go to graw[slot].
+__global__ void __launch_bounds__(256) k_guide_infer_hits(Wavefront w, DevScene scene) {
+  __shared__ __align__(16) _Float16 sW[guide::kWeights];
+  __shared__ float sB[guide::kBiases];
+  guide::load_weights_lds_frag(sW, sB, w.net.w, w.net.b);
+  __syncthreads();
+  const uint32_t n = w.gcnt[G_QUEUE];
+  const int lane = __lane_id();
+  const int v = lane & 15, g = lane >> 4;
+  const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  for (uint32_t tile = blockIdx.x * waves + wave; tile * 16 < n; tile += gridDim.x * waves) {
+    const uint32_t i = tile * 16 + v;
+    float f9[guide::kFeat];
+    uint32_t s = 0;
+    if (i < n) {
+      s = w.gq[i];
+      float4 ro = w.ray_o[s], rd = w.ray_d[s], hh = w.hit[s];
+      f3 d = mk(rd.x, rd.y, rd.z);
+      f3 p = d * hh.x + mk(ro.x, ro.y, ro.z);
+      f3 gn = guide::geometric_normal(scene, p, __float_as_uint(hh.w), w.hit_geom[s]);
+      f9[0] = p.x, f9[1] = p.y, f9[2] = p.z;
+      f9[3] = gn.x, f9[4] = gn.y, f9[5] = gn.z;
+      f9[6] = -d.x, f9[7] = -d.y, f9[8] = -d.z;
+    } else {
+#pragma unroll
+      for (int k = 0; k < guide::kFeat; k++) f9[k] = 0.f;
+    }
+    guide::FeatScalars fs = guide::feat_scalars(f9, w.net.lo, w.net.inv_ext);
+    guide::h4 act[4][4];
+    guide::f4v out[3];
+    guide::forward_tile<true>(sW, sB, fs, act, out, nullptr);
+    if (i < n) {
+      _Float16* dst = w.graw + size_t(s) * guide::kOut + 4 * g;
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        guide::h4 o;
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = (_Float16)out[t][j];
+        *reinterpret_cast<guide::h4*>(dst + 16 * t) = o;
+      }
     }
   }
 }
 
 // Partition the extend queue by material (misses go with the diffuse bucket)
 // so each shade kernel runs one BSDF's code.
+// GUIDE: guided vertices also go to the guide queue.
+template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x;
+  const bool quirks = (w.flags & WF_QUIRKS) != 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     int key = -1;
     uint32_t s = 0;
+    bool gd = false;
     if (i < n) {
       s = w.queue[in][i];
       uint32_t geom = w.hit_geom[s];
       key = geom == 0xffffffffu ? BSDF_DIFFUSE : scene.bsdfs[scene.shapes[geom].bsdf].type;
+      if (GUIDE) gd = guided_hit(w, scene, geom, quirks);
     }
-    uint32_t* c4[4] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3]};
-    bool p4[4] = {key == 0, key == 1, key == 2, key == 3};
-    uint32_t r4[4];
-    block_append<4>(c4, p4, r4, lds);
-    if (key >= 0) {
-      uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : w.mq[3];
-      uint32_t r = key == 0 ? r4[0] : key == 1 ? r4[1] : key == 2 ? r4[2] : r4[3];
-      q[r] = s;
+    if (GUIDE) {
+      uint32_t* c5[5] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
+                         &w.gcnt[G_QUEUE]};
+      bool p5[5] = {key == 0, key == 1, key == 2, key == 3, gd};
+      uint32_t r5[5];
+      block_append<5>(c5, p5, r5, lds);
+      if (key >= 0) {
+        uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : w.mq[3];
+        uint32_t r = key == 0 ? r5[0] : key == 1 ? r5[1] : key == 2 ? r5[2] : r5[3];
+        q[r] = s;
+      }
+      if (gd) w.gq[r5[4]] = s;
+    } else {
+      uint32_t* c4[4] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3]};
+      bool p4[4] = {key == 0, key == 1, key == 2, key == 3};
+      uint32_t r4[4];
+      block_append<4>(c4, p4, r4, lds);
+      if (key >= 0) {
+        uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : w.mq[3];
+        uint32_t r = key == 0 ? r4[0] : key == 1 ? r4[1] : key == 2 ? r4[2] : r4[3];
+        q[r] = s;
+      }
     }
   }
-}
-
-__global__ void k_guide_reset(uint32_t* gcnt) {
-  if (threadIdx.x == 0) gcnt[G_QUEUE] = 0;
 }
 
 struct ShadowJob {
@@ -592,7 +641,9 @@ void LaunchInitPass(const Wavefront& w, bool first, hipStream_t s) {
 void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_regen, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
-void LaunchReset(const Wavefront& w, int out, hipStream_t s) { hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w.cnt, out); }
+void LaunchReset(const Wavefront& w, int out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w.cnt, w.gcnt, out);
+}
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_extend, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene, in);
 }
@@ -603,9 +654,8 @@ void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const La
     hipLaunchKernelGGL(k_shade_path<false>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 template <bool G>
-static void LaunchSortedImpl(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_classify, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+static void LaunchMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
+                            hipStream_t s) {
   // the diffuse kernel also takes the misses, so it always runs
   hipLaunchKernelGGL((k_shade_path<G, BSDF_DIFFUSE>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
   if (mask & (1u << BSDF_ROUGH_METAL))
@@ -616,17 +666,27 @@ static void LaunchSortedImpl(const Wavefront& w, const DevScene& scene, int in, 
     hipLaunchKernelGGL((k_shade_path<G, BSDF_DISNEY>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 
-void LaunchShadeSorted(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
-                       hipStream_t s) {
-  if (w.flags & (WF_GUIDE | WF_RECORD))
-    LaunchSortedImpl<true>(w, scene, in, mask, c, s);
+void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
+  if (w.flags & WF_GUIDE)
+    hipLaunchKernelGGL(k_classify<true>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
   else
-    LaunchSortedImpl<false>(w, scene, in, mask, c, s);
+    hipLaunchKernelGGL(k_classify<false>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+}
+
+void LaunchShadeMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
+                          hipStream_t s) {
+  if (w.flags & (WF_GUIDE | WF_RECORD))
+    LaunchMaterials<true>(w, scene, in, mask, c, s);
+  else
+    LaunchMaterials<false>(w, scene, in, mask, c, s);
 }
 
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_guide_reset, dim3(1), dim3(64), 0, s, w.gcnt);
   hipLaunchKernelGGL(k_guide_prep, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+}
+
+void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_guide_infer_hits, dim3(blocks), dim3(256), 0, s, w, scene);
 }
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);

@@ -61,6 +61,7 @@ constexpr uint32_t kTrainSteps = 96;             // training steps after each tr
 constexpr float kTrainLr = 4e-3f;
 constexpr float kGuideAlpha = 0.5f;
 constexpr float kTrainWmax = 64.f;               // cap of a record's normalised weight
+constexpr uint32_t kFirstTrainPaths = 2u << 20;   // paths in the first training pass
 constexpr float kGuideMinRough = 0.f;            // glossy BSDFs below this alpha are not guided
 
 // Training hyper-parameters; RAD_GUIDE_STEPS / RAD_GUIDE_LR / RAD_GUIDE_ALPHA
@@ -119,35 +120,36 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * 10 + Align(pass * 4) * 3 + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
   if (j.guided)
-    j.bytes += Align(Q * 4 * guide::kFeat) + Align(Q * 4) * 3 + Align(Q * 2 * guide::kOut) +
+    j.bytes += Align(Q * 4) * 3 + Align(Q * 2 * guide::kOut) +
                Align(uint64_t(kRecordCap) * sizeof(guide::TrainRecord)) + Align(kTrainBatch * 4) + 256 +
                Align(guide::kWeights * 2) + Align(guide::kBiases * 4);
   return j;
 }
 
-// Sample passes: without training, as many rows as the result planes hold;
-// with online training, passes of 1, 1, 2, 4, ... rows (each followed by
-// training on the records it produced) until a quarter of the samples are
-// spent, then the rest.
-void PlanPasses(rad_job* job, uint32_t maxRows) {
+// Sample passes: without training, as few passes as the result planes allow,
+// of equal size; with online training, first passes of r, 2r, 4r, ... rows
+// (each followed by training on the records it produced; r rows hold about
+// 2M paths, enough records for a useful first step) until a quarter of the
+// samples are spent, then the rest.
+void PlanPasses(rad_job* job, uint32_t npix, uint32_t maxRows) {
   const uint32_t spp = job->desc.spp;
   uint32_t done = 0;
   if (job->desc.guiding == 1) {
-    uint32_t r = 1;
-    bool first = true;
-    while (done + r <= std::max(1u, spp / 4) && r <= maxRows) {
+    const uint32_t budget = std::max(1u, spp / 4);
+    uint32_t r = std::max(1u, std::min(budget, (kFirstTrainPaths + npix - 1) / npix));
+    while (done + r <= budget && r <= maxRows) {
       job->pass_rows.push_back(r);
       job->pass_train.push_back(1);
       done += r;
-      if (!first) r *= 2;
-      first = false;
+      r *= 2;
     }
   }
-  while (done < spp) {
-    uint32_t r = std::min(maxRows, spp - done);
+  const uint32_t rest = spp - done;
+  const uint32_t n = (rest + maxRows - 1) / maxRows;
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t r = rest / n + (k < rest % n ? 1u : 0u);
     job->pass_rows.push_back(r);
     job->pass_train.push_back(0);
-    done += r;
   }
 }
 
@@ -201,7 +203,7 @@ void RunJob(rad_job* job) {
     const int inferBlocks = std::max(64, job->scene->ctx->cu_count * 3);
     const bool prof = (job->desc.flags & RAD_RENDER_PROFILE) != 0;
     const size_t readback = CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t) + G_WORDS * 4;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[7] = {};
     if (prof)
       for (auto& e : ev) HipCheck(hipEventCreate(&e), "hipEventCreate");
     auto t0 = std::chrono::steady_clock::now();
@@ -234,24 +236,27 @@ void RunJob(rad_job* job) {
         if (prof) HipCheck(hipEventRecord(ev[1], st), "event");
         LaunchExtend(w, scene, in, cfg, st);
         if (prof) HipCheck(hipEventRecord(ev[2], st), "event");
-        if (w.flags & WF_GUIDE) {
-          LaunchGuidePrep(w, scene, in, cfg, st);
-          guide::LaunchInfer(w.net, w.gfeat, &w.gcnt[G_QUEUE], 0, w.gq, w.graw, inferBlocks, st);
-        }
-        if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
         const bool vol = w.integrator == RAD_INTEGRATOR_VOL_PATH;
+        const bool guided = (w.flags & WF_GUIDE) != 0;
+        if (!vol && job->sorted)
+          LaunchClassify(w, scene, in, cfg, st);  // with guiding, also the guide queue
+        else if (!vol && guided)
+          LaunchGuidePrep(w, scene, in, cfg, st);
+        if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
+        if (!vol && guided) LaunchGuideInfer(w, scene, inferBlocks, st);
+        if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
         if (vol)
           LaunchShadeVol(w, scene, in, cfg, st);
         else if (job->sorted)
-          LaunchShadeSorted(w, scene, in, job->mat_mask, cfg, st);
+          LaunchShadeMaterials(w, scene, in, job->mat_mask, cfg, st);
         else
           LaunchShadePath(w, scene, in, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
+        if (prof) HipCheck(hipEventRecord(ev[5], st), "event");
         if (vol)
           LaunchVolShadow(w, scene, cfg, st);
         else
           LaunchShadow(w, scene, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[5], st), "event");
+        if (prof) HipCheck(hipEventRecord(ev[6], st), "event");
         HipCheck(hipGetLastError(), "wavefront launch");
         HipCheck(hipMemcpyAsync(job->h_cnt, w.cnt, CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t),
                                 hipMemcpyDeviceToHost, st),
@@ -262,14 +267,15 @@ void RunJob(rad_job* job) {
         HipCheck(hipStreamSynchronize(st), "wavefront iteration");
         job->iterations++;
         if (prof) {
-          // event pairs -> RAD_KERNEL_REGEN, EXTEND, GUIDE, SHADE, SHADOW
-          const int slot[5] = {RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_GUIDE, RAD_KERNEL_SHADE, RAD_KERNEL_SHADOW};
-          for (int k = 0; k < 5; k++) {
-            if (slot[k] == RAD_KERNEL_GUIDE && !(w.flags & WF_GUIDE)) continue;
+          // event intervals -> stages; classify (or guide-queue gathering) counts as shading
+          const int slot[6] = {RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_SHADE, RAD_KERNEL_GUIDE, RAD_KERNEL_SHADE,
+                               RAD_KERNEL_SHADOW};
+          for (int k = 0; k < 6; k++) {
+            if (slot[k] == RAD_KERNEL_GUIDE && !guided) continue;
             float ms = 0.f;
             HipCheck(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "event time");
             job->kernel_ms[slot[k]] += ms;
-            job->kernel_launches[slot[k]]++;
+            if (k != 2) job->kernel_launches[slot[k]]++;
           }
         }
         const uint64_t* hs = reinterpret_cast<const uint64_t*>(job->h_cnt + CNT_WORDS);
@@ -363,7 +369,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   const JobShape js = Shape(scene, desc);
   const uint32_t npix = js.npix, Q = js.Q;
   job->total = uint64_t(npix) * desc.spp;
-  PlanPasses(job.get(), js.rows);
+  PlanPasses(job.get(), js.npix, js.rows);
   // one arena for every per-slot, per-sample and per-pixel array
   if (reuse && reuse->bytes >= js.bytes) {
     job->arena = std::move(reuse);
@@ -402,7 +408,6 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.cnt = (uint32_t*)take(CNT_WORDS * 4 + ST_WORDS * 8);
   w.stats = reinterpret_cast<unsigned long long*>(w.cnt + CNT_WORDS);
   if (js.guided) {
-    w.gfeat = (float*)take(u4 * guide::kFeat);
     w.gq = (uint32_t*)take(u4);
     w.gpend = (uint32_t*)take(u4);
     w.glast = (uint32_t*)take(u4);

@@ -64,7 +64,6 @@ struct Wavefront {
   uint32_t* cnt;   // CNT_* words
   unsigned long long* stats;  // ST_* words
   // learned guiding (null when off)
-  float* gfeat;      // per guide-queue entry: position, geometric normal, outgoing direction
   uint32_t* gq;      // guide queue: slots whose vertex is guided this iteration
   _Float16* graw;    // per slot: 48 raw network outputs
   uint32_t* gpend;   // per slot: training record still waiting for its radiance baseline
@@ -158,12 +157,15 @@ void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const Launch
 void LaunchReset(const Wavefront& w, int out, hipStream_t s);
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
-// Material-sorted shading: bucket the extend queue by BSDF type, then one
-// specialised shade kernel per type present (mat_mask bit = BSDF type).
-void LaunchShadeSorted(const Wavefront& w, const DevScene& scene, int in, uint32_t mat_mask, const LaunchCfg& c,
-                       hipStream_t s);
+// Material-sorted shading: classify buckets the extend queue by BSDF type
+// (with WF_GUIDE it also fills the guide queue), then one specialised shade
+// kernel per type present (mat_mask bit = BSDF type).
+void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
+void LaunchShadeMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mat_mask, const LaunchCfg& c,
+                          hipStream_t s);
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
+void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, hipStream_t s);
 void LaunchAccum(const Wavefront& w, hipStream_t s);
 // vol_path (defined in volpath_kernels.hip).
 void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
