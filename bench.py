@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     p.add_argument("--unsorted", action="store_true", help="shade with one kernel (no material sort)")
     p.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
-    p.add_argument("--builder", choices=["host", "device"], default="host", help="BVH builder")
+    p.add_argument("--builder", choices=["host", "device"], default="device", help="BVH builder")
     p.add_argument("--no-serving", action="store_true", help="skip the serving-latency measurement")
     p.add_argument("--guide-scale", type=float, default=1.0, help="image scale of the guiding workload")
     p.add_argument("--serve-requests", type=int, default=40)

@@ -269,9 +269,10 @@ typedef struct rad_tune_report {
 
 enum { RAD_BVH_DEVICE = 0, RAD_BVH_HOST = 1 };
 typedef struct rad_bvh_opts {
-  int32_t builder;         /* RAD_BVH_DEVICE: GPU (Morton-ordered binned SAH); RAD_BVH_HOST: CPU binned SAH */
+  int32_t builder;         /* RAD_BVH_DEVICE: GPU binned SAH (3-axis centroid bins, exact sweeps for small
+                              subtrees); RAD_BVH_HOST: CPU binned SAH */
   int32_t max_leaf_size;   /* 0 = default (4) */
-} rad_bvh_opts;         /* opts = NULL selects the host builder */
+} rad_bvh_opts;         /* opts = NULL selects the device builder */
 
 /* Structural check of the built BVH (node / primitive buffers read back). */
 typedef struct rad_bvh_report {

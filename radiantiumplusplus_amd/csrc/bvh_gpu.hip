@@ -252,8 +252,8 @@ __device__ __forceinline__ void grow(float lo[3], float hi[3], const Bin& b) {
 }
 
 // One thread per large task: best binned SAH split over the three axes,
-// children allocated; a child of at most max_leaf primitives becomes a leaf,
-// one of at most kSmall a small task, a larger one a task of the next level.
+// children allocated; a child of one primitive becomes a leaf, one of at
+// most kSmall a small task, a larger one a task of the next level.
 __global__ void k_split(Task* tasks, uint32_t nTasks, const Bin* bins, N2* n2, Task* next, SmallTask* smalls,
                         BuildCtr* ctr, uint32_t maxLeaf, Caps caps) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,13 +320,13 @@ __global__ void k_split(Task* tasks, uint32_t nTasks, const Bin* bins, N2* n2, T
   for (int s = 0; s < 2; s++) {
     const uint32_t m = ce[s] - cb[s];
     tk.child_task[s] = kNone;
-    if (m <= maxLeaf) {
+    if (m == 1) {
       N2 leaf;
       for (int a = 0; a < 3; a++) leaf.lo[a] = clo[s][a], leaf.hi[a] = chi[s][a];
       leaf.left = cb[s];
       leaf.count = m;
       n2[child + s] = leaf;
-    } else if (m <= kSmall) {
+    } else if (m <= kSmall) {  // the wave builder also decides leaves of at most max_leaf
       const uint32_t j = atomicAdd(&ctr->smalls, 1u);
       if (j >= caps.smalls) {
         atomicOr(&ctr->error, 2u);
@@ -403,10 +403,11 @@ __global__ void k_scatter(uint32_t n, const uint32_t* owner, const Task* tasks, 
   owner2[p] = o;
 }
 
-// One wave per small task (at most kSmall = 64 primitives, one per lane):
-// the whole subtree, splitting each range at the best SAH position over the
+// One wave per small task (2 to kSmall = 64 primitives, one per lane): the
+// whole subtree, splitting each range at the best SAH position over the
 // centroid order of every axis (ties broken by position), with the range
-// partitioned stably through LDS.
+// partitioned stably through LDS.  Ranges of at most max_leaf primitives
+// become leaves unless the split is cheaper (the host builder's rule).
 constexpr int kSmallWaves = 4;
 __global__ void __launch_bounds__(64 * kSmallWaves) k_small(const SmallTask* smalls, uint32_t nSmall, uint32_t* ids,
                                                             float4* blo, float4* bhi, N2* n2, BuildCtr* ctr,
@@ -445,16 +446,6 @@ __global__ void __launch_bounds__(64 * kSmallWaves) k_small(const SmallTask* sma
       const float4 mhi = act ? Hi[lane] : make_float4(-kInf, -kInf, -kInf, 0.f);
       float nlo[3] = {wave_min(mlo.x), wave_min(mlo.y), wave_min(mlo.z)};
       float nhi[3] = {wave_max(mhi.x), wave_max(mhi.y), wave_max(mhi.z)};
-      if (uint32_t(cnt) <= maxLeaf) {
-        if (lane == 0) {
-          N2 leaf;
-          for (int a = 0; a < 3; a++) leaf.lo[a] = nlo[a], leaf.hi[a] = nhi[a];
-          leaf.left = st.begin + uint32_t(b);
-          leaf.count = uint32_t(cnt);
-          n2[node] = leaf;
-        }
-        continue;
-      }
       // candidate: this lane's primitive is the first of the right side in
       // (centroid[a], position) order
       const float4 mc = centroid(mlo, mhi);
@@ -490,6 +481,22 @@ __global__ void __launch_bounds__(64 * kSmallWaves) k_small(const SmallTask* sma
         const float oc = __shfl_xor(wc, o);
         const int ol = __shfl_xor(wl, o);
         if (oc < wc || (oc == wc && ol < wl)) wc = oc, wl = ol;
+      }
+      // a range of at most max_leaf primitives stays a leaf unless splitting
+      // is cheaper: leaf cost n against 1 + SAH cost / parent area
+      if (uint32_t(cnt) <= maxLeaf) {
+        const float A = half_area(nlo, nhi);
+        const float splitCost = A > 0.f && wl < 64 ? 1.f + wc / A : kInf;
+        if (cnt <= 1 || float(cnt) <= splitCost) {
+          if (lane == 0) {
+            N2 leaf;
+            for (int a = 0; a < 3; a++) leaf.lo[a] = nlo[a], leaf.hi[a] = nhi[a];
+            leaf.left = st.begin + uint32_t(b);
+            leaf.count = uint32_t(cnt);
+            n2[node] = leaf;
+          }
+          continue;
+        }
       }
       const int winner = wl;
       int axis = 0;
@@ -721,7 +728,7 @@ void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int
   const uint32_t maxNodes = 2 * n;
   dN2.alloc(size_t(maxNodes) * sizeof(N2));
   const uint32_t maxTasks = n / (kSmall + 1) + 2;
-  const uint32_t maxSmall = n / uint32_t(maxLeaf + 1) + 2;
+  const uint32_t maxSmall = n / 2 + 2;  // small tasks hold at least two primitives
   dTasks0.alloc(size_t(maxTasks) * sizeof(Task));
   dTasks1.alloc(size_t(maxTasks) * sizeof(Task));
   dSmall.alloc(size_t(maxSmall) * sizeof(SmallTask));
@@ -744,7 +751,7 @@ void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int
   float4* hi2 = dHi2.as<float4>();
   uint32_t levels = 0, nSmall = 0;
   BuildCtr c0{1, 0, 0, 0};
-  if (n <= uint32_t(maxLeaf)) {
+  if (n == 1) {
     hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(64), 0, st, lo, hi, n, dN2.as<N2>());
   } else if (n <= kSmall) {
     SmallTask root{0, 0, n, 0};
@@ -814,7 +821,7 @@ void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int
   BuildCtr hc{};
   HipCheck(hipMemcpyAsync(&hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st), "node count");
   HipCheck(hipStreamSynchronize(st), "BVH build");
-  const uint32_t n2count = n <= uint32_t(maxLeaf) ? 1u : hc.nodes;
+  const uint32_t n2count = hc.nodes;  // the counter starts at 1 (the root)
   nodesOut.alloc(size_t(n2count) * sizeof(BvhNode4));
   DeviceBuffer dJobs0, dJobs1, dJc;
   dJobs0.alloc(size_t(n2count) * sizeof(Job));

@@ -43,7 +43,7 @@ rad_status Guard(F&& f) {
 }  // namespace
 
 namespace rad {
-constexpr int kDefaultBuilder = RAD_BVH_HOST;
+constexpr int kDefaultBuilder = RAD_BVH_DEVICE;
 }  // namespace rad
 
 namespace rad {

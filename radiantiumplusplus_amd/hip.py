@@ -111,7 +111,7 @@ class Context:
 class Scene:
     """A scene resident in HBM plus its acceleration structure."""
 
-    def __init__(self, ctx: Context, desc: F.SceneDesc, keepalive=None, builder=F.RAD_BVH_HOST, max_leaf=0,
+    def __init__(self, ctx: Context, desc: F.SceneDesc, keepalive=None, builder=F.RAD_BVH_DEVICE, max_leaf=0,
                  build_bvh=True):
         self.ctx = ctx
         self.desc = desc
@@ -127,7 +127,7 @@ class Scene:
         desc = builder_obj.to_desc()
         return cls(ctx, desc, keepalive=builder_obj, **kw)
 
-    def build_bvh(self, builder=F.RAD_BVH_HOST, max_leaf=0):
+    def build_bvh(self, builder=F.RAD_BVH_DEVICE, max_leaf=0):
         opts = F.BvhOpts(builder, max_leaf)
         check(lib().rad_hip_bvh_build(self.h, C.byref(opts)))
 
