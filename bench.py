@@ -45,12 +45,15 @@ def parse():
     p.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     p.add_argument("--unsorted", action="store_true", help="shade with one kernel (no material sort)")
     p.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
+    p.add_argument("--launch", default=None, metavar="SLOTS,BPC,TPC",
+                   help="fixed launch configuration (path slots, shade blocks per CU, trace blocks per CU); no autotune")
     p.add_argument("--builder", choices=["host", "device"], default="device", help="BVH builder")
     p.add_argument("--no-serving", action="store_true", help="skip the serving-latency measurement")
     p.add_argument("--guide-scale", type=float, default=1.0, help="image scale of the guiding workload")
     p.add_argument("--serve-requests", type=int, default=40)
-    p.add_argument("--summarize", nargs=5, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
-                   help="print a text summary of rocprofv3 csv output directories")
+    p.add_argument("--summarize", nargs=6, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "PMC_JSON"),
+                   help="print a text summary of rocprofv3 csv output directories; PMC_JSON: the bench line of "
+                        "the counter runs (their segment count gives the traffic per ray)")
     return p.parse_args()
 
 
@@ -65,14 +68,20 @@ def _csv_rows(d, suffix):
 
 
 def _short(name):
-    return name.split("(")[0].replace("void ", "")
+    return name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+
+
+TRAFFIC_TAG = "TRAFFIC k_extend bytes_per_ray="
+PROFILE_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_extend_large_mesh.txt")
 
 
 def summarize(dirs, copy_mib=1024):
     """Kernel-trace statistics plus FETCH_SIZE / WRITE_SIZE per dispatch,
-    scaled by the readings of the plain copy of `copy_mib` MiB."""
-    trace, fetch, write, cfetch, cwrite = dirs
-    out = ["kernel                 calls   total_ms    avg_us    min_us    max_us   vgpr  scratch"]
+    scaled by the readings of the plain copy of `copy_mib` MiB, and the
+    closest-hit kernel's traffic per traced ray."""
+    trace, fetch, write, cfetch, cwrite, pmc_json = dirs
+    W = 44
+    out = [f"{'kernel':{W}s} calls   total_ms    avg_us    min_us    max_us   vgpr  scratch"]
     stats = {}
     for r in _csv_rows(trace, "kernel_stats.csv"):
         stats[_short(r["Name"])] = r
@@ -81,7 +90,7 @@ def summarize(dirs, copy_mib=1024):
         res.setdefault(_short(r["Kernel_Name"]), (r["VGPR_Count"], r["Scratch_Size"]))
     for k, r in sorted(stats.items(), key=lambda kv: -float(kv[1]["TotalDurationNs"])):
         v, sc = res.get(k, ("?", "?"))
-        out.append(f"{k[:22]:22s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:10.2f} "
+        out.append(f"{k[:W]:{W}s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:10.2f} "
                    f"{float(r['AverageNs']) / 1e3:9.1f} {float(r['MinNs']) / 1e3:9.1f} {float(r['MaxNs']) / 1e3:9.1f} "
                    f"{v:>6s} {sc:>8s}")
 
@@ -100,15 +109,36 @@ def summarize(dirs, copy_mib=1024):
     out.append(f"plain copy of {copy_mib} MiB: FETCH_SIZE {sum(kf) / len(kf):.0f} KiB, WRITE_SIZE "
                f"{sum(kw) / len(kw):.0f} KiB per dispatch -> scale factors fetch x{f_scale:.3f}, write x{w_scale:.3f}")
     out.append("")
-    out.append("kernel                 dispatches  read_MB/launch  write_MB/launch  (scaled)")
+    out.append(f"{'kernel':{W}s} dispatches  read_MB/launch  write_MB/launch  (scaled)")
     f, w = per_kernel(fetch), per_kernel(write)
     for k in sorted(f, key=lambda k: -sum(f[k])):
         if k not in w:
             continue
         rd = sum(f[k]) / len(f[k]) * f_scale * 1024 / 1e6
         wr = sum(w[k]) / len(w[k]) * w_scale * 1024 / 1e6
-        out.append(f"{k[:22]:22s} {len(f[k]):10d} {rd:15.1f} {wr:16.1f}")
+        out.append(f"{k[:W]:{W}s} {len(f[k]):10d} {rd:15.1f} {wr:16.1f}")
+    ext = [k for k in f if k.endswith("k_extend")]
+    if ext and ext[0] in w:
+        k = ext[0]
+        line = json.loads(open(pmc_json).read().strip().splitlines()[-1])
+        rays = line["stats"]["segments_per_step"] * line["steps"]
+        total = (sum(f[k]) * f_scale + sum(w[k]) * w_scale) * 1024
+        out.append("")
+        out.append(f"k_extend: {rays:.0f} rays traced in the counter runs, {total / 1e9:.2f} GB read + written beyond L2")
+        out.append(f"{TRAFFIC_TAG}{total / rays:.1f}")
     return "\n".join(out)
+
+
+def traffic_per_ray():
+    """The closest-hit kernel's measured bytes per ray from the committed
+    counter summary (None when absent)."""
+    try:
+        for ln in open(PROFILE_SUMMARY):
+            if ln.startswith(TRAFFIC_TAG):
+                return float(ln[len(TRAFFIC_TAG):])
+    except OSError:
+        pass
+    return None
 
 
 def copy_probe(mib):
@@ -299,7 +329,8 @@ def main():
     W, H = b.camera["width"], b.camera["height"]
     spp = b.sampler["sample_count"]
     tune = None
-    if not args.no_autotune:
+    launch = [int(x) for x in args.launch.split(",")] if args.launch else None
+    if launch is None and not args.no_autotune:
         # launch configuration picked by timing short renders (images do not depend on it)
         rep = scene.autotune(b.render_desc(spp=max(1, min(spp, (8 << 20) // (W * H))), seed=1))
         tune = {"path_slots": rep.path_slots, "blocks_per_cu": rep.blocks_per_cu,
@@ -309,6 +340,8 @@ def main():
         print(f"# autotune: slots {rep.path_slots}, blocks/CU {rep.blocks_per_cu}, "
               f"trace blocks/CU {rep.trace_blocks_per_cu}", file=sys.stderr, flush=True)
     rd = b.render_desc(spp=spp, seed=1, flags=F.RAD_RENDER_PROFILE | (F.RAD_RENDER_UNSORTED if args.unsorted else 0))
+    if launch:
+        rd.path_slots, rd.blocks_per_cu, rd.trace_blocks_per_cu = launch
     print(f"# scene {name}: {info.primitive_count} primitives, {info.node_count} BVH nodes, "
           f"build {info.bvh_build_ms:.0f} ms, generation {gen_s:.1f} s, upload+build {setup_s:.1f} s",
           file=sys.stderr, flush=True)
@@ -361,8 +394,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (traffic_per_ray() * segs / max(ext_launch, 1)) if traffic_per_ray() else None,
+            "traffic_note": "bytes per launch beyond L2: FETCH_SIZE + WRITE_SIZE per traced ray from the rocprofv3 "
+                            "--pmc runs in profiles/k_extend_large_mesh.txt (scaled by a plain copy), times this "
+                            "run's rays per launch",
             "avg_launch_ms": ext_ms / max(ext_launch, 1),
+            "launch": {"path_slots": stats[0].slots, "blocks_per_cu": stats[0].blocks_per_cu,
+                       "trace_blocks_per_cu": stats[0].trace_blocks_per_cu},
             "rays_per_s": segs / (ext_ms * 1e-3) if ext_ms > 0 else 0.0,
         },
         "stats": {
