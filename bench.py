@@ -35,7 +35,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="large_mesh", choices=["large_mesh", "many_balls", "volume_box"])
     p.add_argument("--spp", type=int, default=None)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
@@ -356,8 +356,8 @@ def main():
         _, _, st = scene.render(rd)
         times.append(time.time() - t)
         stats.append(st)
-        print(f"# step {i}: {times[-1] * 1e3:.1f} ms, {st.segments} segments, {st.iterations} iterations",
-              file=sys.stderr, flush=True)
+        print(f"# step {i}: {times[-1] * 1e3:.1f} ms (render {st.render_ms:.1f} ms), {st.segments} segments, "
+              f"{st.iterations} iterations", file=sys.stderr, flush=True)
     total = time.time() - t_all
     samples = W * H * spp * args.steps
     ms_step = 1e3 * total / args.steps

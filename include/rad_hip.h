@@ -284,6 +284,8 @@ typedef struct rad_bvh_report {
   uint32_t leaf_box_violations;      /* triangles / spheres not inside their leaf box */
   float sah_cost;                    /* traversal 1, intersection 1, relative to the root box area */
   int32_t builder;
+  uint32_t quantized_box_violations; /* children of the traversal's 64-byte nodes whose dequantised box
+                                        does not contain the full-precision one */
 } rad_bvh_report;
 typedef struct rad_scene_info {
   uint64_t primitive_count;

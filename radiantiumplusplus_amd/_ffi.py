@@ -273,6 +273,7 @@ class BvhReport(C.Structure):
         ("leaf_box_violations", C.c_uint32),
         ("sah_cost", C.c_float),
         ("builder", C.c_int32),
+        ("quantized_box_violations", C.c_uint32),
     ]
 
 

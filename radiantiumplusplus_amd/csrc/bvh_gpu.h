@@ -22,4 +22,8 @@ struct DeviceBvhStats {
 void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int maxLeaf, hipStream_t st,
                      DeviceBuffer& nodesOut, DeviceBuffer& primsOut, DeviceBvhStats& stats);
 
+// The traversal's 64-byte copy of m nodes (device buffers); throws if a
+// child box cannot be bounded.
+void QuantizeBvh4(const DeviceBuffer& nodes, uint32_t m, hipStream_t st, DeviceBuffer& out);
+
 }  // namespace rad

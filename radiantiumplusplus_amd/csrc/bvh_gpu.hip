@@ -622,6 +622,14 @@ __global__ void k_collapse(const Job* jobs, uint32_t nJobs, const N2* n2, BvhNod
     kid[bi] = l;
     kid[nk++] = l + 1;
   }
+  // the node's inner children get consecutive ids, so siblings share cache lines
+  int inner = 0;
+  for (int s = 0; s < nk; s++) inner += n2[kid[s]].count > 0 ? 0 : 1;
+  uint32_t base = 0, qbase = 0;
+  if (inner) {
+    base = atomicAdd(&ctr[0], uint32_t(inner));
+    qbase = atomicAdd(&ctr[1], uint32_t(inner));
+  }
   BvhNode4 o;
   for (int s = 0; s < 4; s++) {
     if (s >= nk) {
@@ -642,11 +650,11 @@ __global__ void k_collapse(const Job* jobs, uint32_t nJobs, const N2* n2, BvhNod
       o.child[s] = int32_t(c.left);
       o.count[s] = c.count;
     } else {
-      uint32_t id = atomicAdd(&ctr[0], 1u);
-      uint32_t q = atomicAdd(&ctr[1], 1u);
-      next[q] = Job{kid[s], id};
-      o.child[s] = int32_t(id);
+      next[qbase] = Job{kid[s], base};
+      o.child[s] = int32_t(base);
       o.count[s] = 0;
+      base++;
+      qbase++;
     }
   }
   out[jb.n4] = o;
@@ -678,9 +686,98 @@ __global__ void k_sah(const BvhNode4* nodes, uint32_t m, float rootA, double* co
   if (__lane_id() == 0 && c != 0.f) atomicAdd(cost, double(c));
 }
 
+// ---------------------------------------------------------------- quantised copy
+// Step 2^e per axis, e the smallest with 255 * 2^e >= the node's extent
+// (raised further if rounding keeps a bound out of reach); bounds rounded
+// outward and verified with the traversal's own dequantisation.
+__device__ bool quantize_axis(const float* clo, const float* chi, const bool* valid, float o, float ext, uint32_t& biased,
+                              uint32_t& qlo, uint32_t& qhi) {
+  int e = -126;
+  if (ext > 0.f) {
+    frexpf(ext / 255.f, &e);  // ext / 255 = m 2^e, m in [0.5, 1)
+    e = max(e, -126);
+  }
+  for (; e <= 127; e++) {
+    const uint32_t b = uint32_t(e + 127);
+    const float step = quant_step(b);
+    bool ok = true;
+    uint32_t lo = 0, hi = 0;
+    for (int i = 0; i < 4 && ok; i++) {
+      if (!valid[i]) continue;
+      float fl = fminf(fmaxf(floorf((clo[i] - o) / step), 0.f), 255.f);
+      float fh = fminf(fmaxf(ceilf((chi[i] - o) / step), 0.f), 255.f);
+      uint32_t a = uint32_t(fl), c = uint32_t(fh);
+      while (a > 0 && dequant(a, step, o) > clo[i]) a--;
+      while (c < 255 && dequant(c, step, o) < chi[i]) c++;
+      if (dequant(a, step, o) > clo[i] || dequant(c, step, o) < chi[i]) ok = false;
+      lo |= a << (8 * i);
+      hi |= c << (8 * i);
+    }
+    if (ok) {
+      biased = b;
+      qlo = lo;
+      qhi = hi;
+      return true;
+    }
+  }
+  return false;
+}
+
+__global__ void k_quantize(const BvhNode4* in, uint32_t m, BvhNodeQ* out, uint32_t* error) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const BvhNode4 nd = in[i];
+  bool valid[4];
+  float o[3] = {kInf, kInf, kInf}, h[3] = {-kInf, -kInf, -kInf};
+  for (int c = 0; c < 4; c++) {
+    valid[c] = nd.child[c] >= 0;
+    if (!valid[c]) continue;
+    o[0] = fminf(o[0], nd.lox[c]), o[1] = fminf(o[1], nd.loy[c]), o[2] = fminf(o[2], nd.loz[c]);
+    h[0] = fmaxf(h[0], nd.hix[c]), h[1] = fmaxf(h[1], nd.hiy[c]), h[2] = fmaxf(h[2], nd.hiz[c]);
+  }
+  BvhNodeQ q;
+  uint32_t counts = 0;
+  for (int c = 0; c < 4; c++) {
+    q.child[c] = nd.child[c];
+    counts |= (valid[c] ? min(nd.count[c], 255u) : 0u) << (8 * c);
+  }
+  q.counts = counts;
+  q.pad = 0;
+  if (!(valid[0] || valid[1] || valid[2] || valid[3])) {
+    q.ox = q.oy = q.oz = 0.f;
+    q.exps = 127u | 127u << 8 | 127u << 16;
+    q.qlox = q.qhix = q.qloy = q.qhiy = q.qloz = q.qhiz = 0;
+    out[i] = q;
+    return;
+  }
+  uint32_t bx = 127, by = 127, bz = 127;
+  bool ok = quantize_axis(nd.lox, nd.hix, valid, o[0], h[0] - o[0], bx, q.qlox, q.qhix) &&
+            quantize_axis(nd.loy, nd.hiy, valid, o[1], h[1] - o[1], by, q.qloy, q.qhiy) &&
+            quantize_axis(nd.loz, nd.hiz, valid, o[2], h[2] - o[2], bz, q.qloz, q.qhiz);
+  if (!ok || nd.count[0] > 255u || nd.count[1] > 255u || nd.count[2] > 255u || nd.count[3] > 255u) atomicOr(error, 1u);
+  q.ox = o[0];
+  q.oy = o[1];
+  q.oz = o[2];
+  q.exps = bx | by << 8 | bz << 16;
+  out[i] = q;
+}
+
 unsigned Grid(uint64_t n, unsigned t = 256) { return unsigned((n + t - 1) / t); }
 
 }  // namespace
+
+void QuantizeBvh4(const DeviceBuffer& nodes, uint32_t m, hipStream_t st, DeviceBuffer& out) {
+  out.alloc(size_t(std::max(m, 1u)) * sizeof(BvhNodeQ));
+  DeviceBuffer err;
+  err.alloc(16);
+  HipCheck(hipMemsetAsync(err.ptr, 0, 4, st), "quantize flag");
+  if (m) hipLaunchKernelGGL(k_quantize, dim3(Grid(m)), dim3(256), 0, st, nodes.as<BvhNode4>(), m, out.as<BvhNodeQ>(),
+                            err.as<uint32_t>());
+  uint32_t e = 0;
+  HipCheck(hipMemcpyAsync(&e, err.ptr, 4, hipMemcpyDeviceToHost, st), "quantize flag");
+  HipCheck(hipStreamSynchronize(st), "quantize");
+  if (e) throw std::runtime_error("BVH node bounds cannot be quantised (non-finite boxes?)");
+}
 
 void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int maxLeaf, hipStream_t st,
                      DeviceBuffer& nodesOut, DeviceBuffer& primsOut, DeviceBvhStats& stats) {

@@ -196,9 +196,11 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
       depth = r.depth;
       sah = r.sah_cost;
     }
+    QuantizeBvh4(scene->d_nodes, nodeCount, scene->ctx->stream, scene->d_qnodes);
     scene->builder = builder;
     scene->bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     scene->dev.nodes = scene->d_nodes.as<BvhNode4>();
+    scene->dev.qnodes = scene->d_qnodes.as<BvhNodeQ>();
     scene->dev.prims = scene->d_prims.as<float4>();
     scene->dev.prim_count = n;
     scene->dev.node_count = nodeCount;
@@ -272,8 +274,11 @@ rad_status rad_hip_bvh_check(rad_scene* scene, rad_bvh_report* rep) {
     HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
     const uint32_t m = scene->dev.node_count, n = scene->dev.prim_count;
     std::vector<BvhNode4> nodes(m);
+    std::vector<BvhNodeQ> qnodes(m);
     std::vector<float4> prims(3 * size_t(n));
     HipCheck(hipMemcpy(nodes.data(), scene->d_nodes.ptr, size_t(m) * sizeof(BvhNode4), hipMemcpyDeviceToHost), "nodes");
+    HipCheck(hipMemcpy(qnodes.data(), scene->d_qnodes.ptr, size_t(m) * sizeof(BvhNodeQ), hipMemcpyDeviceToHost),
+             "quantised nodes");
     HipCheck(hipMemcpy(prims.data(), scene->d_prims.ptr, prims.size() * sizeof(float4), hipMemcpyDeviceToHost), "prims");
     std::memset(rep, 0, sizeof(*rep));
     std::vector<uint8_t> seen(n, 0), visited(m, 0);
@@ -290,6 +295,23 @@ rad_status rad_hip_bvh_check(rad_scene* scene, rad_bvh_report* rep) {
       visited[x] = 1;
       rep->depth = std::max(rep->depth, d);
       const BvhNode4& nd = nodes[x];
+      {  // the traversal copy: same children and counts, dequantised boxes around the exact ones
+        const BvhNodeQ& qn = qnodes[x];
+        const float sx = quant_step(qn.exps), sy = quant_step(qn.exps >> 8), sz = quant_step(qn.exps >> 16);
+        for (int s = 0; s < 4; s++) {
+          if (qn.child[s] != nd.child[s]) {
+            rep->quantized_box_violations++;
+            continue;
+          }
+          if (nd.child[s] < 0) continue;
+          const int sh = 8 * s;
+          if (((qn.counts >> sh) & 0xffu) != nd.count[s] ||
+              !(dequant((qn.qlox >> sh) & 0xffu, sx, qn.ox) <= nd.lox[s] && dequant((qn.qhix >> sh) & 0xffu, sx, qn.ox) >= nd.hix[s] &&
+                dequant((qn.qloy >> sh) & 0xffu, sy, qn.oy) <= nd.loy[s] && dequant((qn.qhiy >> sh) & 0xffu, sy, qn.oy) >= nd.hiy[s] &&
+                dequant((qn.qloz >> sh) & 0xffu, sz, qn.oz) <= nd.loz[s] && dequant((qn.qhiz >> sh) & 0xffu, sz, qn.oz) >= nd.hiz[s]))
+            rep->quantized_box_violations++;
+        }
+      }
       for (int s = 0; s < 4; s++) {
         if (nd.child[s] < 0) continue;
         if (nd.count[s] > 0) {

@@ -27,6 +27,32 @@ struct alignas(128) BvhNode4 {
 };
 static_assert(sizeof(BvhNode4) == 128, "BVH node must be one 128-byte line");
 
+// The traversal's copy of a BvhNode4 in 64 bytes: the node box origin, a
+// power-of-two step per axis, and every child's bounds as 8-bit multiples of
+// the step, rounded outward, so each child box contains the exact one.  A
+// dequantised bound is fmaf(q, step, origin): one rounding, computed the same
+// way when the node is built, where the outward rounding is verified.
+struct alignas(64) BvhNodeQ {
+  float ox, oy, oz;
+  uint32_t exps;                    // biased exponents of the x, y, z steps (bytes 0, 1, 2)
+  uint32_t qlox, qhix, qloy, qhiy;  // child i's bound in byte i
+  uint32_t qloz, qhiz;
+  uint32_t counts;                  // child i's leaf primitive count in byte i (0: inner)
+  uint32_t pad;
+  int32_t child[4];                 // as in BvhNode4
+};
+static_assert(sizeof(BvhNodeQ) == 64, "quantised BVH node must be 64 bytes");
+
+__host__ __device__ inline float quant_step(uint32_t biased) {
+  union {
+    uint32_t u;
+    float f;
+  } x;
+  x.u = (biased & 0xffu) << 23;
+  return x.f;
+}
+__host__ __device__ inline float dequant(uint32_t q, float step, float origin) { return fmaf(float(q), step, origin); }
+
 struct ShapeRec {
   int32_t type;       // SHAPE_*
   int32_t bsdf;
@@ -104,7 +130,8 @@ struct CameraRec {
 };
 
 struct DevScene {
-  const BvhNode4* nodes;
+  const BvhNode4* nodes;   // full-precision nodes (checks, statistics)
+  const BvhNodeQ* qnodes;  // traversal copy
   const float4* prims;
   uint32_t prim_count, node_count;
   const ShapeRec* shapes;

@@ -457,6 +457,6 @@ CameraRec MakeCameraRec(const rad_camera_desc& c) {
 }  // namespace rad
 
 uint64_t rad_scene::device_bytes() const {
-  return d_nodes.bytes + d_prims.bytes + d_shapes.bytes + d_pos.bytes + d_nrm.bytes + d_uv.bytes + d_idx.bytes +
+  return d_nodes.bytes + d_qnodes.bytes + d_prims.bytes + d_shapes.bytes + d_pos.bytes + d_nrm.bytes + d_uv.bytes + d_idx.bytes +
          d_bsdfs.bytes + d_lights.bytes + d_cdf.bytes + d_media.bytes + d_grid.bytes;
 }

@@ -1,4 +1,5 @@
-// Device ray traversal over the 4-wide BVH: closest hit
+// Device ray traversal over the 4-wide BVH (its 64-byte quantised copy):
+// closest hit
 // (Embree::RayIntersectPreliminary, embree.cpp:87-111) and occlusion
 // (Embree::RayIntersect, embree.cpp:79-85).
 #pragma once
@@ -208,22 +209,26 @@ __device__ __forceinline__ void cswap(float& da, uint32_t& na, float& db, uint32
 // others (far first).  Returns true when the ray is finished.
 template <bool ANY>
 __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, TraceStack& st) {
-  const float4* q = reinterpret_cast<const float4*>(s.nodes) + 8 * size_t(r.node);
-  float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
-  int4 ch = *reinterpret_cast<const int4*>(q + 6);
-  uint4 cn = *reinterpret_cast<const uint4*>(q + 7);
-  const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
-  const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
-  const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+  // one 64-byte quantised node: origin + steps, 8-bit child bounds, counts, children
+  const uint4* q = reinterpret_cast<const uint4*>(s.qnodes) + 4 * size_t(r.node);
+  const uint4 hdr = q[0], bxy = q[1], bz = q[2];
+  const int4 ch = *reinterpret_cast<const int4*>(q + 3);
+  const float ox = __uint_as_float(hdr.x), oy = __uint_as_float(hdr.y), oz = __uint_as_float(hdr.z);
+  const float sx = quant_step(hdr.w), sy = quant_step(hdr.w >> 8), sz = quant_step(hdr.w >> 16);
   const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
-  const uint32_t cnt[4] = {cn.x, cn.y, cn.z, cn.w};
+  uint32_t cnt[4];
   float tn[4];
   bool m[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    float t0x = fmaf(lo_x[i], r.inv.x, -r.oinv.x), t1x = fmaf(hi_x[i], r.inv.x, -r.oinv.x);
-    float t0y = fmaf(lo_y[i], r.inv.y, -r.oinv.y), t1y = fmaf(hi_y[i], r.inv.y, -r.oinv.y);
-    float t0z = fmaf(lo_z[i], r.inv.z, -r.oinv.z), t1z = fmaf(hi_z[i], r.inv.z, -r.oinv.z);
+    const int sh = 8 * i;
+    cnt[i] = (bz.z >> sh) & 0xffu;
+    const float lo_x = dequant((bxy.x >> sh) & 0xffu, sx, ox), hi_x = dequant((bxy.y >> sh) & 0xffu, sx, ox);
+    const float lo_y = dequant((bxy.z >> sh) & 0xffu, sy, oy), hi_y = dequant((bxy.w >> sh) & 0xffu, sy, oy);
+    const float lo_z = dequant((bz.x >> sh) & 0xffu, sz, oz), hi_z = dequant((bz.y >> sh) & 0xffu, sz, oz);
+    float t0x = fmaf(lo_x, r.inv.x, -r.oinv.x), t1x = fmaf(hi_x, r.inv.x, -r.oinv.x);
+    float t0y = fmaf(lo_y, r.inv.y, -r.oinv.y), t1y = fmaf(hi_y, r.inv.y, -r.oinv.y);
+    float t0z = fmaf(lo_z, r.inv.z, -r.oinv.z), t1z = fmaf(hi_z, r.inv.z, -r.oinv.z);
     float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
     float tfb = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), r.tfar));
     // slack for the rounding of the fused slab arithmetic

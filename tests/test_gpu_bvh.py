@@ -59,9 +59,10 @@ def test_device_bvh_terrain_parity_and_structure(hip_ctx):
     assert r.builder == F.RAD_BVH_DEVICE
     assert r.structure_errors == 0 and r.unreferenced_or_repeated == 0
     assert r.child_box_violations == 0 and r.leaf_box_violations == 0
+    assert r.quantized_box_violations == 0
     assert r.max_leaf <= 4
     h = hip.Scene.from_builder(hip_ctx, b, builder=F.RAD_BVH_HOST).bvh_check()
-    assert h.structure_errors == 0 and h.unreferenced_or_repeated == 0
+    assert h.structure_errors == 0 and h.unreferenced_or_repeated == 0 and h.quantized_box_violations == 0
     print(f"SAH device {r.sah_cost:.2f} host {h.sah_cost:.2f} ratio {r.sah_cost / h.sah_cost:.3f}; "
           f"depth {r.depth} / {h.depth}; nodes {r.node_count} / {h.node_count}")
     assert r.sah_cost <= 1.10 * h.sah_cost
@@ -80,6 +81,7 @@ def test_device_bvh_small_counts(hip_ctx, n):
     g = hip.Scene.from_builder(hip_ctx, b, builder=F.RAD_BVH_DEVICE)
     r = g.bvh_check()
     assert r.structure_errors == 0 and r.unreferenced_or_repeated == 0 and r.leaf_box_violations == 0
+    assert r.quantized_box_violations == 0
     lo, hi = o.bounds()
     rays = ray_batch(lo, hi, 4000, n)
     check_parity(o, rays, g.closest(rays), o.closest(rays, brute=True))
