@@ -318,7 +318,7 @@ def main():
     from radiantiumplusplus_amd import _ffi as F
     from radiantiumplusplus_amd import hip
 
-    t0 = time.time()
+    t0 = t0_all = time.time()
     b, name = make_scene(args.config, args)
     gen_s = time.time() - t0
     ctx = hip.Context(0)
@@ -347,7 +347,7 @@ def main():
           file=sys.stderr, flush=True)
     for i in range(args.warmup):
         _, _, st = scene.render(rd)
-        print(f"# warmup {i}: {st.render_ms:.1f} ms", file=sys.stderr, flush=True)
+        print(f"# warmup {i}: {st.render_ms:.1f} ms (t = {time.time() - t0_all:.1f} s)", file=sys.stderr, flush=True)
     times, stats = [], []
     t_all = time.time()
     for i in range(args.steps):

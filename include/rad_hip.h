@@ -83,7 +83,13 @@ enum { /* BSDF factories (build/factory.cpp:274-283) on the hot path */
   RAD_BSDF_DIFFUSE = 0,
   RAD_BSDF_ROUGH_METAL = 1,
   RAD_BSDF_ROUGH_GLASS = 2,
-  RAD_BSDF_DISNEY = 3
+  RAD_BSDF_DISNEY = 3,
+  RAD_BSDF_PLASTIC = 4,        /* bsdf/plastic.cpp */
+  RAD_BSDF_ROUGH_PLASTIC = 5,  /* bsdf/rough_plastic.cpp */
+  RAD_BSDF_PERFECT_MIRROR = 6, /* bsdf/perfect_mirror.cpp */
+  RAD_BSDF_PERFECT_GLASS = 7,  /* bsdf/perfect_glass.cpp */
+  RAD_BSDF_TWO_SIDE = 8,       /* bsdf/two_side.cpp ("two_side" form): wraps bsdfs[nested] */
+  RAD_BSDF_MASK = 9            /* bsdf/mask.cpp: wraps bsdfs[nested] with a constant opacity */
 };
 
 enum { RAD_MICROFACET_GGX = 0, RAD_MICROFACET_BECKMANN = 1 };
@@ -121,15 +127,30 @@ typedef struct rad_bsdf_desc {
   float disney_eta;        /* used when RAD_DISNEY_HAS_ETA */
   float specular;          /* used otherwise (default 0.5) */
   float specular_sampling_rate, clearcoat_sampling_rate, diffuse_sampling_rate;
+  /* plastic / rough_plastic (plastic.cpp:20-40): int_ior / ext_ior above (1.49, 1.000277),
+     rough_plastic's roughness in alpha_u, distribution, sample_visible */
+  float diffuse[3];        /* "diffuse" (default 0.5) */
+  float specular_reflectance[3]; /* "specular" (default 1) */
+  int32_t nonlinear;       /* "no_linear" */
+  /* two_side / mask */
+  int32_t nested;          /* index of the wrapped BSDF in the scene's bsdf array */
+  float opacity;           /* mask "opacity" (default 0.5, clamped to [0, 1]) */
 } rad_bsdf_desc;
 
-enum { RAD_LIGHT_AREA = 0, RAD_LIGHT_SKYBOX = 1 };
+enum {
+  RAD_LIGHT_AREA = 0,
+  RAD_LIGHT_SKYBOX = 1,
+  RAD_LIGHT_POINT = 2,      /* light/point.cpp */
+  RAD_LIGHT_PROJECTION = 3  /* light/projection.cpp (constant irradiance) */
+};
 
 typedef struct rad_light_desc {
   int32_t type;            /* RAD_LIGHT_* */
   int32_t shape;           /* area: emitting shape index (set by the loader) */
-  float radiance[3];       /* constant radiance (area default 1, skybox 1) */
-  float to_world[16];      /* skybox orientation */
+  float radiance[3];       /* area / skybox radiance; point intensity; projection irradiance x scale */
+  float to_world[16];      /* skybox orientation; point: world position in the translation column;
+                              projection: the light's frame (looks along +z) */
+  float fov;               /* projection field of view in degrees (default 90) */
 } rad_light_desc;
 
 enum { RAD_MEDIUM_HOMOGENEOUS = 0, RAD_MEDIUM_HETEROGENEOUS = 1 };

@@ -1,6 +1,8 @@
 // ORACLE / TEST INFRASTRUCTURE ONLY (see ref_math.h).
-// BSDFs of the restatement: diffuse, rough_metal, rough_glass, disney, with
-// the GGX / Beckmann microfacet distributions and the Fresnel terms.
+// BSDFs of the restatement: diffuse, rough_metal, rough_glass, disney,
+// plastic, rough_plastic, perfect_mirror, perfect_glass and the two_side /
+// mask wrappers, with the GGX / Beckmann microfacet distributions and the
+// Fresnel terms.
 #include "ref_scene.h"
 
 namespace ref {
@@ -642,9 +644,246 @@ struct Disney final : Bsdf {
   }
 };
 
+// ---------------------------------------------------------------- plastic (bsdf/plastic.cpp:18-132)
+// Fresnel::DiffuseReflectance (fresnel.cpp:79-84): the polynomial fit of the
+// hemispherical Fresnel average, Horner-evaluated with fused multiply-adds
+// as math_base.h:90-105 does.
+static Float DiffuseReflectance(Float eta) {
+  Float invEta = Rcp(eta);
+  if (eta < 1) return Fmadd(Float(0.0636f), invEta, Fmadd(eta, Fmadd(eta, Float(-1.4399f), Float(0.7099f)), Float(0.6681f)));
+  const Float c[6] = {0.919317f, -3.4793f, 6.75335f, -7.80989f, 4.98554f, -1.36881f};
+  Float acc = c[5];
+  for (int i = 4; i >= 0; i--) acc = Fmadd(invEta, acc, c[i]);
+  return acc;
+}
+
+// Shared by plastic and rough_plastic: sampling weight from the luminance of
+// the (constant) diffuse and specular colours, and the diffuse term under
+// the dielectric coating.
+struct PlasticBase : Bsdf {
+  Float eta = 1.49f / 1.000277f, fdrInt = 0, invEta2 = 1, specularSampleWeight = 0.5f;
+  Spectrum diffuse, specular;
+  bool nonlinear = false;
+  void Init() {
+    fdrInt = DiffuseReflectance(Float(1) / eta);
+    invEta2 = Float(1) / (eta * eta);
+    double dm = Luminance(diffuse), sm = Luminance(specular);
+    specularSampleWeight = Float(sm / (dm + sm));
+  }
+  Spectrum DiffuseTerm(Float fi, Float fo, Float cosThetaO) const {
+    Spectrum num = diffuse * (1 / PI) * invEta2 * (1 - fi) * (1 - fo) * cosThetaO;
+    if (nonlinear) {
+      return Spectrum(num.x * (1 / (1 - diffuse.x * fdrInt)), num.y * (1 / (1 - diffuse.y * fdrInt)),
+                      num.z * (1 / (1 - diffuse.z * fdrInt)));
+    }
+    return num / (1 - fdrInt);
+  }
+};
+
+struct Plastic final : PlasticBase {
+  std::pair<BsdfSampleResult, Spectrum> Sample(const SurfaceInteraction& si, Float lobeXi, const V2& dirXi) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi);
+    if (cosThetaI <= 0) return {{}, Spectrum(0)};
+    Float fi = FresnelDielectric(cosThetaI, eta).r;
+    Float probSpec = fi * specularSampleWeight;
+    Float probDiff = (1 - fi) * (1 - specularSampleWeight);
+    probSpec = probSpec / (probSpec + probDiff);
+    probDiff = 1 - probSpec;
+    BsdfSampleResult bsr{};
+    bsr.Eta = 1;
+    if (lobeXi < probSpec) {
+      bsr.Wo = V3(-si.Wi.x, -si.Wi.y, si.Wi.z);
+      bsr.Pdf = probSpec;
+      bsr.TypeMask = BSDF_REFLECTION | BSDF_DELTA;
+      return {bsr, specular * fi};
+    }
+    bsr.Wo = Warp::SquareToCosineHemisphere(dirXi);
+    bsr.Pdf = Warp::SquareToCosineHemispherePdf(bsr.Wo) * probDiff;
+    bsr.TypeMask = BSDF_REFLECTION | BSDF_DIFFUSE;
+    Float cosThetaO = Frame::CosTheta(bsr.Wo);
+    Float fo = FresnelDielectric(cosThetaO, eta).r;
+    return {bsr, DiffuseTerm(fi, fo, cosThetaO)};
+  }
+  Spectrum Eval(const SurfaceInteraction& si, const V3& wo) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi), cosThetaO = Frame::CosTheta(wo);
+    if (cosThetaI <= 0 || cosThetaO <= 0) return Spectrum(0);
+    Float fi = FresnelDielectric(cosThetaI, eta).r;
+    Float fo = FresnelDielectric(cosThetaO, eta).r;
+    return DiffuseTerm(fi, fo, cosThetaO);
+  }
+  Float Pdf(const SurfaceInteraction& si, const V3& wo) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi), cosThetaO = Frame::CosTheta(wo);
+    if (cosThetaI <= 0 || cosThetaO <= 0) return 0;
+    Float fi = FresnelDielectric(cosThetaI, eta).r;
+    Float probSpec = fi * specularSampleWeight;
+    Float probDiff = (1 - fi) * (1 - specularSampleWeight);
+    probDiff = probDiff / (probDiff + probSpec);
+    return Warp::SquareToCosineHemispherePdf(wo) * probDiff;
+  }
+};
+
+// ---------------------------------------------------------------- rough_plastic (bsdf/rough_plastic.cpp:18-250)
+struct RoughPlastic final : PlasticBase {
+  Microfacet dist;
+  Float ProbSpec(Float fi) const {
+    Float probSpec = fi * specularSampleWeight;
+    Float probDiff = (1 - fi) * (1 - specularSampleWeight);
+    return probSpec / (probSpec + probDiff);
+  }
+  std::pair<BsdfSampleResult, Spectrum> Sample(const SurfaceInteraction& si, Float lobeXi, const V2& dirXi) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi);
+    if (cosThetaI <= 0) return {{}, Spectrum(0)};
+    Float probSpec = ProbSpec(FresnelDielectric(cosThetaI, eta).r);
+    BsdfSampleResult bsr{};
+    bsr.Eta = 1;
+    if (lobeXi < probSpec) {
+      auto [wh, pdf] = dist.Sample(si.Wi, dirXi);
+      if (pdf <= 0) return {bsr, Spectrum(0)};
+      bsr.Wo = Reflect(si.Wi, wh);
+      bsr.TypeMask = BSDF_GLOSSY | BSDF_REFLECTION;
+    } else {
+      bsr.Wo = Warp::SquareToCosineHemisphere(dirXi);
+      bsr.TypeMask = BSDF_DIFFUSE | BSDF_REFLECTION;
+    }
+    bsr.Pdf = Pdf(si, bsr.Wo);
+    if (bsr.Pdf <= 0) return {bsr, Spectrum(0)};
+    return {bsr, Eval(si, bsr.Wo)};
+  }
+  Spectrum Eval(const SurfaceInteraction& si, const V3& wo) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi), cosThetaO = Frame::CosTheta(wo);
+    if (cosThetaI <= 0 || cosThetaO <= 0) return Spectrum(0);
+    V3 wh = Normalized(wo + si.Wi);
+    Float F = FresnelDielectric(Dot(si.Wi, wh), eta).r;
+    Spectrum spec = specular * dist.D(wh) * dist.G(si.Wi, wo, wh) * F / (4 * cosThetaI);
+    Float fo = FresnelDielectric(Dot(wo, wh), eta).r;
+    return spec + DiffuseTerm(F, fo, cosThetaO);
+  }
+  Float Pdf(const SurfaceInteraction& si, const V3& wo) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi), cosThetaO = Frame::CosTheta(wo);
+    if (cosThetaI <= 0 || cosThetaO <= 0) return 0;
+    Float probSpec = ProbSpec(FresnelDielectric(cosThetaI, eta).r);
+    Float probDiff = 1 - probSpec;
+    V3 wh = Normalized(wo + si.Wi);
+    Float specPdf = dist.Pdf(si.Wi, wh) / (4 * Dot(wo, wh)) * probSpec;
+    Float diffPdf = Warp::SquareToCosineHemispherePdf(wo) * probDiff;
+    return specPdf + diffPdf;
+  }
+};
+
+// ---------------------------------------------------------------- perfect_mirror (bsdf/perfect_mirror.cpp:16-62)
+struct PerfectMirror final : Bsdf {
+  Spectrum reflectance, eta, k;
+  std::pair<BsdfSampleResult, Spectrum> Sample(const SurfaceInteraction& si, Float, const V2&) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi);
+    if (cosThetaI <= 0) return {{}, Spectrum(0)};
+    BsdfSampleResult bsr{};
+    bsr.Wo = V3(-si.Wi.x, -si.Wi.y, si.Wi.z);
+    bsr.Eta = 1;
+    bsr.TypeMask = flags;
+    bsr.Pdf = 1;
+    return {bsr, Mul(reflectance, FresnelConductor(cosThetaI, eta, k))};
+  }
+  Spectrum Eval(const SurfaceInteraction&, const V3&) const override { return Spectrum(0); }
+  Float Pdf(const SurfaceInteraction&, const V3&) const override { return 0; }
+};
+
+// ---------------------------------------------------------------- perfect_glass (bsdf/perfect_glass.cpp:14-83)
+struct PerfectGlass final : Bsdf {
+  Spectrum reflectance, transmittance;
+  Float eta = 1.5046f / 1.000277f;
+  std::pair<BsdfSampleResult, Spectrum> Sample(const SurfaceInteraction& si, Float lobeXi, const V2&) const override {
+    Float cosThetaI = Frame::CosTheta(si.Wi);
+    DielectricF fr = FresnelDielectric(cosThetaI, eta);
+    Float ri = fr.r, ti = 1 - ri;
+    BsdfSampleResult bsr{};
+    if (lobeXi <= ri) {
+      bsr.Wo = V3(-si.Wi.x, -si.Wi.y, si.Wi.z);
+      bsr.Eta = 1;
+      bsr.Pdf = ri;
+      bsr.TypeMask = BSDF_DELTA | BSDF_REFLECTION;
+      return {bsr, reflectance * ri};
+    }
+    bsr.Wo = V3(-fr.etaTI * si.Wi.x, -fr.etaTI * si.Wi.y, fr.cosThetaT);
+    bsr.Eta = fr.etaIT;
+    bsr.Pdf = ti;
+    bsr.TypeMask = BSDF_DELTA | BSDF_TRANSMISSION;
+    Spectrum f = transmittance * ti;
+    return {bsr, f * Sqr(fr.etaTI)};
+  }
+  Spectrum Eval(const SurfaceInteraction&, const V3&) const override { return Spectrum(0); }
+  Float Pdf(const SurfaceInteraction&, const V3&) const override { return 0; }
+};
+
+// ---------------------------------------------------------------- two_side (bsdf/two_side.cpp:60-137, "two_side" form)
+struct TwoSide final : Bsdf {
+  std::unique_ptr<Bsdf> nested;
+  std::pair<BsdfSampleResult, Spectrum> Sample(const SurfaceInteraction& si, Float lobeXi, const V2& dirXi) const override {
+    Float c = Frame::CosTheta(si.Wi);
+    if (c > 0) return nested->Sample(si, lobeXi, dirXi);
+    if (c < 0) {
+      SurfaceInteraction t = si;
+      t.Wi.z = -t.Wi.z;
+      auto [bsr, f] = nested->Sample(t, lobeXi, dirXi);
+      bsr.Wo.z = -bsr.Wo.z;
+      return {bsr, f};
+    }
+    BsdfSampleResult bsr{};
+    bsr.Pdf = 0;
+    return {bsr, Spectrum(0)};
+  }
+  Spectrum Eval(const SurfaceInteraction& si, const V3& wo) const override {
+    Float c = Frame::CosTheta(si.Wi);
+    if (c > 0) return nested->Eval(si, wo);
+    if (c < 0) {
+      SurfaceInteraction t = si;
+      t.Wi.z = -t.Wi.z;
+      return nested->Eval(t, V3(wo.x, wo.y, -wo.z));
+    }
+    return Spectrum(0);
+  }
+  Float Pdf(const SurfaceInteraction& si, const V3& wo) const override {
+    Float c = Frame::CosTheta(si.Wi);
+    if (c > 0) return nested->Pdf(si, wo);
+    if (c < 0) {
+      SurfaceInteraction t = si;
+      t.Wi.z = -t.Wi.z;
+      return nested->Pdf(t, V3(wo.x, wo.y, -wo.z));
+    }
+    return 0;
+  }
+};
+
+// ---------------------------------------------------------------- mask (bsdf/mask.cpp:8-62)
+// The pass-through sample keeps the reference's bookkeeping: type Null (not
+// delta), pdf 1 - opacity, weight 1.
+struct Mask final : Bsdf {
+  std::unique_ptr<Bsdf> nested;
+  Float opacity = 0.5f;
+  std::pair<BsdfSampleResult, Spectrum> Sample(const SurfaceInteraction& si, Float lobeXi, const V2& dirXi) const override {
+    if (lobeXi < opacity) {
+      lobeXi /= opacity;
+      auto [bsr, f] = nested->Sample(si, lobeXi, dirXi);
+      bsr.Pdf *= opacity;
+      return {bsr, f * opacity};
+    }
+    BsdfSampleResult bsr{};
+    bsr.Pdf = 1 - opacity;
+    bsr.Eta = 1;
+    bsr.Wo = -si.Wi;
+    bsr.TypeMask = 0;
+    return {bsr, Spectrum(1)};
+  }
+  Spectrum Eval(const SurfaceInteraction& si, const V3& wo) const override { return nested->Eval(si, wo) * opacity; }
+  Float Pdf(const SurfaceInteraction& si, const V3& wo) const override { return nested->Pdf(si, wo) * opacity; }
+};
+
 float RefErfInv(float a) { return ErfInv(a); }
 
-std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks) {
+std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks) { return MakeBsdf(&d, 1, 0, quirks, 0); }
+
+std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_t index, bool quirks, int depth) {
+  if (index >= count || depth > 2) throw std::runtime_error("bsdf: nested index out of range");
+  const rad_bsdf_desc& d = all[index];
   auto spec = [](const float* v) { return Spectrum(v[0], v[1], v[2]); };
   Microfacet md;
   md.beckmann = d.distribution == RAD_MICROFACET_BECKMANN;
@@ -718,6 +957,61 @@ std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks) {
       b->flags = BSDF_DIFFUSE | BSDF_REFLECTION;
       if (b->hasClearcoat) b->flags |= BSDF_GLOSSY;
       if (b->hasSpecTrans) b->flags |= BSDF_TRANSMISSION;
+      out = std::move(b);
+      break;
+    }
+    case RAD_BSDF_PLASTIC:
+    case RAD_BSDF_ROUGH_PLASTIC: {
+      std::unique_ptr<PlasticBase> b;
+      if (d.type == RAD_BSDF_PLASTIC) {
+        b = std::make_unique<Plastic>();
+      } else {
+        auto r = std::make_unique<RoughPlastic>();
+        r->dist = md;
+        r->dist.alphaY = md.alphaX;  // isotropic "alpha"
+        b = std::move(r);
+      }
+      b->flags = BSDF_DIFFUSE | BSDF_REFLECTION | BSDF_DELTA;  // both declare Delta (plastic.cpp:22, rough_plastic.cpp:22)
+      b->eta = d.int_ior / d.ext_ior;
+      b->diffuse = spec(d.diffuse);
+      b->specular = spec(d.specular_reflectance);
+      b->nonlinear = d.nonlinear != 0;
+      b->Init();
+      out = std::move(b);
+      break;
+    }
+    case RAD_BSDF_PERFECT_MIRROR: {
+      auto b = std::make_unique<PerfectMirror>();
+      b->flags = BSDF_DELTA | BSDF_REFLECTION;
+      b->reflectance = spec(d.reflectance);
+      b->eta = spec(d.eta);
+      b->k = spec(d.k);
+      out = std::move(b);
+      break;
+    }
+    case RAD_BSDF_PERFECT_GLASS: {
+      auto b = std::make_unique<PerfectGlass>();
+      b->flags = BSDF_DELTA | BSDF_REFLECTION | BSDF_TRANSMISSION;
+      b->reflectance = spec(d.reflectance);
+      b->transmittance = spec(d.transmittance);
+      b->eta = d.int_ior / d.ext_ior;
+      out = std::move(b);
+      break;
+    }
+    case RAD_BSDF_TWO_SIDE: {
+      auto b = std::make_unique<TwoSide>();
+      if (d.nested < 0) throw std::runtime_error("two_side: no nested bsdf");
+      b->nested = MakeBsdf(all, count, uint32_t(d.nested), quirks, depth + 1);
+      b->flags = b->nested->flags;
+      out = std::move(b);
+      break;
+    }
+    case RAD_BSDF_MASK: {
+      auto b = std::make_unique<Mask>();
+      if (d.nested < 0) throw std::runtime_error("mask: no nested bsdf");
+      b->nested = MakeBsdf(all, count, uint32_t(d.nested), quirks, depth + 1);
+      b->opacity = std::clamp(d.opacity, Float(0), Float(1));
+      b->flags = b->nested->flags;
       out = std::move(b);
       break;
     }

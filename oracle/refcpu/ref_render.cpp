@@ -596,9 +596,9 @@ int ref_render(const ref_scene* s, const ref_render_params* p, float* fb, float*
 }
 
 // Per-function probes: BSDF eval / pdf / sample in shading space.
-int ref_bsdf_eval(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out) {
+int ref_bsdf_eval(const rad_bsdf_desc* d, uint32_t count, int quirks, const float* in, uint32_t n, float* out) {
   try {
-    auto b = MakeBsdf(*d, quirks != 0);
+    auto b = MakeBsdf(d, count, 0, quirks != 0);
     for (uint32_t i = 0; i < n; i++) {
       SurfaceInteraction si{};
       si.Wi = V3(in[6 * i], in[6 * i + 1], in[6 * i + 2]);
@@ -616,9 +616,9 @@ int ref_bsdf_eval(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t 
   }
 }
 
-int ref_bsdf_sample(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out) {
+int ref_bsdf_sample(const rad_bsdf_desc* d, uint32_t count, int quirks, const float* in, uint32_t n, float* out) {
   try {
-    auto b = MakeBsdf(*d, quirks != 0);
+    auto b = MakeBsdf(d, count, 0, quirks != 0);
     for (uint32_t i = 0; i < n; i++) {
       SurfaceInteraction si{};
       si.Wi = V3(in[6 * i], in[6 * i + 1], in[6 * i + 2]);

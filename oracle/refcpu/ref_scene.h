@@ -252,6 +252,8 @@ struct Bsdf {
   virtual Float Pdf(const SurfaceInteraction& si, const V3& wo) const = 0;
 };
 std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks);
+// bsdfs[index] of a scene's array (two_side / mask resolve their nested entry)
+std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_t index, bool quirks, int depth = 0);
 float RefErfInv(float a);
 
 // ---- lights (render/light.h) ----

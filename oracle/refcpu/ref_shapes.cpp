@@ -409,6 +409,60 @@ struct SkyboxLight final : Light {  // light/skybox.cpp:24-141 (constant map)
   }
 };
 
+struct PointLight final : Light {  // light/point.cpp:10-45
+  V3 pos;
+  Spectrum intensity;
+  Spectrum Eval(const SurfaceInteraction&) const override { return Spectrum(0); }
+  std::pair<DirectionSampleResult, Spectrum> SampleDirection(const Interaction& ref, const V2&) const override {
+    DirectionSampleResult dsr{};
+    dsr.P = pos;
+    dsr.Pdf = 1;
+    dsr.IsDelta = true;
+    dsr.Dir = dsr.P - ref.P;
+    Float dist2 = Dot(dsr.Dir, dsr.Dir);
+    dsr.Dist = std::sqrt(dist2);
+    Float invDist = Rcp(dsr.Dist);
+    dsr.Dir = dsr.Dir * invDist;
+    return {dsr, intensity * Sqr(invDist)};
+  }
+  Float PdfDirection(const Interaction&, const DirectionSampleResult&) const override { return 0; }
+};
+
+// light/projection.cpp:12-63 with a constant irradiance (aspect 1)
+struct ProjectionLight final : Light {
+  Transform toWorld, toClip;
+  Spectrum irradiance;  // irradiance x scale
+  void Init(const float* tw, Float fov) {
+    toWorld = Transform(Mat4::FromRowMajor(tw));
+    Float aspect = 1, farC = 10000, nearC = Float(0.0001);
+    Float recip = 1 / (farC - nearC);
+    Float cot = 1 / std::tan(Radian(fov / 2));
+    Float P[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, farC * recip, -nearC * farC * recip, 0, 0, 1, 0};
+    Float S[16] = {-0.5f, 0, 0, 0, 0, -0.5f * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    Float T[16] = {1, 0, 0, -1, 0, 1, 0, -1 / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
+    toClip = Transform(Mat4::FromRowMajor(S) * Mat4::FromRowMajor(T) * Mat4::FromRowMajor(P));
+  }
+  Spectrum Eval(const SurfaceInteraction&) const override { return Spectrum(0); }
+  std::pair<DirectionSampleResult, Spectrum> SampleDirection(const Interaction& ref, const V2&) const override {
+    V3 local = toWorld.ApplyAffineToLocal(ref.P);
+    V3 c = toClip.ApplyAffineToWorld(local);
+    bool inRange = c.x >= 0 && c.x <= 1 && c.y >= 0 && c.y <= 1 && local.z > 0;
+    if (!inRange) return {DirectionSampleResult{}, Spectrum(0)};
+    DirectionSampleResult dsr{};
+    dsr.P = V3(toWorld.ToWorld.m[0][3], toWorld.ToWorld.m[1][3], toWorld.ToWorld.m[2][3]);
+    dsr.N = Normalized(toWorld.ApplyLinearToWorld(V3(0, 0, 1)));
+    dsr.UV = V2(c.x, c.y);
+    dsr.Pdf = 1;
+    dsr.IsDelta = true;
+    dsr.Dir = dsr.P - ref.P;
+    Float sq = Dot(dsr.Dir, dsr.Dir);
+    dsr.Dist = std::sqrt(sq);
+    dsr.Dir = dsr.Dir * Rcp(dsr.Dist);
+    return {dsr, irradiance * PI / (Sqr(local.z) * Dot(-dsr.Dir, dsr.N))};
+  }
+  Float PdfDirection(const Interaction&, const DirectionSampleResult&) const override { return 0; }
+};
+
 // ---------------------------------------------------------------- camera
 void Camera::Init(const rad_camera_desc& d) {  // perspective.cpp:15-70
   nearClip = d.near_clip;
@@ -455,7 +509,7 @@ static const uint32_t kCubeT[12][3] = {{0, 1, 2},    {3, 0, 2},    {4, 5, 6},   
 
 void Scene::Create(const rad_scene_desc& desc, bool q) {
   quirks = q;
-  for (uint32_t i = 0; i < desc.bsdf_count; i++) bsdfs.push_back(MakeBsdf(desc.bsdfs[i], quirks));
+  for (uint32_t i = 0; i < desc.bsdf_count; i++) bsdfs.push_back(MakeBsdf(desc.bsdfs, desc.bsdf_count, i, quirks));
   for (uint32_t i = 0; i < desc.medium_count; i++) media.push_back(MakeMedium(desc.media[i]));
   globalMedium = desc.global_medium >= 0 ? media[desc.global_medium].get() : nullptr;
   for (uint32_t i = 0; i < desc.light_count; i++) {
@@ -463,6 +517,16 @@ void Scene::Create(const rad_scene_desc& desc, bool q) {
     if (ld.type == RAD_LIGHT_AREA) {
       auto l = std::make_unique<AreaLight>();
       l->radiance = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
+      lights.push_back(std::move(l));
+    } else if (ld.type == RAD_LIGHT_POINT) {
+      auto l = std::make_unique<PointLight>();
+      l->intensity = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
+      l->pos = V3(ld.to_world[3], ld.to_world[7], ld.to_world[11]);
+      lights.push_back(std::move(l));
+    } else if (ld.type == RAD_LIGHT_PROJECTION) {
+      auto l = std::make_unique<ProjectionLight>();
+      l->irradiance = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
+      l->Init(ld.to_world, ld.fov);
       lights.push_back(std::move(l));
     } else {
       auto l = std::make_unique<SkyboxLight>();

@@ -34,8 +34,9 @@ int ref_intersect_any(const ref_scene* s, const rad_ray* rays, uint32_t n, int b
 float ref_intersect_prim(const ref_scene* s, const rad_ray* ray, uint32_t geom, uint32_t prim);
 void ref_camera_ray(const ref_scene* s, float sx, float sy, rad_ray* out);
 int ref_render(const ref_scene* s, const ref_render_params* p, float* fb, float* sumsq, double* seconds);
-int ref_bsdf_eval(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out);
-int ref_bsdf_sample(const rad_bsdf_desc* d, int quirks, const float* in, uint32_t n, float* out);
+// d[0] is probed; two_side / mask find their nested entry in d[0 .. count)
+int ref_bsdf_eval(const rad_bsdf_desc* d, uint32_t count, int quirks, const float* in, uint32_t n, float* out);
+int ref_bsdf_sample(const rad_bsdf_desc* d, uint32_t count, int quirks, const float* in, uint32_t n, float* out);
 int ref_grid_eval(const rad_medium_desc* d, const float* in, uint32_t n, float* out);
 float ref_erfinv(float a);
 float ref_medium_transmittance(const ref_scene* s, int medium, const rad_ray* ray, uint32_t channel, uint32_t seed);

@@ -9,8 +9,10 @@ RAD_OK = 0
 
 RAD_SHAPE_MESH, RAD_SHAPE_SPHERE, RAD_SHAPE_RECTANGLE, RAD_SHAPE_CUBE = 0, 1, 2, 3
 RAD_BSDF_DIFFUSE, RAD_BSDF_ROUGH_METAL, RAD_BSDF_ROUGH_GLASS, RAD_BSDF_DISNEY = 0, 1, 2, 3
+RAD_BSDF_PLASTIC, RAD_BSDF_ROUGH_PLASTIC, RAD_BSDF_PERFECT_MIRROR, RAD_BSDF_PERFECT_GLASS = 4, 5, 6, 7
+RAD_BSDF_TWO_SIDE, RAD_BSDF_MASK = 8, 9
 RAD_MICROFACET_GGX, RAD_MICROFACET_BECKMANN = 0, 1
-RAD_LIGHT_AREA, RAD_LIGHT_SKYBOX = 0, 1
+RAD_LIGHT_AREA, RAD_LIGHT_SKYBOX, RAD_LIGHT_POINT, RAD_LIGHT_PROJECTION = 0, 1, 2, 3
 RAD_MEDIUM_HOMOGENEOUS, RAD_MEDIUM_HETEROGENEOUS = 0, 1
 RAD_PHASE_ISOTROPIC, RAD_PHASE_HG = 0, 1
 RAD_WRAP_CLAMP, RAD_WRAP_REPEAT = 0, 1
@@ -87,6 +89,11 @@ class BsdfDesc(C.Structure):
         ("specular_sampling_rate", C.c_float),
         ("clearcoat_sampling_rate", C.c_float),
         ("diffuse_sampling_rate", C.c_float),
+        ("diffuse", F3),
+        ("specular_reflectance", F3),
+        ("nonlinear", C.c_int32),
+        ("nested", C.c_int32),
+        ("opacity", C.c_float),
     ]
 
 
@@ -96,6 +103,7 @@ class LightDesc(C.Structure):
         ("shape", C.c_int32),
         ("radiance", F3),
         ("to_world", F16),
+        ("fov", C.c_float),
     ]
 
 

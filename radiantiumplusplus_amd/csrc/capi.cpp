@@ -227,7 +227,7 @@ rad_status rad_hip_autotune(rad_scene* scene, const rad_render_desc* probe, rad_
       d.blocks_per_cu = bpc;
       d.trace_blocks_per_cu = tpc;
       d.flags &= ~uint32_t(RAD_RENDER_PROFILE);
-      rad_job* j = StartJob(scene, d);
+      rad_job* j = StartJob(scene, d, TakeSpareArena(scene));
       rad_job_stats st{};
       try {
         WaitJob(j, nullptr, nullptr, &st);
@@ -451,7 +451,7 @@ rad_status rad_hip_render_start(rad_scene* scene, const rad_render_desc* desc, r
   *out = nullptr;
   return Guard([&] {
     RequireBvh(scene);
-    *out = StartJob(scene, *desc);
+    *out = StartJob(scene, *desc, TakeSpareArena(scene));
   });
 }
 

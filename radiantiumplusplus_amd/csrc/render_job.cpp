@@ -517,9 +517,22 @@ void DestroyJob(rad_job* job) {
     job->th.join();
   }
   (void)hipSetDevice(job->scene->ctx->device);
-  if (job->stream) (void)hipStreamDestroy(job->stream);
+  if (job->stream) {
+    (void)hipStreamSynchronize(job->stream);  // nothing of this job may still write the arena
+    (void)hipStreamDestroy(job->stream);
+  }
   if (job->h_cnt) (void)hipHostFree(job->h_cnt);
+  if (job->arena) {  // keep the larger arena on the scene for its next job
+    rad_scene* sc = job->scene;
+    std::lock_guard<std::mutex> lk(sc->arena_mu);
+    if (!sc->spare_arena || sc->spare_arena->bytes < job->arena->bytes) sc->spare_arena = std::move(job->arena);
+  }
   delete job;
+}
+
+std::unique_ptr<DeviceBuffer> TakeSpareArena(rad_scene* scene) {
+  std::lock_guard<std::mutex> lk(scene->arena_mu);
+  return std::move(scene->spare_arena);
 }
 
 }  // namespace rad

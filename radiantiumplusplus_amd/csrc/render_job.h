@@ -22,6 +22,8 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
 bool JobComplete(const rad_job* job);
 // Detach the job's arena (joins its thread first) so a later job can reuse it.
 std::unique_ptr<DeviceBuffer> TakeArena(rad_job* job);
+// The scene's spare arena (left by the last destroyed job), or null.
+std::unique_ptr<DeviceBuffer> TakeSpareArena(rad_scene* scene);
 void PollJob(rad_job* job, uint64_t* done, uint64_t* total, int32_t* complete);
 void CancelJob(rad_job* job);
 void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats);

@@ -91,5 +91,10 @@ struct rad_scene {
   std::mutex guide_mu;  // held by a render that trains the guide
   // launch configuration chosen by rad_hip_autotune (0 = library default)
   uint32_t tuned_slots = 0, tuned_blocks_per_cu = 0, tuned_trace_blocks_per_cu = 0;
+  // The arena of the last finished render job, kept for the next one: fresh
+  // device allocations are cleared by the driver in the background, which
+  // competed with the first renders after each allocation.
+  std::unique_ptr<rad::DeviceBuffer> spare_arena;
+  std::mutex arena_mu;
   uint64_t device_bytes() const;
 };

@@ -246,6 +246,37 @@ class SceneBuilder:
         self.bsdfs.append(d)
         return len(self.bsdfs) - 1
 
+    def plastic(self, diffuse=0.5, specular=1.0, int_ior=1.49, ext_ior=1.000277, nonlinear=False):
+        self.bsdfs.append({"type": "plastic", "diffuse": _f3(diffuse), "specular": _f3(specular),
+                           "int_ior": float(int_ior), "ext_ior": float(ext_ior), "no_linear": bool(nonlinear)})
+        return len(self.bsdfs) - 1
+
+    def rough_plastic(self, alpha=0.1, diffuse=0.5, specular=1.0, int_ior=1.49, ext_ior=1.000277, nonlinear=False,
+                      distribution="ggx", sample_visible=True):
+        self.bsdfs.append({"type": "rough_plastic", "alpha": float(alpha), "diffuse": _f3(diffuse),
+                           "specular": _f3(specular), "int_ior": float(int_ior), "ext_ior": float(ext_ior),
+                           "no_linear": bool(nonlinear), "distribution": distribution,
+                           "sample_visible": bool(sample_visible)})
+        return len(self.bsdfs) - 1
+
+    def perfect_mirror(self, reflectance=1.0, eta=0.0, k=1.0):
+        self.bsdfs.append({"type": "perfect_mirror", "reflectance": _f3(reflectance), "eta": _f3(eta), "k": _f3(k)})
+        return len(self.bsdfs) - 1
+
+    def perfect_glass(self, int_ior=1.5046, ext_ior=1.000277, reflectance=1.0, transmittance=1.0):
+        self.bsdfs.append({"type": "perfect_glass", "int_ior": float(int_ior), "ext_ior": float(ext_ior),
+                           "reflectance": _f3(reflectance), "transmittance": _f3(transmittance)})
+        return len(self.bsdfs) - 1
+
+    def two_side(self, inner):
+        """`two_side` with the same BSDF on both faces (the "two_side" key form)."""
+        self.bsdfs.append({"type": "two_side", "two_side": dict(self.bsdfs[inner])})
+        return len(self.bsdfs) - 1
+
+    def mask(self, inner, opacity=0.5):
+        self.bsdfs.append({"type": "mask", "opacity": float(opacity), "nested": dict(self.bsdfs[inner])})
+        return len(self.bsdfs) - 1
+
     # -- media -----------------------------------------------------------
     def homogeneous(self, sigma_t=1.0, albedo=0.75, scale=1.0, g=None):
         phase = {"type": "isotropic_phase"} if g is None else {"type": "hg_phase", "g": float(g)}
@@ -298,6 +329,23 @@ class SceneBuilder:
             self.shapes[shape].bsdf = self.diffuse(0.5)
             self.shapes[shape].json_shape["auto_bsdf"] = True
         return li
+
+    def point_light(self, position, intensity=1.0):
+        """`point` light (light/point.cpp) at a world position."""
+        tw = np.eye(4, dtype=np.float32)
+        tw[:3, 3] = position
+        self.lights.append({"type": "point", "radiance": _f3(intensity), "shape": -1, "to_world": tw,
+                            "position": [float(x) for x in position]})
+        return len(self.lights) - 1
+
+    def projection_light(self, to_world, irradiance=1.0, scale=1.0, fov=90.0):
+        """`projection` light (light/projection.cpp) with a constant irradiance,
+        placed by to_world and projecting along its +z."""
+        irr, sc = _f3(irradiance), _f3(scale)
+        self.lights.append({"type": "projection", "radiance": [float(np.float32(a) * np.float32(b)) for a, b in zip(irr, sc)],
+                            "irradiance": irr, "scale": sc, "fov": float(fov), "shape": -1,
+                            "to_world": np.asarray(to_world, np.float32)})
+        return len(self.lights) - 1
 
     def skybox(self, radiance, to_world=IDENTITY):
         self.lights.append({"type": "skybox", "radiance": _f3(radiance), "shape": -1,
@@ -381,9 +429,55 @@ class SceneBuilder:
             b.specular_sampling_rate = d.get("main_specular_sampling_rate", 1.0)
             b.clearcoat_sampling_rate = d.get("clearcoat_sampling_rate", 1.0)
             b.diffuse_sampling_rate = d.get("diffuse_reflectance_sampling_rate", 1.0)
+        elif t in ("plastic", "rough_plastic"):
+            b.type = F.RAD_BSDF_PLASTIC if t == "plastic" else F.RAD_BSDF_ROUGH_PLASTIC
+            b.diffuse[:] = d.get("diffuse", [0.5] * 3)
+            b.specular_reflectance[:] = d.get("specular", [1.0] * 3)
+            b.int_ior = d.get("int_ior", 1.49)
+            b.ext_ior = d.get("ext_ior", 1.000277)
+            b.nonlinear = 1 if d.get("no_linear", False) else 0
+            b.distribution = F.RAD_MICROFACET_BECKMANN if d.get("distribution") == "beckmann" else F.RAD_MICROFACET_GGX
+            b.sample_visible = 1 if d.get("sample_visible", True) else 0
+            b.alpha_u = b.alpha_v = d.get("alpha", 0.1)
+        elif t == "perfect_mirror":
+            b.type = F.RAD_BSDF_PERFECT_MIRROR
+            b.reflectance[:] = d.get("reflectance", [1.0] * 3)
+            b.eta[:] = d.get("eta", [0.0] * 3)
+            b.k[:] = d.get("k", [1.0] * 3)
+        elif t == "perfect_glass":
+            b.type = F.RAD_BSDF_PERFECT_GLASS
+            b.reflectance[:] = d.get("reflectance", [1.0] * 3)
+            b.transmittance[:] = d.get("transmittance", [1.0] * 3)
+            b.int_ior = d.get("int_ior", 1.5046)
+            b.ext_ior = d.get("ext_ior", 1.000277)
+        elif t in ("two_side", "mask"):
+            b.type = F.RAD_BSDF_TWO_SIDE if t == "two_side" else F.RAD_BSDF_MASK
+            b.opacity = d.get("opacity", 0.5)
+            b.nested = -1  # set by bsdf_array
         else:
             raise ValueError(f"unsupported bsdf {t}")
         return b
+
+    def bsdf_array(self):
+        """The BSDF descriptors, the wrappers' nested BSDFs appended after the
+        builder's own entries (shape indices stay valid)."""
+        out = [self.bsdf_desc(d) for d in self.bsdfs]
+
+        def nest(i, d):
+            inner = d.get("two_side") if d["type"] == "two_side" else d.get("nested")
+            j = len(out)
+            out.append(self.bsdf_desc(inner))
+            out[i].nested = j
+            if inner["type"] in ("two_side", "mask"):
+                nest(j, inner)
+
+        for i, d in enumerate(self.bsdfs):
+            if d["type"] in ("two_side", "mask"):
+                nest(i, d)
+        arr = (F.BsdfDesc * max(len(out), 1))()
+        for i, b in enumerate(out):
+            arr[i] = b
+        return arr, len(out)
 
     def to_desc(self):
         self._keep = []
@@ -407,12 +501,12 @@ class SceneBuilder:
                 d.indices = self._ptr(idx, C.c_uint32)
             d.center[:] = s.center
             d.radius = s.radius
-        bsdfs = (F.BsdfDesc * max(len(self.bsdfs), 1))()
-        for i, b in enumerate(self.bsdfs):
-            bsdfs[i] = self.bsdf_desc(b)
+        bsdfs, bsdf_count = self.bsdf_array()
         lights = (F.LightDesc * max(len(self.lights), 1))()
         for i, l in enumerate(self.lights):
-            lights[i].type = F.RAD_LIGHT_AREA if l["type"] == "area" else F.RAD_LIGHT_SKYBOX
+            lights[i].type = {"area": F.RAD_LIGHT_AREA, "skybox": F.RAD_LIGHT_SKYBOX, "point": F.RAD_LIGHT_POINT,
+                              "projection": F.RAD_LIGHT_PROJECTION}[l["type"]]
+            lights[i].fov = l.get("fov", 90.0)
             lights[i].shape = l.get("shape", -1)
             lights[i].radiance[:] = l["radiance"]
             lights[i].to_world[:] = [float(x) for x in np.asarray(l.get("to_world", IDENTITY), np.float32).reshape(-1)]
@@ -437,7 +531,7 @@ class SceneBuilder:
         desc = F.SceneDesc()
         desc.shape_count = n
         desc.shapes = shapes
-        desc.bsdf_count = len(self.bsdfs)
+        desc.bsdf_count = bsdf_count
         desc.bsdfs = bsdfs
         desc.light_count = len(self.lights)
         desc.lights = lights
@@ -498,6 +592,12 @@ class SceneBuilder:
         for l in self.lights:
             if l["type"] == "skybox":
                 entities.append({"to_world": fmt(l["to_world"]), "light": {"type": "skybox", "radiance": l["radiance"]}})
+            elif l["type"] == "point":
+                entities.append({"light": {"type": "point", "position": l["position"], "intensity": l["radiance"]}})
+            elif l["type"] == "projection":
+                entities.append({"to_world": fmt(l["to_world"]),
+                                 "light": {"type": "projection", "irradiance": l["irradiance"], "scale": l["scale"],
+                                           "fov": l["fov"]}})
         cam = self.camera
         camj = {"type": "perspective", "fov": cam["fov"], "near": cam["near"], "far": cam["far"],
                 "resolution": [cam["width"], cam["height"]], "to_world": fmt(cam["to_world"])}

@@ -52,8 +52,8 @@ def lib():
         L.ref_intersect_prim.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
         L.ref_camera_ray.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
         L.ref_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
-        L.ref_bsdf_eval.argtypes = [C.POINTER(F.BsdfDesc), C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
-        L.ref_bsdf_sample.argtypes = [C.POINTER(F.BsdfDesc), C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ref_bsdf_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ref_bsdf_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_grid_eval.argtypes = [C.POINTER(F.MediumDesc), C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_erfinv.restype = C.c_float
         L.ref_erfinv.argtypes = [C.c_float]
@@ -128,11 +128,20 @@ class OracleScene:
         return fb, sq, secs.value
 
 
+def _desc_array(desc):
+    """A single descriptor, or (array, count) from SceneBuilder.bsdf_array()
+    with the probed BSDF first."""
+    if isinstance(desc, tuple):
+        return C.cast(desc[0], C.c_void_p), desc[1]
+    return C.cast(C.pointer(desc), C.c_void_p), 1
+
+
 def bsdf_eval(desc, wi, wo, quirks=True):
     n = len(wi)
     inp = np.ascontiguousarray(np.concatenate([wi, wo], 1), np.float32)
     out = np.zeros((n, 4), np.float32)
-    if lib().ref_bsdf_eval(C.byref(desc), 1 if quirks else 0, inp.ctypes.data, n, out.ctypes.data):
+    arr, cnt = _desc_array(desc)
+    if lib().ref_bsdf_eval(arr, cnt, 1 if quirks else 0, inp.ctypes.data, n, out.ctypes.data):
         raise RuntimeError(lib().ref_last_error().decode())
     return out
 
@@ -141,7 +150,8 @@ def bsdf_sample(desc, wi, lobe, xi, quirks=True):
     n = len(wi)
     inp = np.ascontiguousarray(np.concatenate([wi, lobe.reshape(-1, 1), xi], 1), np.float32)
     out = np.zeros((n, 8), np.float32)
-    if lib().ref_bsdf_sample(C.byref(desc), 1 if quirks else 0, inp.ctypes.data, n, out.ctypes.data):
+    arr, cnt = _desc_array(desc)
+    if lib().ref_bsdf_sample(arr, cnt, 1 if quirks else 0, inp.ctypes.data, n, out.ctypes.data):
         raise RuntimeError(lib().ref_last_error().decode())
     return out
 

@@ -311,3 +311,178 @@ def test_sample_filter_counts_dropped_samples():
     fb1, _, _ = s.render(spp=4, threads=2)
     fb2, _, _ = s.render(spp=16, threads=2)
     assert np.allclose(fb1[0, 0], 1.0) and np.allclose(fb2[0, 0], 1.0)
+
+
+# ------------------------------------------------------------------ remaining BSDFs and lights
+def fresnel_dielectric(cos_i, eta):
+    """Unpolarised dielectric reflectance (exact), eta = n_t / n_i."""
+    cos_i = np.asarray(cos_i, np.float64)
+    eta = np.where(cos_i >= 0, eta, 1 / eta)
+    c = np.abs(cos_i)
+    s2t = (1 - c * c) / (eta * eta)
+    tir = s2t >= 1
+    ct = np.sqrt(np.maximum(0, 1 - s2t))
+    rs = (c - eta * ct) / (c + eta * ct)
+    rp = (eta * c - ct) / (eta * c + ct)
+    return np.where(tir, 1.0, 0.5 * (rs * rs + rp * rp))
+
+
+def test_plastic_diffuse_term_and_fresnel_average():
+    """plastic.cpp: f = diffuse/pi * (1/eta^2) (1 - F(wi)) (1 - F(wo)) cos_o / (1 - Fdr(1/eta)),
+    Fdr the hemispherical average of the internal Fresnel reflectance (the
+    polynomial fit of fresnel.cpp:79-84, checked here against quadrature)."""
+    eta = 1.49 / 1.000277
+    d = bsdf("plastic", diffuse=(0.3, 0.5, 0.7))
+    wi, wo = hemi(256), hemi(256)
+    out = O.bsdf_eval(d, wi, wo)
+    fi, fo = fresnel_dielectric(wi[:, 2], eta), fresnel_dielectric(wo[:, 2], eta)
+    mu = (np.arange(200000) + 0.5) / 200000
+    fdr = np.mean(fresnel_dielectric(mu, 1 / eta) * 2 * mu)
+    expect = np.array([0.3, 0.5, 0.7]) / np.pi / eta ** 2 * ((1 - fi) * (1 - fo) * wo[:, 2])[:, None] / (1 - fdr)
+    assert np.allclose(out[:, :3], expect, rtol=2e-3)
+    # pdf: cosine lobe times the diffuse selection probability
+    w = 0.2126 * 1 + 0.7152 * 1 + 0.0722 * 1
+    wd = 0.212671 * 0.3 + 0.715160 * 0.5 + 0.072169 * 0.7
+    ps = fi * (w / (w + wd))
+    pdd = (1 - fi) * (1 - w / (w + wd))
+    assert np.allclose(out[:, 3], wo[:, 2] / np.pi * pdd / (pdd + ps), rtol=2e-3)
+
+
+def test_plastic_sampling_albedo():
+    """E[f / pdf] over plastic's two lobes = specular F(wi) + the quadrature of
+    the diffuse term (probabilities normalised correctly)."""
+    d = bsdf("plastic", diffuse=(0.6, 0.4, 0.2), specular=(0.9, 0.9, 0.9))
+    wi = unit(np.array([[0.3, 0.2, 0.8]], np.float32))
+    samp = albedo_by_sampling(d, wi, 400_000, quirks=True)
+    quad = albedo_by_quadrature(d, wi, 1_000_000) + 0.9 * fresnel_dielectric(wi[0, 2], 1.49 / 1.000277)
+    assert np.allclose(samp, quad, rtol=0.01)
+
+
+@pytest.mark.parametrize("dist", ["ggx", "beckmann"])
+def test_rough_plastic_consistent_and_unbiased(dist):
+    d = bsdf("rough_plastic", alpha=0.3, diffuse=(0.5, 0.3, 0.2), distribution=dist)
+    n = 2000
+    wi = hemi(n)
+    s = O.bsdf_sample(d, wi, RNG.random(n, dtype=np.float32), RNG.random((n, 2), dtype=np.float32), quirks=False)
+    ok = s[:, 3] > 0
+    e = O.bsdf_eval(d, wi[ok], s[ok, :3], quirks=False)
+    assert np.allclose(e[:, 3], s[ok, 3], rtol=2e-3, atol=1e-6)
+    assert np.allclose(e[:, :3], s[ok, 4:7], rtol=2e-3, atol=1e-6)
+    w0 = unit(np.array([[0.5, -0.2, 0.7]], np.float32))
+    assert np.allclose(albedo_by_sampling(d, w0, 400_000, quirks=False), albedo_by_quadrature(d, w0, 2_000_000),
+                       rtol=0.02)
+
+
+def test_perfect_mirror_closed_form():
+    eta, k = np.array([0.2, 0.9, 1.1]), np.array([3.9, 2.4, 2.1])
+    d = bsdf("perfect_mirror", reflectance=(0.9, 0.8, 0.7), eta=eta, k=k)
+    wi = np.array([[0, 0, 1], [0.6, 0, 0.8]], np.float32)
+    s = O.bsdf_sample(d, wi, np.full(2, 0.5, np.float32), np.full((2, 2), 0.5, np.float32))
+    assert np.allclose(s[:, :3], wi * np.array([-1, -1, 1]), atol=1e-7)
+    assert np.all(s[:, 3] == 1)
+    F1 = ((eta - 1) ** 2 + k ** 2) / ((eta + 1) ** 2 + k ** 2)
+    assert np.allclose(s[0, 4:7], np.array([0.9, 0.8, 0.7]) * F1, rtol=1e-5)
+    assert np.all(O.bsdf_eval(d, wi, wi)[:, 3] == 0)
+
+
+def test_perfect_glass_snell_and_weights():
+    eta = 1.5
+    d = bsdf("perfect_glass", int_ior=eta, ext_ior=1.0, transmittance=(0.9, 0.9, 0.9))
+    wi = unit(np.array([[0.5, 0.1, 0.6]] * 2 + [[0.4, 0.0, -0.7]], np.float32))
+    ri = fresnel_dielectric(wi[:, 2], eta)
+    s = O.bsdf_sample(d, wi, np.array([0.0, 0.999, 0.999], np.float32), np.zeros((3, 2), np.float32))
+    # lobe 0: reflection with probability F
+    assert np.allclose(s[0, :3], wi[0] * [-1, -1, 1], atol=1e-7) and np.isclose(s[0, 3], ri[0], rtol=1e-5)
+    for i, (ni, nt) in ((1, (1.0, eta)), (2, (eta, 1.0))):
+        wo = s[i, :3]
+        sin_i, sin_t = np.linalg.norm(wi[i, :2]), np.linalg.norm(wo[:2])
+        assert np.sign(wo[2]) == -np.sign(wi[i, 2])
+        assert np.isclose(ni * sin_i, nt * sin_t, rtol=1e-5)
+        assert np.isclose(s[i, 3], 1 - ri[i], rtol=1e-5)
+        assert np.allclose(s[i, 4:7] / s[i, 3], 0.9 * (ni / nt) ** 2, rtol=1e-5)
+        assert np.isclose(s[i, 7], nt / ni, rtol=1e-6)
+
+
+def test_two_side_mirrors_the_lower_hemisphere():
+    b = scenes.SceneBuilder()
+    inner = b.rough_metal(alpha=0.3)
+    ts = b.two_side(inner)
+    b.bsdfs = [b.bsdfs[ts], b.bsdfs[inner]]
+    arr = b.bsdf_array()
+    base = b.bsdf_desc(b.bsdfs[1])
+    wi, wo = hemi(128), hemi(128)
+    flip = np.array([1, 1, -1], np.float32)
+    lo = O.bsdf_eval(arr, wi * flip, wo * flip)
+    hi = O.bsdf_eval(base, wi, wo)
+    assert np.array_equal(lo, hi)
+    assert np.array_equal(O.bsdf_eval(arr, wi, wo), hi)
+
+
+def test_mask_scales_and_passes_through():
+    b = scenes.SceneBuilder()
+    inner = b.diffuse((0.4, 0.5, 0.6))
+    m = b.mask(inner, opacity=0.3)
+    b.bsdfs = [b.bsdfs[m], b.bsdfs[inner]]
+    arr = b.bsdf_array()
+    base = b.bsdf_desc(b.bsdfs[1])
+    wi, wo = hemi(64), hemi(64)
+    assert np.allclose(O.bsdf_eval(arr, wi, wo), O.bsdf_eval(base, wi, wo) * 0.3, rtol=1e-6)
+    n = 100000
+    wis = np.tile(hemi(1), (n, 1))
+    s = O.bsdf_sample(arr, wis, RNG.random(n, dtype=np.float32), RNG.random((n, 2), dtype=np.float32))
+    through = np.all(np.isclose(s[:, :3], -wis), axis=1)
+    assert abs(through.mean() - 0.7) < 0.01
+    assert np.allclose(s[through, 3], 0.7) and np.allclose(s[through, 4:7], 1.0)
+
+
+def light_plane_scene(light):
+    """A diffuse plane y = 0 (rho 0.5) seen from above by a 24x24 camera looking down."""
+    b = scenes.SceneBuilder()
+    b.rectangle(to_world=scenes.trs(axis=(1, 0, 0), angle=-90, scale=(4, 4, 4)), bsdf=b.diffuse(0.5))
+    light(b)
+    b.set_camera(24, 24, 60, origin=(0, 5, 0.001), target=(0, 0, 0), up=(0, 0, 1), far=1000)
+    b.set_render("path", spp=16, max_depth=-1, rr_depth=3, seed=1)
+    return b
+
+
+def plane_points(o):
+    """Pixel-centre camera rays on the y = 0 plane."""
+    pts = np.zeros((24, 24, 3))
+    for y in range(24):
+        for x in range(24):
+            r = o.camera_ray(x + 0.5, y + 0.5)
+            t = -r["o"][1] / r["d"][1]
+            pts[y, x] = r["o"] + t * r["d"]
+    return pts
+
+
+def test_point_light_closed_form():
+    """L = rho/pi * I cos(theta) / d^2 (light/point.cpp:24-38)."""
+    I = np.array([20.0, 10.0, 5.0])
+    b = light_plane_scene(lambda b: b.point_light((0.3, 2.0, -0.2), I))
+    o = O.OracleScene(b)
+    fb, _ = o.render(spp=64, seed=3)[:2]
+    p = plane_points(o)
+    d = np.array([0.3, 2.0, -0.2]) - p
+    dist = np.linalg.norm(d, axis=-1)
+    expect = 0.5 / np.pi * I * ((d[..., 1] / dist) / dist ** 2)[..., None]
+    assert np.allclose(fb, expect, rtol=0.03)
+
+
+def test_projection_light_closed_form():
+    """A projection light 3 above the plane looking down (constant irradiance
+    E0, scale s): inside the frustum Li = E0 s pi d / z^3 (light/projection.cpp:45-60),
+    so the plane's irradiance is E0 s pi / z^2 and L = rho E0 s / z^2; outside
+    the frustum nothing."""
+    tw = np.array([[1, 0, 0, 0], [0, 0, -1, 3], [0, 1, 0, 0], [0, 0, 0, 1]], np.float32)  # +z -> -y
+    b = light_plane_scene(lambda b: b.projection_light(tw, irradiance=(2.0, 1.0, 0.5), scale=1.5, fov=40.0))
+    o = O.OracleScene(b)
+    fb, _ = o.render(spp=16, seed=3)[:2]
+    p = plane_points(o)
+    half = 3 * math.tan(math.radians(20))
+    inside = (np.abs(p[..., 0]) < 0.9 * half) & (np.abs(p[..., 2]) < 0.9 * half)
+    outside = (np.abs(p[..., 0]) > 1.1 * half) | (np.abs(p[..., 2]) > 1.1 * half)
+    assert inside.sum() > 20 and outside.sum() > 20
+    expect = 0.5 * np.array([2.0, 1.0, 0.5]) * 1.5 / 9
+    assert np.allclose(fb[inside], expect, rtol=1e-3)
+    assert np.all(fb[outside] == 0)
