@@ -1,5 +1,7 @@
-// Device BSDFs on the path: diffuse, rough_metal, rough_glass, disney, with the
-// GGX / Beckmann microfacet distributions and Fresnel terms.  Directions are in
+// Device BSDFs on the path: diffuse, rough_metal, rough_glass, disney,
+// plastic, rough_plastic, perfect_mirror, perfect_glass and the two_side /
+// mask wrappers, with the GGX / Beckmann microfacet distributions and
+// Fresnel terms.  Directions are in
 // the shading frame (z = shading normal); `wi` points away from the surface
 // toward the previous vertex (SurfaceInteraction::Wi).
 #pragma once
@@ -625,7 +627,122 @@ __device__ inline f3 disney_sample(const BsdfRec& b, bool quirks, f3 wi, float l
   return disney_f(b, quirks, wi, s.wo);
 }
 
-// ---------------------------------------------------------------- dispatch
+// ---------------------------------------------------------------- plastic (plastic.cpp:18-132)
+// The diffuse term under the dielectric coating, shared with rough_plastic:
+// diffuse / pi (1 / eta^2) (1 - Fi) (1 - Fo) cos_o / (1 - Fdr), or the
+// per-channel 1 / (1 - diffuse Fdr) with "no_linear".
+__device__ inline f3 plastic_diffuse(const BsdfRec& b, float fi, float fo, float co) {
+  f3 num = b.diffuse3 * kInvPi * b.inv_eta2 * (1.f - fi) * (1.f - fo) * co;
+  if (b.nonlinear)
+    return mk(num.x * (1.f / (1.f - b.diffuse3.x * b.fdr_int)), num.y * (1.f / (1.f - b.diffuse3.y * b.fdr_int)),
+              num.z * (1.f / (1.f - b.diffuse3.z * b.fdr_int)));
+  return num / (1.f - b.fdr_int);
+}
+__device__ inline float plastic_prob_spec(const BsdfRec& b, float fi) {
+  float ps = fi * b.spec_weight, pd = (1.f - fi) * (1.f - b.spec_weight);
+  return ps / (ps + pd);
+}
+__device__ inline f3 cosine_dir(float u1, float u2) {  // SquareToCosineHemisphere (warp.cpp:33-50)
+  float r = sqrtf(u1), ang = 2.f * kPi * u2;
+  float x = r * cosf(ang), y = r * sinf(ang);
+  return mk(x, y, safe_sqrt(1.f - sqr(x) - sqr(y)));
+}
+__device__ inline f3 plastic_eval(const BsdfRec& b, f3 wi, f3 wo, float& pdf) {
+  pdf = 0.f;
+  if (wi.z <= 0.f || wo.z <= 0.f) return splat(0.f);
+  float fi = fresnel_dielectric(wi.z, b.eta).r, fo = fresnel_dielectric(wo.z, b.eta).r;
+  float ps = fi * b.spec_weight, pd = (1.f - fi) * (1.f - b.spec_weight);
+  pdf = wo.z * kInvPi * (pd / (pd + ps));
+  return plastic_diffuse(b, fi, fo, wo.z);
+}
+__device__ inline f3 plastic_sample(const BsdfRec& b, f3 wi, float lobe, float u1, float u2, BsdfSample& s) {
+  s.pdf = 0.f;
+  if (wi.z <= 0.f) return splat(0.f);
+  float fi = fresnel_dielectric(wi.z, b.eta).r;
+  float ps = plastic_prob_spec(b, fi);
+  s.eta = 1.f;
+  if (lobe < ps) {
+    s.wo = mk(-wi.x, -wi.y, wi.z);
+    s.pdf = ps;
+    s.lobes = LOBE_REFLECTION | LOBE_DELTA;
+    return b.spec3 * fi;
+  }
+  s.wo = cosine_dir(u1, u2);
+  s.pdf = s.wo.z * kInvPi * (1.f - ps);
+  s.lobes = LOBE_REFLECTION | LOBE_DIFFUSE;
+  return plastic_diffuse(b, fi, fresnel_dielectric(s.wo.z, b.eta).r, s.wo.z);
+}
+
+// ---------------------------------------------------------------- rough_plastic (rough_plastic.cpp:18-250)
+__device__ inline Microfacet mf_iso(const BsdfRec& b, bool quirks) {
+  return Microfacet{b.alpha_u, b.alpha_u, b.beckmann != 0, b.sample_visible != 0, quirks};
+}
+__device__ inline f3 rough_plastic_eval(const BsdfRec& b, bool quirks, f3 wi, f3 wo, float& pdf) {
+  pdf = 0.f;
+  if (wi.z <= 0.f || wo.z <= 0.f) return splat(0.f);
+  Microfacet mf = mf_iso(b, quirks);
+  f3 wh = normalize(wo + wi);
+  float F = fresnel_dielectric(dot(wi, wh), b.eta).r;
+  f3 spec = b.spec3 * mf.D(wh) * mf.G(wi, wo, wh) * F / (4.f * wi.z);
+  float fo = fresnel_dielectric(dot(wo, wh), b.eta).r;
+  float ps = plastic_prob_spec(b, fresnel_dielectric(wi.z, b.eta).r);
+  pdf = mf.Pdf(wi, wh) / (4.f * dot(wo, wh)) * ps + wo.z * kInvPi * (1.f - ps);
+  return spec + plastic_diffuse(b, F, fo, wo.z);
+}
+__device__ inline f3 rough_plastic_sample(const BsdfRec& b, bool quirks, f3 wi, float lobe, float u1, float u2,
+                                          BsdfSample& s) {
+  s.pdf = 0.f;
+  if (wi.z <= 0.f) return splat(0.f);
+  float ps = plastic_prob_spec(b, fresnel_dielectric(wi.z, b.eta).r);
+  s.eta = 1.f;
+  if (lobe < ps) {
+    Microfacet mf = mf_iso(b, quirks);
+    float mpdf;
+    f3 m = mf.Sample(wi, u1, u2, mpdf);
+    if (mpdf <= 0.f) return splat(0.f);
+    s.wo = reflect_wh(wi, m);
+    s.lobes = LOBE_GLOSSY | LOBE_REFLECTION;
+  } else {
+    s.wo = cosine_dir(u1, u2);
+    s.lobes = LOBE_DIFFUSE | LOBE_REFLECTION;
+  }
+  float pdf;
+  f3 f = rough_plastic_eval(b, quirks, wi, s.wo, pdf);
+  if (!(pdf > 0.f)) return splat(0.f);
+  s.pdf = pdf;
+  return f;
+}
+
+// ---------------------------------------------------------------- perfect_mirror / perfect_glass
+// (perfect_mirror.cpp:24-43, perfect_glass.cpp:26-65): delta lobes only, so
+// Eval and Pdf are zero.
+__device__ inline f3 mirror_sample(const BsdfRec& b, f3 wi, BsdfSample& s) {
+  s.pdf = 0.f;
+  if (wi.z <= 0.f) return splat(0.f);
+  s.wo = mk(-wi.x, -wi.y, wi.z);
+  s.eta = 1.f;
+  s.pdf = 1.f;
+  s.lobes = LOBE_DELTA | LOBE_REFLECTION;
+  return mul(b.reflectance, fresnel_conductor(wi.z, b.eta3, b.k3));
+}
+__device__ inline f3 perfect_glass_sample(const BsdfRec& b, f3 wi, float lobe, BsdfSample& s) {
+  Dielectric fd = fresnel_dielectric(wi.z, b.eta);
+  float ri = fd.r, ti = 1.f - ri;
+  if (lobe <= ri) {
+    s.wo = mk(-wi.x, -wi.y, wi.z);
+    s.eta = 1.f;
+    s.pdf = ri;
+    s.lobes = LOBE_DELTA | LOBE_REFLECTION;
+    return b.reflectance * ri;
+  }
+  s.wo = mk(-fd.eta_ti * wi.x, -fd.eta_ti * wi.y, fd.cos_t);
+  s.eta = fd.eta_it;
+  s.pdf = ti;
+  s.lobes = LOBE_DELTA | LOBE_TRANSMISSION;
+  return b.transmittance * ti * sqr(fd.eta_ti);
+}
+
+// ---------------------------------------------------------------- dispatch over the base BSDFs
 __device__ inline f3 bsdf_eval(const BsdfRec& b, bool quirks, f3 wi, f3 wo, float& pdf) {
   switch (b.type) {
     case BSDF_DIFFUSE:
@@ -634,9 +751,16 @@ __device__ inline f3 bsdf_eval(const BsdfRec& b, bool quirks, f3 wi, f3 wo, floa
       return metal_eval(b, quirks, wi, wo, pdf);
     case BSDF_ROUGH_GLASS:
       return glass_eval(b, quirks, wi, wo, pdf);
-    default:
+    case BSDF_DISNEY:
       pdf = disney_pdf(b, quirks, wi, wo);
       return disney_f(b, quirks, wi, wo);
+    case BSDF_PLASTIC:
+      return plastic_eval(b, wi, wo, pdf);
+    case BSDF_ROUGH_PLASTIC:
+      return rough_plastic_eval(b, quirks, wi, wo, pdf);
+    default:  // perfect mirror / glass
+      pdf = 0.f;
+      return splat(0.f);
   }
 }
 
@@ -648,15 +772,93 @@ __device__ inline f3 bsdf_sample(const BsdfRec& b, bool quirks, f3 wi, float lob
       return metal_sample(b, quirks, wi, u1, u2, s);
     case BSDF_ROUGH_GLASS:
       return glass_sample(b, quirks, wi, lobe, u1, u2, s);
-    default:
+    case BSDF_DISNEY:
       return disney_sample(b, quirks, wi, lobe, u1, u2, s);
+    case BSDF_PLASTIC:
+      return plastic_sample(b, wi, lobe, u1, u2, s);
+    case BSDF_ROUGH_PLASTIC:
+      return rough_plastic_sample(b, quirks, wi, lobe, u1, u2, s);
+    case BSDF_PERFECT_MIRROR:
+      return mirror_sample(b, wi, s);
+    default:
+      return perfect_glass_sample(b, wi, lobe, s);
   }
 }
 
+// ---------------------------------------------------------------- two_side / mask wrappers
+// (two_side.cpp:60-137, mask.cpp:17-53).  At most two wrappers enclose a
+// base BSDF (the scene loader checks).  two_side evaluates the lower
+// hemisphere through the mirrored directions; mask scales by its opacity and
+// otherwise passes the path straight through (weight 1, pdf 1 - opacity, no
+// lobe type, as the reference records it).
+__device__ inline f3 bsdf_eval_w(const BsdfRec* all, const BsdfRec& top, bool quirks, f3 wi, f3 wo, float& pdf) {
+  const BsdfRec* b = &top;
+  float scale = 1.f;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (b->type == BSDF_MASK) {
+      scale *= b->opacity;
+      b = &all[b->nested];
+    } else if (b->type == BSDF_TWO_SIDE) {
+      if (wi.z == 0.f) {
+        pdf = 0.f;
+        return splat(0.f);
+      }
+      if (wi.z < 0.f) {
+        wi.z = -wi.z;
+        wo.z = -wo.z;
+      }
+      b = &all[b->nested];
+    }
+  }
+  f3 f = bsdf_eval(*b, quirks, wi, wo, pdf);
+  pdf *= scale;
+  return f * scale;
+}
+
+__device__ inline f3 bsdf_sample_w(const BsdfRec* all, const BsdfRec& top, bool quirks, f3 wi, float lobe, float u1,
+                                   float u2, BsdfSample& s) {
+  const BsdfRec* b = &top;
+  float scale = 1.f;
+  bool flip = false;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (b->type == BSDF_MASK) {
+      const float op = b->opacity;
+      if (!(lobe < op)) {
+        s.wo = -wi;
+        if (flip) s.wo.z = -s.wo.z;
+        s.pdf = (1.f - op) * scale;
+        s.eta = 1.f;
+        s.lobes = 0u;
+        return splat(scale);
+      }
+      lobe /= op;
+      scale *= op;
+      b = &all[b->nested];
+    } else if (b->type == BSDF_TWO_SIDE) {
+      if (wi.z == 0.f) {
+        s.pdf = 0.f;
+        return splat(0.f);
+      }
+      if (wi.z < 0.f) {
+        wi.z = -wi.z;
+        flip = !flip;
+      }
+      b = &all[b->nested];
+    }
+  }
+  f3 f = bsdf_sample(*b, quirks, wi, lobe, u1, u2, s);
+  if (flip) s.wo.z = -s.wo.z;
+  s.pdf *= scale;
+  return f * scale;
+}
+
 // Material-specialised dispatch for kernels that shade one BSDF type
-// (MAT = BSDF_*; any other value keeps the run-time switch).
+// (MAT = BSDF_DIFFUSE .. BSDF_DISNEY); any other MAT handles every type,
+// wrappers included, at run time.
 template <int MAT>
-__device__ __forceinline__ f3 bsdf_eval_t(const BsdfRec& b, bool quirks, f3 wi, f3 wo, float& pdf) {
+__device__ __forceinline__ f3 bsdf_eval_t(const BsdfRec* all, const BsdfRec& b, bool quirks, f3 wi, f3 wo, float& pdf) {
   if constexpr (MAT == BSDF_DIFFUSE) {
     return diffuse_eval(b, wi, wo, pdf);
   } else if constexpr (MAT == BSDF_ROUGH_METAL) {
@@ -667,13 +869,13 @@ __device__ __forceinline__ f3 bsdf_eval_t(const BsdfRec& b, bool quirks, f3 wi, 
     pdf = disney_pdf(b, quirks, wi, wo);
     return disney_f(b, quirks, wi, wo);
   } else {
-    return bsdf_eval(b, quirks, wi, wo, pdf);
+    return bsdf_eval_w(all, b, quirks, wi, wo, pdf);
   }
 }
 
 template <int MAT>
-__device__ __forceinline__ f3 bsdf_sample_t(const BsdfRec& b, bool quirks, f3 wi, float lobe, float u1, float u2,
-                                            BsdfSample& s) {
+__device__ __forceinline__ f3 bsdf_sample_t(const BsdfRec* all, const BsdfRec& b, bool quirks, f3 wi, float lobe,
+                                            float u1, float u2, BsdfSample& s) {
   if constexpr (MAT == BSDF_DIFFUSE) {
     return diffuse_sample(b, wi, u1, u2, s);
   } else if constexpr (MAT == BSDF_ROUGH_METAL) {
@@ -683,7 +885,7 @@ __device__ __forceinline__ f3 bsdf_sample_t(const BsdfRec& b, bool quirks, f3 wi
   } else if constexpr (MAT == BSDF_DISNEY) {
     return disney_sample(b, quirks, wi, lobe, u1, u2, s);
   } else {
-    return bsdf_sample(b, quirks, wi, lobe, u1, u2, s);
+    return bsdf_sample_w(all, b, quirks, wi, lobe, u1, u2, s);
   }
 }
 

@@ -480,6 +480,8 @@ static rad_status BsdfProbe(const rad_bsdf_desc* bsdf, uint32_t quirks, const fl
   return Guard([&] {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw StatusError(RAD_ERR_HIP, "no HIP device available");
+    if (bsdf->type == RAD_BSDF_TWO_SIDE || bsdf->type == RAD_BSDF_MASK)
+      throw StatusError(RAD_ERR_UNSUPPORTED, "the probes take a base BSDF (two_side / mask wrap another record)");
     BsdfRec b = MakeBsdfRec(*bsdf);
     size_t inBytes = size_t(n) * 6 * sizeof(float), outBytes = size_t(n) * (sample ? 8 : 4) * sizeof(float);
     DeviceBuffer din, dout;

@@ -129,6 +129,7 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u
 // close to their product distribution.
 __device__ __forceinline__ bool guidable(const BsdfRec& b, bool quirks, float min_rough) {
   if (b.lobes & (LOBE_DELTA | LOBE_TRANSMISSION)) return false;
+  if (b.type > BSDF_DISNEY) return false;  // plastics carry a delta lobe; wrappers pass paths through
   if (b.type == BSDF_DIFFUSE) return true;
   const float rough = b.type == BSDF_DISNEY ? b.roughness * b.roughness : fminf(b.alpha_u, b.alpha_v);
   if (rough < min_rough) return false;

@@ -502,9 +502,42 @@ struct Builder {
       } else {
         b.specular = FloatOr(n, "specular", 0.5f);
       }
-    } else if (type == "plastic" || type == "rough_plastic" || type == "perfect_glass" || type == "perfect_mirror" ||
-               type == "two_side" || type == "mask") {
-      throw LoadError(RAD_ERR_UNSUPPORTED, "bsdf '" + type + "' is not on the GPU path yet");
+    } else if (type == "plastic" || type == "rough_plastic") {  // plastic.cpp:20-40, rough_plastic.cpp:20-55
+      b.type = type == "plastic" ? RAD_BSDF_PLASTIC : RAD_BSDF_ROUGH_PLASTIC;
+      b.int_ior = FloatOr(n, "int_ior", 1.49f);
+      b.ext_ior = FloatOr(n, "ext_ior", 1.000277f);
+      Color(n, "diffuse", b.diffuse, 0.5f);
+      Color(n, "specular", b.specular_reflectance, 1.f);
+      b.nonlinear = BoolOr(n, "no_linear", false) ? 1 : 0;
+      if (b.type == RAD_BSDF_ROUGH_PLASTIC) {
+        std::string d = StrOr(n, "distribution", "ggx");
+        b.distribution = d == "beckmann" ? RAD_MICROFACET_BECKMANN : RAD_MICROFACET_GGX;
+        b.sample_visible = BoolOr(n, "sample_visible", true) ? 1 : 0;
+        b.alpha_u = b.alpha_v = Mono(n, "alpha", 0.1f);
+      }
+    } else if (type == "perfect_mirror") {  // perfect_mirror.cpp:16-22
+      b.type = RAD_BSDF_PERFECT_MIRROR;
+      Color(n, "reflectance", b.reflectance, 1.f);
+      Color(n, "eta", b.eta, 0.f);
+      Color(n, "k", b.k, 1.f);
+    } else if (type == "perfect_glass") {  // perfect_glass.cpp:16-23
+      b.type = RAD_BSDF_PERFECT_GLASS;
+      Color(n, "reflectance", b.reflectance, 1.f);
+      Color(n, "transmittance", b.transmittance, 1.f);
+      b.int_ior = FloatOr(n, "int_ior", 1.5046f);
+      b.ext_ior = FloatOr(n, "ext_ior", 1.000277f);
+    } else if (type == "two_side") {  // two_side.cpp:12-58
+      const Value* inner = n.find("two_side");
+      if (!inner)  // the out_side / in_side form reads uninitialised pointers in the reference (two_side.cpp:41-59)
+        throw LoadError(RAD_ERR_UNSUPPORTED, "two_side: only the \"two_side\" form is supported");
+      b.type = RAD_BSDF_TWO_SIDE;
+      b.nested = MakeBsdf(*inner);
+    } else if (type == "mask") {  // mask.cpp:8-17
+      const Value* inner = n.find("nested");
+      if (!inner) throw LoadError(RAD_ERR_INVALID_ARG, "mask: missing nested bsdf");
+      b.type = RAD_BSDF_MASK;
+      b.opacity = Mono(n, "opacity", 0.5f);
+      b.nested = MakeBsdf(*inner);
     } else {
       throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + type);
     }
@@ -658,8 +691,24 @@ struct Builder {
         } else if (lt == "skybox") {
           l.type = RAD_LIGHT_SKYBOX;
           Color(*lightNode, "radiance", l.radiance, 1.f);
-        } else if (lt == "point" || lt == "projection") {
-          throw LoadError(RAD_ERR_UNSUPPORTED, "light '" + lt + "' is not on the GPU path yet");
+        } else if (lt == "point") {  // point.cpp:10-17: world position = to_world * position
+          l.type = RAD_LIGHT_POINT;
+          Color(*lightNode, "intensity", l.radiance, 1.f);
+          float pos[3] = {0, 0, 0};
+          VecOr<3>(*lightNode, "position", pos, {0, 0, 0});
+          float w[3];
+          for (int r = 0; r < 3; r++)
+            w[r] = ent.toWorld.m[r * 4] * pos[0] + ent.toWorld.m[r * 4 + 1] * pos[1] + ent.toWorld.m[r * 4 + 2] * pos[2] +
+                   ent.toWorld.m[r * 4 + 3];
+          const float id[16] = {1, 0, 0, w[0], 0, 1, 0, w[1], 0, 0, 1, w[2], 0, 0, 0, 1};
+          std::memcpy(l.to_world, id, sizeof(id));
+        } else if (lt == "projection") {  // projection.cpp:14-36
+          l.type = RAD_LIGHT_PROJECTION;
+          float irr[3], sc[3];
+          Color(*lightNode, "irradiance", irr, 1.f);
+          VecOr<3>(*lightNode, "scale", sc, {1, 1, 1});
+          for (int c = 0; c < 3; c++) l.radiance[c] = irr[c] * sc[c];
+          l.fov = FloatOr(*lightNode, "fov", 90.f);
         } else {
           throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + lt);
         }

@@ -135,7 +135,7 @@ __global__ void k_reset(uint32_t* cnt, uint32_t* gcnt, int out) {
     cnt[CNT_SHADOW] = 0;
     cnt[CNT_FETCH_EXT] = 0;
     cnt[CNT_FETCH_SH] = 0;
-    for (int k = 0; k < 4; k++) cnt[CNT_MAT + k] = 0;
+    for (int k = 0; k < kMatQueues; k++) cnt[CNT_MAT + k] = 0;
     if (gcnt) gcnt[G_QUEUE] = 0;
   }
 }
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             if (ls.pdf > 0.f) {
               f3 wo = si.sh.to_local(ls.dir);
               float bp;
-              f3 f = bsdf_eval_t<MAT>(b, quirks, si.wi, wo, bp);
+              f3 f = bsdf_eval_t<MAT>(scene.bsdfs, b, quirks, si.wi, wo, bp);
               if (a > 0.f) bp = a * guide::mixture_pdf(mx, ls.dir) + (1.f - a) * bp;
               // Scene::IsOcclude setup (scene.cpp:53-63)
               f3 o = offset_p(si.p, si.n, ls.p - si.p);
@@ -285,14 +285,14 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           if (useGuide) {
             f3 wg = guide::mixture_sample(mx, ul, u1, u2);
             float bp;
-            f = bsdf_eval_t<MAT>(b, quirks, si.wi, si.sh.to_local(wg), bp);
+            f = bsdf_eval_t<MAT>(scene.bsdfs, b, quirks, si.wi, si.sh.to_local(wg), bp);
             wo = wg;
             pdf = a * guide::mixture_pdf(mx, wg) + (1.f - a) * bp;
             bs.eta = 1.f;
             bs.lobes = LOBE_GLOSSY;
             if (is_zero(f)) pdf = 0.f;  // zero-valued paths end here without bias
           } else {
-            f = bsdf_sample_t<MAT>(b, quirks, si.wi, ul, u1, u2, bs);
+            f = bsdf_sample_t<MAT>(scene.bsdfs, b, quirks, si.wi, ul, u1, u2, bs);
             wo = si.sh.to_world(bs.wo);
             pdf = bs.pdf;
             if (a > 0.f && pdf > 0.f) pdf = a * guide::mixture_pdf(mx, wo) + (1.f - a) * bs.pdf;
@@ -440,7 +440,10 @@ __global__ void __launch_bounds__(256) k_guide_infer_hits(Wavefront w, DevScene 
 
 // Partition the extend queue by material (misses go with the diffuse bucket)
 // so each shade kernel runs one BSDF's code.
-// GUIDE: guided vertices also go to the guide queue.
+// GUIDE: guided vertices also go to the guide queue.  Diffuse, rough metal,
+// rough glass and Disney get queues of their own (misses join the diffuse
+// queue); every other type shares the last queue and its run-time-dispatch
+// kernel.
 template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
@@ -455,32 +458,30 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
     if (i < n) {
       s = w.queue[in][i];
       uint32_t geom = w.hit_geom[s];
-      key = geom == 0xffffffffu ? BSDF_DIFFUSE : scene.bsdfs[scene.shapes[geom].bsdf].type;
+      key = geom == 0xffffffffu ? BSDF_DIFFUSE : min(scene.bsdfs[scene.shapes[geom].bsdf].type, kMatOther);
       if (GUIDE) gd = guided_hit(w, scene, geom, quirks);
     }
-    if (GUIDE) {
+    uint32_t r6[6] = {0, 0, 0, 0, 0, 0};
+    if (GUIDE) {  // lds: 6 counters x (4 waves + 1) = 30 words
+      uint32_t* c6[6] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
+                         &w.cnt[CNT_MAT + 4], &w.gcnt[G_QUEUE]};
+      bool p6[6] = {key == 0, key == 1, key == 2, key == 3, key == 4, gd};
+      block_append<6>(c6, p6, r6, lds);
+    } else {
       uint32_t* c5[5] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
-                         &w.gcnt[G_QUEUE]};
-      bool p5[5] = {key == 0, key == 1, key == 2, key == 3, gd};
+                         &w.cnt[CNT_MAT + 4]};
+      bool p5[5] = {key == 0, key == 1, key == 2, key == 3, key == 4};
       uint32_t r5[5];
       block_append<5>(c5, p5, r5, lds);
-      if (key >= 0) {
-        uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : w.mq[3];
-        uint32_t r = key == 0 ? r5[0] : key == 1 ? r5[1] : key == 2 ? r5[2] : r5[3];
-        q[r] = s;
-      }
-      if (gd) w.gq[r5[4]] = s;
-    } else {
-      uint32_t* c4[4] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3]};
-      bool p4[4] = {key == 0, key == 1, key == 2, key == 3};
-      uint32_t r4[4];
-      block_append<4>(c4, p4, r4, lds);
-      if (key >= 0) {
-        uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : w.mq[3];
-        uint32_t r = key == 0 ? r4[0] : key == 1 ? r4[1] : key == 2 ? r4[2] : r4[3];
-        q[r] = s;
-      }
+#pragma unroll
+      for (int k = 0; k < 5; k++) r6[k] = r5[k];
     }
+    if (key >= 0) {
+      uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : key == 3 ? w.mq[3] : w.mq[4];
+      uint32_t r = key == 0 ? r6[0] : key == 1 ? r6[1] : key == 2 ? r6[2] : key == 3 ? r6[3] : r6[4];
+      q[r] = s;
+    }
+    if (GUIDE && gd) w.gq[r6[5]] = s;
   }
 }
 
@@ -664,6 +665,8 @@ static void LaunchMaterials(const Wavefront& w, const DevScene& scene, int in, u
     hipLaunchKernelGGL((k_shade_path<G, BSDF_ROUGH_GLASS>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
   if (mask & (1u << BSDF_DISNEY))
     hipLaunchKernelGGL((k_shade_path<G, BSDF_DISNEY>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  if (mask & (1u << kMatOther))
+    hipLaunchKernelGGL((k_shade_path<G, kMatOther>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 
 void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {

@@ -117,7 +117,7 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   Q = std::max<uint64_t>(1, std::min(Q, pass));
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
-  j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * 10 + Align(pass * 4) * 3 + Align(uint64_t(j.npix) * 16) * 2 +
+  j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * (6 + kMatQueues) + Align(pass * 4) * 3 + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
   if (j.guided)
     j.bytes += Align(Q * 4) * 3 + Align(Q * 2 * guide::kOut) +
@@ -401,7 +401,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.queue[0] = (uint32_t*)take(u4);
   w.queue[1] = (uint32_t*)take(u4);
   w.dead = (uint32_t*)take(u4);
-  for (int k = 0; k < 4; k++) w.mq[k] = (uint32_t*)take(u4);
+  for (int k = 0; k < kMatQueues; k++) w.mq[k] = (uint32_t*)take(u4);
   w.res = (float*)take(size_t(js.rows) * npix * 4 * 3);
   w.pacc = (float4*)take(size_t(npix) * 16);
   w.pacc2 = (float4*)take(size_t(npix) * 16);
@@ -447,7 +447,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.flags = (desc.flags & RAD_RENDER_FIX_QUIRKS) ? 0u : WF_QUIRKS;
   w.integrator = desc.integrator;
   job->sorted = desc.integrator == RAD_INTEGRATOR_PATH && !(desc.flags & RAD_RENDER_UNSORTED);
-  for (const BsdfRec& b : scene->host.bsdfs) job->mat_mask |= 1u << b.type;
+  for (const BsdfRec& b : scene->host.bsdfs) job->mat_mask |= 1u << std::min<int32_t>(b.type, kMatOther);
   HipCheck(hipHostMalloc((void**)&job->h_cnt, CNT_WORDS * 4 + ST_WORDS * 8 + 2 * G_WORDS * 4, hipHostMallocDefault),
            "hipHostMalloc");
   HipCheck(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking), "hipStreamCreate");

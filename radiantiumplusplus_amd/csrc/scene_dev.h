@@ -72,7 +72,18 @@ struct ShapeRec {
   float cdf_sum, cdf_norm;
 };
 
-enum : int32_t { BSDF_DIFFUSE = 0, BSDF_ROUGH_METAL = 1, BSDF_ROUGH_GLASS = 2, BSDF_DISNEY = 3 };
+enum : int32_t {
+  BSDF_DIFFUSE = 0,
+  BSDF_ROUGH_METAL = 1,
+  BSDF_ROUGH_GLASS = 2,
+  BSDF_DISNEY = 3,
+  BSDF_PLASTIC = 4,
+  BSDF_ROUGH_PLASTIC = 5,
+  BSDF_PERFECT_MIRROR = 6,
+  BSDF_PERFECT_GLASS = 7,
+  BSDF_TWO_SIDE = 8,  // wraps bsdfs[nested]
+  BSDF_MASK = 9       // wraps bsdfs[nested]
+};
 enum : uint32_t {
   LOBE_DIFFUSE = 1,
   LOBE_GLOSSY = 2,
@@ -95,14 +106,25 @@ struct BsdfRec {
   float roughness, anisotropic, spec_trans, sheen, sheen_tint, flatness, spec_tint, metallic, clearcoat,
       clearcoat_gloss;
   float spec_rate, cc_rate, diff_rate;
+  // plastic / rough_plastic (plastic.cpp:20-40): colours, 1 / eta^2, the
+  // internal Fresnel average Fdr(1 / eta), the specular sampling weight
+  f3 diffuse3, spec3;
+  float inv_eta2, fdr_int, spec_weight;
+  int32_t nonlinear;
+  // two_side / mask
+  int32_t nested;
+  float opacity;
 };
 
-enum : int32_t { LIGHT_AREA = 0, LIGHT_SKYBOX = 1 };
+enum : int32_t { LIGHT_AREA = 0, LIGHT_SKYBOX = 1, LIGHT_POINT = 2, LIGHT_PROJECTION = 3 };
 struct LightRec {
   int32_t type;
   int32_t shape;
-  f3 radiance;
-  Affine to_world;  // skybox orientation
+  f3 radiance;      // area / skybox radiance, point intensity, projection irradiance x scale
+  Affine to_world;  // skybox orientation; point position (translation); projection frame
+  Affine to_local;  // projection: world -> light frame
+  float clip[16];   // projection: light frame -> [0,1]^2 image plane (row-major, projective)
+  f3 axis;          // projection: normalised +z of the frame in world space
 };
 
 enum : int32_t { MEDIUM_HOMOGENEOUS = 0, MEDIUM_HETEROGENEOUS = 1 };

@@ -139,6 +139,19 @@ void GrowBounds(float* b, f3 p) {
 
 }  // namespace
 
+// Fresnel::DiffuseReflectance (fresnel.cpp:79-84), Horner-evaluated with
+// fused multiply-adds as math_base.h:90-105 does.
+static float DiffuseReflectance(float eta) {
+  float invEta = 1.0f / eta;
+  if (eta < 1.f) return std::fma(0.0636f, invEta, std::fma(eta, std::fma(eta, -1.4399f, 0.7099f), 0.6681f));
+  const float c[6] = {0.919317f, -3.4793f, 6.75335f, -7.80989f, 4.98554f, -1.36881f};
+  float acc = c[5];
+  for (int i = 4; i >= 0; i--) acc = std::fma(invEta, acc, c[i]);
+  return acc;
+}
+
+static float Lum(const float* c) { return (c[0] * 0.212671f) + (c[1] * 0.715160f) + (c[2] * 0.072169f); }
+
 BsdfRec MakeBsdfRec(const rad_bsdf_desc& d) {
   BsdfRec b{};
   auto v3 = [](const float* a) { return mk(a[0], a[1], a[2]); };
@@ -194,15 +207,60 @@ BsdfRec MakeBsdfRec(const rad_bsdf_desc& d) {
       if (hasSpecTrans) b.lobes |= LOBE_TRANSMISSION;
       break;
     }
+    case RAD_BSDF_PLASTIC:
+    case RAD_BSDF_ROUGH_PLASTIC: {  // plastic.cpp:20-40, rough_plastic.cpp:20-55
+      b.lobes = LOBE_DIFFUSE | LOBE_REFLECTION | LOBE_DELTA;
+      b.eta = d.int_ior / d.ext_ior;
+      b.inv_eta = 1.0f / b.eta;
+      b.diffuse3 = v3(d.diffuse);
+      b.spec3 = v3(d.specular_reflectance);
+      b.nonlinear = d.nonlinear != 0;
+      b.fdr_int = DiffuseReflectance(1.0f / b.eta);
+      b.inv_eta2 = 1.0f / (b.eta * b.eta);
+      double dm = Lum(d.diffuse), sm = Lum(d.specular_reflectance);
+      b.spec_weight = float(sm / (dm + sm));
+      b.alpha_v = b.alpha_u;  // isotropic "alpha"
+      break;
+    }
+    case RAD_BSDF_PERFECT_MIRROR:
+      b.lobes = LOBE_DELTA | LOBE_REFLECTION;  // perfect_mirror.cpp:18
+      break;
+    case RAD_BSDF_PERFECT_GLASS:
+      b.lobes = LOBE_DELTA | LOBE_REFLECTION | LOBE_TRANSMISSION;  // perfect_glass.cpp:17
+      b.eta = d.int_ior / d.ext_ior;
+      b.inv_eta = d.ext_ior / d.int_ior;
+      break;
+    case RAD_BSDF_TWO_SIDE:
+    case RAD_BSDF_MASK:  // lobes: the wrapped BSDF's, filled in once the array is complete
+      b.nested = d.nested;
+      b.opacity = std::min(std::max(d.opacity, 0.0f), 1.0f);
+      break;
     default:
       throw std::runtime_error("unsupported BSDF type " + std::to_string(d.type));
   }
   return b;
 }
 
+// Wrappers: the nested index must name another record, and at most two
+// wrappers may enclose a base BSDF (the device resolves two levels).
+static void ResolveWrappers(std::vector<BsdfRec>& bsdfs) {
+  const int32_t n = int32_t(bsdfs.size());
+  for (int32_t i = 0; i < n; i++) {
+    int32_t j = i, depth = 0;
+    while (bsdfs[j].type == BSDF_TWO_SIDE || bsdfs[j].type == BSDF_MASK) {
+      int32_t k = bsdfs[j].nested;
+      if (k < 0 || k >= n || k == j) throw std::runtime_error("bsdf " + std::to_string(i) + ": nested bsdf out of range");
+      if (++depth > 2) throw std::runtime_error("bsdf " + std::to_string(i) + ": more than two nested two_side / mask");
+      j = k;
+    }
+    bsdfs[i].lobes = bsdfs[j].lobes;
+  }
+}
+
 void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
   out = HostScene();
   for (uint32_t i = 0; i < desc.bsdf_count; i++) out.bsdfs.push_back(MakeBsdfRec(desc.bsdfs[i]));
+  ResolveWrappers(out.bsdfs);
   if (desc.light_count == 0) throw std::runtime_error("scene has no light (Scene::SampleLight needs one)");
   // media
   for (uint32_t i = 0; i < desc.medium_count; i++) {
@@ -258,6 +316,23 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
     r.shape = -1;
     r.radiance = mk(l.radiance[0], l.radiance[1], l.radiance[2]);
     r.to_world = M4::From(l.to_world).ToAffine();
+    if (l.type == RAD_LIGHT_PROJECTION) {  // projection.cpp:14-36 with a constant irradiance (aspect 1)
+      M4 tw = M4::From(l.to_world);
+      r.to_local = tw.Inverse().ToAffine();
+      const float aspect = 1.f, farC = 10000.f, nearC = 0.0001f;
+      const float recip = 1.f / (farC - nearC);
+      const float cot = 1.f / std::tan(l.fov * 0.5f * (3.14159265358979323846f / 180.f));
+      const float P[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, farC * recip, -nearC * farC * recip, 0, 0, 1, 0};
+      const float S[16] = {-0.5f, 0, 0, 0, 0, -0.5f * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+      const float T[16] = {1, 0, 0, -1, 0, 1, 0, -1 / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
+      M4 clip = M4::From(S) * M4::From(T) * M4::From(P);
+      for (int k = 0; k < 16; k++) r.clip[k] = clip.m[k];
+      float ax = tw.m[2], ay = tw.m[6], az = tw.m[10];
+      float len = std::sqrt(ax * ax + ay * ay + az * az);
+      r.axis = mk(ax / len, ay / len, az / len);
+    } else if (l.type != RAD_LIGHT_AREA && l.type != RAD_LIGHT_SKYBOX && l.type != RAD_LIGHT_POINT) {
+      throw std::runtime_error("unsupported light type " + std::to_string(l.type));
+    }
     out.lights.push_back(r);
     if (l.type == RAD_LIGHT_SKYBOX) out.env_light = int32_t(i);  // the last env light wins (scene.cpp:25-33)
   }

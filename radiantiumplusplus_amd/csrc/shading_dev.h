@@ -222,6 +222,47 @@ __device__ inline LightSample sample_area_shape(const DevScene& s, const ShapeRe
 }
 
 // Scene::SampleLightDirection (scene.cpp:87-96)
+// Point and projection lights (light/point.cpp:24-38, projection.cpp:40-60
+// with a constant irradiance), kept out of line: rare, and inlined they
+// raise the register count of every shading kernel.
+__device__ __noinline__ LightSample sample_delta_light(const LightRec& L, f3 refp) {
+  LightSample ls;
+  ls.delta = true;
+  ls.pdf = 1.f;
+  const f3 pos = mk(L.to_world.m[3], L.to_world.m[7], L.to_world.m[11]);
+  if (L.type == LIGHT_POINT) {
+    f3 d = pos - refp;
+    ls.p = pos;
+    ls.n = splat(0.f);
+    ls.dist = sqrtf(dot(d, d));
+    const float inv = 1.f / ls.dist;
+    ls.dir = d * inv;
+    ls.li = L.radiance * sqr(inv);
+    return ls;
+  }
+  const f3 local = L.to_local.point(refp);
+  const float* c = L.clip;
+  const float cw = c[12] * local.x + c[13] * local.y + c[14] * local.z + c[15];
+  const float u = (c[0] * local.x + c[1] * local.y + c[2] * local.z + c[3]) / cw;
+  const float v = (c[4] * local.x + c[5] * local.y + c[6] * local.z + c[7]) / cw;
+  if (!(u >= 0.f && u <= 1.f && v >= 0.f && v <= 1.f && local.z > 0.f)) {
+    ls.pdf = 0.f;
+    ls.li = splat(0.f);
+    ls.p = refp;
+    ls.n = splat(0.f);
+    ls.dir = mk(0.f, 0.f, 1.f);
+    ls.dist = 0.f;
+    return ls;
+  }
+  f3 d = pos - refp;
+  ls.p = pos;
+  ls.n = L.axis;
+  ls.dist = sqrtf(dot(d, d));
+  ls.dir = d * (1.f / ls.dist);
+  ls.li = L.radiance * kPi / (sqr(local.z) * dot(-ls.dir, ls.n));
+  return ls;
+}
+
 __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul, float u1, float u2) {
   uint32_t index = 0;
   float lpdf = 1.f;
@@ -244,6 +285,8 @@ __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul,
     ls.pdf = 1.f / (4.f * kPi);
     ls.delta = false;
     ls.li = L.radiance;
+  } else if (L.type == LIGHT_POINT || L.type == LIGHT_PROJECTION) {
+    ls = sample_delta_light(L, refp);
   } else {  // light/area.cpp:50-75
     ls = sample_area_shape(s, s.shapes[L.shape], refp, u1, u2);
     ls.li = dot(-ls.dir, ls.n) >= 0.f ? L.radiance : splat(0.f);

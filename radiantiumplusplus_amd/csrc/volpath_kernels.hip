@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
                 if (ls.pdf > 0.f) {
                   f3 wo = si.sh.to_local(ls.dir);
                   float bp;
-                  f3 f = bsdf_eval(b, quirks, si.wi, wo, bp);
+                  f3 f = bsdf_eval_w(scene.bsdfs, b, quirks, si.wi, wo, bp);
                   float wgt = ls.delta ? 1.f : mis_weight(ls.pdf, bp);
                   f3 c = mul(mul(T, f), ls.li) * wgt / ls.pdf;
                   if (!is_zero(c)) {
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
               float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
               dim += 3;
               BsdfSample bs;
-              f3 f = bsdf_sample(b, quirks, si.wi, ul, u1, u2, bs);
+              f3 f = bsdf_sample_w(scene.bsdfs, b, quirks, si.wi, ul, u1, u2, bs);
               if (bs.pdf > 0.f) {
                 T = mul(T, f) / bs.pdf;
                 eta *= bs.eta;

@@ -33,13 +33,18 @@ enum : uint32_t {
   CNT_NEXT = 4,       // next sample id of the pass
   CNT_FETCH_EXT = 5,  // extend-kernel work counter
   CNT_FETCH_SH = 6,   // shadow-kernel work counter
-  CNT_MAT = 8,        // 4 words: per-material queue counts (diffuse + misses, metal, glass, Disney)
-  CNT_WORDS = 12
+  CNT_MAT = 8,        // 5 words: per-material queue counts (diffuse + misses, metal, glass, Disney, others)
+  CNT_WORDS = 14      // even, so the 64-bit statistics that follow stay aligned
 };
 enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_WORDS = 4 };
 enum : uint32_t { WF_QUIRKS = 1u, WF_GUIDE = 2u, WF_RECORD = 4u };
 enum : uint32_t { G_QUEUE = 0, G_RECORDS = 1, G_WSUM = 2, G_WORDS = 4 };
 enum : uint32_t { MED_UNSET = 0xffffffffu };
+
+// Sorted shading queues: BSDF types 0..3 (diffuse, rough metal, rough glass,
+// Disney) have their own; kMatOther takes every other type.
+constexpr int kMatOther = 4;
+constexpr int kMatQueues = 5;
 
 struct Wavefront {
   float4* ray_o;   // origin, tmin
@@ -53,7 +58,7 @@ struct Wavefront {
   float4* hit;     // t, u, v, prim (bits)
   uint32_t* hit_geom;
   uint32_t* queue[2];
-  uint32_t* mq[4];   // the extend queue partitioned by material (k_classify)
+  uint32_t* mq[5];   // the extend queue partitioned by material (k_classify); [4]: every other type
   uint32_t* dead;
   float4* sh_o;    // shadow origin, tmax
   float4* sh_d;    // shadow direction, slot (bits)

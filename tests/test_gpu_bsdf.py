@@ -21,6 +21,12 @@ CASES = [
     ("disney", {"base_color": (0.6, 0.4, 0.3), "roughness": 0.3, "spec_trans": 0.7, "clearcoat": 0.5,
                 "clearcoat_gloss": 0.5, "sheen": 0.4, "sheen_tint": 0.5, "spec_tint": 0.2, "flatness": 0.3,
                 "anisotropic": 0.4}),
+    ("plastic", {"diffuse": (0.6, 0.3, 0.2), "specular": (0.8, 0.8, 0.8)}),
+    ("plastic", {"diffuse": (0.6, 0.3, 0.2), "nonlinear": True, "int_ior": 1.7}),
+    ("rough_plastic", {"alpha": 0.25, "diffuse": (0.2, 0.5, 0.6)}),
+    ("rough_plastic", {"alpha": 0.3, "distribution": "beckmann", "nonlinear": True}),
+    ("perfect_mirror", {"reflectance": (0.9, 0.8, 0.7), "eta": (0.2, 0.9, 1.1), "k": (3.9, 2.4, 2.1)}),
+    ("perfect_glass", {"int_ior": 1.5, "ext_ior": 1.0, "transmittance": (0.9, 0.8, 0.9)}),
 ]
 
 
@@ -48,7 +54,7 @@ def close(a, b, frac=0.999):
 def test_eval_matches_oracle(kind, kw, quirks):
     rng = np.random.default_rng(42)
     d = desc(kind, kw)
-    wi, wo = dirs(20000, rng, both=kind == "rough_glass" or "spec_trans" in kw), dirs(20000, rng)
+    wi, wo = dirs(20000, rng, both=kind in ("rough_glass", "perfect_glass") or "spec_trans" in kw), dirs(20000, rng)
     g = hip.bsdf_eval(d, wi, wo, quirks=quirks)
     o = O.bsdf_eval(d, wi, wo, quirks=quirks)
     assert close(g, o)
@@ -81,7 +87,7 @@ def test_sample_matches_oracle(kind, kw, quirks):
     rng = np.random.default_rng(7)
     d = desc(kind, kw)
     n = 20000
-    wi = dirs(n, rng, both=kind == "rough_glass" or "spec_trans" in kw)
+    wi = dirs(n, rng, both=kind in ("rough_glass", "perfect_glass") or "spec_trans" in kw)
     lobe, xi = rng.random(n, dtype=np.float32), rng.random((n, 2), dtype=np.float32)
     g = hip.bsdf_sample(d, wi, lobe, xi, quirks=quirks)
     o = O.bsdf_sample(d, wi, lobe, xi, quirks=quirks)

@@ -167,8 +167,10 @@ def test_bsdf_vars_and_disney_flags(tmp_path):
 
 
 @pytest.mark.parametrize("bad,code", [
-    ({"type": "plastic"}, 4),      # not on the GPU path yet -> RAD_ERR_UNSUPPORTED
-    ({"type": "nonexistent"}, 1),  # unknown factory -> RAD_ERR_INVALID_ARG
+    ({"type": "plastic", "diffuse": {"type": "chessboard"}}, 4),  # textures: RAD_ERR_UNSUPPORTED
+    ({"type": "two_side", "out_side": {"type": "diffuse"}}, 4),   # the reference's broken form
+    ({"type": "mask"}, 1),                                         # no nested bsdf
+    ({"type": "nonexistent"}, 1),                                  # unknown factory -> RAD_ERR_INVALID_ARG
 ])
 def test_unsupported_and_unknown_bsdfs_are_rejected(tmp_path, bad, code):
     sc = dict(BASE)
@@ -207,3 +209,46 @@ def test_exr_writer(tmp_path):
     # last scanline block holds B, G, R planes of row 4
     tail = np.frombuffer(raw[-7 * 3 * 4:], np.float32).reshape(3, 7)
     assert np.array_equal(tail[2], img[4, :, 0]) and np.array_equal(tail[0], img[4, :, 2])
+
+
+def test_remaining_bsdfs_and_lights_load_with_reference_defaults(tmp_path):
+    """plastic.cpp / rough_plastic.cpp / perfect_*.cpp / two_side.cpp /
+    mask.cpp / point.cpp / projection.cpp defaults, nested BSDFs appended
+    and referenced by index."""
+    sc = dict(BASE)
+    sc["scene"] = [
+        {"shape": {"type": "sphere"}, "bsdf": {"type": "plastic"}},
+        {"shape": {"type": "sphere"}, "bsdf": {"type": "rough_plastic", "alpha": 0.3, "distribution": "beckmann"}},
+        {"shape": {"type": "sphere"}, "bsdf": {"type": "perfect_mirror"}},
+        {"shape": {"type": "sphere"}, "bsdf": {"type": "perfect_glass"}},
+        {"shape": {"type": "sphere"}, "bsdf": {"type": "mask", "opacity": 0.25,
+                                               "nested": {"type": "two_side", "two_side": {"type": "diffuse"}}}},
+        {"to_world": {"translate": [1, 2, 3]}, "light": {"type": "point", "position": [0, 1, 0], "intensity": 4}},
+        {"light": {"type": "projection", "irradiance": [1, 2, 3], "scale": 2, "fov": 30}},
+    ]
+    h = load(_write_scene(tmp_path, sc))
+    try:
+        d = hip.lib().rad_loaded_scene_desc(h).contents
+        b = [d.bsdfs[i] for i in range(d.bsdf_count)]
+        types = [x.type for x in b]
+        assert types[:4] == [F.RAD_BSDF_PLASTIC, F.RAD_BSDF_ROUGH_PLASTIC, F.RAD_BSDF_PERFECT_MIRROR,
+                             F.RAD_BSDF_PERFECT_GLASS]
+        p = b[0]
+        assert p.int_ior == pytest.approx(1.49) and p.ext_ior == pytest.approx(1.000277)
+        assert list(p.diffuse) == pytest.approx([0.5] * 3) and list(p.specular_reflectance) == pytest.approx([1] * 3)
+        assert b[1].alpha_u == pytest.approx(0.3) and b[1].distribution == F.RAD_MICROFACET_BECKMANN
+        assert list(b[2].eta) == [0, 0, 0] and list(b[2].k) == [1, 1, 1]
+        assert b[3].int_ior == pytest.approx(1.5046)
+        shape_bsdf = d.shapes[4].bsdf
+        m = b[shape_bsdf]
+        assert m.type == F.RAD_BSDF_MASK and m.opacity == pytest.approx(0.25)
+        ts = b[m.nested]
+        assert ts.type == F.RAD_BSDF_TWO_SIDE and b[ts.nested].type == F.RAD_BSDF_DIFFUSE
+        lights = [d.lights[i] for i in range(d.light_count)]
+        pt = [l for l in lights if l.type == F.RAD_LIGHT_POINT][0]
+        assert [pt.to_world[3], pt.to_world[7], pt.to_world[11]] == pytest.approx([1, 3, 3])
+        assert list(pt.radiance) == pytest.approx([4] * 3)
+        pj = [l for l in lights if l.type == F.RAD_LIGHT_PROJECTION][0]
+        assert list(pj.radiance) == pytest.approx([2, 4, 6]) and pj.fov == pytest.approx(30)
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
