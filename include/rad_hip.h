@@ -270,6 +270,8 @@ typedef struct rad_job_stats {
   float guide_loss;        /* mean loss of the last training step */
   float guide_alpha;       /* probability of sampling the guide in the final pass */
   uint32_t blocks_per_cu, trace_blocks_per_cu; /* launch configuration used */
+  uint64_t suspended;      /* extension traversals cut off at the per-launch node-visit budget and resumed
+                              by the next wavefront iteration (results do not depend on it) */
 } rad_job_stats;
 
 /* Launch-configuration autotuner.  Times short renders of `probe` (its spp is

@@ -194,6 +194,7 @@ class JobStats(C.Structure):
         ("guide_alpha", C.c_float),
         ("blocks_per_cu", C.c_uint32),
         ("trace_blocks_per_cu", C.c_uint32),
+        ("suspended", C.c_uint64),
     ]
 
 

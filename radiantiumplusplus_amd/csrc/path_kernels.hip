@@ -21,6 +21,7 @@ __global__ void k_init_pass(Wavefront w, int first) {
     w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);
     w.dead[s] = s;
     if (w.glast) w.glast[s] = w.gpend[s] = 0xffffffffu;
+    if (w.susp) w.susp[s] = kNoSusp;
   }
   if (s == 0) {
     w.cnt[CNT_Q0] = 0;
@@ -30,6 +31,8 @@ __global__ void k_init_pass(Wavefront w, int first) {
     w.cnt[CNT_NEXT] = 0;
     w.cnt[CNT_FETCH_EXT] = 0;
     w.cnt[CNT_FETCH_SH] = 0;
+    w.cnt[CNT_SUSP] = 0;
+    w.cnt[CNT_SUSP + 1] = 0;
     if (first)
       for (int i = 0; i < ST_WORDS; i++) w.stats[i] = 0ull;
   }
@@ -136,15 +139,36 @@ __global__ void k_reset(uint32_t* cnt, uint32_t* gcnt, int out) {
     cnt[CNT_FETCH_EXT] = 0;
     cnt[CNT_FETCH_SH] = 0;
     for (int k = 0; k < kMatQueues; k++) cnt[CNT_MAT + k] = 0;
+    cnt[CNT_SUSP + out] = 0;  // resume pool `out` was drained by the previous extend
     if (gcnt) gcnt[G_QUEUE] = 0;
   }
 }
 
 struct ExtendJob {
+  static constexpr bool kSuspend = true;
   const uint32_t* queue;
   const float4 *ray_o, *ray_d;
   float4* hit;
   uint32_t* hit_geom;
+  uint32_t* susp;                 // null: traversals run to the end
+  const ResumeState* rin;         // pool this launch resumes from
+  ResumeState* rout;              // pool this launch suspends into
+  uint32_t* rcount;               // entries of rout
+  uint32_t cap, budget;
+  __device__ __forceinline__ void resume(uint32_t s, TraceRay& r, TraceStack& st) {
+    if (!susp) return;
+    const uint32_t j = susp[s];
+    if (j == kNoSusp) return;
+    restore_trace(rin[j], r, st);
+    susp[s] = kNoSusp;
+  }
+  __device__ __forceinline__ bool suspend(uint32_t s, const TraceRay& r, const TraceStack& st) {
+    const uint32_t j = atomicAdd(rcount, 1u);
+    if (j >= cap) return false;  // pool full: the ray finishes in this launch
+    save_trace(rout[j], r, st);
+    susp[s] = j;
+    return true;
+  }
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
     uint32_t s = queue[i];
     float4 a = ray_o[s], b = ray_d[s];
@@ -162,7 +186,8 @@ struct ExtendJob {
 };
 
 __global__ void __launch_bounds__(kTraceBlock) k_extend(Wavefront w, DevScene scene, int in) {
-  ExtendJob job{w.queue[in], w.ray_o, w.ray_d, w.hit, w.hit_geom};
+  ExtendJob job{w.queue[in], w.ray_o, w.ray_d, w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
+                &w.cnt[CNT_SUSP + 1 - in], w.resume_cap, w.susp ? w.visit_budget : 0xffffffffu};
   trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
 }
 
@@ -188,8 +213,12 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     guide::TrainRecord rec;
-    if (i < n) {
-      s = queue[i];
+    if (i < n) s = queue[i];
+    // a traversal suspended by this iteration's extend continues next iteration
+    // (the material queues never hold one)
+    const bool held = MAT < 0 && i < n && w.susp && w.susp[s] != kNoSusp;
+    if (held) alive = true;
+    if (i < n && !held) {
       float4 ro4 = w.ray_o[s], rd4 = w.ray_d[s];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float4 hh = w.hit[s];
@@ -382,7 +411,7 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
     uint32_t s = 0;
     if (i < n) {
       s = w.queue[in][i];
-      g = guided_hit(w, scene, w.hit_geom[s], quirks);
+      g = !(w.susp && w.susp[s] != kNoSusp) && guided_hit(w, scene, w.hit_geom[s], quirks);
     }
     uint32_t* c1[1] = {&w.gcnt[G_QUEUE]};
     bool p1[1] = {g};
@@ -446,7 +475,7 @@ __global__ void __launch_bounds__(256) k_guide_infer_hits(Wavefront w, DevScene 
 // kernel.
 template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
-  __shared__ uint32_t lds[32];
+  __shared__ uint32_t lds[40];
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
@@ -454,38 +483,43 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
     uint32_t i = base + threadIdx.x;
     int key = -1;
     uint32_t s = 0;
-    bool gd = false;
+    bool gd = false, held = false;
     if (i < n) {
       s = w.queue[in][i];
-      uint32_t geom = w.hit_geom[s];
-      key = geom == 0xffffffffu ? BSDF_DIFFUSE : min(scene.bsdfs[scene.shapes[geom].bsdf].type, kMatOther);
-      if (GUIDE) gd = guided_hit(w, scene, geom, quirks);
+      held = w.susp && w.susp[s] != kNoSusp;  // traversal suspended: straight to the next extend queue
+      if (!held) {
+        uint32_t geom = w.hit_geom[s];
+        key = geom == 0xffffffffu ? BSDF_DIFFUSE : min(scene.bsdfs[scene.shapes[geom].bsdf].type, kMatOther);
+        if (GUIDE) gd = guided_hit(w, scene, geom, quirks);
+      }
     }
-    uint32_t r6[6] = {0, 0, 0, 0, 0, 0};
-    if (GUIDE) {  // lds: 6 counters x (4 waves + 1) = 30 words
-      uint32_t* c6[6] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
-                         &w.cnt[CNT_MAT + 4], &w.gcnt[G_QUEUE]};
-      bool p6[6] = {key == 0, key == 1, key == 2, key == 3, key == 4, gd};
-      block_append<6>(c6, p6, r6, lds);
+    uint32_t r7[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (GUIDE) {  // lds: 7 counters x (4 waves + 1) = 35 words
+      uint32_t* c7[7] = {&w.cnt[CNT_MAT],     &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
+                         &w.cnt[CNT_MAT + 4], &w.cnt[1 - in],      &w.gcnt[G_QUEUE]};
+      bool p7[7] = {key == 0, key == 1, key == 2, key == 3, key == 4, held, gd};
+      block_append<7>(c7, p7, r7, lds);
     } else {
-      uint32_t* c5[5] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
-                         &w.cnt[CNT_MAT + 4]};
-      bool p5[5] = {key == 0, key == 1, key == 2, key == 3, key == 4};
-      uint32_t r5[5];
-      block_append<5>(c5, p5, r5, lds);
+      uint32_t* c6[6] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
+                         &w.cnt[CNT_MAT + 4], &w.cnt[1 - in]};
+      bool p6[6] = {key == 0, key == 1, key == 2, key == 3, key == 4, held};
+      uint32_t r6[6];
+      block_append<6>(c6, p6, r6, lds);
 #pragma unroll
-      for (int k = 0; k < 5; k++) r6[k] = r5[k];
+      for (int k = 0; k < 6; k++) r7[k] = r6[k];
     }
+    if (held) w.queue[1 - in][r7[5]] = s;
     if (key >= 0) {
       uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : key == 3 ? w.mq[3] : w.mq[4];
-      uint32_t r = key == 0 ? r6[0] : key == 1 ? r6[1] : key == 2 ? r6[2] : key == 3 ? r6[3] : r6[4];
+      uint32_t r = key == 0 ? r7[0] : key == 1 ? r7[1] : key == 2 ? r7[2] : key == 3 ? r7[3] : r7[4];
       q[r] = s;
     }
-    if (GUIDE && gd) w.gq[r6[5]] = s;
+    if (GUIDE && gd) w.gq[r7[6]] = s;
   }
 }
 
 struct ShadowJob {
+  static constexpr bool kSuspend = false;
   const float4 *sh_o, *sh_d, *sh_c;
   float4* lsum;
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
@@ -555,6 +589,7 @@ __global__ void k_final(Wavefront w, float* fb, float* sq) {
 }
 
 struct ClosestJob {
+  static constexpr bool kSuspend = false;
   const float4* rays;
   float* hits;
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
@@ -577,6 +612,7 @@ struct ClosestJob {
 };
 
 struct AnyJob {
+  static constexpr bool kSuspend = false;
   const float4* rays;
   uint8_t* occ;
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {

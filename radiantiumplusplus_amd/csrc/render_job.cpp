@@ -33,7 +33,7 @@ struct rad_job {
   double render_ms = 0;
   uint32_t iterations = 0;
   uint32_t passes = 0;
-  uint64_t segments = 0, shadow_rays = 0;
+  uint64_t segments = 0, shadow_rays = 0, suspended = 0;
   uint64_t guide_records = 0;
   float guide_loss = 0.f, guide_alpha = 0.f;
   uint64_t stats[rad::ST_WORDS] = {};
@@ -63,6 +63,8 @@ constexpr float kGuideAlpha = 0.5f;
 constexpr float kTrainWmax = 64.f;               // cap of a record's normalised weight
 constexpr uint32_t kFirstTrainPaths = 2u << 20;   // paths in the first training pass
 constexpr float kGuideMinRough = 0.f;            // glossy BSDFs below this alpha are not guided
+constexpr uint32_t kVisitBudget = 512;           // node visits of a path ray per extend launch
+constexpr uint32_t kResumeCap = 16384;           // suspended traversals per extend launch
 
 // Training hyper-parameters; RAD_GUIDE_STEPS / RAD_GUIDE_LR / RAD_GUIDE_ALPHA
 // in the environment override the defaults above.
@@ -82,6 +84,12 @@ float GuideMinRough() {
   const char* e = std::getenv("RAD_GUIDE_MINROUGH");
   return e ? float(std::atof(e)) : kGuideMinRough;
 }
+// RAD_VISIT_BUDGET overrides kVisitBudget; 0 lets every traversal finish in
+// the launch that started it.
+uint32_t VisitBudget() {
+  const char* e = std::getenv("RAD_VISIT_BUDGET");
+  return e ? uint32_t(std::max(0, std::atoi(e))) : kVisitBudget;
+}
 float GuideAlpha() {
   const char* e = std::getenv("RAD_GUIDE_ALPHA");
   return e ? std::min(0.95f, std::max(0.05f, float(std::atof(e)))) : kGuideAlpha;
@@ -91,6 +99,7 @@ size_t Align(size_t v) { return (v + 255) & ~size_t(255); }
 struct JobShape {
   uint32_t npix, rows, Q;
   bool guided;
+  uint32_t resume_cap;  // 0: no suspended traversals (vol_path, or budget 0)
   uint64_t bytes;
 };
 
@@ -117,8 +126,10 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   Q = std::max<uint64_t>(1, std::min(Q, pass));
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
+  j.resume_cap = desc.integrator == RAD_INTEGRATOR_PATH && VisitBudget() > 0 ? uint32_t(std::min<uint64_t>(Q, kResumeCap)) : 0;
   j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * (6 + kMatQueues) + Align(pass * 4) * 3 + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
+  if (j.resume_cap) j.bytes += Align(Q * 4) + 2 * Align(uint64_t(j.resume_cap) * sizeof(ResumeState));
   if (j.guided)
     j.bytes += Align(Q * 4) * 3 + Align(Q * 2 * guide::kOut) +
                Align(uint64_t(kRecordCap) * sizeof(guide::TrainRecord)) + Align(kTrainBatch * 4) + 256 +
@@ -280,7 +291,9 @@ void RunJob(rad_job* job) {
         }
         const uint64_t* hs = reinterpret_cast<const uint64_t*>(job->h_cnt + CNT_WORDS);
         job->done.store(hs[ST_SAMPLES]);
-        job->segments += job->h_cnt[in];
+        // a suspended traversal counts once, in the launch that finishes it
+        job->segments += job->h_cnt[in] - std::min(job->h_cnt[CNT_SUSP + 1 - in], w.resume_cap);
+        job->suspended += std::min(job->h_cnt[CNT_SUSP + 1 - in], w.resume_cap);
         job->shadow_rays += job->h_cnt[CNT_SHADOW];
         uint32_t cont = job->h_cnt[1 - in], dead = job->h_cnt[CNT_DEAD];
         if (cont == 0 && dead == 0) break;
@@ -407,6 +420,13 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.pacc2 = (float4*)take(size_t(npix) * 16);
   w.cnt = (uint32_t*)take(CNT_WORDS * 4 + ST_WORDS * 8);
   w.stats = reinterpret_cast<unsigned long long*>(w.cnt + CNT_WORDS);
+  if (js.resume_cap) {
+    w.susp = (uint32_t*)take(u4);
+    w.resume[0] = (ResumeState*)take(size_t(js.resume_cap) * sizeof(ResumeState));
+    w.resume[1] = (ResumeState*)take(size_t(js.resume_cap) * sizeof(ResumeState));
+    w.resume_cap = js.resume_cap;
+    w.visit_budget = VisitBudget();
+  }
   if (js.guided) {
     w.gq = (uint32_t*)take(u4);
     w.gpend = (uint32_t*)take(u4);
@@ -480,6 +500,7 @@ void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats) {
     stats->bvh_build_ms = job->scene->bvh_ms;
     stats->samples = job->stats[ST_SAMPLES];
     stats->segments = job->segments;
+    stats->suspended = job->suspended;
     stats->shadow_rays = job->shadow_rays;
     stats->medium_steps = job->stats[ST_MEDIUM];
     stats->iterations = job->iterations;

@@ -180,6 +180,53 @@ struct TraceRay {
   Hit h;
 };
 
+// A closest-hit traversal interrupted at a launch's visit budget: the node
+// to visit next, the pending stack, the closest hit so far.  Resuming it
+// replays nothing, so the result is the one an uninterrupted traversal gives.
+struct ResumeState {
+  uint32_t node, sp, hit, prim, geom;
+  float tfar, t, u, v;
+  uint32_t pad[3];
+  uint32_t stack_node[kStackDepth];
+  float stack_dist[kStackDepth];
+};
+
+__device__ inline void save_trace(ResumeState& rs, const TraceRay& r, const TraceStack& st) {
+  rs.node = r.node;
+  rs.sp = uint32_t(st.sp);
+  rs.hit = r.hit ? 1u : 0u;
+  rs.prim = r.h.prim;
+  rs.geom = r.h.geom;
+  rs.tfar = r.tfar;
+  rs.t = r.h.t;
+  rs.u = r.h.u;
+  rs.v = r.h.v;
+  for (int k = 0; k < st.sp; k++) {
+    if (k < st.lo) {
+      rs.stack_node[k] = st.ov_node[k];
+      rs.stack_dist[k] = st.ov_dist[k];
+    } else {
+      const int j = (k & (kLdsStack - 1)) * kTraceBlock;
+      rs.stack_node[k] = st.node[j];
+      rs.stack_dist[k] = st.dist[j];
+    }
+  }
+}
+
+// After trace_begin on the same ray.
+__device__ inline void restore_trace(const ResumeState& rs, TraceRay& r, TraceStack& st) {
+  r.node = rs.node;
+  r.hit = rs.hit != 0;
+  r.h.prim = rs.prim;
+  r.h.geom = rs.geom;
+  r.tfar = rs.tfar;
+  r.h.t = rs.t;
+  r.h.u = rs.u;
+  r.h.v = rs.v;
+  const int n = min(int(rs.sp), kStackDepth);
+  for (int k = 0; k < n; k++) st.push(rs.stack_node[k], rs.stack_dist[k]);
+}
+
 __device__ __forceinline__ void trace_begin(TraceRay& r, TraceStack& st, f3 o, f3 d, float tmin, float tmax) {
   r.o = o;
   r.d = d;
@@ -290,6 +337,14 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
 // each wave takes the 64 items at its own position once).
 // Job: bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag)
 //      void finish(uint32_t i, uint32_t tag, const TraceRay& r)
+//      static constexpr bool kSuspend; with it, a ray that has visited
+//      job.budget nodes in this launch is offered to the job, which may keep
+//      it for a later launch (per-ray costs are heavy-tailed: near-grazing
+//      rays over dense meshes visit thousands of nodes, and a launch cannot
+//      end before its slowest ray):
+//      uint32_t budget;
+//      void resume(uint32_t tag, TraceRay& r, TraceStack& st)   (after trace_begin)
+//      bool suspend(uint32_t tag, const TraceRay& r, const TraceStack& st)
 // Launch with kTraceBlock threads per block; every lane must call this.
 template <bool ANY, class Job>
 __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint32_t* fetch, Job& job) {
@@ -304,7 +359,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
   st.dist = lds_dist + threadIdx.x;
   TraceRay r;
   bool active = false, exhausted = false;
-  uint32_t idx = 0, tag = 0;
+  uint32_t idx = 0, tag = 0, visits = 0;
   uint32_t cbase = 0, cleft = 0;  // the wave's chunk (wave-uniform)
   for (;;) {
     if (!exhausted || cleft > 0) {
@@ -349,6 +404,8 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
           if (job.load(i, o, d, tmin, tmax, tag)) {
             idx = i;
             trace_begin(r, st, o, d, tmin, tmax);
+            if constexpr (Job::kSuspend) job.resume(tag, r, st);
+            visits = 0;
             active = true;
           }
         }
@@ -359,9 +416,16 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
       continue;
     }
     for (int k = 0; k < 256; k++) {
-      if (active && trace_step<ANY>(s, r, st)) {
-        job.finish(idx, tag, r);
-        active = false;
+      if (active) {
+        if (trace_step<ANY>(s, r, st)) {
+          job.finish(idx, tag, r);
+          active = false;
+        } else if constexpr (Job::kSuspend) {
+          if (++visits >= job.budget) {
+            if (job.suspend(tag, r, st)) active = false;
+            visits = 0;
+          }
+        }
       }
       unsigned long long busy = __ballot(active);
       if (busy == 0ull) break;

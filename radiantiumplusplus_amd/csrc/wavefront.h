@@ -20,6 +20,7 @@
 
 #include "guide.h"
 #include "scene_dev.h"
+#include "traverse.h"
 
 namespace rad {
 
@@ -34,7 +35,8 @@ enum : uint32_t {
   CNT_FETCH_EXT = 5,  // extend-kernel work counter
   CNT_FETCH_SH = 6,   // shadow-kernel work counter
   CNT_MAT = 8,        // 5 words: per-material queue counts (diffuse + misses, metal, glass, Disney, others)
-  CNT_WORDS = 14      // even, so the 64-bit statistics that follow stay aligned
+  CNT_SUSP = 14,      // 2 words: traversals suspended into resume pool 0 / 1
+  CNT_WORDS = 16      // even, so the 64-bit statistics that follow stay aligned
 };
 enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_WORDS = 4 };
 enum : uint32_t { WF_QUIRKS = 1u, WF_GUIDE = 2u, WF_RECORD = 4u };
@@ -44,6 +46,7 @@ enum : uint32_t { MED_UNSET = 0xffffffffu };
 // Sorted shading queues: BSDF types 0..3 (diffuse, rough metal, rough glass,
 // Disney) have their own; kMatOther takes every other type.
 constexpr int kMatOther = 4;
+constexpr uint32_t kNoSusp = 0xffffffffu;
 constexpr int kMatQueues = 5;
 
 struct Wavefront {
@@ -58,6 +61,13 @@ struct Wavefront {
   float4* hit;     // t, u, v, prim (bits)
   uint32_t* hit_geom;
   uint32_t* queue[2];
+  // Traversals cut off at visit_budget node visits (path jobs; null = never):
+  // per slot the resume-pool index, or kNoSusp.  Iteration `in` resumes from
+  // resume[in] and suspends into resume[1 - in] (count in CNT_SUSP + 1 - in);
+  // a suspended slot skips shading and goes straight to the next extend queue.
+  uint32_t* susp;
+  ResumeState* resume[2];
+  uint32_t resume_cap, visit_budget;
   uint32_t* mq[5];   // the extend queue partitioned by material (k_classify); [4]: every other type
   uint32_t* dead;
   float4* sh_o;    // shadow origin, tmax

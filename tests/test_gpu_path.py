@@ -201,3 +201,31 @@ def test_remaining_types_sorted_equals_unsorted(hip_ctx):
     a = s.render(b.render_desc(spp=8, seed=5))[0]
     u = s.render(b.render_desc(spp=8, seed=5, flags=F.RAD_RENDER_UNSORTED))[0]
     assert np.array_equal(a, u)
+
+
+@pytest.mark.parametrize("budget", [1, 3, 24])
+def test_suspended_traversals_leave_images_unchanged(hip_ctx, monkeypatch, budget):
+    """Extension traversals cut off at a per-launch node-visit budget resume
+    in the next iteration exactly where they stopped: the image and the ray
+    count equal those of uncut traversals, sorted or not, guided or not.
+    Budget 1 also overflows the resume pool, whose refused rays finish in
+    their own launch."""
+    b = scenes.many_balls(seed=0, width=128, height=128, spp=16, n_spheres=200)
+    s = hip.Scene.from_builder(hip_ctx, b)
+    s.set_guide_params(None)
+    for guiding in (0, 2):
+        def desc(flags=0):
+            d = b.render_desc(spp=8, seed=5, flags=flags)
+            d.guiding = guiding
+            return d
+        monkeypatch.setenv("RAD_VISIT_BUDGET", "0")
+        ref, _, st0 = s.render(desc())
+        assert st0.suspended == 0
+        monkeypatch.setenv("RAD_VISIT_BUDGET", str(budget))
+        for flags in (0, F.RAD_RENDER_UNSORTED):
+            img, _, st = s.render(desc(flags))
+            print(f"guiding {guiding} flags {flags} budget {budget}: suspended {st.suspended} "
+                  f"iterations {st0.iterations} -> {st.iterations}")
+            assert st.suspended > 0
+            assert st.segments == st0.segments
+            assert np.array_equal(img, ref)
