@@ -423,8 +423,14 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
 
 // Network inference for the guide queue: the inputs (hit position, geometric
 // normal, outgoing direction) are formed from the slot's ray and hit, and
-// the 48 raw outputs go to graw[slot].
-__global__ void __launch_bounds__(256) k_guide_infer_hits(Wavefront w, DevScene scene) {
+// the 48 raw outputs go to graw[slot].  A wave takes 64 queue entries at a
+// time, one per lane, so the chain of dependent loads behind the inputs
+// (queue -> slot -> ray and hit -> shape -> triangle) is walked for 64
+// vertices at once; the MLP then runs as four 16-vertex tiles whose inputs
+// are shuffled in from the lanes that gathered them.  The weights sit in LDS
+// once per block of kInferBlock threads.
+constexpr int kInferBlock = 512;
+__global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, DevScene scene) {
   __shared__ __align__(16) _Float16 sW[guide::kWeights];
   __shared__ float sB[guide::kBiases];
   guide::load_weights_lds_frag(sW, sB, w.net.w, w.net.b);
@@ -433,8 +439,8 @@ __global__ void __launch_bounds__(256) k_guide_infer_hits(Wavefront w, DevScene 
   const int lane = __lane_id();
   const int v = lane & 15, g = lane >> 4;
   const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-  for (uint32_t tile = blockIdx.x * waves + wave; tile * 16 < n; tile += gridDim.x * waves) {
-    const uint32_t i = tile * 16 + v;
+  for (uint32_t base = (blockIdx.x * waves + wave) * 64; base < n; base += gridDim.x * waves * 64) {
+    const uint32_t i = base + lane;
     float f9[guide::kFeat];
     uint32_t s = 0;
     if (i < n) {
@@ -450,18 +456,25 @@ __global__ void __launch_bounds__(256) k_guide_infer_hits(Wavefront w, DevScene 
 #pragma unroll
       for (int k = 0; k < guide::kFeat; k++) f9[k] = 0.f;
     }
-    guide::FeatScalars fs = guide::feat_scalars(f9, w.net.lo, w.net.inv_ext);
-    guide::h4 act[4][4];
-    guide::f4v out[3];
-    guide::forward_tile<true>(sW, sB, fs, act, out, nullptr);
-    if (i < n) {
-      _Float16* dst = w.graw + size_t(s) * guide::kOut + 4 * g;
+    const guide::FeatScalars mine = guide::feat_scalars(f9, w.net.lo, w.net.inv_ext);
+    for (int t16 = 0; t16 < 4 && base + 16 * t16 < n; t16++) {  // wave-uniform bound
+      const int src = 16 * t16 + v;
+      guide::FeatScalars fs;
 #pragma unroll
-      for (int t = 0; t < 3; t++) {
-        guide::h4 o;
+      for (int k = 0; k < 7; k++) fs.s[k] = __shfl(mine.s[k], src);
+      const uint32_t sv = __shfl(s, src);
+      guide::h4 act[4][4];
+      guide::f4v out[3];
+      guide::forward_tile<true>(sW, sB, fs, act, out, nullptr);
+      if (base + src < n) {
+        _Float16* dst = w.graw + size_t(sv) * guide::kOut + 4 * g;
 #pragma unroll
-        for (int j = 0; j < 4; j++) o[j] = (_Float16)out[t][j];
-        *reinterpret_cast<guide::h4*>(dst + 16 * t) = o;
+        for (int t = 0; t < 3; t++) {
+          guide::h4 o;
+#pragma unroll
+          for (int j = 0; j < 4; j++) o[j] = (_Float16)out[t][j];
+          *reinterpret_cast<guide::h4*>(dst + 16 * t) = o;
+        }
       }
     }
   }
@@ -725,7 +738,7 @@ void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const La
 }
 
 void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_guide_infer_hits, dim3(blocks), dim3(256), 0, s, w, scene);
+  hipLaunchKernelGGL(k_guide_infer_hits, dim3(blocks), dim3(kInferBlock), 0, s, w, scene);
 }
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);

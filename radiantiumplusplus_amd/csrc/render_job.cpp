@@ -211,7 +211,7 @@ void RunJob(rad_job* job) {
     LaunchCfg cfg{std::max(64, cus * bpc), std::max(64, cus * tpc)};
     job->blocks_per_cu = uint32_t(bpc);
     job->trace_blocks_per_cu = uint32_t(tpc);
-    const int inferBlocks = std::max(64, job->scene->ctx->cu_count * 3);
+    const int inferBlocks = std::max(64, job->scene->ctx->cu_count * 2);  // two 512-thread blocks per CU
     const bool prof = (job->desc.flags & RAD_RENDER_PROFILE) != 0;
     const size_t readback = CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t) + G_WORDS * 4;
     hipEvent_t ev[7] = {};
