@@ -108,7 +108,17 @@ enum {
   RAD_DISNEY_HAS_ETA = 1u << 8 /* "eta" given (else "specular" is used) */
 };
 
-/* Constant-parameter BSDF record.  Defaults are the reference constructors'. */
+/* BSDF parameters that may be textured (every Texture<T> member of the
+ * reference's BSDFs; "alpha" maps to both alpha slots). */
+enum {
+  RAD_TEX_REFLECTANCE = 0, RAD_TEX_TRANSMITTANCE, RAD_TEX_ETA, RAD_TEX_K, RAD_TEX_ALPHA_U, RAD_TEX_ALPHA_V,
+  RAD_TEX_BASE_COLOR, RAD_TEX_ROUGHNESS, RAD_TEX_ANISOTROPIC, RAD_TEX_SPEC_TRANS, RAD_TEX_SHEEN, RAD_TEX_SHEEN_TINT,
+  RAD_TEX_FLATNESS, RAD_TEX_SPEC_TINT, RAD_TEX_METALLIC, RAD_TEX_CLEARCOAT, RAD_TEX_CLEARCOAT_GLOSS,
+  RAD_TEX_DIFFUSE, RAD_TEX_SPECULAR, RAD_TEX_OPACITY,
+  RAD_TEX_SLOTS
+};
+
+/* BSDF record.  Defaults are the reference constructors'. */
 typedef struct rad_bsdf_desc {
   int32_t type;            /* RAD_BSDF_* */
   int32_t distribution;    /* RAD_MICROFACET_* (rough_metal, rough_glass) */
@@ -135,6 +145,9 @@ typedef struct rad_bsdf_desc {
   /* two_side / mask */
   int32_t nested;          /* index of the wrapped BSDF in the scene's bsdf array */
   float opacity;           /* mask "opacity" (default 0.5, clamped to [0, 1]) */
+  /* Textured parameters (ConfigNodeReadTexture, build/config_node_ext.h:17-45):
+   * tex[RAD_TEX_*] = 1 + index into rad_scene_desc.textures, 0 = the constant above. */
+  int32_t tex[RAD_TEX_SLOTS];
 } rad_bsdf_desc;
 
 enum {
@@ -171,6 +184,31 @@ typedef struct rad_medium_desc {
   const float* grid;       /* ((z*Y + y)*X + x)*C floats */
 } rad_medium_desc;
 
+/* Textures (texture/bitmap.cpp, texture/chessboard.cpp).  A bitmap holds
+ * linear texel values as the reference's image assets deliver them
+ * (image_stb.cpp: 8-bit / 255, sRGB -> linear for 3 channels, rows flipped
+ * so that texel row 0 is the image's bottom row); texel (x, y) channel c is
+ * texels[(y * width + x) * channels + c].  The library builds the mip chain
+ * (bitmap.cpp:25-61) itself. */
+enum { RAD_TEXTURE_BITMAP = 0, RAD_TEXTURE_CHESSBOARD = 1 };
+enum { RAD_FILTER_NEAREST = 0, RAD_FILTER_BILINEAR = 1, RAD_FILTER_TRILINEAR = 2, RAD_FILTER_ANISOTROPIC = 3 };
+typedef struct rad_texture_desc {
+  int32_t type;            /* RAD_TEXTURE_* */
+  int32_t channels;        /* 3 (colour parameters) or 1 (scalar parameters) */
+  /* bitmap */
+  uint32_t width, height;
+  const float* texels;
+  int32_t filter;          /* RAD_FILTER_* (default bilinear) */
+  int32_t wrap;            /* RAD_WRAP_* (default clamp) */
+  int32_t max_level;       /* mip levels: -1 = full chain (trilinear / anisotropic default), 0 = level 0 only */
+  float anisotropic_level; /* default 8 for anisotropic, else 0 */
+  /* chessboard */
+  float to_uv[16];         /* "to_uv" (row-major, default identity) */
+  int32_t tex_a, tex_b;    /* 1 + index of a bitmap texture for color_a / color_b, 0 = the constants */
+  float color_a[3], color_b[3]; /* defaults 0.4 / 0.2 (mono: channel 0) */
+  int32_t remap;           /* "remap" */
+} rad_texture_desc;
+
 typedef struct rad_camera_desc {
   float to_world[16];      /* camera-to-world (LookAtLeftHand or explicit) */
   float fov;               /* degrees, horizontal (camera/perspective.cpp:35) */
@@ -190,6 +228,8 @@ typedef struct rad_scene_desc {
   const rad_medium_desc* media;
   int32_t global_medium;   /* index into media, -1 = none */
   rad_camera_desc camera;
+  uint32_t texture_count;
+  const rad_texture_desc* textures;
 } rad_scene_desc;
 
 /* ---- rays and hits --------------------------------------------------------- */
@@ -394,6 +434,16 @@ rad_status rad_hip_bsdf_eval(const rad_bsdf_desc* bsdf, uint32_t quirks, const f
 rad_status rad_hip_bsdf_sample(const rad_bsdf_desc* bsdf, uint32_t quirks, const float* in, uint32_t n, float* out);
 /* Heterogeneous grid lookup (volume/grid.cpp:41-75): in = n x {p[3]} in [0,1]^3, out = n x 3. */
 rad_status rad_hip_grid_eval(const rad_medium_desc* medium, const float* in, uint32_t n, float* out);
+/* Texture lookup (Texture<T>::Eval, texture.h:57-62, with bitmap.cpp / chessboard.cpp):
+ * textures[index] of an array of count, in = n x {uv[2], duvdx[2], duvdy[2]}, out = n x 3
+ * (mono textures fill channel 0). */
+rad_status rad_hip_texture_eval(const rad_texture_desc* textures, uint32_t count, uint32_t index, const float* in,
+                                uint32_t n, float* out);
+/* Texture coordinates at the first hit of the camera ray through each screen
+ * position with their screen-space derivatives (perspective.cpp:84-99 and
+ * SurfaceInteraction::BSDF(const RayDifferential&), interaction.cpp:83-111):
+ * in = n x {sx, sy} pixels, out = n x {uv[2], duvdx[2], duvdy[2]} (zeros on a miss). */
+rad_status rad_hip_camera_footprint(rad_scene* scene, const float* in, uint32_t n, float* out);
 
 
 /* ---- render serving ---------------------------------------------------------------

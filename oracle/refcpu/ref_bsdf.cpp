@@ -881,7 +881,8 @@ float RefErfInv(float a) { return ErfInv(a); }
 
 std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks) { return MakeBsdf(&d, 1, 0, quirks, 0); }
 
-std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_t index, bool quirks, int depth) {
+std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_t index, bool quirks, int depth,
+                               const Float* specWeights) {
   if (index >= count || depth > 2) throw std::runtime_error("bsdf: nested index out of range");
   const rad_bsdf_desc& d = all[index];
   auto spec = [](const float* v) { return Spectrum(v[0], v[1], v[2]); };
@@ -977,6 +978,7 @@ std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_
       b->specular = spec(d.specular_reflectance);
       b->nonlinear = d.nonlinear != 0;
       b->Init();
+      if (specWeights && !std::isnan(specWeights[index])) b->specularSampleWeight = specWeights[index];
       out = std::move(b);
       break;
     }
@@ -1001,7 +1003,7 @@ std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_
     case RAD_BSDF_TWO_SIDE: {
       auto b = std::make_unique<TwoSide>();
       if (d.nested < 0) throw std::runtime_error("two_side: no nested bsdf");
-      b->nested = MakeBsdf(all, count, uint32_t(d.nested), quirks, depth + 1);
+      b->nested = MakeBsdf(all, count, uint32_t(d.nested), quirks, depth + 1, specWeights);
       b->flags = b->nested->flags;
       out = std::move(b);
       break;
@@ -1009,7 +1011,7 @@ std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_
     case RAD_BSDF_MASK: {
       auto b = std::make_unique<Mask>();
       if (d.nested < 0) throw std::runtime_error("mask: no nested bsdf");
-      b->nested = MakeBsdf(all, count, uint32_t(d.nested), quirks, depth + 1);
+      b->nested = MakeBsdf(all, count, uint32_t(d.nested), quirks, depth + 1, specWeights);
       b->opacity = std::clamp(d.opacity, Float(0), Float(1));
       b->flags = b->nested->flags;
       out = std::move(b);

@@ -205,8 +205,9 @@ static Float MisWeight(Float a, Float b) {
   return std::isfinite(w) ? w : 0;
 }
 
-Spectrum PathLi(const Ray& ray0, const Scene& scene, Sampler& sampler, int maxDepth, int rrDepth) {
+Spectrum PathLi(const Ray& ray0, const Scene& scene, Sampler& sampler, int maxDepth, int rrDepth, const RayDiff& diff) {
   Ray ray = ray0;
+  RayDiff rd = diff;  // only the camera ray carries differentials (SpawnRay drops them)
   Spectrum throughput(1), result(0);
   Float eta = 1;
   int depth = 0;
@@ -231,7 +232,9 @@ Spectrum PathLi(const Ray& ray0, const Scene& scene, Sampler& sampler, int maxDe
     }
     if (!anyHit || (maxDepth >= 0 && depth + 1 >= maxDepth)) break;
     if (si.shape->IsLight()) break;
-    const Bsdf* bsdf = si.shape->bsdf;
+    ComputeUVDifferentials(si, rd);  // si.BSDF(ray) (path.cpp:172)
+    std::unique_ptr<Bsdf> resolved;
+    const Bsdf* bsdf = scene.HitBsdf(si, resolved);
     if (bsdf->HasAnyTypeExceptDelta()) {
       Float sl = sampler.Next1D();
       V2 xi = sampler.Next2D();
@@ -254,6 +257,7 @@ Spectrum PathLi(const Ray& ray0, const Scene& scene, Sampler& sampler, int maxDe
       throughput = Mul(throughput, f) / bsr.Pdf;
       eta *= bsr.Eta;
       ray = si.SpawnRay(si.ToWorld(bsr.Wo));
+      rd.has = false;
       prevBsdfPdf = bsr.Pdf;
       prevSi = si;
       isSpecularPath = (bsr.TypeMask & BSDF_DELTA) != 0;
@@ -423,7 +427,9 @@ Spectrum VolPathLi(const Ray& ray0, const Scene& scene, Sampler& sampler, int ma
         depth--;
         continue;
       }
-      const Bsdf* bsdf = si.shape->bsdf;
+      // vol_path.cpp:28 drops the camera ray's differentials: textures see a zero footprint
+      std::unique_ptr<Bsdf> resolved;
+      const Bsdf* bsdf = scene.HitBsdf(si, resolved);
       if (bsdf->HasAnyTypeExceptDelta()) {
         auto [li, dsr] = VolSampleLight(scene, sampler, medium, channel, si);
         if (dsr.Pdf > 0) {
@@ -564,9 +570,10 @@ int ref_render(const ref_scene* s, const ref_render_params* p, float* fb, float*
         for (uint32_t i = 0; i < spp; i++) {
           Float jx = dist(rng);
           Float jy = dist(rng);
-          Ray ray = scene.camera.SampleRay(V2(x + jx, y + jy));
+          RayDiff rd;
+          Ray ray = scene.camera.SampleRayDifferential(V2(x + jx, y + jy), rd);
           Spectrum li = p->integrator == RAD_INTEGRATOR_VOL_PATH ? VolPathLi(ray, scene, sampler, p->max_depth, p->rr_depth)
-                                                                 : PathLi(ray, scene, sampler, p->max_depth, p->rr_depth);
+                                                                 : PathLi(ray, scene, sampler, p->max_depth, p->rr_depth, rd);
           bool bad = HasNaN(li);
           if (quirks) {  // spectrum.h:30-31 checks the first channel only
             bad = bad || std::isinf(li.x) || li.x < 0;
@@ -637,6 +644,42 @@ int ref_bsdf_sample(const rad_bsdf_desc* d, uint32_t count, int quirks, const fl
 }
 
 float ref_erfinv(float a) { return ref::RefErfInv(a); }
+
+int ref_texture_eval(const rad_texture_desc* t, uint32_t count, uint32_t index, const float* in, uint32_t n, float* out) {
+  try {
+    std::vector<std::unique_ptr<Texture>> tex;
+    BuildTextures(t, count, tex);
+    if (index >= count) throw std::runtime_error("texture index out of range");
+    for (uint32_t i = 0; i < n; i++) {
+      const float* q = in + 6 * i;
+      V3 v = tex[index]->Eval(V2(q[0], q[1]), V2(q[2], q[3]), V2(q[4], q[5]));
+      out[3 * i] = v.x;
+      out[3 * i + 1] = v.y;
+      out[3 * i + 2] = v.z;
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Texture coordinates and their screen-space derivatives at the first hit of
+// the camera ray through each screen position (0 where the ray misses).
+int ref_camera_footprint(const ref_scene* s, const float* in, uint32_t n, float* out) {
+  const Scene& scene = s->scene;
+  for (uint32_t i = 0; i < n; i++) {
+    RayDiff rd;
+    Ray ray = scene.camera.SampleRayDifferential(V2(in[2 * i], in[2 * i + 1]), rd);
+    SurfaceInteraction si{};
+    float* o = out + 6 * i;
+    for (int k = 0; k < 6; k++) o[k] = 0.f;
+    if (!scene.RayIntersect(ray, si) || !si.shape) continue;
+    ComputeUVDifferentials(si, rd);
+    o[0] = si.UV.x, o[1] = si.UV.y, o[2] = si.dUVdX.x, o[3] = si.dUVdX.y, o[4] = si.dUVdY.x, o[5] = si.dUVdY.y;
+  }
+  return 0;
+}
 
 int ref_grid_eval(const rad_medium_desc* d, const float* in, uint32_t n, float* out) {
   GridVolume g;

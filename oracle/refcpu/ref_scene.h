@@ -51,6 +51,7 @@ struct SurfaceInteraction : Interaction {
   Frame Shading;
   V2 UV;
   V3 dPdU, dPdV, dNdU, dNdV;
+  V2 dUVdX, dUVdY;  // texture-space footprint (camera rays only, interaction.cpp:83-111)
   V3 Wi;
   void ComputeShadingFrame() {  // interaction.cpp:45-53
     if (IsZero(dPdU)) {
@@ -163,6 +164,7 @@ struct Bounds {
 struct Shape {
   Float surfaceArea = 0;
   const Bsdf* bsdf = nullptr;
+  int32_t bsdfIndex = -1;
   const Light* light = nullptr;
   const Medium* inside = nullptr;
   const Medium* outside = nullptr;
@@ -253,8 +255,51 @@ struct Bsdf {
 };
 std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc& d, bool quirks);
 // bsdfs[index] of a scene's array (two_side / mask resolve their nested entry)
-std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_t index, bool quirks, int depth = 0);
+// specWeights (optional, per array entry, NaN = none): the plastic specular
+// sampling weight taken from textured diffuse / specular parameters.
+std::unique_ptr<Bsdf> MakeBsdf(const rad_bsdf_desc* all, uint32_t count, uint32_t index, bool quirks, int depth = 0,
+                               const Float* specWeights = nullptr);
 float RefErfInv(float a);
+
+// ---- textures (render/texture.h, texture/bitmap.cpp, texture/chessboard.cpp) ----
+struct TexImage {  // BlockArray2D<T> (the block tiling is only a storage layout); mono in .x
+  uint32_t w = 0, h = 0;
+  std::vector<V3> px;
+  const V3& At(uint32_t x, uint32_t y) const;
+};
+struct Texture {
+  int32_t type = RAD_TEXTURE_BITMAP, channels = 3;
+  // bitmap
+  std::vector<TexImage> mips;
+  int32_t filter = RAD_FILTER_BILINEAR, wrap = RAD_WRAP_CLAMP;
+  Float aniso = 0;
+  // chessboard
+  Transform toUV;
+  const Texture* a = nullptr;
+  const Texture* b = nullptr;
+  V3 colorA, colorB;
+  bool remap = false;
+
+  V3 Eval(const V2& uv, const V2& dx, const V2& dy) const;
+  double MeanLuminance() const;
+  void BuildMips(TexImage level0, int32_t maxLevel);
+  V3 BilinearAt(const TexImage& img, uint32_t u, uint32_t v) const;
+  V3 Bilinear4(const TexImage& img, uint32_t pu, uint32_t pv, Float fu, Float fv) const;
+  V3 Nearest(const TexImage& img, const V2& uv) const;
+  V3 Bilinear(const TexImage& img, const V2& uv) const;
+  V3 Isotropic(const V2& uv, size_t level) const;
+  V3 Ewa(const V2& uv, size_t level, const V2& d0, const V2& d1) const;
+  V3 Trilinear(const V2& uv, const V2& dx, const V2& dy) const;
+  V3 Anisotropic(const V2& uv, const V2& dx, const V2& dy) const;
+};
+void BuildTextures(const rad_texture_desc* descs, uint32_t count, std::vector<std::unique_ptr<Texture>>& out);
+
+// RayDifferential of a camera ray (ray.h:35-58, perspective.cpp:84-99).
+struct RayDiff {
+  bool has = false;
+  V3 Ox, Oy, Dx, Dy;
+};
+void ComputeUVDifferentials(SurfaceInteraction& si, const RayDiff& rd);
 
 // ---- lights (render/light.h) ----
 struct Scene;
@@ -301,8 +346,10 @@ struct Camera {
   int width = 1280, height = 720;
   V2 rcpResolution;
   Transform cameraToWorld, cameraToClip;
+  V3 dx, dy;  // camera-space offset of one pixel on the near plane
   void Init(const rad_camera_desc& d);
   Ray SampleRay(const V2& screenPosition) const;
+  Ray SampleRayDifferential(const V2& screenPosition, RayDiff& rd) const;
 };
 
 // ---- sampler (sampler/independent.cpp) ----
@@ -354,8 +401,15 @@ struct Scene {
   Accel accel;
   bool quirks = true;
   bool brute = false;
+  // textured parameters: the BSDF descriptors, which of them (or their nested
+  // BSDFs) read a texture, and the plastic weights from texel means
+  std::vector<rad_bsdf_desc> bsdfDescs;
+  std::vector<bool> bsdfTextured;
+  std::vector<Float> specWeights;
+  std::vector<std::unique_ptr<Texture>> textures;
 
   void Create(const rad_scene_desc& desc, bool quirks);
+  const Bsdf* HitBsdf(const SurfaceInteraction& si, std::unique_ptr<Bsdf>& holder) const;
   bool RayIntersect(const Ray& ray) const { return brute ? accel.OccludedBrute(ray) : accel.Occluded(ray); }
   bool RayIntersect(const Ray& ray, SurfaceInteraction& si) const;
   bool IsOcclude(const Interaction& ref, const V3& p) const;
@@ -370,7 +424,8 @@ struct Scene {
 };
 
 // ---- integrators (renderer/path.cpp, renderer/vol_path.cpp) ----
-Spectrum PathLi(const Ray& ray, const Scene& scene, Sampler& sampler, int maxDepth, int rrDepth);
+Spectrum PathLi(const Ray& ray, const Scene& scene, Sampler& sampler, int maxDepth, int rrDepth,
+                const RayDiff& diff = RayDiff());
 Spectrum VolPathLi(const Ray& ray, const Scene& scene, Sampler& sampler, int maxDepth, int rrDepth);
 
 }  // namespace ref

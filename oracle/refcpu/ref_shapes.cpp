@@ -482,6 +482,10 @@ void Camera::Init(const rad_camera_desc& d) {  // perspective.cpp:15-70
   scale = Mat4::FromRowMajor(S);
   translate = Mat4::FromRowMajor(T);
   cameraToClip = Transform(scale * translate * perspective);
+  // perspective.cpp:55-58
+  V3 c0 = cameraToClip.ApplyAffineToLocal(V3(0, 0, 0));
+  dx = cameraToClip.ApplyAffineToLocal(V3(Float(1) / Float(width), 0, 0)) - c0;
+  dy = cameraToClip.ApplyAffineToLocal(V3(0, Float(1) / Float(height), 0)) - c0;
 }
 
 Ray Camera::SampleRay(const V2& sp) const {  // perspective.cpp:84-99 (differentials unused)
@@ -492,6 +496,18 @@ Ray Camera::SampleRay(const V2& sp) const {  // perspective.cpp:84-99 (different
   V3 o = cameraToWorld.ApplyAffineToWorld(V3(0, 0, 0));
   V3 d = Normalized(cameraToWorld.ApplyLinearToWorld(dir));
   return Ray{o, d, nearClip * invZ, farClip * invZ};
+}
+
+Ray Camera::SampleRayDifferential(const V2& sp, RayDiff& rd) const {  // perspective.cpp:84-99
+  V3 ndc(sp.x * rcpResolution.x, sp.y * rcpResolution.y, 0);
+  V3 nearP = cameraToClip.ApplyAffineToLocal(ndc);
+  Ray r = SampleRay(sp);
+  rd.has = true;
+  rd.Ox = r.O;
+  rd.Oy = r.O;
+  rd.Dx = cameraToWorld.ApplyLinearToWorld(Normalized(nearP + dx));
+  rd.Dy = cameraToWorld.ApplyLinearToWorld(Normalized(nearP + dy));
+  return r;
 }
 
 // ---------------------------------------------------------------- scene
@@ -509,7 +525,34 @@ static const uint32_t kCubeT[12][3] = {{0, 1, 2},    {3, 0, 2},    {4, 5, 6},   
 
 void Scene::Create(const rad_scene_desc& desc, bool q) {
   quirks = q;
-  for (uint32_t i = 0; i < desc.bsdf_count; i++) bsdfs.push_back(MakeBsdf(desc.bsdfs, desc.bsdf_count, i, quirks));
+  BuildTextures(desc.textures, desc.texture_count, textures);
+  bsdfDescs.assign(desc.bsdfs, desc.bsdfs + desc.bsdf_count);
+  specWeights.assign(desc.bsdf_count, std::numeric_limits<Float>::quiet_NaN());
+  bsdfTextured.assign(desc.bsdf_count, false);
+  for (uint32_t i = 0; i < desc.bsdf_count; i++) {
+    const rad_bsdf_desc& d = desc.bsdfs[i];
+    for (int k = 0; k < RAD_TEX_SLOTS; k++)
+      if (d.tex[k] < 0 || uint32_t(d.tex[k]) > desc.texture_count) throw std::runtime_error("bsdf: texture index out of range");
+    // plastic.cpp:29-42 / rough_plastic.cpp:42-55: the sampling weight from the texel means
+    if ((d.type == RAD_BSDF_PLASTIC || d.type == RAD_BSDF_ROUGH_PLASTIC) &&
+        (d.tex[RAD_TEX_DIFFUSE] || d.tex[RAD_TEX_SPECULAR])) {
+      double dm = d.tex[RAD_TEX_DIFFUSE] ? textures[d.tex[RAD_TEX_DIFFUSE] - 1]->MeanLuminance()
+                                         : double(Luminance(Spectrum(d.diffuse[0], d.diffuse[1], d.diffuse[2])));
+      double sm = d.tex[RAD_TEX_SPECULAR]
+                      ? textures[d.tex[RAD_TEX_SPECULAR] - 1]->MeanLuminance()
+                      : double(Luminance(Spectrum(d.specular_reflectance[0], d.specular_reflectance[1], d.specular_reflectance[2])));
+      specWeights[i] = Float(sm / (dm + sm));
+    }
+  }
+  for (uint32_t i = 0; i < desc.bsdf_count; i++) {
+    for (int32_t j = int32_t(i), guard = 0; j >= 0 && uint32_t(j) < desc.bsdf_count && guard < 3; guard++) {
+      const rad_bsdf_desc& d = desc.bsdfs[j];
+      for (int k = 0; k < RAD_TEX_SLOTS; k++) bsdfTextured[i] = bsdfTextured[i] || d.tex[k] != 0;
+      j = (d.type == RAD_BSDF_TWO_SIDE || d.type == RAD_BSDF_MASK) ? d.nested : -1;
+    }
+  }
+  for (uint32_t i = 0; i < desc.bsdf_count; i++)
+    bsdfs.push_back(MakeBsdf(desc.bsdfs, desc.bsdf_count, i, quirks, 0, specWeights.data()));
   for (uint32_t i = 0; i < desc.medium_count; i++) media.push_back(MakeMedium(desc.media[i]));
   globalMedium = desc.global_medium >= 0 ? media[desc.global_medium].get() : nullptr;
   for (uint32_t i = 0; i < desc.light_count; i++) {
@@ -607,6 +650,7 @@ void Scene::Create(const rad_scene_desc& desc, bool q) {
       throw std::runtime_error("unknown shape type");
     }
     s->index = i;
+    s->bsdfIndex = sd.bsdf;
     if (sd.bsdf >= 0) s->bsdf = bsdfs[sd.bsdf].get();
     if (sd.light >= 0) {
       s->light = lights[sd.light].get();

@@ -39,6 +39,10 @@ int ref_bsdf_eval(const rad_bsdf_desc* d, uint32_t count, int quirks, const floa
 int ref_bsdf_sample(const rad_bsdf_desc* d, uint32_t count, int quirks, const float* in, uint32_t n, float* out);
 int ref_grid_eval(const rad_medium_desc* d, const float* in, uint32_t n, float* out);
 float ref_erfinv(float a);
+/* textures[index] of an array: in = n x {uv[2], duvdx[2], duvdy[2]}, out = n x 3 */
+int ref_texture_eval(const rad_texture_desc* t, uint32_t count, uint32_t index, const float* in, uint32_t n, float* out);
+/* first-hit uv and d(uv)/dx, d(uv)/dy of camera rays: in = n x {sx, sy}, out = n x 6 */
+int ref_camera_footprint(const ref_scene* s, const float* in, uint32_t n, float* out);
 float ref_medium_transmittance(const ref_scene* s, int medium, const rad_ray* ray, uint32_t channel, uint32_t seed);
 
 #ifdef __cplusplus

@@ -94,6 +94,37 @@ class BsdfDesc(C.Structure):
         ("nonlinear", C.c_int32),
         ("nested", C.c_int32),
         ("opacity", C.c_float),
+        ("tex", C.c_int32 * 20),
+    ]
+
+
+# textured BSDF parameter slots (rad_bsdf_desc.tex)
+(RAD_TEX_REFLECTANCE, RAD_TEX_TRANSMITTANCE, RAD_TEX_ETA, RAD_TEX_K, RAD_TEX_ALPHA_U, RAD_TEX_ALPHA_V,
+ RAD_TEX_BASE_COLOR, RAD_TEX_ROUGHNESS, RAD_TEX_ANISOTROPIC, RAD_TEX_SPEC_TRANS, RAD_TEX_SHEEN, RAD_TEX_SHEEN_TINT,
+ RAD_TEX_FLATNESS, RAD_TEX_SPEC_TINT, RAD_TEX_METALLIC, RAD_TEX_CLEARCOAT, RAD_TEX_CLEARCOAT_GLOSS,
+ RAD_TEX_DIFFUSE, RAD_TEX_SPECULAR, RAD_TEX_OPACITY) = range(20)
+RAD_TEX_SLOTS = 20
+RAD_TEXTURE_BITMAP, RAD_TEXTURE_CHESSBOARD = 0, 1
+RAD_FILTER_NEAREST, RAD_FILTER_BILINEAR, RAD_FILTER_TRILINEAR, RAD_FILTER_ANISOTROPIC = 0, 1, 2, 3
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("channels", C.c_int32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("texels", C.POINTER(C.c_float)),
+        ("filter", C.c_int32),
+        ("wrap", C.c_int32),
+        ("max_level", C.c_int32),
+        ("anisotropic_level", C.c_float),
+        ("to_uv", F16),
+        ("tex_a", C.c_int32),
+        ("tex_b", C.c_int32),
+        ("color_a", F3),
+        ("color_b", F3),
+        ("remap", C.c_int32),
     ]
 
 
@@ -146,6 +177,8 @@ class SceneDesc(C.Structure):
         ("media", C.POINTER(MediumDesc)),
         ("global_medium", C.c_int32),
         ("camera", CameraDesc),
+        ("texture_count", C.c_uint32),
+        ("textures", C.POINTER(TextureDesc)),
     ]
 
 

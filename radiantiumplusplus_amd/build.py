@@ -43,6 +43,7 @@ SOURCES = [
     "capi.cpp",
     "path_kernels.hip",
     "volpath_kernels.hip",
+    "texture_kernels.hip",
     "guide_kernels.hip",
     "server.cpp",
     "host/json.cpp",

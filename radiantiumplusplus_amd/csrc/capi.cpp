@@ -63,6 +63,8 @@ void UploadScene(rad_scene* s) {
   s->d_cdf.upload(h.light_cdf.data(), h.light_cdf.size() * sizeof(float), st);
   s->d_media.upload(h.media.data(), h.media.size() * sizeof(MediumRec), st);
   s->d_grid.upload(h.grid.data(), h.grid.size() * sizeof(float), st);
+  s->d_tex.upload(h.textures.data(), h.textures.size() * sizeof(TexRec), st);
+  s->d_texels.upload(h.texels.data(), h.texels.size() * sizeof(float), st);
   DevScene& d = s->dev;
   d = DevScene{};
   d.shapes = s->d_shapes.as<ShapeRec>();
@@ -80,6 +82,8 @@ void UploadScene(rad_scene* s) {
   d.media = s->d_media.as<MediumRec>();
   d.global_medium = h.global_medium;
   d.grid_data = s->d_grid.as<float>();
+  d.textures = s->d_tex.as<TexRec>();
+  d.texels = s->d_texels.as<float>();
   d.cam = h.cam;
   // skybox world sphere (skybox.cpp:136-141, bounding_sphere.cpp:15-18)
   f3 lo = mk(h.world_lo[0], h.world_lo[1], h.world_lo[2]), hi = mk(h.world_hi[0], h.world_hi[1], h.world_hi[2]);
@@ -406,6 +410,48 @@ rad_status rad_hip_intersect_closest(rad_scene* scene, const rad_ray* rays, uint
     HipCheck(hipGetLastError(), "closest-hit launch");
     if (n) HipCheck(hipMemcpyAsync(hits, scene->d_hits.ptr, sizeof(rad_hit) * size_t(n), hipMemcpyDeviceToHost, st), "copy hits");
     HipCheck(hipStreamSynchronize(st), "closest-hit");
+  });
+}
+
+rad_status rad_hip_camera_footprint(rad_scene* scene, const float* in, uint32_t n, float* out) {
+  if (!scene || (n && (!in || !out))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  return Guard([&] {
+    std::lock_guard<std::mutex> lk(scene->mu);
+    RequireBvh(scene);
+    HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
+    hipStream_t st = scene->ctx->stream;
+    DeviceBuffer din, dout;
+    din.alloc(std::max<size_t>(16, size_t(n) * 8));
+    dout.alloc(std::max<size_t>(16, size_t(n) * 24));
+    if (n) HipCheck(hipMemcpyAsync(din.ptr, in, size_t(n) * 8, hipMemcpyHostToDevice, st), "copy positions");
+    LaunchFootprint(scene->dev, din.as<float>(), n, dout.as<float>(), st);
+    HipCheck(hipGetLastError(), "footprint launch");
+    if (n) HipCheck(hipMemcpyAsync(out, dout.ptr, size_t(n) * 24, hipMemcpyDeviceToHost, st), "copy footprints");
+    HipCheck(hipStreamSynchronize(st), "footprint");
+  });
+}
+
+rad_status rad_hip_texture_eval(const rad_texture_desc* textures, uint32_t count, uint32_t index, const float* in,
+                                uint32_t n, float* out) {
+  if (!textures || index >= count || (n && (!in || !out))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument or index out of range");
+  return Guard([&] {
+    int dev = 0;
+    if (hipGetDeviceCount(&dev) != hipSuccess || dev == 0) throw StatusError(RAD_ERR_HIP, "no HIP device available");
+    std::vector<TexRec> recs;
+    std::vector<float> texels;
+    BuildTextureSet(textures, count, recs, texels);
+    DeviceBuffer dt, dx, din, dout;
+    dt.upload(recs.data(), recs.size() * sizeof(TexRec), nullptr);
+    dx.upload(texels.data(), texels.size() * sizeof(float), nullptr);
+    din.alloc(std::max<size_t>(16, size_t(n) * 24));
+    dout.alloc(std::max<size_t>(16, size_t(n) * 12));
+    if (n) HipCheck(hipMemcpy(din.ptr, in, size_t(n) * 24, hipMemcpyHostToDevice), "copy lookups");
+    DevScene d{};
+    d.textures = dt.as<TexRec>();
+    d.texels = dx.as<float>();
+    LaunchTexEval(d, index, din.as<float>(), n, dout.as<float>(), nullptr);
+    HipCheck(hipGetLastError(), "texture probe launch");
+    if (n) HipCheck(hipMemcpy(out, dout.ptr, size_t(n) * 12, hipMemcpyDeviceToHost), "copy results");
   });
 }
 

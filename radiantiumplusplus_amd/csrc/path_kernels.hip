@@ -5,6 +5,7 @@
 #include "guide_dev.h"
 #include "guide_mlp.h"
 #include "shading_dev.h"
+#include "texture_dev.h"
 #include "wavefront.h"
 
 namespace rad {
@@ -266,9 +267,24 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           f3 le = dot(si.sh.to_world(si.wi), si.n) >= 0.f ? scene.lights[sr.light].radiance : splat(0.f);
           L += mul(T, le) * wgt;
         }
-        bool stop = (w.max_depth >= 0 && int32_t(depth) + 1 >= w.max_depth) || sr.light >= 0;
+        // a shape with neither light nor BSDF ends the path (the reference dereferences a null BSDF there)
+        bool stop = (w.max_depth >= 0 && int32_t(depth) + 1 >= w.max_depth) || sr.light >= 0 || sr.bsdf < 0;
         if (!stop) {
-          const BsdfRec& b = scene.bsdfs[sr.bsdf];
+          const BsdfRec* all = scene.bsdfs;
+          const BsdfRec* bp0 = &scene.bsdfs[sr.bsdf];
+          BsdfRec chain[3];
+          if constexpr (MAT < 0 || MAT == kMatOther) {  // textured BSDFs are classified into the generic queue
+            if (bp0->tex_any) {
+              // only the camera ray carries differentials (path.cpp:172; SpawnRay drops them)
+              const uint32_t W = uint32_t(scene.cam.width);
+              const float sx = float(m.x % W) + rng_u(key, 0), sy = float(m.x / W) + rng_u(key, 1);
+              const TexCoord tc = hit_texcoord(scene, si, hh, geom, depth == 0, sx, sy);
+              resolve_chain(scene, *bp0, tc, chain);
+              all = chain;
+              bp0 = &chain[0];
+            }
+          }
+          const BsdfRec& b = *bp0;
           const bool guided = GUIDE && guide::guidable(b, quirks, w.net.min_rough);
           guide::Mixture mx;
           if (guided && guideOn) mx = guide::decode(w.graw + size_t(s) * guide::kOut);
@@ -280,7 +296,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             if (ls.pdf > 0.f) {
               f3 wo = si.sh.to_local(ls.dir);
               float bp;
-              f3 f = bsdf_eval_t<MAT>(scene.bsdfs, b, quirks, si.wi, wo, bp);
+              f3 f = bsdf_eval_t<MAT>(all, b, quirks, si.wi, wo, bp);
               if (a > 0.f) bp = a * guide::mixture_pdf(mx, ls.dir) + (1.f - a) * bp;
               // Scene::IsOcclude setup (scene.cpp:53-63)
               f3 o = offset_p(si.p, si.n, ls.p - si.p);
@@ -314,14 +330,14 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           if (useGuide) {
             f3 wg = guide::mixture_sample(mx, ul, u1, u2);
             float bp;
-            f = bsdf_eval_t<MAT>(scene.bsdfs, b, quirks, si.wi, si.sh.to_local(wg), bp);
+            f = bsdf_eval_t<MAT>(all, b, quirks, si.wi, si.sh.to_local(wg), bp);
             wo = wg;
             pdf = a * guide::mixture_pdf(mx, wg) + (1.f - a) * bp;
             bs.eta = 1.f;
             bs.lobes = LOBE_GLOSSY;
             if (is_zero(f)) pdf = 0.f;  // zero-valued paths end here without bias
           } else {
-            f = bsdf_sample_t<MAT>(scene.bsdfs, b, quirks, si.wi, ul, u1, u2, bs);
+            f = bsdf_sample_t<MAT>(all, b, quirks, si.wi, ul, u1, u2, bs);
             wo = si.sh.to_world(bs.wo);
             pdf = bs.pdf;
             if (a > 0.f && pdf > 0.f) pdf = a * guide::mixture_pdf(mx, wo) + (1.f - a) * bs.pdf;
@@ -480,7 +496,7 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
   }
 }
 
-// Partition the extend queue by material (misses go with the diffuse bucket)
+// Partition the extend queue by material (misses and BSDF-less emitters go with the diffuse bucket)
 // so each shade kernel runs one BSDF's code.
 // GUIDE: guided vertices also go to the guide queue.  Diffuse, rough metal,
 // rough glass and Disney get queues of their own (misses join the diffuse
@@ -502,7 +518,13 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
       held = w.susp && w.susp[s] != kNoSusp;  // traversal suspended: straight to the next extend queue
       if (!held) {
         uint32_t geom = w.hit_geom[s];
-        key = geom == 0xffffffffu ? BSDF_DIFFUSE : min(scene.bsdfs[scene.shapes[geom].bsdf].type, kMatOther);
+        const int32_t bi = geom == 0xffffffffu ? -1 : scene.shapes[geom].bsdf;
+        if (bi < 0) {  // misses and shapes without a BSDF (emitters): the always-launched diffuse kernel
+          key = BSDF_DIFFUSE;
+        } else {
+          const BsdfRec& br = scene.bsdfs[bi];
+          key = br.tex_any ? kMatOther : min(br.type, kMatOther);
+        }
         if (GUIDE) gd = guided_hit(w, scene, geom, quirks);
       }
     }

@@ -467,7 +467,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.flags = (desc.flags & RAD_RENDER_FIX_QUIRKS) ? 0u : WF_QUIRKS;
   w.integrator = desc.integrator;
   job->sorted = desc.integrator == RAD_INTEGRATOR_PATH && !(desc.flags & RAD_RENDER_UNSORTED);
-  for (const BsdfRec& b : scene->host.bsdfs) job->mat_mask |= 1u << std::min<int32_t>(b.type, kMatOther);
+  for (const BsdfRec& b : scene->host.bsdfs) job->mat_mask |= 1u << (b.tex_any ? kMatOther : std::min<int32_t>(b.type, kMatOther));
   HipCheck(hipHostMalloc((void**)&job->h_cnt, CNT_WORDS * 4 + ST_WORDS * 8 + 2 * G_WORDS * 4, hipHostMallocDefault),
            "hipHostMalloc");
   HipCheck(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking), "hipStreamCreate");

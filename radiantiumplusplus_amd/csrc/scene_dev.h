@@ -114,6 +114,29 @@ struct BsdfRec {
   // two_side / mask
   int32_t nested;
   float opacity;
+  // textured parameters: bit k of tex_mask set = slot k (RAD_TEX_*) reads
+  // textures[tex[k]]; tex_any: this record or a record it wraps is textured
+  uint32_t tex_mask;
+  uint32_t tex_any;
+  uint16_t tex[20];
+};
+
+// A texture (texture/bitmap.cpp, texture/chessboard.cpp).  Bitmap texels are
+// float, `channels` per texel, level l at texels + off[l], row-major.
+constexpr int kMaxMips = 16;
+enum : int32_t { TEX_BITMAP = 0, TEX_CHESSBOARD = 1 };
+struct TexRec {
+  int32_t type, channels, filter, wrap;
+  uint32_t levels;
+  float aniso;
+  uint32_t off[kMaxMips];
+  uint32_t w[kMaxMips], h[kMaxMips];
+  // chessboard: to_uv (affine with the homogeneous divide), nested bitmaps
+  // (-1 = the constant colour), remap
+  float to_uv[16];
+  int32_t tex_a, tex_b;
+  f3 color_a, color_b;
+  int32_t remap;
 };
 
 enum : int32_t { LIGHT_AREA = 0, LIGHT_SKYBOX = 1, LIGHT_POINT = 2, LIGHT_PROJECTION = 3 };
@@ -149,6 +172,7 @@ struct CameraRec {
   float near_clip, far_clip;
   int32_t width, height;
   float rcp_w, rcp_h;
+  f3 dx, dy;             // camera-space offset of one pixel on the near plane (perspective.cpp:55-58)
 };
 
 struct DevScene {
@@ -171,6 +195,8 @@ struct DevScene {
   const MediumRec* media;
   int32_t global_medium;
   const float* grid_data;
+  const TexRec* textures;
+  const float* texels;
   CameraRec cam;
   f3 world_center;
   float world_radius;

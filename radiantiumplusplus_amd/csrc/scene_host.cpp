@@ -241,6 +241,124 @@ BsdfRec MakeBsdfRec(const rad_bsdf_desc& d) {
   return b;
 }
 
+// ---------------------------------------------------------------- textures
+// A bitmap's mip chain (texture/bitmap.cpp:27-61): levels of half size (at
+// least 1), each texel the mean of four bilinear lookups at integer
+// coordinates 2i, 2i + 1 of the level above; with those coordinates the
+// reference's bilinear weights are exactly 1/2 towards the next texel
+// (clamped at the last one), after the integer wrap of the coordinates.
+// Odd sizes at level 0 disable the chain; max_level < 0 asks for
+// 1 + floor(log2(max(w, h))) levels.
+static void BuildMipChain(const rad_texture_desc& d, TexRec& t, std::vector<float>& pool) {
+  const uint32_t C = uint32_t(d.channels);
+  int32_t maxLevel = d.max_level;
+  if (maxLevel != 0 && ((d.width % 2 != 0) || (d.height % 2 != 0))) maxLevel = 0;
+  uint32_t levels = maxLevel < 0 ? 1 + uint32_t(std::log2(double(std::max(d.width, d.height)))) : uint32_t(maxLevel);
+  levels = std::max(levels, 1u);
+  if (levels > uint32_t(kMaxMips)) throw std::runtime_error("bitmap: more than 16 mip levels");
+  t.levels = levels;
+  if (pool.size() + size_t(d.width) * d.height * C * 2 > 0xffffffffull) throw std::runtime_error("textures exceed 4G texels");
+  t.off[0] = uint32_t(pool.size());
+  t.w[0] = d.width;
+  t.h[0] = d.height;
+  pool.insert(pool.end(), d.texels, d.texels + size_t(d.width) * d.height * C);
+  for (uint32_t l = 1; l < levels; l++) {
+    const uint32_t W = t.w[l - 1], H = t.h[l - 1];
+    const uint32_t w = std::max(1u, W / 2), h = std::max(1u, H / 2);
+    t.off[l] = uint32_t(pool.size());
+    t.w[l] = w;
+    t.h[l] = h;
+    pool.resize(pool.size() + size_t(w) * h * C);
+    const float* src = pool.data() + t.off[l - 1];
+    float* dst = pool.data() + t.off[l];
+    auto wrapi = [&](uint32_t v, uint32_t n) { return d.wrap == RAD_WRAP_REPEAT ? v % n : std::min(v, n - 1); };
+    auto tap = [&](uint32_t u, uint32_t v, uint32_t c) {
+      const uint32_t pu = wrapi(u, W), pv = wrapi(v, H);
+      const uint32_t au = std::min(pu + 1, W - 1), av = std::min(pv + 1, H - 1);
+      const float a = src[(size_t(pv) * W + pu) * C + c], b = src[(size_t(pv) * W + au) * C + c];
+      const float e = src[(size_t(av) * W + pu) * C + c], f = src[(size_t(av) * W + au) * C + c];
+      return (a * 0.5f + b * 0.5f) * 0.5f + (e * 0.5f + f * 0.5f) * 0.5f;
+    };
+    for (uint32_t j = 0; j < h; j++)
+      for (uint32_t i = 0; i < w; i++)
+        for (uint32_t c = 0; c < C; c++)
+          dst[(size_t(j) * w + i) * C + c] =
+              (tap(2 * i, 2 * j, c) + tap(2 * i + 1, 2 * j, c) + tap(2 * i, 2 * j + 1, c) + tap(2 * i + 1, 2 * j + 1, c)) *
+              0.25f;
+  }
+}
+
+void BuildTextureSet(const rad_texture_desc* descs, uint32_t n, std::vector<TexRec>& textures, std::vector<float>& texels) {
+  if (n && !descs) throw std::runtime_error("texture_count without textures");
+  textures.assign(n, TexRec{});
+  texels.clear();
+  for (uint32_t i = 0; i < n; i++) {
+    const rad_texture_desc& d = descs[i];
+    TexRec& t = textures[i];
+    t.type = d.type;
+    t.channels = d.channels;
+    if (d.channels != 1 && d.channels != 3) throw std::runtime_error("texture " + std::to_string(i) + ": channels must be 1 or 3");
+    t.tex_a = t.tex_b = -1;
+    if (d.type == RAD_TEXTURE_BITMAP) {
+      if (!d.texels || d.width == 0 || d.height == 0) throw std::runtime_error("bitmap " + std::to_string(i) + ": empty image");
+      if (d.filter < RAD_FILTER_NEAREST || d.filter > RAD_FILTER_ANISOTROPIC) throw std::runtime_error("bitmap: unknown filter");
+      t.filter = d.filter;
+      t.wrap = d.wrap;
+      t.aniso = d.anisotropic_level;
+      BuildMipChain(d, t, texels);
+    } else if (d.type == RAD_TEXTURE_CHESSBOARD) {
+      std::memcpy(t.to_uv, d.to_uv, sizeof(t.to_uv));
+      t.remap = d.remap != 0;
+      t.color_a = d.channels == 3 ? mk(d.color_a[0], d.color_a[1], d.color_a[2]) : mk(d.color_a[0], 0.f, 0.f);
+      t.color_b = d.channels == 3 ? mk(d.color_b[0], d.color_b[1], d.color_b[2]) : mk(d.color_b[0], 0.f, 0.f);
+      auto nested = [&](int32_t k) {
+        if (k == 0) return -1;
+        if (k < 1 || uint32_t(k) > n || descs[k - 1].type != RAD_TEXTURE_BITMAP)
+          throw std::runtime_error("chessboard: color_a / color_b must be constants or bitmaps");
+        return k - 1;
+      };
+      t.tex_a = nested(d.tex_a);
+      t.tex_b = nested(d.tex_b);
+    } else {
+      throw std::runtime_error("texture " + std::to_string(i) + ": unknown type");
+    }
+  }
+}
+
+// Texel mean of a texture's level 0 as Texture<T>::Read sees it (a chessboard
+// reads 0 as a single texel, chessboard.cpp:29-30, 57-59).
+static double TextureMean(const HostScene& h, uint32_t id) {
+  const TexRec& t = h.textures[id];
+  if (t.type == TEX_CHESSBOARD) return 0.0;
+  const float* p = h.texels.data() + t.off[0];
+  double sum = 0;
+  const size_t n = size_t(t.w[0]) * t.h[0];
+  for (size_t i = 0; i < n; i++) sum += t.channels == 3 ? Lum(p + 3 * i) : p[i];
+  return sum / double(n);
+}
+
+// Textured slots of a BSDF record; plastics take their specular sampling
+// weight from the texel means (plastic.cpp:29-42, rough_plastic.cpp:42-55).
+static void AttachTextures(const rad_bsdf_desc& d, const HostScene& h, BsdfRec& b) {
+  b.tex_mask = 0;
+  for (int k = 0; k < RAD_TEX_SLOTS; k++) {
+    if (d.tex[k] == 0) continue;
+    if (d.tex[k] < 0 || uint32_t(d.tex[k]) > h.textures.size()) throw std::runtime_error("bsdf: texture index out of range");
+    const TexRec& t = h.textures[d.tex[k] - 1];
+    const bool colour = k == RAD_TEX_REFLECTANCE || k == RAD_TEX_TRANSMITTANCE || k == RAD_TEX_ETA || k == RAD_TEX_K ||
+                        k == RAD_TEX_BASE_COLOR || k == RAD_TEX_DIFFUSE || k == RAD_TEX_SPECULAR;
+    if ((t.channels == 3) != colour) throw std::runtime_error("bsdf: texture channel count does not match its parameter");
+    b.tex_mask |= 1u << k;
+    b.tex[k] = uint16_t(d.tex[k] - 1);
+  }
+  if ((d.type == RAD_BSDF_PLASTIC || d.type == RAD_BSDF_ROUGH_PLASTIC) && (d.tex[RAD_TEX_DIFFUSE] || d.tex[RAD_TEX_SPECULAR])) {
+    double dm = d.tex[RAD_TEX_DIFFUSE] ? TextureMean(h, uint32_t(d.tex[RAD_TEX_DIFFUSE] - 1)) : double(Lum(d.diffuse));
+    double sm = d.tex[RAD_TEX_SPECULAR] ? TextureMean(h, uint32_t(d.tex[RAD_TEX_SPECULAR] - 1))
+                                        : double(Lum(d.specular_reflectance));
+    b.spec_weight = float(sm / (dm + sm));
+  }
+}
+
 // Wrappers: the nested index must name another record, and at most two
 // wrappers may enclose a base BSDF (the device resolves two levels).
 static void ResolveWrappers(std::vector<BsdfRec>& bsdfs) {
@@ -255,11 +373,24 @@ static void ResolveWrappers(std::vector<BsdfRec>& bsdfs) {
     }
     bsdfs[i].lobes = bsdfs[j].lobes;
   }
+  for (int32_t i = 0; i < n; i++) {  // textured: the record or one it wraps reads a texture
+    uint32_t any = 0;
+    for (int32_t j = i, guard = 0; guard < 3; guard++) {
+      any |= bsdfs[j].tex_mask;
+      if (bsdfs[j].type != BSDF_TWO_SIDE && bsdfs[j].type != BSDF_MASK) break;
+      j = bsdfs[j].nested;
+    }
+    bsdfs[i].tex_any = any != 0;
+  }
 }
 
 void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
   out = HostScene();
-  for (uint32_t i = 0; i < desc.bsdf_count; i++) out.bsdfs.push_back(MakeBsdfRec(desc.bsdfs[i]));
+  BuildTextureSet(desc.textures, desc.texture_count, out.textures, out.texels);
+  for (uint32_t i = 0; i < desc.bsdf_count; i++) {
+    out.bsdfs.push_back(MakeBsdfRec(desc.bsdfs[i]));
+    AttachTextures(desc.bsdfs[i], out, out.bsdfs.back());
+  }
   ResolveWrappers(out.bsdfs);
   if (desc.light_count == 0) throw std::runtime_error("scene has no light (Scene::SampleLight needs one)");
   // media
@@ -352,6 +483,10 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
       if (desc.lights[d.light].type != RAD_LIGHT_AREA) throw std::runtime_error("only area lights attach to shapes");
       out.lights[d.light].shape = int32_t(si);
     }
+    if (d.type == RAD_SHAPE_CUBE && d.bsdf >= 0 && out.bsdfs[d.bsdf].tex_any)
+      throw std::runtime_error("shape " + std::to_string(si) +
+                               ": textured BSDF on a cube (the reference never initialises cube texture coordinates, "
+                               "cube.cpp:115, 136)");
     M4 tw = M4::From(d.to_world);
     if (d.type == RAD_SHAPE_MESH || d.type == RAD_SHAPE_CUBE) {
       s.type = SHAPE_MESH;
@@ -526,6 +661,10 @@ CameraRec MakeCameraRec(const rad_camera_desc& c) {
   M4 inv = toClip.Inverse();
   std::memcpy(cam.clip_to_camera.m, inv.m, sizeof(inv.m));
   cam.camera_to_world = M4::From(c.to_world).ToAffine();
+  // perspective.cpp:55-58: one pixel step on the near plane, in camera space
+  const f3 c0 = cam.clip_to_camera.point_h(mk(0.f, 0.f, 0.f));
+  cam.dx = cam.clip_to_camera.point_h(mk(1.0f / float(c.width), 0.f, 0.f)) - c0;
+  cam.dy = cam.clip_to_camera.point_h(mk(0.f, 1.0f / float(c.height), 0.f)) - c0;
   return cam;
 }
 
@@ -533,5 +672,5 @@ CameraRec MakeCameraRec(const rad_camera_desc& c) {
 
 uint64_t rad_scene::device_bytes() const {
   return d_nodes.bytes + d_qnodes.bytes + d_prims.bytes + d_shapes.bytes + d_pos.bytes + d_nrm.bytes + d_uv.bytes + d_idx.bytes +
-         d_bsdfs.bytes + d_lights.bytes + d_cdf.bytes + d_media.bytes + d_grid.bytes;
+         d_bsdfs.bytes + d_lights.bytes + d_cdf.bytes + d_media.bytes + d_grid.bytes + d_tex.bytes + d_texels.bytes;
 }

@@ -31,6 +31,8 @@ struct HostScene {
   std::vector<float> light_cdf;
   std::vector<MediumRec> media;
   std::vector<float> grid;
+  std::vector<TexRec> textures;
+  std::vector<float> texels;  // every bitmap's mip chain
   int32_t env_light = -1;
   int32_t global_medium = -1;
   CameraRec cam{};
@@ -46,6 +48,9 @@ struct HostScene {
 void AssembleScene(const rad_scene_desc& desc, HostScene& out);
 
 CameraRec MakeCameraRec(const rad_camera_desc& c);
+
+// Texture records and the texel pool (each bitmap's mip chain) of a texture array.
+void BuildTextureSet(const rad_texture_desc* descs, uint32_t n, std::vector<TexRec>& textures, std::vector<float>& texels);
 
 // Reference-default BSDF record from the desc (eta derivations of disney.cpp:86-101 etc.).
 BsdfRec MakeBsdfRec(const rad_bsdf_desc& d);
@@ -79,7 +84,8 @@ struct rad_scene {
   rad_ctx* ctx = nullptr;
   rad::HostScene host;
   rad::DevScene dev{};
-  rad::DeviceBuffer d_nodes, d_qnodes, d_prims, d_shapes, d_pos, d_nrm, d_uv, d_idx, d_bsdfs, d_lights, d_cdf, d_media, d_grid;
+  rad::DeviceBuffer d_nodes, d_qnodes, d_prims, d_shapes, d_pos, d_nrm, d_uv, d_idx, d_bsdfs, d_lights, d_cdf, d_media, d_grid,
+      d_tex, d_texels;
   rad::DeviceBuffer d_rays, d_hits, d_occ;  // staging for the host-pointer intersect entry points
   std::mutex mu;
   bool bvh_ready = false;

@@ -5,6 +5,7 @@
 
 #include "medium_dev.h"
 #include "scene_host.h"
+#include "texture_dev.h"
 #include "wavefront.h"
 
 namespace rad {
@@ -177,7 +178,16 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
               next_depth = depth;
               alive = true;
             } else {
-              const BsdfRec& b = scene.bsdfs[sr.bsdf];
+              const BsdfRec* all = scene.bsdfs;
+              const BsdfRec* bp0 = &scene.bsdfs[sr.bsdf];
+              BsdfRec chain[3];
+              if (bp0->tex_any) {  // vol_path.cpp:28 drops the camera ray's differentials
+                const TexCoord tc = hit_texcoord(scene, si, hh, geom, false, 0.f, 0.f);
+                resolve_chain(scene, *bp0, tc, chain);
+                all = chain;
+                bp0 = &chain[0];
+              }
+              const BsdfRec& b = *bp0;
               if (b.lobes & LOBE_NO_DELTA) {  // NEE through media (vol_path.cpp:178-190)
                 float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
                 uint32_t sdim = dim;
@@ -186,7 +196,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
                 if (ls.pdf > 0.f) {
                   f3 wo = si.sh.to_local(ls.dir);
                   float bp;
-                  f3 f = bsdf_eval_w(scene.bsdfs, b, quirks, si.wi, wo, bp);
+                  f3 f = bsdf_eval_w(all, b, quirks, si.wi, wo, bp);
                   float wgt = ls.delta ? 1.f : mis_weight(ls.pdf, bp);
                   f3 c = mul(mul(T, f), ls.li) * wgt / ls.pdf;
                   if (!is_zero(c)) {
@@ -201,7 +211,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
               float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
               dim += 3;
               BsdfSample bs;
-              f3 f = bsdf_sample_w(scene.bsdfs, b, quirks, si.wi, ul, u1, u2, bs);
+              f3 f = bsdf_sample_w(all, b, quirks, si.wi, ul, u1, u2, bs);
               if (bs.pdf > 0.f) {
                 T = mul(T, f) / bs.pdf;
                 eta *= bs.eta;

@@ -192,6 +192,8 @@ void LaunchClosest(const DevScene& scene, const void* rays, uint32_t n, void* hi
 void LaunchAny(const DevScene& scene, const void* rays, uint32_t n, uint8_t* occ, hipStream_t s);
 
 // Per-function probes.
+void LaunchTexEval(const DevScene& s, uint32_t id, const float* in, uint32_t n, float* out, hipStream_t st);
+void LaunchFootprint(const DevScene& s, const float* in, uint32_t n, float* out, hipStream_t st);
 void LaunchBsdfEval(const BsdfRec& b, bool quirks, const float* in, uint32_t n, float* out, hipStream_t s);
 void LaunchBsdfSample(const BsdfRec& b, bool quirks, const float* in, uint32_t n, float* out, hipStream_t s);
 

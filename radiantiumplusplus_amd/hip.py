@@ -59,6 +59,8 @@ def lib():
                 "rad_hip_bsdf_eval": (i32, [C.POINTER(F.BsdfDesc), u32, vp, u32, vp]),
                 "rad_hip_bsdf_sample": (i32, [C.POINTER(F.BsdfDesc), u32, vp, u32, vp]),
                 "rad_hip_grid_eval": (i32, [C.POINTER(F.MediumDesc), vp, u32, vp]),
+                "rad_hip_texture_eval": (i32, [C.POINTER(F.TextureDesc), u32, u32, vp, u32, vp]),
+                "rad_hip_camera_footprint": (i32, [vp, vp, u32, vp]),
                 "rad_scene_load_json": (i32, [C.c_char_p, C.POINTER(vp)]),
                 "rad_loaded_scene_desc": (C.POINTER(F.SceneDesc), [vp]),
                 "rad_loaded_scene_render": (C.POINTER(F.RenderDesc), [vp]),
@@ -146,6 +148,13 @@ class Scene:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         out = np.zeros(len(rays), np.uint8)
         check(lib().rad_hip_intersect_any(self.h, rays.ctypes.data, len(rays), out.ctypes.data))
+        return out
+
+    def camera_footprint(self, screen):
+        """uv and d(uv)/dx, d(uv)/dy at the first hit of camera rays through screen positions (n x 2)."""
+        pts = np.ascontiguousarray(screen, np.float32)
+        out = np.zeros((len(pts), 6), np.float32)
+        check(lib().rad_hip_camera_footprint(self.h, pts.ctypes.data, len(pts), out.ctypes.data))
         return out
 
     def autotune(self, probe: F.RenderDesc):
@@ -250,6 +259,14 @@ def bsdf_sample(desc, wi, lobe, xi, quirks=True):
     inp = np.ascontiguousarray(np.concatenate([wi, lobe.reshape(-1, 1), xi], 1), np.float32)
     out = np.zeros((n, 8), np.float32)
     check(lib().rad_hip_bsdf_sample(C.byref(desc), 1 if quirks else 0, inp.ctypes.data, n, out.ctypes.data))
+    return out
+
+
+def texture_eval(textures, index, lookups):
+    """textures: ctypes array of TextureDesc; lookups: n x {uv[2], duvdx[2], duvdy[2]} -> n x 3."""
+    q = np.ascontiguousarray(lookups, np.float32)
+    out = np.zeros((len(q), 3), np.float32)
+    check(lib().rad_hip_texture_eval(textures, len(textures), index, q.ctypes.data, len(q), out.ctypes.data))
     return out
 
 

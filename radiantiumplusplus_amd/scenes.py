@@ -186,9 +186,26 @@ def perlin3d(res, octaves, seed):
 
 # --------------------------------------------------------------------------- builder
 def _f3(v):
+    if isinstance(v, dict):  # a texture (SceneBuilder.bitmap / chessboard)
+        return v
     if np.isscalar(v):
         return [float(v)] * 3
     return [float(a) for a in v]
+
+
+def _f1(v):
+    return v if isinstance(v, dict) else float(v)
+
+
+_FILTERS = {"nearest": 0, "bilinear": 1, "trilinear": 2, "anisotropic": 3}
+# BSDF parameter -> (rad_bsdf_desc.tex slot, colour?)
+_TEX_SLOTS = {
+    "reflectance": (0, True), "transmittance": (1, True), "eta": (2, True), "k": (3, True),
+    "alpha_u": (4, False), "alpha_v": (5, False), "base_color": (6, True), "roughness": (7, False),
+    "anisotropic": (8, False), "spec_trans": (9, False), "sheen": (10, False), "sheen_tint": (11, False),
+    "flatness": (12, False), "spec_tint": (13, False), "metallic": (14, False), "clearcoat": (15, False),
+    "clearcoat_gloss": (16, False), "diffuse": (17, True), "specular": (18, True), "opacity": (19, False),
+}
 
 
 @dataclass
@@ -226,14 +243,14 @@ class SceneBuilder:
 
     def rough_metal(self, alpha=0.1, reflectance=1.0, eta=(0.1999899, 0.9220823, 1.0998751),
                     k=(3.9046354, 2.4476316, 2.1376500), distribution="ggx", sample_visible=True):
-        self.bsdfs.append({"type": "rough_metal", "alpha": float(alpha), "reflectance": _f3(reflectance),
+        self.bsdfs.append({"type": "rough_metal", "alpha": _f1(alpha), "reflectance": _f3(reflectance),
                            "eta": _f3(eta), "k": _f3(k), "distribution": distribution,
                            "sample_visible": bool(sample_visible)})
         return len(self.bsdfs) - 1
 
     def rough_glass(self, alpha=0.1, int_ior=1.5046, ext_ior=1.000277, reflectance=1.0, transmittance=1.0,
                     distribution="ggx", sample_visible=True):
-        self.bsdfs.append({"type": "rough_glass", "alpha": float(alpha), "int_ior": float(int_ior),
+        self.bsdfs.append({"type": "rough_glass", "alpha": _f1(alpha), "int_ior": float(int_ior),
                            "ext_ior": float(ext_ior), "reflectance": _f3(reflectance),
                            "transmittance": _f3(transmittance), "distribution": distribution,
                            "sample_visible": bool(sample_visible)})
@@ -242,7 +259,7 @@ class SceneBuilder:
     def disney(self, **params):
         d = {"type": "disney"}
         for k, v in params.items():
-            d[k] = _f3(v) if k == "base_color" else float(v)
+            d[k] = _f3(v) if k == "base_color" else _f1(v)
         self.bsdfs.append(d)
         return len(self.bsdfs) - 1
 
@@ -253,7 +270,7 @@ class SceneBuilder:
 
     def rough_plastic(self, alpha=0.1, diffuse=0.5, specular=1.0, int_ior=1.49, ext_ior=1.000277, nonlinear=False,
                       distribution="ggx", sample_visible=True):
-        self.bsdfs.append({"type": "rough_plastic", "alpha": float(alpha), "diffuse": _f3(diffuse),
+        self.bsdfs.append({"type": "rough_plastic", "alpha": _f1(alpha), "diffuse": _f3(diffuse),
                            "specular": _f3(specular), "int_ior": float(int_ior), "ext_ior": float(ext_ior),
                            "no_linear": bool(nonlinear), "distribution": distribution,
                            "sample_visible": bool(sample_visible)})
@@ -274,8 +291,59 @@ class SceneBuilder:
         return len(self.bsdfs) - 1
 
     def mask(self, inner, opacity=0.5):
-        self.bsdfs.append({"type": "mask", "opacity": float(opacity), "nested": dict(self.bsdfs[inner])})
+        self.bsdfs.append({"type": "mask", "opacity": _f1(opacity), "nested": dict(self.bsdfs[inner])})
         return len(self.bsdfs) - 1
+
+    # -- textures --------------------------------------------------------
+    @staticmethod
+    def bitmap(texels, filter="bilinear", wrap="clamp", max_level=None, anisotropic_level=None, name=None):
+        """A `bitmap` texture over linear texels (H x W x 3 colour, or H x W mono),
+        texel row 0 being the image's bottom row (the reference's flipped load)."""
+        t = np.ascontiguousarray(texels, np.float32)
+        if t.ndim == 3 and t.shape[2] == 1:
+            t = t[:, :, 0]
+        return {"type": "bitmap", "_texels": t, "filter": filter, "wrap": wrap, "max_level": max_level,
+                "anisotropic_level": anisotropic_level, "asset_name": name}
+
+    @staticmethod
+    def chessboard(color_a=0.4, color_b=0.2, to_uv=None, remap=False):
+        """A `chessboard` texture; color_a / color_b are constants or bitmaps."""
+        return {"type": "chessboard", "color_a": color_a, "color_b": color_b,
+                "to_uv": None if to_uv is None else np.asarray(to_uv, np.float32), "remap": bool(remap)}
+
+    def _texture(self, tex, colour, out):
+        """Append the TextureDesc of `tex` (and of the bitmaps it nests) to out; 1-based index."""
+        td = F.TextureDesc()
+        td.channels = 3 if colour else 1
+        td.to_uv[:] = [float(x) for x in IDENTITY.reshape(-1)]
+        if tex["type"] == "bitmap":
+            t = tex["_texels"]
+            if colour != (t.ndim == 3):
+                raise ValueError("bitmap channel count does not match the parameter")
+            if colour and t.shape[2] != 3:
+                raise ValueError("colour bitmaps need 3 channels")
+            td.type = F.RAD_TEXTURE_BITMAP
+            td.height, td.width = t.shape[0], t.shape[1]
+            td.texels = self._ptr(t, C.c_float)
+            td.filter = _FILTERS[tex["filter"]]
+            td.wrap = F.RAD_WRAP_REPEAT if tex["wrap"] == "repeat" else F.RAD_WRAP_CLAMP
+            mip = tex["filter"] in ("trilinear", "anisotropic")
+            td.max_level = tex["max_level"] if tex["max_level"] is not None else (-1 if mip else 0)
+            aniso = tex["anisotropic_level"]
+            td.anisotropic_level = aniso if aniso is not None else (8.0 if tex["filter"] == "anisotropic" else 0.0)
+        else:
+            td.type = F.RAD_TEXTURE_CHESSBOARD
+            if tex["to_uv"] is not None:
+                td.to_uv[:] = [float(x) for x in tex["to_uv"].reshape(-1)]
+            td.remap = 1 if tex["remap"] else 0
+            for key, slot, cfield in (("color_a", "tex_a", "color_a"), ("color_b", "tex_b", "color_b")):
+                v = tex[key]
+                if isinstance(v, dict):
+                    setattr(td, slot, self._texture(v, colour, out))
+                else:
+                    getattr(td, cfield)[:] = _f3(v) if colour else [float(v), 0.0, 0.0]
+        out.append(td)
+        return len(out)
 
     # -- media -----------------------------------------------------------
     def homogeneous(self, sigma_t=1.0, albedo=0.75, scale=1.0, g=None):
@@ -379,7 +447,34 @@ class SceneBuilder:
         self._keep.append(arr)
         return arr.ctypes.data_as(C.POINTER(typ))
 
-    def bsdf_desc(self, d):
+    def bsdf_desc(self, d, textures=None):
+        """BsdfDesc of one BSDF dict; textured parameters are appended to
+        `textures` (a list of TextureDesc) and referenced by index."""
+        textured = {}
+        consts = {}
+        for key, v in d.items():
+            if isinstance(v, dict) and v.get("type") in ("bitmap", "chessboard"):
+                textured[key] = v
+            else:
+                consts[key] = v
+        b = self._bsdf_consts(consts)
+        for key, tex in textured.items():
+            if textures is None:
+                raise ValueError("textured parameters need a texture list")
+            keys = ["alpha_u", "alpha_v"] if key == "alpha" else [key]
+            idx = None
+            for k2 in keys:
+                slot, colour = _TEX_SLOTS[k2]
+                if idx is None:
+                    idx = self._texture(tex, colour, textures)
+                b.tex[slot] = idx
+        if d["type"] == "disney":  # a textured parameter counts as given (ConfigHasValue)
+            for key in textured:
+                if key in F.DISNEY_FLAGS:
+                    b.disney_flags |= F.DISNEY_FLAGS[key]
+        return b
+
+    def _bsdf_consts(self, d):
         b = F.BsdfDesc()
         t = d["type"]
         b.sample_visible = 1
@@ -458,15 +553,18 @@ class SceneBuilder:
             raise ValueError(f"unsupported bsdf {t}")
         return b
 
-    def bsdf_array(self):
+    def bsdf_array(self, textures=None):
         """The BSDF descriptors, the wrappers' nested BSDFs appended after the
-        builder's own entries (shape indices stay valid)."""
-        out = [self.bsdf_desc(d) for d in self.bsdfs]
+        builder's own entries (shape indices stay valid); textures collects
+        the TextureDesc of textured parameters."""
+        if textures is None:
+            textures = []
+        out = [self.bsdf_desc(d, textures) for d in self.bsdfs]
 
         def nest(i, d):
             inner = d.get("two_side") if d["type"] == "two_side" else d.get("nested")
             j = len(out)
-            out.append(self.bsdf_desc(inner))
+            out.append(self.bsdf_desc(inner, textures))
             out[i].nested = j
             if inner["type"] in ("two_side", "mask"):
                 nest(j, inner)
@@ -501,7 +599,11 @@ class SceneBuilder:
                 d.indices = self._ptr(idx, C.c_uint32)
             d.center[:] = s.center
             d.radius = s.radius
-        bsdfs, bsdf_count = self.bsdf_array()
+        tex_list = []
+        bsdfs, bsdf_count = self.bsdf_array(tex_list)
+        textures = (F.TextureDesc * max(len(tex_list), 1))()
+        for i, td in enumerate(tex_list):
+            textures[i] = td
         lights = (F.LightDesc * max(len(self.lights), 1))()
         for i, l in enumerate(self.lights):
             lights[i].type = {"area": F.RAD_LIGHT_AREA, "skybox": F.RAD_LIGHT_SKYBOX, "point": F.RAD_LIGHT_POINT,
@@ -544,7 +646,9 @@ class SceneBuilder:
         desc.camera.far_clip = cam["far"]
         desc.camera.width = cam["width"]
         desc.camera.height = cam["height"]
-        self._keep += [shapes, bsdfs, lights, media]
+        desc.texture_count = len(tex_list)
+        desc.textures = textures
+        self._keep += [shapes, bsdfs, lights, media, textures]
         return desc
 
     def render_desc(self, spp=None, seed=None, flags=0, path_slots=0, guiding=0, request_id=0):

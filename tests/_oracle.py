@@ -55,6 +55,9 @@ def lib():
         L.ref_bsdf_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_bsdf_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_grid_eval.argtypes = [C.POINTER(F.MediumDesc), C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ref_texture_eval.argtypes = [C.POINTER(F.TextureDesc), C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                       C.c_void_p]
+        L.ref_camera_footprint.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_erfinv.restype = C.c_float
         L.ref_erfinv.argtypes = [C.c_float]
         L.ref_medium_transmittance.restype = C.c_float
@@ -106,6 +109,12 @@ class OracleScene:
         r = np.zeros(1, RAY_DTYPE)
         lib().ref_camera_ray(self.h, sx, sy, r.ctypes.data)
         return r[0]
+
+    def camera_footprint(self, screen):
+        pts = np.ascontiguousarray(screen, np.float32)
+        out = np.zeros((len(pts), 6), np.float32)
+        lib().ref_camera_footprint(self.h, pts.ctypes.data, len(pts), out.ctypes.data)
+        return out
 
     def render(self, spp=None, seed=None, threads=0, window=None, want_sumsq=True, max_depth=None, rr_depth=None,
                tile_step=0):
@@ -160,4 +169,13 @@ def grid_eval(medium_desc, pts):
     pts = np.ascontiguousarray(pts, np.float32)
     out = np.zeros((len(pts), 3), np.float32)
     lib().ref_grid_eval(C.byref(medium_desc), pts.ctypes.data, len(pts), out.ctypes.data)
+    return out
+
+
+def texture_eval(textures, index, lookups):
+    """textures: ctypes array of TextureDesc; lookups: n x {uv, duvdx, duvdy} -> n x 3."""
+    q = np.ascontiguousarray(lookups, np.float32)
+    out = np.zeros((len(q), 3), np.float32)
+    if lib().ref_texture_eval(textures, len(textures), index, q.ctypes.data, len(q), out.ctypes.data) != 0:
+        raise RuntimeError(lib().ref_last_error().decode())
     return out

