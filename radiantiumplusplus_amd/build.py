@@ -48,6 +48,7 @@ SOURCES = [
     "server.cpp",
     "host/json.cpp",
     "host/loader.cpp",
+    "host/image.cpp",
     "host/exr.cpp",
 ]
 CLI_SOURCES = ["host/cli.cpp", "host/renderer.cpp"]
@@ -101,7 +102,7 @@ def build(force=True, jobs=None, verbose=False):
             if verbose and log.strip():
                 print(log, file=sys.stderr)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stderr[-4000:]}")

@@ -3,6 +3,7 @@
 // rad_scene_desc, with the asset readers it relies on: Wavefront OBJ
 // (module/rad.core/src/wavefront_obj_reader.cpp:37-463) and mitsuba VOL v3
 // (module/rad.core/src/volume_reader.cpp:65-124).
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "../../../include/rad_hip.h"
+#include "image.h"
 #include "json.h"
 
 namespace rad {
@@ -379,6 +381,8 @@ struct rad_loaded_scene {
   std::vector<rad_medium_desc> media;
   std::deque<Model> models;
   std::deque<Grid> grids;
+  std::deque<std::vector<float>> images;  // linear texels of the image assets
+  std::vector<rad_texture_desc> textures;
 };
 
 namespace {
@@ -389,7 +393,14 @@ struct Builder {
   std::map<std::string, const ObjFile*> objAssets;
   std::map<std::string, std::unique_ptr<ObjFile>> objStore;
   std::map<std::string, const Grid*> volAssets;
+  struct ImageAsset {
+    const std::vector<float>* texels;
+    uint32_t width, height;
+    int channels;
+  };
+  std::map<std::string, ImageAsset> imageAssets;
   std::map<std::string, int> bsdfVars;
+  rad_bsdf_desc* cur = nullptr;  // BSDF being built (its textured slots)
 
   std::string Resolve(const std::string& loc) const {
     if (!loc.empty() && loc[0] == '/') return loc;
@@ -408,36 +419,136 @@ struct Builder {
       } else if (type == "volume_data_mitsuba_vol") {
         out.grids.push_back(ReadMitsubaVol(Resolve(loc)));
         volAssets[name] = &out.grids.back();
-      } else if (type == "image_default" || type == "image_exr" || type == "image_hdr" || type == "volume_data_vdb") {
-        // images only feed bitmap textures, which are rejected where used
+      } else if (type == "image_default") {  // asset/image_stb.cpp:12-100
+        const int channel = IntOr(a, "channel", 3);
+        const bool flip = BoolOr(a, "is_flip_y", true);
+        const bool toLinear = BoolOr(a, "is_to_linear", channel == 3);
+        rad::host::Image8 img;
+        try {
+          img = rad::host::ReadImage8(Resolve(loc), channel, flip);
+        } catch (const std::runtime_error& e) {
+          throw LoadError(RAD_ERR_IO, e.what());
+        }
+        std::vector<float> t(img.px.size());
+        for (size_t i = 0; i < t.size(); i++) {
+          float v = float(img.px[i]) / 255.0f;
+          if (toLinear && channel == 3)  // Color24f::ToLinear (color.h:41-46)
+            v = v <= 0.04045f ? v * (1.0f / 12.92f) : std::pow((v + 0.055f) * (1.0f / 1.055f), 2.4f);
+          t[i] = v;
+        }
+        out.images.push_back(std::move(t));
+        imageAssets[name] = ImageAsset{&out.images.back(), img.width, img.height, channel};
+      } else if (type == "image_hdr") {  // asset/image_hdr.cpp:10-50
+        rad::host::ImageF img;
+        try {
+          img = rad::host::ReadHdr(Resolve(loc), BoolOr(a, "is_flip_y", false));
+        } catch (const std::runtime_error& e) {
+          throw LoadError(RAD_ERR_IO, e.what());
+        }
+        out.images.push_back(std::move(img.px));
+        imageAssets[name] = ImageAsset{&out.images.back(), img.width, img.height, 3};
+      } else if (type == "image_exr" || type == "volume_data_vdb") {
+        // readers this build does not carry: rejected where a texture or medium uses them
       } else {
         throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + type);
       }
     }
   }
 
-  // ConfigNodeReadTexture for a constant: number or [r,g,b]
-  void Color(const Value& node, const char* key, float* out3, float def) {
+  // A texture node (texture/bitmap.cpp:264-317, texture/chessboard.cpp:17-31);
+  // returns 1 + its index in out.textures.
+  int32_t MakeTexture(const Value& n, int channels) {
+    std::string type = Str(n, "type");
+    rad_texture_desc t{};
+    t.channels = channels;
+    const float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    std::memcpy(t.to_uv, ident, sizeof(ident));
+    if (type == "bitmap") {
+      std::string asset = Str(n, "asset_name");
+      auto it = imageAssets.find(asset);
+      if (it == imageAssets.end()) throw LoadError(RAD_ERR_INVALID_ARG, "unknown image asset: " + asset);
+      if (it->second.channels != channels)
+        throw LoadError(RAD_ERR_INVALID_ARG, "loaded image " + asset + (channels == 3 ? " is mono" : " is rgb"));
+      t.type = RAD_TEXTURE_BITMAP;
+      t.width = it->second.width;
+      t.height = it->second.height;
+      t.texels = it->second.texels->data();
+      std::string filter = StrOr(n, "filter", "bilinear");
+      t.filter = filter == "nearest" ? RAD_FILTER_NEAREST
+                 : filter == "trilinear" ? RAD_FILTER_TRILINEAR
+                 : filter == "anisotropic" ? RAD_FILTER_ANISOTROPIC
+                                           : RAD_FILTER_BILINEAR;  // unknown names fall back to bilinear
+      t.wrap = StrOr(n, "wrap", "clamp") == "repeat" ? RAD_WRAP_REPEAT : RAD_WRAP_CLAMP;
+      const bool mip = t.filter == RAD_FILTER_TRILINEAR || t.filter == RAD_FILTER_ANISOTROPIC;
+      t.max_level = IntOr(n, "max_level", mip ? -1 : 0);
+      if (!mip && t.max_level < 0) t.max_level = 0;
+      t.anisotropic_level = FloatOr(n, "anisotropic_level", t.filter == RAD_FILTER_ANISOTROPIC ? 8.f : 0.f);
+    } else if (type == "chessboard") {
+      t.type = RAD_TEXTURE_CHESSBOARD;
+      if (const Value* tu = n.find("to_uv")) {
+        M4 m = AsTransform(*tu);
+        std::memcpy(t.to_uv, m.m, sizeof(t.to_uv));
+      }
+      t.remap = BoolOr(n, "remap", false) ? 1 : 0;
+      auto side = [&](const char* key, float def, float* color, int32_t& tex) {
+        const Value* v = n.find(key);
+        color[0] = color[1] = color[2] = def;
+        if (!v) return;
+        if (v->is_object()) {
+          if (Str(*v, "type") != "bitmap")
+            throw LoadError(RAD_ERR_UNSUPPORTED, std::string("chessboard ") + key + ": only constants and bitmaps nest");
+          tex = MakeTexture(*v, channels);
+        } else if (channels == 3) {
+          Vec<3>(*v, color, key);
+        } else {
+          color[0] = Num(*v, key);
+        }
+      };
+      side("color_a", 0.4f, t.color_a, t.tex_a);
+      side("color_b", 0.2f, t.color_b, t.tex_b);
+    } else {
+      throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + type);
+    }
+    out.textures.push_back(t);
+    return int32_t(out.textures.size());
+  }
+
+  // ConfigNodeReadTexture (config_node_ext.h:17-45): a number / [r,g,b]
+  // constant, or a texture object bound to the BSDF's slot.
+  void Color(const Value& node, const char* key, float* out3, float def, int slot) {
     const Value* v = node.find(key);
-    if (!v) {
-      out3[0] = out3[1] = out3[2] = def;
+    out3[0] = out3[1] = out3[2] = def;
+    if (!v) return;
+    if (v->is_object()) {
+      cur->tex[slot] = MakeTexture(*v, 3);
       return;
     }
-    if (v->is_object())
-      throw LoadError(RAD_ERR_UNSUPPORTED, std::string("textured '") + key + "' is not supported on the GPU path yet");
     Vec<3>(*v, out3, key);
   }
-  float Mono(const Value& node, const char* key, float def) {
+  // Light colours: textured emitters (area.cpp, skybox.cpp, projection.cpp
+  // image textures) are not carried by this build.
+  void LightColor(const Value& node, const char* key, float* out3, float def) {
+    const Value* v = node.find(key);
+    out3[0] = out3[1] = out3[2] = def;
+    if (!v) return;
+    if (v->is_object()) throw LoadError(RAD_ERR_UNSUPPORTED, std::string("textured light '") + key + "' is not supported");
+    Vec<3>(*v, out3, key);
+  }
+  float Mono(const Value& node, const char* key, float def, int slot) {
     const Value* v = node.find(key);
     if (!v) return def;
-    if (v->is_object())
-      throw LoadError(RAD_ERR_UNSUPPORTED, std::string("textured '") + key + "' is not supported on the GPU path yet");
+    if (v->is_object()) {
+      cur->tex[slot] = MakeTexture(*v, 1);
+      return def;
+    }
     return Num(*v, key);
   }
 
   int MakeBsdf(const Value& n) {
     std::string type = Str(n, "type");
     rad_bsdf_desc b{};
+    rad_bsdf_desc* outer = cur;
+    cur = &b;
     b.sample_visible = 1;
     b.alpha_u = b.alpha_v = 0.1f;
     b.int_ior = 1.5046f;
@@ -450,48 +561,54 @@ struct Builder {
       b.distribution = d == "beckmann" ? RAD_MICROFACET_BECKMANN : RAD_MICROFACET_GGX;
       b.sample_visible = BoolOr(n, "sample_visible", true) ? 1 : 0;
       if (n.find("alpha")) {
-        b.alpha_u = b.alpha_v = Mono(n, "alpha", 0.1f);
+        b.alpha_u = b.alpha_v = Mono(n, "alpha", 0.1f, RAD_TEX_ALPHA_U);
+        b.tex[RAD_TEX_ALPHA_V] = b.tex[RAD_TEX_ALPHA_U];
       } else {
-        b.alpha_u = Mono(n, "alpha_u", 0.1f);
-        b.alpha_v = Mono(n, "alpha_v", 0.1f);
+        b.alpha_u = Mono(n, "alpha_u", 0.1f, RAD_TEX_ALPHA_U);
+        b.alpha_v = Mono(n, "alpha_v", 0.1f, RAD_TEX_ALPHA_V);
       }
     };
     if (type == "diffuse") {  // diffuse.cpp:40-44
       b.type = RAD_BSDF_DIFFUSE;
-      Color(n, "reflectance", b.reflectance, 0.5f);
+      Color(n, "reflectance", b.reflectance, 0.5f, RAD_TEX_REFLECTANCE);
     } else if (type == "rough_metal") {  // rough_metal.cpp:22-44
       b.type = RAD_BSDF_ROUGH_METAL;
-      Color(n, "reflectance", b.reflectance, 1.f);
+      Color(n, "reflectance", b.reflectance, 1.f, RAD_TEX_REFLECTANCE);
       const float eta[3] = {0.1999899f, 0.9220823f, 1.0998751f}, k[3] = {3.9046354f, 2.4476316f, 2.1376500f};
-      if (n.find("eta")) Color(n, "eta", b.eta, 0); else std::memcpy(b.eta, eta, sizeof(eta));
-      if (n.find("k")) Color(n, "k", b.k, 0); else std::memcpy(b.k, k, sizeof(k));
+      if (n.find("eta")) Color(n, "eta", b.eta, 0, RAD_TEX_ETA); else std::memcpy(b.eta, eta, sizeof(eta));
+      if (n.find("k")) Color(n, "k", b.k, 0, RAD_TEX_K); else std::memcpy(b.k, k, sizeof(k));
       dist();
     } else if (type == "rough_glass") {  // rough_glass.cpp:22-48
       b.type = RAD_BSDF_ROUGH_GLASS;
-      Color(n, "reflectance", b.reflectance, 1.f);
-      Color(n, "transmittance", b.transmittance, 1.f);
+      Color(n, "reflectance", b.reflectance, 1.f, RAD_TEX_REFLECTANCE);
+      Color(n, "transmittance", b.transmittance, 1.f, RAD_TEX_TRANSMITTANCE);
       b.int_ior = FloatOr(n, "int_ior", 1.5046f);
       b.ext_ior = FloatOr(n, "ext_ior", 1.000277f);
       dist();
     } else if (type == "disney") {  // disney.cpp:62-110
       b.type = RAD_BSDF_DISNEY;
-      Color(n, "base_color", b.base_color, 0.5f);
-      b.roughness = Mono(n, "roughness", 0.5f);
+      Color(n, "base_color", b.base_color, 0.5f, RAD_TEX_BASE_COLOR);
+      b.roughness = Mono(n, "roughness", 0.5f, RAD_TEX_ROUGHNESS);
       struct F {
         const char* key;
         uint32_t bit;
         float* dst;
-      } fl[] = {{"anisotropic", RAD_DISNEY_HAS_ANISOTROPIC, &b.anisotropic}, {"spec_trans", RAD_DISNEY_HAS_SPEC_TRANS, &b.spec_trans},
-                {"sheen", RAD_DISNEY_HAS_SHEEN, &b.sheen},                   {"sheen_tint", RAD_DISNEY_HAS_SHEEN_TINT, &b.sheen_tint},
-                {"flatness", RAD_DISNEY_HAS_FLATNESS, &b.flatness},          {"spec_tint", RAD_DISNEY_HAS_SPEC_TINT, &b.spec_tint},
-                {"metallic", RAD_DISNEY_HAS_METALLIC, &b.metallic},          {"clearcoat", RAD_DISNEY_HAS_CLEARCOAT, &b.clearcoat}};
+        int slot;
+      } fl[] = {{"anisotropic", RAD_DISNEY_HAS_ANISOTROPIC, &b.anisotropic, RAD_TEX_ANISOTROPIC},
+                {"spec_trans", RAD_DISNEY_HAS_SPEC_TRANS, &b.spec_trans, RAD_TEX_SPEC_TRANS},
+                {"sheen", RAD_DISNEY_HAS_SHEEN, &b.sheen, RAD_TEX_SHEEN},
+                {"sheen_tint", RAD_DISNEY_HAS_SHEEN_TINT, &b.sheen_tint, RAD_TEX_SHEEN_TINT},
+                {"flatness", RAD_DISNEY_HAS_FLATNESS, &b.flatness, RAD_TEX_FLATNESS},
+                {"spec_tint", RAD_DISNEY_HAS_SPEC_TINT, &b.spec_tint, RAD_TEX_SPEC_TINT},
+                {"metallic", RAD_DISNEY_HAS_METALLIC, &b.metallic, RAD_TEX_METALLIC},
+                {"clearcoat", RAD_DISNEY_HAS_CLEARCOAT, &b.clearcoat, RAD_TEX_CLEARCOAT}};
       for (auto& e : fl) {
         const Value* v = n.find(e.key);
         // ConfigHasValue: present and (not a number, or a non-zero number)
         if (v && (!v->is_number() || v->num != 0)) b.disney_flags |= e.bit;
-        *e.dst = Mono(n, e.key, 0.f);
+        *e.dst = Mono(n, e.key, 0.f, e.slot);
       }
-      b.clearcoat_gloss = Mono(n, "clearcoat_gloss", 0.f);
+      b.clearcoat_gloss = Mono(n, "clearcoat_gloss", 0.f, RAD_TEX_CLEARCOAT_GLOSS);
       b.specular_sampling_rate = FloatOr(n, "main_specular_sampling_rate", 1.f);
       b.clearcoat_sampling_rate = FloatOr(n, "clearcoat_sampling_rate", 1.f);
       b.diffuse_sampling_rate = FloatOr(n, "diffuse_reflectance_sampling_rate", 1.f);
@@ -506,24 +623,25 @@ struct Builder {
       b.type = type == "plastic" ? RAD_BSDF_PLASTIC : RAD_BSDF_ROUGH_PLASTIC;
       b.int_ior = FloatOr(n, "int_ior", 1.49f);
       b.ext_ior = FloatOr(n, "ext_ior", 1.000277f);
-      Color(n, "diffuse", b.diffuse, 0.5f);
-      Color(n, "specular", b.specular_reflectance, 1.f);
+      Color(n, "diffuse", b.diffuse, 0.5f, RAD_TEX_DIFFUSE);
+      Color(n, "specular", b.specular_reflectance, 1.f, RAD_TEX_SPECULAR);
       b.nonlinear = BoolOr(n, "no_linear", false) ? 1 : 0;
       if (b.type == RAD_BSDF_ROUGH_PLASTIC) {
         std::string d = StrOr(n, "distribution", "ggx");
         b.distribution = d == "beckmann" ? RAD_MICROFACET_BECKMANN : RAD_MICROFACET_GGX;
         b.sample_visible = BoolOr(n, "sample_visible", true) ? 1 : 0;
-        b.alpha_u = b.alpha_v = Mono(n, "alpha", 0.1f);
+        b.alpha_u = b.alpha_v = Mono(n, "alpha", 0.1f, RAD_TEX_ALPHA_U);
+        b.tex[RAD_TEX_ALPHA_V] = b.tex[RAD_TEX_ALPHA_U];
       }
     } else if (type == "perfect_mirror") {  // perfect_mirror.cpp:16-22
       b.type = RAD_BSDF_PERFECT_MIRROR;
-      Color(n, "reflectance", b.reflectance, 1.f);
-      Color(n, "eta", b.eta, 0.f);
-      Color(n, "k", b.k, 1.f);
+      Color(n, "reflectance", b.reflectance, 1.f, RAD_TEX_REFLECTANCE);
+      Color(n, "eta", b.eta, 0.f, RAD_TEX_ETA);
+      Color(n, "k", b.k, 1.f, RAD_TEX_K);
     } else if (type == "perfect_glass") {  // perfect_glass.cpp:16-23
       b.type = RAD_BSDF_PERFECT_GLASS;
-      Color(n, "reflectance", b.reflectance, 1.f);
-      Color(n, "transmittance", b.transmittance, 1.f);
+      Color(n, "reflectance", b.reflectance, 1.f, RAD_TEX_REFLECTANCE);
+      Color(n, "transmittance", b.transmittance, 1.f, RAD_TEX_TRANSMITTANCE);
       b.int_ior = FloatOr(n, "int_ior", 1.5046f);
       b.ext_ior = FloatOr(n, "ext_ior", 1.000277f);
     } else if (type == "two_side") {  // two_side.cpp:12-58
@@ -532,15 +650,18 @@ struct Builder {
         throw LoadError(RAD_ERR_UNSUPPORTED, "two_side: only the \"two_side\" form is supported");
       b.type = RAD_BSDF_TWO_SIDE;
       b.nested = MakeBsdf(*inner);
+      cur = &b;
     } else if (type == "mask") {  // mask.cpp:8-17
       const Value* inner = n.find("nested");
       if (!inner) throw LoadError(RAD_ERR_INVALID_ARG, "mask: missing nested bsdf");
       b.type = RAD_BSDF_MASK;
-      b.opacity = Mono(n, "opacity", 0.5f);
+      b.opacity = Mono(n, "opacity", 0.5f, RAD_TEX_OPACITY);
       b.nested = MakeBsdf(*inner);
+      cur = &b;
     } else {
       throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + type);
     }
+    cur = outer;
     out.bsdfs.push_back(b);
     return int(out.bsdfs.size() - 1);
   }
@@ -685,15 +806,15 @@ struct Builder {
         l.shape = -1;
         if (lt == "area") {
           l.type = RAD_LIGHT_AREA;
-          Color(*lightNode, "radiance", l.radiance, 1.f);
+          LightColor(*lightNode, "radiance", l.radiance, 1.f);
           if (shape < 0) throw LoadError(RAD_ERR_INVALID_ARG, "area light entity without a shape");
           l.shape = shape;
         } else if (lt == "skybox") {
           l.type = RAD_LIGHT_SKYBOX;
-          Color(*lightNode, "radiance", l.radiance, 1.f);
+          LightColor(*lightNode, "radiance", l.radiance, 1.f);
         } else if (lt == "point") {  // point.cpp:10-17: world position = to_world * position
           l.type = RAD_LIGHT_POINT;
-          Color(*lightNode, "intensity", l.radiance, 1.f);
+          LightColor(*lightNode, "intensity", l.radiance, 1.f);
           float pos[3] = {0, 0, 0};
           VecOr<3>(*lightNode, "position", pos, {0, 0, 0});
           float w[3];
@@ -705,7 +826,7 @@ struct Builder {
         } else if (lt == "projection") {  // projection.cpp:14-36
           l.type = RAD_LIGHT_PROJECTION;
           float irr[3], sc[3];
-          Color(*lightNode, "irradiance", irr, 1.f);
+          LightColor(*lightNode, "irradiance", irr, 1.f);
           VecOr<3>(*lightNode, "scale", sc, {1, 1, 1});
           for (int c = 0; c < 3; c++) l.radiance[c] = irr[c] * sc[c];
           l.fov = FloatOr(*lightNode, "fov", 90.f);
@@ -773,6 +894,8 @@ struct Builder {
     out.desc.medium_count = uint32_t(out.media.size());
     out.desc.media = out.media.data();
     out.desc.global_medium = globalMedium;
+    out.desc.texture_count = uint32_t(out.textures.size());
+    out.desc.textures = out.textures.data();
   }
 
   int MakeShape(const Value& n, const M4& toWorld) {

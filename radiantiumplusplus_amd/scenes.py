@@ -684,8 +684,7 @@ class SceneBuilder:
             else:
                 e["shape"] = {"type": "cube"}
             if s.bsdf >= 0 and not s.json_shape.get("auto_bsdf"):
-                b = dict(self.bsdfs[s.bsdf])
-                e["bsdf"] = b
+                e["bsdf"] = self._json_value(self.bsdfs[s.bsdf], directory, assets)
             if s.light >= 0:
                 l = self.lights[s.light]
                 e["light"] = {"type": "area", "radiance": l["radiance"]}
@@ -714,6 +713,37 @@ class SceneBuilder:
             json.dump(scene, f, indent=1)
         return path
 
+    def _json_value(self, v, directory, assets):
+        """A BSDF / texture value as JSON; bitmaps become 8-bit PNG image assets
+        (texels must be multiples of 1/255; rows written top first, loaded
+        with is_flip_y so that texel row 0 is the bottom row)."""
+        if isinstance(v, dict) and v.get("type") == "bitmap":
+            t = v["_texels"]
+            u8 = np.rint(t * 255.0)
+            if np.abs(u8 / 255.0 - t).max() > 1e-6 or u8.min() < 0 or u8.max() > 255:
+                raise ValueError("bitmap texels must be k / 255 to be written as an 8-bit image")
+            name = v.get("asset_name") or f"tex{len(assets)}"
+            fname = f"{name}.png"
+            write_png(os.path.join(directory, fname), u8.astype(np.uint8)[::-1])
+            assets.append({"type": "image_default", "name": name, "location": fname, "channel": 3 if t.ndim == 3 else 1,
+                           "is_to_linear": False, "is_flip_y": True})
+            j = {"type": "bitmap", "asset_name": name, "filter": v["filter"], "wrap": v["wrap"]}
+            if v["max_level"] is not None:
+                j["max_level"] = int(v["max_level"])
+            if v["anisotropic_level"] is not None:
+                j["anisotropic_level"] = float(v["anisotropic_level"])
+            return j
+        if isinstance(v, dict) and v.get("type") == "chessboard":
+            j = {"type": "chessboard", "remap": v["remap"],
+                 "color_a": self._json_value(v["color_a"], directory, assets),
+                 "color_b": self._json_value(v["color_b"], directory, assets)}
+            if v["to_uv"] is not None:
+                j["to_uv"] = [float(x) for x in v["to_uv"].reshape(-1)]
+            return j
+        if isinstance(v, dict):
+            return {k: self._json_value(x, directory, assets) for k, x in v.items()}
+        return v
+
     def _medium_json(self, idx, directory, assets):
         m = self.media[idx]
         j = {"type": m["type"], "albedo": m["albedo"], "scale": m["scale"], "phase": m["phase"]}
@@ -727,6 +757,22 @@ class SceneBuilder:
                 assets.append({"type": "volume_data_mitsuba_vol", "name": aname, "location": fname})
             j["sigma_t"] = {"type": "volume_grid", "asset_name": aname}
         return j
+
+
+def write_png(path, u8):
+    """8-bit grey (H x W) or RGB (H x W x 3) PNG, row 0 written first."""
+    import struct
+    import zlib
+    u8 = np.ascontiguousarray(u8, np.uint8)
+    h, w = u8.shape[:2]
+    raw = b"".join(b"\x00" + u8[y].tobytes() for y in range(h))
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 0 if u8.ndim == 2 else 2, 0, 0, 0))
+                + chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
 
 
 def write_obj(path, positions, normals, uvs, indices):

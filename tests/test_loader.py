@@ -167,7 +167,7 @@ def test_bsdf_vars_and_disney_flags(tmp_path):
 
 
 @pytest.mark.parametrize("bad,code", [
-    ({"type": "plastic", "diffuse": {"type": "chessboard"}}, 4),  # textures: RAD_ERR_UNSUPPORTED
+    ({"type": "plastic", "diffuse": {"type": "bitmap", "asset_name": "none"}}, 1),  # unknown image asset
     ({"type": "two_side", "out_side": {"type": "diffuse"}}, 4),   # the reference's broken form
     ({"type": "mask"}, 1),                                         # no nested bsdf
     ({"type": "nonexistent"}, 1),                                  # unknown factory -> RAD_ERR_INVALID_ARG
@@ -250,5 +250,292 @@ def test_remaining_bsdfs_and_lights_load_with_reference_defaults(tmp_path):
         assert list(pt.radiance) == pytest.approx([4] * 3)
         pj = [l for l in lights if l.type == F.RAD_LIGHT_PROJECTION][0]
         assert list(pj.radiance) == pytest.approx([2, 4, 6]) and pj.fov == pytest.approx(30)
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
+
+
+
+# ---------------------------------------------------------------- images and textures
+def _png_bytes(rows, color_type, depth, palette=None, filters=(0,), width=None):
+    """PNG encoder for the tests: rows are lists of already packed byte rows;
+    row y uses filter filters[y % len(filters)] (0 none, 1 sub, 2 up, 3 average, 4 Paeth)."""
+    import struct
+    import zlib
+    h = len(rows)
+    samples = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[color_type]
+    bpp = max(1, samples * depth // 8)
+    out = b""
+    prev = bytes(len(rows[0]))
+    for y, row in enumerate(rows):
+        ft = filters[y % len(filters)]
+        enc = bytearray()
+        for i, v in enumerate(row):
+            a = row[i - bpp] if i >= bpp else 0
+            b = prev[i]
+            c = prev[i - bpp] if i >= bpp else 0
+            if ft == 0:
+                pr = 0
+            elif ft == 1:
+                pr = a
+            elif ft == 2:
+                pr = b
+            elif ft == 3:
+                pr = (a + b) >> 1
+            else:
+                p_ = a + b - c
+                pa, pb, pc = abs(p_ - a), abs(p_ - b), abs(p_ - c)
+                pr = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+            enc.append((v - pr) & 0xFF)
+        out += bytes([ft]) + bytes(enc)
+        prev = row
+    w_pixels = width if width is not None else len(rows[0]) * 8 // (samples * depth)
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+
+    data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w_pixels, h, depth, color_type, 0, 0, 0))
+    if palette is not None:
+        data += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
+    return data + chunk(b"IDAT", zlib.compress(out, 9)) + chunk(b"IEND", b"")
+
+
+def _texels_of(h, idx):
+    d = desc_of(h)
+    t = d.textures[idx]
+    n = t.width * t.height * t.channels
+    return t, np.ctypeslib.as_array(t.texels, (n,)).reshape(t.height, t.width, t.channels).copy()
+
+
+def _load_images(tmp_path, files, assets, params):
+    """Scene with one diffuse sphere per image asset, each textured by it."""
+    for name, data in files.items():
+        (tmp_path / name).write_bytes(data)
+    sc = dict(BASE)
+    sc["assets"] = assets
+    sc["scene"] = [{"shape": {"type": "sphere"}, "bsdf": {"type": "rough_metal", **p}} for p in params]
+    sc["scene"].append({"shape": {"type": "rectangle"}, "light": {"type": "area"}})
+    return load(_write_scene(tmp_path, sc))
+
+
+def _stb_rgb(src, comp):
+    """stb_image's conversion of 8-bit samples (comp per pixel) to RGB."""
+    if comp <= 2:
+        return np.repeat(src[..., :1], 3, 2)
+    return src[..., :3]
+
+
+def _stb_grey(src, comp):
+    if comp <= 2:
+        return src[..., :1]
+    r, g, b = (src[..., k].astype(np.int32) for k in range(3))
+    return ((r * 77 + g * 150 + 29 * b) >> 8)[..., None].astype(np.uint8)
+
+
+@pytest.mark.parametrize("ctype,depth", [(0, 8), (0, 16), (0, 2), (2, 8), (2, 16), (3, 8), (3, 4), (4, 8), (6, 8), (6, 16)])
+def test_png_decoding_matches_stb_conversions(tmp_path, ctype, depth):
+    """asset/image_stb.cpp with stb_image's rules: 16-bit samples keep the high
+    byte, sub-byte grey rescales, palettes expand, alpha is dropped, RGB -> grey
+    is (77 r + 150 g + 29 b) >> 8, values / 255, rows flipped (is_flip_y)."""
+    rng = np.random.default_rng(ctype * 100 + depth)
+    W, H = 13, 7
+    samples = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    palette = rng.integers(0, 256, (1 << depth, 3)) if ctype == 3 else None
+    if depth == 16:
+        vals = rng.integers(0, 65536, (H, W, samples))
+        rows = [vals[y].astype(">u2").tobytes() for y in range(H)]
+        px8 = (vals >> 8).astype(np.uint8)
+    elif depth < 8:
+        vals = rng.integers(0, 1 << depth, (H, W))
+        rows = []
+        for y in range(H):
+            bits = "".join(format(int(v), f"0{depth}b") for v in vals[y])
+            bits += "0" * (-len(bits) % 8)
+            rows.append(bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8)))
+        px8 = (vals * {1: 255, 2: 0x55, 4: 0x11}[depth]).astype(np.uint8)[..., None] if ctype == 0 else None
+        if ctype == 3:
+            px8 = palette[vals].astype(np.uint8)
+    else:
+        vals = rng.integers(0, 256, (H, W, samples))
+        rows = [vals[y].astype(np.uint8).tobytes() for y in range(H)]
+        px8 = vals.astype(np.uint8)
+        if ctype == 3:
+            px8 = palette[vals[..., 0]].astype(np.uint8)
+    comp = 3 if ctype == 3 else samples
+    data = _png_bytes(rows, ctype, depth, palette, filters=(0, 1, 2, 3, 4), width=W)
+    assets = [{"type": "image_default", "name": "c", "location": "a.png", "channel": 3, "is_to_linear": False},
+              {"type": "image_default", "name": "m", "location": "a.png", "channel": 1}]
+    h = _load_images(tmp_path, {"a.png": data}, assets,
+                     [{"reflectance": {"type": "bitmap", "asset_name": "c"}},
+                      {"alpha": {"type": "bitmap", "asset_name": "m"}}])
+    try:
+        t, rgb = _texels_of(h, 0)
+        assert (t.width, t.height, t.channels) == (W, H, 3)
+        assert np.array_equal(rgb, (_stb_rgb(px8, comp)[::-1].astype(np.float32) / np.float32(255)))
+        t, grey = _texels_of(h, 1)
+        assert t.channels == 1
+        assert np.array_equal(grey, _stb_grey(px8, comp)[::-1].astype(np.float32) / np.float32(255))
+        d = desc_of(h)
+        # "alpha" binds both roughness slots to the one texture
+        assert d.bsdfs[1].tex[F.RAD_TEX_ALPHA_U] == d.bsdfs[1].tex[F.RAD_TEX_ALPHA_V] == 2
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
+
+
+def test_srgb_to_linear_and_pnm(tmp_path):
+    """Colour images default to is_to_linear (Color24f::ToLinear, color.h:41-46);
+    binary PPM / PGM read like PNG."""
+    rng = np.random.default_rng(7)
+    img = rng.integers(0, 256, (5, 6, 3)).astype(np.uint8)
+    ppm = b"P6\n# comment\n6 5\n255\n" + img.tobytes()
+    pgm = b"P5 6 5 255\n" + img[..., 0].tobytes()
+    assets = [{"type": "image_default", "name": "c", "location": "a.ppm"},
+              {"type": "image_default", "name": "g", "location": "a.pgm", "channel": 1, "is_flip_y": False}]
+    h = _load_images(tmp_path, {"a.ppm": ppm, "a.pgm": pgm}, assets,
+                     [{"reflectance": {"type": "bitmap", "asset_name": "c"}},
+                      {"alpha": {"type": "bitmap", "asset_name": "g"}}])
+    try:
+        _, rgb = _texels_of(h, 0)
+        v = img[::-1].astype(np.float32) / np.float32(255)
+        lin = np.where(v <= np.float32(0.04045), v * np.float32(1 / 12.92),
+                       np.power((v + np.float32(0.055)) * np.float32(1 / 1.055), np.float32(2.4)))
+        assert np.allclose(rgb, lin, rtol=1e-6, atol=0)
+        assert rgb[0, 0, 0] == pytest.approx(lin[0, 0, 0], rel=1e-6)
+        _, grey = _texels_of(h, 1)
+        assert np.array_equal(grey[..., 0], img[..., 0].astype(np.float32) / np.float32(255))
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
+
+
+def _rgbe(rgb):
+    """Float RGB -> RGBE bytes (the usual frexp encoding)."""
+    m = rgb.max(-1)
+    out = np.zeros(rgb.shape[:-1] + (4,), np.uint8)
+    nz = m > 1e-32
+    f, e = np.frexp(m[nz])
+    sc = f * 256.0 / m[nz]
+    out[nz, :3] = np.floor(rgb[nz] * sc[:, None]).astype(np.uint8)
+    out[nz, 3] = (e + 128).astype(np.uint8)
+    return out
+
+
+@pytest.mark.parametrize("rle", [False, True])
+def test_radiance_hdr(tmp_path, rle):
+    """asset/image_hdr.cpp: RGBE texels (r, g, b) 2^(e - 136), no flip by default."""
+    rng = np.random.default_rng(8)
+    W, H = 20, 4
+    rgb = rng.uniform(0, 4, (H, W, 3))
+    rgb[1, 3:11] = 0.75  # a run for the RLE coder
+    q = _rgbe(rgb)
+    body = b""
+    for y in range(H):
+        if not rle:
+            body += q[y].tobytes()
+            continue
+        body += bytes([2, 2, W >> 8, W & 255])
+        for c in range(4):
+            ch = q[y, :, c]
+            x = 0
+            while x < W:
+                run = 1
+                while x + run < W and ch[x + run] == ch[x] and run < 127:
+                    run += 1
+                if run >= 3:
+                    body += bytes([128 + run, ch[x]])
+                    x += run
+                else:
+                    n = 1
+                    while x + n < W and n < 128 and not (x + n + 2 < W and ch[x + n] == ch[x + n + 1] == ch[x + n + 2]):
+                        n += 1
+                    body += bytes([n]) + bytes(ch[x:x + n])
+                    x += n
+    data = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n" % (H, W) + body
+    h = _load_images(tmp_path, {"a.hdr": data}, [{"type": "image_hdr", "name": "e", "location": "a.hdr"}],
+                     [{"reflectance": {"type": "bitmap", "asset_name": "e"}}])
+    try:
+        _, tex = _texels_of(h, 0)
+        scale = np.where(q[..., 3] > 0, np.ldexp(np.float32(1), q[..., 3].astype(np.int32) - 136), 0).astype(np.float32)
+        want = q[..., :3].astype(np.float32) * scale[..., None]
+        assert np.array_equal(tex, want)
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
+
+
+def test_texture_nodes_and_defaults(tmp_path):
+    """bitmap.cpp:264-317 defaults (bilinear, clamp, mip chain only for
+    trilinear / anisotropic, anisotropic level 8) and chessboard.cpp:17-31
+    (to_uv, colours 0.4 / 0.2, nested bitmaps, remap); textured lights and
+    mismatched channel counts are rejected."""
+    img = np.zeros((4, 4, 3), np.uint8)
+    (tmp_path / "a.png").write_bytes(_png_bytes([img[y].tobytes() for y in range(4)], 2, 8))
+    assets = [{"type": "image_default", "name": "c", "location": "a.png"}]
+    params = [{"reflectance": {"type": "bitmap", "asset_name": "c"}},
+              {"reflectance": {"type": "bitmap", "asset_name": "c", "filter": "trilinear", "wrap": "repeat"}},
+              {"reflectance": {"type": "bitmap", "asset_name": "c", "filter": "anisotropic", "max_level": 2}},
+              {"reflectance": {"type": "chessboard", "to_uv": {"scale": [2, 3, 1]}, "remap": True,
+                               "color_b": {"type": "bitmap", "asset_name": "c", "filter": "nearest"}}},
+              {"alpha": {"type": "chessboard", "color_a": 0.3}}]
+    h = _load_images(tmp_path, {}, assets, params)
+    try:
+        d = desc_of(h)
+        ts = [d.textures[i] for i in range(d.texture_count)]
+        assert (ts[0].filter, ts[0].wrap, ts[0].max_level, ts[0].anisotropic_level) == \
+            (F.RAD_FILTER_BILINEAR, F.RAD_WRAP_CLAMP, 0, 0.0)
+        assert (ts[1].filter, ts[1].wrap, ts[1].max_level) == (F.RAD_FILTER_TRILINEAR, F.RAD_WRAP_REPEAT, -1)
+        assert (ts[2].filter, ts[2].max_level, ts[2].anisotropic_level) == (F.RAD_FILTER_ANISOTROPIC, 2, 8.0)
+        # the chessboard's nested bitmap is listed before it
+        cb = ts[4]
+        assert cb.type == F.RAD_TEXTURE_CHESSBOARD and cb.remap == 1 and cb.tex_a == 0 and cb.tex_b == 4
+        assert list(cb.color_a) == pytest.approx([0.4] * 3)
+        assert ts[3].filter == F.RAD_FILTER_NEAREST
+        assert np.allclose(np.array(list(cb.to_uv)).reshape(4, 4), scenes.trs(scale=(2, 3, 1)))
+        mono = ts[5]
+        assert mono.channels == 1 and mono.color_a[0] == pytest.approx(0.3) and mono.color_b[0] == pytest.approx(0.2)
+        assert d.bsdfs[3].tex[F.RAD_TEX_REFLECTANCE] == 5
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
+    for bad, code in [({"alpha": {"type": "bitmap", "asset_name": "c"}}, 1),  # rgb image on a scalar parameter
+                      ({"reflectance": {"type": "chessboard", "color_a": {"type": "chessboard"}}}, 4)]:
+        with pytest.raises(hip.RadError) as e:
+            _load_images(tmp_path, {}, assets, [bad])
+        assert e.value.status == code
+    sc = dict(BASE)
+    sc["scene"] = [{"shape": {"type": "rectangle"}, "light": {"type": "area", "radiance": {"type": "chessboard"}}}]
+    with pytest.raises(hip.RadError) as e:
+        load(_write_scene(tmp_path, sc))
+    assert e.value.status == 4
+
+
+def test_textured_scene_round_trip_renders_identically(tmp_path):
+    """Builder textures written as PNG assets and loaded back give the oracle
+    the same texels: bit-identical renders."""
+    rng = np.random.default_rng(9)
+    b = scenes.SceneBuilder()
+    img = rng.integers(0, 256, (16, 32, 3)).astype(np.float32) / np.float32(255)
+    mono = rng.integers(10, 120, (8, 8)).astype(np.float32) / np.float32(255)
+    b.rectangle(to_world=scenes.trs(axis=(1, 0, 0), angle=-90, scale=(6, 6, 6)),
+                bsdf=b.diffuse(b.bitmap(img, filter="trilinear", wrap="repeat")))
+    b.sphere(center=(0, 1, 0), radius=1,
+             bsdf=b.rough_metal(alpha=b.bitmap(mono, filter="nearest"),
+                                reflectance=b.chessboard((0.8, 0.2, 0.2), b.bitmap(img), to_uv=scenes.trs(scale=(4, 4, 1)))))
+    light = b.rectangle(to_world=scenes.trs(translate=(0, 5, -1), axis=(1, 0, 0), angle=90))
+    b.area_light(light, 8.0)
+    b.set_camera(24, 16, 55, origin=(0, 3, -8), target=(0, 0.8, 0), far=1000)
+    b.set_render("path", spp=4, max_depth=5, rr_depth=3, seed=3)
+    path = b.write_json(str(tmp_path))
+    h = load(path)
+    try:
+        d = desc_of(h)
+
+        class Wrapped:
+            camera = b.camera
+            renderer = b.renderer
+            sampler = b.sampler
+
+            def to_desc(self):
+                return d
+
+        fb1 = O.OracleScene(b).render(spp=4, seed=3, threads=2)[0]
+        fb2 = O.OracleScene(Wrapped()).render(spp=4, seed=3, threads=2)[0]
+        assert np.array_equal(fb1, fb2)
     finally:
         hip.lib().rad_loaded_scene_free(h)
