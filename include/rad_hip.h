@@ -164,6 +164,8 @@ typedef struct rad_light_desc {
   float to_world[16];      /* skybox orientation; point: world position in the translation column;
                               projection: the light's frame (looks along +z) */
   float fov;               /* projection field of view in degrees (default 90) */
+  int32_t tex;             /* area / skybox: 1 + index of an RGB texture for the radiance, 0 = the constant
+                              (light/area.cpp:20-41, light/skybox.cpp:24-49) */
 } rad_light_desc;
 
 enum { RAD_MEDIUM_HOMOGENEOUS = 0, RAD_MEDIUM_HETEROGENEOUS = 1 };

@@ -148,6 +148,24 @@ struct DiscreteDistribution1D {
   }
 };
 
+// distribution.cpp:55-128 (ContinuousDistribution1D / 2D)
+struct Dist1D {
+  std::vector<Float> pdf, cdf;
+  Float sum = 0, normalization = 0;
+  void Build(const Float* p, size_t n);
+  size_t Count() const { return pdf.size(); }
+  // (cell index, continuous position, normalised density)
+  std::tuple<size_t, Float, Float> Sample(Float xi) const;
+};
+struct Dist2D {
+  std::vector<Dist1D> conditional;
+  Dist1D marginal;
+  void Build(const Float* data, size_t nu, size_t nv);
+  // (cell u, cell v, density over [0,1]^2, continuous position in [0,1]^2)
+  std::tuple<size_t, size_t, Float, V2> Sample(const V2& xi) const;
+  Float Pdf(const V2& uv) const;
+};
+
 struct Bounds {
   V3 lo{Inf, Inf, Inf}, hi{-Inf, -Inf, -Inf};
   void Extend(const V3& p) { lo = Min(lo, p); hi = Max(hi, p); }
@@ -293,6 +311,9 @@ struct Texture {
   V3 Anisotropic(const V2& uv, const V2& dx, const V2& dy) const;
 };
 void BuildTextures(const rad_texture_desc* descs, uint32_t count, std::vector<std::unique_ptr<Texture>>& out);
+
+// A rectangle's parametric surface (rectangle.cpp:236-250).
+SurfaceInteraction RectangleParamSurface(const Shape* rect, const V2& uv);
 
 // RayDifferential of a camera ray (ray.h:35-58, perspective.cpp:84-99).
 struct RayDiff {

@@ -361,45 +361,195 @@ PositionSampleResult RectangleShape::SamplePosition(const V2& xi) const {  // :2
   return psr;
 }
 
+// ---------------------------------------------------------------- distributions
+void Dist1D::Build(const Float* p, size_t n) {  // distribution.cpp:55-68
+  pdf.assign(p, p + n);
+  cdf.assign(n + 1, 0);
+  for (size_t i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + pdf[i - 1] / Float(n);
+  sum = cdf[n];
+  for (size_t i = 1; i < n + 1; i++) cdf[i] /= sum;
+  normalization = Float(1.0 / sum);
+}
+
+// distribution.cpp:85-96: the cell below the first cdf entry >= xi (the
+// unsigned index - 1 wraps for xi <= cdf[0] and clamps to the last cell).
+std::tuple<size_t, Float, Float> Dist1D::Sample(Float xi) const {
+  size_t index = size_t(std::lower_bound(cdf.begin(), cdf.end(), xi) - cdf.begin());
+  index = std::clamp(index - 1, size_t(0), cdf.size() - 2);
+  Float du = xi - cdf[index];
+  if ((cdf[index + 1] - cdf[index]) > 0) du /= (cdf[index + 1] - cdf[index]);
+  return {index, (Float(index) + du) / Float(Count()), pdf[index] * normalization};
+}
+
+void Dist2D::Build(const Float* data, size_t nu, size_t nv) {  // distribution.cpp:101-113
+  conditional.resize(nv);
+  for (size_t v = 0; v < nv; v++) conditional[v].Build(data + v * nu, nu);
+  std::vector<Float> m(nv);
+  for (size_t v = 0; v < nv; v++) m[v] = conditional[v].sum;
+  marginal.Build(m.data(), nv);
+}
+
+std::tuple<size_t, size_t, Float, V2> Dist2D::Sample(const V2& xi) const {  // :115-122
+  auto [vj, d1, pdf1] = marginal.Sample(xi.y);
+  auto [vi, d0, pdf0] = conditional[vj].Sample(xi.x);
+  return {vi, vj, pdf0 * pdf1, V2(d0, d1)};
+}
+
+Float Dist2D::Pdf(const V2& uv) const {  // :124-128
+  size_t iu = std::clamp(size_t(uv.x * Float(conditional[0].Count())), size_t(0), conditional[0].Count() - 1);
+  size_t iv = std::clamp(size_t(uv.y * Float(marginal.Count())), size_t(0), marginal.Count() - 1);
+  return conditional[iv].pdf[iu] / marginal.sum;
+}
+
+SurfaceInteraction RectangleParamSurface(const Shape* shape, const V2& uv) {  // rectangle.cpp:236-250
+  const auto* r = static_cast<const RectangleShape*>(shape);
+  SurfaceInteraction si{};
+  si.T = 0;
+  si.P = r->toWorld.ApplyAffineToWorld(V3(uv.x * 2 - 1, uv.y * 2 - 1, 0));
+  si.N = r->frame.N;
+  si.Shading = r->frame;
+  si.UV = uv;
+  si.dPdU = r->frame.S;
+  si.dPdV = r->frame.T;
+  si.shape = shape;
+  return si;
+}
+
 // ---------------------------------------------------------------- lights
-struct AreaLight final : Light {  // light/area.cpp:20-91 (uniform radiance)
+// light/area.cpp:20-148.  A radiance texture larger than one texel is sampled
+// through a 2-D distribution of its luminance at texel corners; the sample
+// lands on the texel's corner uv.  SampleDirection builds its record with
+// SurfaceInteraction::ToDsr, which sets the density to 0 (interaction.cpp:
+// 59-75), so with quirks on such a light receives no light samples and is
+// reached only by BSDF sampling (still MIS-weighted against PdfDirection).
+// Quirks off: the continuous position inside the texel, with the
+// PdfDirection density.
+struct AreaLight final : Light {
   Spectrum radiance;
+  const Texture* tex = nullptr;
+  bool uniform = true;
+  bool quirks = true;
+  Dist2D dist;
+  Spectrum Radiance(const V2& uv) const { return tex ? tex->Eval(uv, V2(), V2()) : radiance; }
   Spectrum Eval(const SurfaceInteraction& si) const override {
-    return Dot(si.ToWorld(si.Wi), si.N) >= 0 ? radiance : Spectrum(0);
+    return Dot(si.ToWorld(si.Wi), si.N) >= 0 ? Radiance(si.UV) : Spectrum(0);
+  }
+  void Init(uint32_t w, uint32_t h) {
+    uniform = !tex || (w == 1 && h == 1);
+    if (uniform) return;
+    std::vector<Float> lum(size_t(w) * h);
+    for (uint32_t y = 0; y < h; y++) {
+      Float v = Float(y) / Float(h);
+      for (uint32_t x = 0; x < w; x++) lum[size_t(y) * w + x] = Luminance(tex->Eval(V2(Float(x) / Float(w), v), V2(), V2()));
+    }
+    dist.Build(lum.data(), w, h);
   }
   std::pair<DirectionSampleResult, Spectrum> SampleDirection(const Interaction& ref, const V2& xi) const override {
-    DirectionSampleResult dsr = shape->SampleDirection(ref, xi);
-    Spectrum r = Dot(-dsr.Dir, dsr.N) >= 0 ? radiance : Spectrum(0);
+    if (uniform) {
+      DirectionSampleResult dsr = shape->SampleDirection(ref, xi);
+      Spectrum r = Dot(-dsr.Dir, dsr.N) >= 0 ? Radiance(dsr.UV) : Spectrum(0);
+      return {dsr, r};
+    }
+    auto [vi, vj, pdf, cont] = dist.Sample(xi);
+    if (pdf <= 0) return {{}, Spectrum(0)};
+    V2 uv = quirks ? V2(Float(vi) / Float(dist.conditional[0].Count()), Float(vj) / Float(dist.marginal.Count())) : cont;
+    SurfaceInteraction surface = RectangleParamSurface(shape, uv);
+    DirectionSampleResult dsr = surface.ToDsr(ref);
+    dsr.Pdf = quirks ? 0 : PdfDirection(ref, dsr);
+    Spectrum r = Dot(-dsr.Dir, dsr.N) >= 0 ? Radiance(surface.UV) : Spectrum(0);
     return {dsr, r};
   }
   Float PdfDirection(const Interaction& ref, const DirectionSampleResult& dsr) const override {
-    return Dot(-dsr.Dir, dsr.N) >= 0 ? shape->PdfDirection(ref, dsr) : 0;
+    if (uniform) return Dot(-dsr.Dir, dsr.N) >= 0 ? shape->PdfDirection(ref, dsr) : 0;
+    SurfaceInteraction si = RectangleParamSurface(shape, dsr.UV);
+    Float pdf = dist.Pdf(dsr.UV);
+    if (pdf <= 0) return 0;
+    Float dp = Dot(dsr.Dir, dsr.N);
+    return pdf * Sqr(dsr.Dist) / (Norm(Cross(si.dPdU, si.dPdV)) * -dp);
   }
 };
 
-struct SkyboxLight final : Light {  // light/skybox.cpp:24-141 (constant map)
+// light/skybox.cpp:24-141, 215-234: a lat-long radiance map; a map larger
+// than one texel is sampled through its sin(theta)-weighted luminance at
+// texel corners.  PdfDirection maps the direction with the affine inverse
+// (translation included, Q15).  Quirks off: the continuous position inside
+// the texel and the linear inverse.
+struct SkyboxLight final : Light {
   Spectrum radiance;
+  const Texture* tex = nullptr;
+  bool uniform = true;
+  bool quirks = true;
+  Dist2D dist;
   Transform toWorld;
   V3 sphereCenter;
   Float sphereRadius = 0;
   bool IsEnv() const override { return true; }
-  Spectrum Eval(const SurfaceInteraction&) const override { return radiance; }
+  static V2 DirUV(const V3& v) {
+    Float phi = std::atan2(v.x, -v.z);
+    phi = (phi < 0) ? (phi + 2 * PI) : phi;
+    Float theta = std::acos(std::clamp(v.y, Float(-1), Float(1)));
+    return V2(phi / (2 * PI), theta / PI);
+  }
+  Spectrum Radiance(const V2& uv) const { return tex ? tex->Eval(uv, V2(), V2()) : radiance; }
+  void Init(uint32_t w, uint32_t h) {
+    uniform = !tex || (w == 1 && h == 1);
+    if (uniform) return;
+    std::vector<Float> lum(size_t(w) * h);
+    for (uint32_t y = 0; y < h; y++) {
+      Float sinTheta = std::sin((Float(y) + Float(0.5)) / Float(h) * PI);
+      Float v = Float(y) / Float(h);
+      for (uint32_t x = 0; x < w; x++) {
+        Float l = Luminance(tex->Eval(V2(Float(x) / Float(w), v), V2(), V2()));
+        lum[size_t(y) * w + x] = l * sinTheta;
+      }
+    }
+    dist.Build(lum.data(), w, h);
+  }
+  Spectrum Eval(const SurfaceInteraction& si) const override {
+    return tex ? Radiance(DirUV(toWorld.ApplyLinearToLocal(-si.Wi))) : radiance;
+  }
   std::pair<DirectionSampleResult, Spectrum> SampleDirection(const Interaction& ref, const V2& xi) const override {
     Float radius = std::max(sphereRadius, Norm(ref.P - sphereCenter));
-    Float dist = 2 * radius;
+    Float dist2 = 2 * radius;
     DirectionSampleResult dsr{};
-    V3 dir = Normalized(Warp::SquareToUniformSphere(xi));
-    V3 worldDir = toWorld.ApplyLinearToWorld(dir);
-    dsr.P = worldDir * dist + ref.P;
+    if (uniform) {
+      V3 dir = Normalized(Warp::SquareToUniformSphere(xi));
+      V3 worldDir = toWorld.ApplyLinearToWorld(dir);
+      dsr.P = worldDir * dist2 + ref.P;
+      dsr.N = -worldDir;
+      dsr.UV = DirUV(dir);
+      dsr.Pdf = Warp::SquareToUniformSpherePdf();
+      dsr.Dir = worldDir;
+      dsr.Dist = dist2;
+      dsr.IsDelta = false;
+      return {dsr, Radiance(dsr.UV)};
+    }
+    auto [vi, vj, pdf, cont] = dist.Sample(xi);
+    if (pdf <= 0) return {dsr, Spectrum(0)};
+    Float y = quirks ? Float(vj) / Float(dist.marginal.Count()) : cont.y;
+    Float x = quirks ? Float(vi) / Float(dist.conditional[0].Count()) : cont.x;
+    Float theta = y * PI, phi = x * 2 * PI;
+    Float sinTheta = std::sin(theta);
+    if (sinTheta == 0) return {dsr, Spectrum(0)};
+    pdf /= (2 * PI * PI * sinTheta);
+    V3 sph(std::cos(phi) * sinTheta, std::sin(phi) * sinTheta, std::cos(theta));  // math_ext.h:108-112
+    V3 worldDir = toWorld.ApplyLinearToWorld(V3(sph.y, sph.z, -sph.x));
+    dsr.P = worldDir * dist2 + ref.P;
     dsr.N = -worldDir;
-    dsr.Pdf = Warp::SquareToUniformSpherePdf();
+    dsr.UV = V2(x, y);
+    dsr.Pdf = pdf;
     dsr.Dir = worldDir;
-    dsr.Dist = dist;
+    dsr.Dist = dist2;
     dsr.IsDelta = false;
-    return {dsr, radiance};
+    return {dsr, Radiance(dsr.UV)};
   }
-  Float PdfDirection(const Interaction&, const DirectionSampleResult&) const override {
-    return Warp::SquareToUniformSpherePdf();
+  Float PdfDirection(const Interaction&, const DirectionSampleResult& dsr) const override {
+    if (uniform) return Warp::SquareToUniformSpherePdf();
+    V3 v = quirks ? toWorld.ApplyAffineToLocal(dsr.Dir) : toWorld.ApplyLinearToLocal(dsr.Dir);
+    Float theta = std::acos(std::clamp(v.y, Float(-1), Float(1)));
+    Float sinTheta = std::sin(theta);
+    if (sinTheta == 0) return 0;
+    return dist.Pdf(DirUV(v)) / (2 * PI * PI * sinTheta);
   }
   void SetScene(const Scene& scene) override {  // skybox.cpp:136-141, bounding_sphere.cpp:15-18
     Bounds b = scene.accel.world;
@@ -557,9 +707,24 @@ void Scene::Create(const rad_scene_desc& desc, bool q) {
   globalMedium = desc.global_medium >= 0 ? media[desc.global_medium].get() : nullptr;
   for (uint32_t i = 0; i < desc.light_count; i++) {
     const rad_light_desc& ld = desc.lights[i];
+    auto texOf = [&](int32_t k) -> const Texture* {
+      if (k == 0) return nullptr;
+      if (k < 0 || uint32_t(k) > desc.texture_count || desc.textures[k - 1].channels != 3)
+        throw std::runtime_error("light: bad radiance texture");
+      return textures[size_t(k - 1)].get();
+    };
+    auto texSize = [&](int32_t k, uint32_t& w, uint32_t& h) {
+      w = h = 1;
+      if (k > 0 && desc.textures[k - 1].type == RAD_TEXTURE_BITMAP) w = desc.textures[k - 1].width, h = desc.textures[k - 1].height;
+    };
+    uint32_t tw = 1, th = 1;
+    texSize(ld.tex, tw, th);
     if (ld.type == RAD_LIGHT_AREA) {
       auto l = std::make_unique<AreaLight>();
       l->radiance = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
+      l->tex = texOf(ld.tex);
+      l->quirks = quirks;
+      l->Init(tw, th);
       lights.push_back(std::move(l));
     } else if (ld.type == RAD_LIGHT_POINT) {
       auto l = std::make_unique<PointLight>();
@@ -575,6 +740,9 @@ void Scene::Create(const rad_scene_desc& desc, bool q) {
       auto l = std::make_unique<SkyboxLight>();
       l->radiance = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
       l->toWorld = Transform(Mat4::FromRowMajor(ld.to_world));
+      l->tex = texOf(ld.tex);
+      l->quirks = quirks;
+      l->Init(tw, th);
       lights.push_back(std::move(l));
     }
   }
@@ -655,6 +823,9 @@ void Scene::Create(const rad_scene_desc& desc, bool q) {
     if (sd.light >= 0) {
       s->light = lights[sd.light].get();
       lights[sd.light]->shape = s.get();
+      auto* al = dynamic_cast<AreaLight*>(lights[sd.light].get());
+      if (al && !al->uniform && sd.type != RAD_SHAPE_RECTANGLE)
+        throw std::runtime_error("textured area light: only rectangles have a parametric surface (shape.h EvalParamSurface)");
     }
     s->inside = sd.medium_inside >= 0 ? media[sd.medium_inside].get() : nullptr;
     s->outside = sd.medium_outside >= 0 ? media[sd.medium_outside].get() : nullptr;

@@ -135,6 +135,7 @@ class LightDesc(C.Structure):
         ("radiance", F3),
         ("to_world", F16),
         ("fov", C.c_float),
+        ("tex", C.c_int32),
     ]
 
 

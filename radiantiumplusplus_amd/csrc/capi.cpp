@@ -65,6 +65,7 @@ void UploadScene(rad_scene* s) {
   s->d_grid.upload(h.grid.data(), h.grid.size() * sizeof(float), st);
   s->d_tex.upload(h.textures.data(), h.textures.size() * sizeof(TexRec), st);
   s->d_texels.upload(h.texels.data(), h.texels.size() * sizeof(float), st);
+  s->d_ldist.upload(h.light_dist.data(), h.light_dist.size() * sizeof(float), st);
   DevScene& d = s->dev;
   d = DevScene{};
   d.shapes = s->d_shapes.as<ShapeRec>();
@@ -84,6 +85,7 @@ void UploadScene(rad_scene* s) {
   d.grid_data = s->d_grid.as<float>();
   d.textures = s->d_tex.as<TexRec>();
   d.texels = s->d_texels.as<float>();
+  d.light_dist = s->d_ldist.as<float>();
   d.cam = h.cam;
   // skybox world sphere (skybox.cpp:136-141, bounding_sphere.cpp:15-18)
   f3 lo = mk(h.world_lo[0], h.world_lo[1], h.world_lo[2]), hi = mk(h.world_hi[0], h.world_hi[1], h.world_hi[2]);

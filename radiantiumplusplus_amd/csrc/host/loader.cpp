@@ -525,8 +525,18 @@ struct Builder {
     }
     Vec<3>(*v, out3, key);
   }
-  // Light colours: textured emitters (area.cpp, skybox.cpp, projection.cpp
-  // image textures) are not carried by this build.
+  // Area / skybox radiance (area.cpp:20-41, skybox.cpp:24-49): a constant,
+  // or a texture (returns 1 + its index).
+  int32_t EmitterColor(const Value& node, const char* key, float* out3) {
+    const Value* v = node.find(key);
+    out3[0] = out3[1] = out3[2] = 1.f;
+    if (!v) return 0;
+    if (v->is_object()) return MakeTexture(*v, 3);
+    Vec<3>(*v, out3, key);
+    return 0;
+  }
+  // Point intensity and projection irradiance: a projection light's image
+  // texture is not carried by this build.
   void LightColor(const Value& node, const char* key, float* out3, float def) {
     const Value* v = node.find(key);
     out3[0] = out3[1] = out3[2] = def;
@@ -806,12 +816,12 @@ struct Builder {
         l.shape = -1;
         if (lt == "area") {
           l.type = RAD_LIGHT_AREA;
-          LightColor(*lightNode, "radiance", l.radiance, 1.f);
+          l.tex = EmitterColor(*lightNode, "radiance", l.radiance);
           if (shape < 0) throw LoadError(RAD_ERR_INVALID_ARG, "area light entity without a shape");
           l.shape = shape;
         } else if (lt == "skybox") {
           l.type = RAD_LIGHT_SKYBOX;
-          LightColor(*lightNode, "radiance", l.radiance, 1.f);
+          l.tex = EmitterColor(*lightNode, "radiance", l.radiance);
         } else if (lt == "point") {  // point.cpp:10-17: world position = to_world * position
           l.type = RAD_LIGHT_POINT;
           LightColor(*lightNode, "intensity", l.radiance, 1.f);

@@ -148,6 +148,14 @@ struct LightRec {
   Affine to_local;  // projection: world -> light frame
   float clip[16];   // projection: light frame -> [0,1]^2 image plane (row-major, projective)
   f3 axis;          // projection: normalised +z of the frame in world space
+  // radiance texture (area / skybox; -1 = constant).  A texture of more than
+  // one texel is sampled through a 2-D luminance distribution at
+  // light_dist + dist_off: lum[h][w] | cdf[h][w + 1] | 1/row sum[h] |
+  // row sum[h] | marginal cdf[h + 1]; msum = the marginal's sum.
+  int32_t tex;
+  int32_t tex_uniform;
+  uint32_t dist_off, dist_w, dist_h;
+  float msum, mnorm;
 };
 
 enum : int32_t { MEDIUM_HOMOGENEOUS = 0, MEDIUM_HETEROGENEOUS = 1 };
@@ -197,6 +205,7 @@ struct DevScene {
   const float* grid_data;
   const TexRec* textures;
   const float* texels;
+  const float* light_dist;
   CameraRec cam;
   f3 world_center;
   float world_radius;

@@ -359,6 +359,57 @@ static void AttachTextures(const rad_bsdf_desc& d, const HostScene& h, BsdfRec& 
   }
 }
 
+// Lookups at a zero footprint (every filter then reads level 0: trilinear and
+// anisotropic fall back to the bilinear lookup, bitmap.cpp:82-134), used to
+// build the emitters' luminance distributions.
+#pragma float_control(push)
+#pragma clang fp contract(off)
+static f3 HostTexel(const HostScene& h, const TexRec& t, uint32_t x, uint32_t y) {
+  const float* p = h.texels.data() + t.off[0] + (size_t(y) * t.w[0] + x) * uint32_t(t.channels);
+  return t.channels == 3 ? mk(p[0], p[1], p[2]) : mk(p[0], 0.f, 0.f);
+}
+static float HostWrap(float v, int32_t wrap) {
+  return wrap == RAD_WRAP_REPEAT ? std::min(std::max(v - std::floor(v), 0.f), 1.f) : std::min(std::max(v, 0.f), 1.f);
+}
+static f3 HostTexLevel0(const HostScene& h, const TexRec& t, float u, float v) {
+  const uint32_t W = t.w[0], H = t.h[0];
+  const float fu = HostWrap(u, t.wrap) * float(W - 1), fv = HostWrap(v, t.wrap) * float(H - 1);
+  const uint32_t pu = uint32_t(fu), pv = uint32_t(fv);
+  if (t.filter == RAD_FILTER_NEAREST) return HostTexel(h, t, pu, pv);
+  const uint32_t apu = std::min(pu + (fu > float(pu) + 0.5f ? 1u : 0xffffffffu), W - 1);
+  const uint32_t apv = std::min(pv + (fv > float(pv) + 0.5f ? 1u : 0xffffffffu), H - 1);
+  const float du = std::min(std::fabs(fu - float(pu) - 0.5f), 1.f), dv = std::min(std::fabs(fv - float(pv) - 0.5f), 1.f);
+  const f3 a = HostTexel(h, t, pu, pv), b = HostTexel(h, t, apu, pv), c = HostTexel(h, t, pu, apv), d = HostTexel(h, t, apu, apv);
+  return (a * (1.f - du) + b * du) * (1.f - dv) + (c * (1.f - du) + d * du) * dv;
+}
+
+// ContinuousDistribution2D (distribution.cpp:55-113) over a w x h luminance
+// map, stored for the device at light_dist + dist_off.
+static void BuildLightDist(const std::vector<float>& lum, uint32_t w, uint32_t h, std::vector<float>& pool, LightRec& r) {
+  r.dist_off = uint32_t(pool.size());
+  r.dist_w = w;
+  r.dist_h = h;
+  pool.insert(pool.end(), lum.begin(), lum.end());
+  const size_t cdfOff = pool.size(), normOff = cdfOff + size_t(h) * (w + 1), sumOff = normOff + h, mOff = sumOff + h;
+  pool.resize(mOff + h + 1);
+  for (uint32_t v = 0; v < h; v++) {
+    float* cdf = pool.data() + cdfOff + size_t(v) * (w + 1);
+    cdf[0] = 0.f;
+    for (uint32_t i = 1; i <= w; i++) cdf[i] = cdf[i - 1] + lum[size_t(v) * w + i - 1] / float(w);
+    const float sum = cdf[w];
+    for (uint32_t i = 1; i <= w; i++) cdf[i] /= sum;
+    pool[normOff + v] = float(1.0 / double(sum));
+    pool[sumOff + v] = sum;
+  }
+  float* m = pool.data() + mOff;
+  m[0] = 0.f;
+  for (uint32_t v = 1; v <= h; v++) m[v] = m[v - 1] + pool[sumOff + v - 1] / float(h);
+  r.msum = m[h];
+  for (uint32_t v = 1; v <= h; v++) m[v] /= r.msum;
+  r.mnorm = float(1.0 / double(r.msum));
+}
+#pragma float_control(pop)
+
 // Wrappers: the nested index must name another record, and at most two
 // wrappers may enclose a base BSDF (the device resolves two levels).
 static void ResolveWrappers(std::vector<BsdfRec>& bsdfs) {
@@ -447,6 +498,30 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
     r.shape = -1;
     r.radiance = mk(l.radiance[0], l.radiance[1], l.radiance[2]);
     r.to_world = M4::From(l.to_world).ToAffine();
+    r.tex = -1;
+    if (l.tex != 0) {  // light/area.cpp:20-41, light/skybox.cpp:24-49
+      if (l.type != RAD_LIGHT_AREA && l.type != RAD_LIGHT_SKYBOX) throw std::runtime_error("only area and skybox radiance may be textured");
+      if (l.tex < 0 || uint32_t(l.tex) > out.textures.size()) throw std::runtime_error("light: texture index out of range");
+      const TexRec& t = out.textures[l.tex - 1];
+      if (t.channels != 3) throw std::runtime_error("light: radiance texture must have 3 channels");
+      r.tex = l.tex - 1;
+      const uint32_t w = t.type == TEX_CHESSBOARD ? 1 : t.w[0], hgt = t.type == TEX_CHESSBOARD ? 1 : t.h[0];
+      r.tex_uniform = w == 1 && hgt == 1;
+      if (!r.tex_uniform) {
+        std::vector<float> lum(size_t(w) * hgt);
+        for (uint32_t y = 0; y < hgt; y++) {
+          const float sinTheta = l.type == RAD_LIGHT_SKYBOX ? std::sin((float(y) + 0.5f) / float(hgt) * kPi) : 1.f;
+          const float v = float(y) / float(hgt);
+          for (uint32_t x = 0; x < w; x++) {
+            const f3 c = HostTexLevel0(out, t, float(x) / float(w), v);
+            const float lm = (c.x * 0.212671f) + (c.y * 0.715160f) + (c.z * 0.072169f);
+            lum[size_t(y) * w + x] = l.type == RAD_LIGHT_SKYBOX ? lm * sinTheta : lm;
+          }
+        }
+        BuildLightDist(lum, w, hgt, out.light_dist, r);
+      }
+    }
+    if (l.type == RAD_LIGHT_SKYBOX) r.to_local = M4::From(l.to_world).Inverse().ToAffine();
     if (l.type == RAD_LIGHT_PROJECTION) {  // projection.cpp:14-36 with a constant irradiance (aspect 1)
       M4 tw = M4::From(l.to_world);
       r.to_local = tw.Inverse().ToAffine();
@@ -481,6 +556,8 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
       throw std::runtime_error("shape " + std::to_string(si) + " references an out-of-range record");
     if (d.light >= 0) {
       if (desc.lights[d.light].type != RAD_LIGHT_AREA) throw std::runtime_error("only area lights attach to shapes");
+      if (out.lights[d.light].tex >= 0 && d.type != RAD_SHAPE_RECTANGLE)
+        throw std::runtime_error("textured area lights need a rectangle (the reference's parametric surface, rectangle.cpp:236-250)");
       out.lights[d.light].shape = int32_t(si);
     }
     if (d.type == RAD_SHAPE_CUBE && d.bsdf >= 0 && out.bsdfs[d.bsdf].tex_any)
@@ -672,5 +749,6 @@ CameraRec MakeCameraRec(const rad_camera_desc& c) {
 
 uint64_t rad_scene::device_bytes() const {
   return d_nodes.bytes + d_qnodes.bytes + d_prims.bytes + d_shapes.bytes + d_pos.bytes + d_nrm.bytes + d_uv.bytes + d_idx.bytes +
-         d_bsdfs.bytes + d_lights.bytes + d_cdf.bytes + d_media.bytes + d_grid.bytes + d_tex.bytes + d_texels.bytes;
+         d_bsdfs.bytes + d_lights.bytes + d_cdf.bytes + d_media.bytes + d_grid.bytes + d_tex.bytes + d_texels.bytes +
+         d_ldist.bytes;
 }

@@ -29,6 +29,7 @@ struct HostScene {
   std::vector<BsdfRec> bsdfs;
   std::vector<LightRec> lights;
   std::vector<float> light_cdf;
+  std::vector<float> light_dist;  // 2-D luminance distributions of textured emitters
   std::vector<MediumRec> media;
   std::vector<float> grid;
   std::vector<TexRec> textures;
@@ -85,7 +86,7 @@ struct rad_scene {
   rad::HostScene host;
   rad::DevScene dev{};
   rad::DeviceBuffer d_nodes, d_qnodes, d_prims, d_shapes, d_pos, d_nrm, d_uv, d_idx, d_bsdfs, d_lights, d_cdf, d_media, d_grid,
-      d_tex, d_texels;
+      d_tex, d_texels, d_ldist;
   rad::DeviceBuffer d_rays, d_hits, d_occ;  // staging for the host-pointer intersect entry points
   std::mutex mu;
   bool bvh_ready = false;

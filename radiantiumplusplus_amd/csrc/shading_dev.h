@@ -3,6 +3,7 @@
 #pragma once
 
 #include "bsdf_dev.h"
+#include "texture_eval.h"
 #include "traverse.h"
 
 namespace rad {
@@ -263,7 +264,128 @@ __device__ __noinline__ LightSample sample_delta_light(const LightRec& L, f3 ref
   return ls;
 }
 
-__device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul, float u1, float u2) {
+// ---- textured emitters (light/area.cpp:20-148, light/skybox.cpp:24-234)
+__device__ inline f3 emitter_radiance(const DevScene& s, const LightRec& L, float u, float v) {
+  return L.tex < 0 ? L.radiance : tex_eval(s, uint32_t(L.tex), TexCoord{u, v, 0.f, 0.f, 0.f, 0.f});
+}
+// lat-long coordinates of a direction in the skybox frame (skybox.cpp:53-63)
+__device__ inline void sky_uv(f3 v, float& u, float& w) {
+  float phi = atan2f(v.x, -v.z);
+  phi = phi < 0.f ? phi + 2.f * kPi : phi;
+  u = phi / (2.f * kPi);
+  w = acosf(fminf(fmaxf(v.y, -1.f), 1.f)) / kPi;
+}
+__device__ inline f3 skybox_radiance(const DevScene& s, const LightRec& L, f3 worldDir) {
+  if (L.tex < 0) return L.radiance;
+  float u, v;
+  sky_uv(L.to_local.vec(worldDir), u, v);
+  return emitter_radiance(s, L, u, v);
+}
+
+// ContinuousDistribution2D::Sample / Pdf (distribution.cpp:85-128) over an
+// emitter's luminance map.  The cell is the one below the first cdf entry
+// >= xi; the reference forms that index in unsigned arithmetic, so xi at or
+// below cdf[0] wraps to the last cell.
+// Also returns the continuous position (index + du) / count.
+__device__ inline uint32_t dist_cell(const float* cdf, uint32_t count, float xi, float& cont) {
+  uint32_t lo = 0, hi = count + 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cdf[mid] < xi)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  const uint32_t i = lo == 0 ? count - 1 : min(lo - 1, count - 1);
+  float du = xi - cdf[i];
+  if ((cdf[i + 1] - cdf[i]) > 0.f) du /= (cdf[i + 1] - cdf[i]);
+  cont = (float(i) + du) / float(count);
+  return i;
+}
+struct DistSample {
+  uint32_t u, v;
+  float pdf;
+  float cu, cv;  // continuous position
+};
+__device__ inline DistSample light_dist_sample(const DevScene& s, const LightRec& L, float xu, float xv) {
+  const uint32_t w = L.dist_w, h = L.dist_h;
+  const float* lum = s.light_dist + L.dist_off;
+  const float* ccdf = lum + size_t(w) * h;
+  const float* cnorm = ccdf + size_t(h) * (w + 1);
+  const float* rsum = cnorm + h;
+  const float* mcdf = rsum + h;
+  DistSample d;
+  d.v = dist_cell(mcdf, h, xv, d.cv);
+  d.u = dist_cell(ccdf + size_t(d.v) * (w + 1), w, xu, d.cu);
+  d.pdf = (lum[size_t(d.v) * w + d.u] * cnorm[d.v]) * (rsum[d.v] * L.mnorm);
+  return d;
+}
+__device__ inline float light_dist_pdf(const DevScene& s, const LightRec& L, float u, float v) {
+  const uint32_t w = L.dist_w, h = L.dist_h;
+  const float fu = u * float(w), fv = v * float(h);
+  const uint32_t iu = fu >= float(w) ? w - 1 : (fu > 0.f ? uint32_t(fu) : 0u);
+  const uint32_t iv = fv >= float(h) ? h - 1 : (fv > 0.f ? uint32_t(fv) : 0u);
+  return s.light_dist[L.dist_off + size_t(iv) * w + iu] / L.msum;
+}
+
+// Textured area light (more than one texel) on its rectangle: sampled at the
+// corner uv of a luminance-distributed texel (area.cpp:56-73).  Its
+// SampleDirection record comes from SurfaceInteraction::ToDsr, whose density
+// is 0 (interaction.cpp:59-75): with quirks on the sample is discarded and the
+// light is reached by BSDF sampling only.  Quirks off: the continuous
+// position inside the texel with the PdfDirection density.
+__device__ __noinline__ LightSample sample_area_textured(const DevScene& s, const LightRec& L, f3 refp, float u1,
+                                                         float u2, bool quirks) {
+  LightSample ls;
+  ls.delta = false;
+  const DistSample d = light_dist_sample(s, L, u1, u2);
+  const ShapeRec& sr = s.shapes[L.shape];
+  const float u = quirks ? float(d.u) / float(L.dist_w) : d.cu, v = quirks ? float(d.v) / float(L.dist_h) : d.cv;
+  ls.p = sr.to_world.point(mk(u * 2.f - 1.f, v * 2.f - 1.f, 0.f));
+  ls.n = sr.frame_n;
+  ls.dir = ls.p - refp;
+  ls.dist = sqrtf(sqnorm(ls.dir));
+  ls.dir = ls.dir / ls.dist;
+  ls.li = dot(-ls.dir, ls.n) >= 0.f ? emitter_radiance(s, L, u, v) : splat(0.f);
+  if (!(d.pdf > 0.f) || quirks) {
+    ls.pdf = 0.f;
+    return ls;
+  }
+  const float pa = light_dist_pdf(s, L, u, v);
+  ls.pdf = pa * sqr(ls.dist) / (norm(cross(sr.frame_s, sr.frame_t)) * -dot(ls.dir, ls.n));
+  return ls;
+}
+
+// Lat-long map of more than one texel (skybox.cpp:92-108, 215-234): the
+// texel's corner, or (quirks off) the continuous position inside it.
+__device__ __noinline__ LightSample sample_skybox_textured(const DevScene& s, const LightRec& L, f3 refp, float u1,
+                                                           float u2, bool quirks) {
+  LightSample ls;
+  ls.delta = false;
+  const float radius = fmaxf(s.world_radius, norm(refp - s.world_center));
+  ls.dist = 2.f * radius;
+  ls.pdf = 0.f;
+  ls.li = splat(0.f);
+  ls.p = refp;
+  ls.n = splat(0.f);
+  ls.dir = mk(0.f, 0.f, 1.f);
+  const DistSample d = light_dist_sample(s, L, u1, u2);
+  if (!(d.pdf > 0.f)) return ls;
+  const float y = quirks ? float(d.v) / float(L.dist_h) : d.cv, x = quirks ? float(d.u) / float(L.dist_w) : d.cu;
+  const float theta = y * kPi, phi = x * 2.f * kPi;
+  const float sinT = sinf(theta);
+  if (sinT == 0.f) return ls;
+  const f3 sph = mk(cosf(phi) * sinT, sinf(phi) * sinT, cosf(theta));  // math_ext.h:108-112
+  const f3 wd = L.to_world.vec(mk(sph.y, sph.z, -sph.x));
+  ls.p = wd * ls.dist + refp;
+  ls.n = -wd;
+  ls.dir = wd;
+  ls.pdf = d.pdf / (2.f * kPi * kPi * sinT);
+  ls.li = emitter_radiance(s, L, x, y);
+  return ls;
+}
+
+__device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul, float u1, float u2, bool quirks) {
   uint32_t index = 0;
   float lpdf = 1.f;
   if (s.light_count >= 2) {
@@ -272,7 +394,9 @@ __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul,
   }
   const LightRec& L = s.lights[index];
   LightSample ls;
-  if (L.type == LIGHT_SKYBOX) {  // skybox.cpp:65-88 (constant map)
+  if (L.type == LIGHT_SKYBOX && L.tex >= 0 && !L.tex_uniform) {
+    ls = sample_skybox_textured(s, L, refp, u1, u2, quirks);
+  } else if (L.type == LIGHT_SKYBOX) {  // skybox.cpp:65-88 (one-texel map)
     float radius = fmaxf(s.world_radius, norm(refp - s.world_center));
     float dist = 2.f * radius;
     float z = fmaf(-2.f, u2, 1.f), r = safe_sqrt(fmaf(-z, z, 1.f)), a = 2.f * kPi * u1;
@@ -284,21 +408,49 @@ __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul,
     ls.dist = dist;
     ls.pdf = 1.f / (4.f * kPi);
     ls.delta = false;
-    ls.li = L.radiance;
+    if (L.tex < 0) {
+      ls.li = L.radiance;
+    } else {
+      float tu, tv;
+      sky_uv(dir, tu, tv);
+      ls.li = emitter_radiance(s, L, tu, tv);
+    }
   } else if (L.type == LIGHT_POINT || L.type == LIGHT_PROJECTION) {
     ls = sample_delta_light(L, refp);
-  } else {  // light/area.cpp:50-75
+  } else if (L.tex >= 0 && !L.tex_uniform) {
+    ls = sample_area_textured(s, L, refp, u1, u2, quirks);
+  } else {  // light/area.cpp:50-75 (a one-texel texture is read at the rectangle's uv = xi)
     ls = sample_area_shape(s, s.shapes[L.shape], refp, u1, u2);
-    ls.li = dot(-ls.dir, ls.n) >= 0.f ? L.radiance : splat(0.f);
+    ls.li = dot(-ls.dir, ls.n) >= 0.f ? emitter_radiance(s, L, u1, u2) : splat(0.f);
   }
   ls.pdf *= lpdf;
   return ls;
 }
 
-// Light::PdfDirection * 1/N for a light hit from `refp` (scene.cpp:98-103).
+// PdfDirection of a textured emitter (area.cpp:77-91, skybox.cpp:110-124).
+__device__ __noinline__ float textured_pdf_dir(const DevScene& s, const LightRec& L, f3 n_shading, f3 dir, float dist,
+                                               float hu, float hv, bool quirks) {
+  if (L.type == LIGHT_SKYBOX) {  // the affine inverse, translation included (Q15), unless quirks are off
+    const f3 v = quirks ? L.to_local.point(dir) : L.to_local.vec(dir);
+    const float theta = acosf(fminf(fmaxf(v.y, -1.f), 1.f));
+    const float sinT = sinf(theta);
+    if (sinT == 0.f) return 0.f;
+    float u, w;
+    sky_uv(v, u, w);
+    return light_dist_pdf(s, L, u, w) / (2.f * kPi * kPi * sinT);
+  }
+  const float pa = light_dist_pdf(s, L, hu, hv);
+  if (pa <= 0.f) return 0.f;
+  const ShapeRec& sr = s.shapes[L.shape];
+  return pa * sqr(dist) / (norm(cross(sr.frame_s, sr.frame_t)) * -dot(dir, n_shading));
+}
+
+// Light::PdfDirection * 1/N for a light hit from `refp` (scene.cpp:98-103);
+// (hu, hv): the hit's texture coordinates on a textured rectangle emitter.
 __device__ inline float light_pdf_dir(const DevScene& s, int32_t light, f3 refp, f3 p, f3 n_shading, f3 dir,
-                                      float dist) {
+                                      float dist, float hu, float hv, bool quirks) {
   const LightRec& L = s.lights[light];
+  if (L.tex >= 0 && !L.tex_uniform) return textured_pdf_dir(s, L, n_shading, dir, dist, hu, hv, quirks) * s.light_pdf;
   if (L.type == LIGHT_SKYBOX) return (1.f / (4.f * kPi)) * s.light_pdf;
   if (!(dot(-dir, n_shading) >= 0.f)) return 0.f;  // area.cpp:77-80
   const ShapeRec& sr = s.shapes[L.shape];

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
           float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
           uint32_t sdim = dim;
           dim += 3;
-          LightSample ls = sample_light(scene, mei.p, ul, u1, u2);
+          LightSample ls = sample_light(scene, mei.p, ul, u1, u2, quirks);
           if (ls.pdf > 0.f) {
             float ph = phase_eval(M, wi, ls.dir);
             float wgt = ls.delta ? 1.f : mis_weight(ls.pdf, ph);
@@ -154,8 +154,8 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
           if (geom == 0xffffffffu) {  // escaped: environment emission, then stop
             if (scene.env_light >= 0) {
               float wgt = 1.f;
-              if (!spec) wgt = mis_weight(prevPdf, light_pdf_dir(scene, scene.env_light, prevP, rd, rd, rd, 0.f));
-              L += mul(T, scene.lights[scene.env_light].radiance) * wgt;
+              if (!spec) wgt = mis_weight(prevPdf, light_pdf_dir(scene, scene.env_light, prevP, rd, rd, rd, 0.f, 0.f, 0.f, quirks));
+              L += mul(T, skybox_radiance(scene, scene.lights[scene.env_light], rd)) * wgt;
             }
           } else {
             SurfaceHit si = surface_hit(scene, ro, rd, hh.x, hh.y, hh.z, __float_as_uint(hh.w), geom);
@@ -166,9 +166,11 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
                 f3 rel = si.p - prevP;
                 float dist = norm(rel);
                 f3 dir = rel / dist;
-                wgt = mis_weight(prevPdf, light_pdf_dir(scene, sr.light, prevP, si.p, si.sh.n, dir, dist));
+                wgt = mis_weight(prevPdf, light_pdf_dir(scene, sr.light, prevP, si.p, si.sh.n, dir, dist, hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f, quirks));
               }
-              f3 le = dot(si.sh.to_world(si.wi), si.n) >= 0.f ? scene.lights[sr.light].radiance : splat(0.f);
+              f3 le = dot(si.sh.to_world(si.wi), si.n) >= 0.f
+                    ? emitter_radiance(scene, scene.lights[sr.light], hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f)
+                    : splat(0.f);
               L += mul(T, le) * wgt;
             } else if (sr.bsdf < 0) {  // medium boundary without a BSDF: pass through (vol_path.cpp:171-176)
               ro = offset_p(si.p, si.n, rd);
@@ -192,7 +194,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
                 float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
                 uint32_t sdim = dim;
                 dim += 3;
-                LightSample ls = sample_light(scene, si.p, ul, u1, u2);
+                LightSample ls = sample_light(scene, si.p, ul, u1, u2, quirks);
                 if (ls.pdf > 0.f) {
                   f3 wo = si.sh.to_local(ls.dir);
                   float bp;

@@ -601,17 +601,21 @@ class SceneBuilder:
             d.radius = s.radius
         tex_list = []
         bsdfs, bsdf_count = self.bsdf_array(tex_list)
-        textures = (F.TextureDesc * max(len(tex_list), 1))()
-        for i, td in enumerate(tex_list):
-            textures[i] = td
         lights = (F.LightDesc * max(len(self.lights), 1))()
         for i, l in enumerate(self.lights):
             lights[i].type = {"area": F.RAD_LIGHT_AREA, "skybox": F.RAD_LIGHT_SKYBOX, "point": F.RAD_LIGHT_POINT,
                               "projection": F.RAD_LIGHT_PROJECTION}[l["type"]]
             lights[i].fov = l.get("fov", 90.0)
             lights[i].shape = l.get("shape", -1)
-            lights[i].radiance[:] = l["radiance"]
+            if isinstance(l["radiance"], dict):  # a radiance texture (area / skybox)
+                lights[i].tex = self._texture(l["radiance"], True, tex_list)
+                lights[i].radiance[:] = [1.0, 1.0, 1.0]
+            else:
+                lights[i].radiance[:] = l["radiance"]
             lights[i].to_world[:] = [float(x) for x in np.asarray(l.get("to_world", IDENTITY), np.float32).reshape(-1)]
+        textures = (F.TextureDesc * max(len(tex_list), 1))()
+        for i, td in enumerate(tex_list):
+            textures[i] = td
         media = (F.MediumDesc * max(len(self.media), 1))()
         for i, m in enumerate(self.media):
             md = media[i]
@@ -687,14 +691,15 @@ class SceneBuilder:
                 e["bsdf"] = self._json_value(self.bsdfs[s.bsdf], directory, assets)
             if s.light >= 0:
                 l = self.lights[s.light]
-                e["light"] = {"type": "area", "radiance": l["radiance"]}
+                e["light"] = {"type": "area", "radiance": self._json_value(l["radiance"], directory, assets)}
             for key, idx in (("in_medium", s.medium_inside), ("out_medium", s.medium_outside)):
                 if idx >= 0:
                     e[key] = self._medium_json(idx, directory, assets)
             entities.append(e)
         for l in self.lights:
             if l["type"] == "skybox":
-                entities.append({"to_world": fmt(l["to_world"]), "light": {"type": "skybox", "radiance": l["radiance"]}})
+                entities.append({"to_world": fmt(l["to_world"]),
+                                 "light": {"type": "skybox", "radiance": self._json_value(l["radiance"], directory, assets)}})
             elif l["type"] == "point":
                 entities.append({"light": {"type": "point", "position": l["position"], "intensity": l["radiance"]}})
             elif l["type"] == "projection":

@@ -201,3 +201,71 @@ def test_json_textured_scene_end_to_end(hip_ctx, tmp_path):
         assert np.array_equal(fb1, fb2)
     finally:
         hip.lib().rad_loaded_scene_free(h)
+
+
+def emitter_scene(integrator="path"):
+    """A textured rectangle emitter (8 x 4 radiance map) and a rotated,
+    translated 16 x 8 environment map over a diffuse floor, a rough-metal
+    sphere and a textured diffuse sphere."""
+    rng = np.random.default_rng(21)
+    b = scenes.SceneBuilder()
+    b.rectangle(to_world=scenes.trs(axis=(1, 0, 0), angle=-90, scale=(5, 5, 5)), bsdf=b.diffuse(0.6))
+    b.sphere(center=(-1.0, 1, 0), radius=1, bsdf=b.rough_metal(alpha=0.3))
+    b.sphere(center=(1.4, 0.7, -0.4), radius=0.7, bsdf=b.diffuse(b.bitmap(rng.uniform(0.2, 0.9, (8, 8, 3)))))
+    light = b.rectangle(to_world=scenes.trs(translate=(0, 4, -1), axis=(1, 0, 0), angle=80, scale=(1.5, 1, 1)))
+    b.area_light(light, b.bitmap(rng.uniform(0, 12, (4, 8, 3)), filter="nearest"))
+    env = rng.uniform(0, 1.5, (8, 16, 3)).astype(np.float32)
+    env[2:4, 3:6] = 20.0  # a bright region the distribution has to find
+    sky_tw = scenes.trs(translate=(0.5, 0.2, -0.3), axis=(0, 1, 0), angle=35)
+    b.skybox(b.bitmap(env, filter="bilinear"), to_world=sky_tw)
+    b.set_camera(64, 40, 60, origin=(0, 2.5, -7), target=(0, 0.8, 0), far=1000)
+    b.set_render(integrator, spp=16, max_depth=6, rr_depth=3, seed=4)
+    return b
+
+
+@pytest.mark.parametrize("integrator,fix", [("path", False), ("path", True), ("vol_path", False)])
+def test_textured_emitters_match_oracle(hip_ctx, integrator, fix):
+    b = emitter_scene(integrator)
+    s = hip.Scene.from_builder(hip_ctx, b)
+    flags = F.RAD_RENDER_WANT_SUMSQ | (F.RAD_RENDER_FIX_QUIRKS if fix else 0)
+    g, gsq, _ = s.render(b.render_desc(spp=4096, seed=1, flags=flags), want_sumsq=True)
+    of, osq, _ = O.OracleScene(b, quirks=not fix).render(spp=1024, seed=99)
+    z, rel = stats.compare(g, gsq, 4096, of, osq, 1024, tile=8)
+    print(f"max |z| {np.abs(z).max():.2f}  relMSE {rel:.2e}")
+    assert np.isfinite(g).all()
+    assert np.abs(z).max() < 4.0
+    assert rel < 2e-3
+
+
+def test_textured_environment_furnace_gpu(hip_ctx):
+    """Constant 8 x 4 map with quirks off: the sphere shows its albedo."""
+    import test_texture_oracle as T
+    b = T.env_furnace("bitmap")
+    s = hip.Scene.from_builder(hip_ctx, b)
+    fb = s.render(b.render_desc(spp=1024, seed=3, flags=F.RAD_RENDER_FIX_QUIRKS))[0]
+    assert np.allclose(fb[0, 0], 1.0, rtol=1e-6)
+    assert np.allclose(fb[6:10, 6:10].mean(axis=(0, 1)), 0.6, atol=0.01)
+
+
+def test_json_textured_emitters_end_to_end(hip_ctx, tmp_path):
+    import ctypes as C
+
+    rng = np.random.default_rng(22)
+    b = scenes.SceneBuilder()
+    b.rectangle(to_world=scenes.trs(axis=(1, 0, 0), angle=-90, scale=(5, 5, 5)), bsdf=b.diffuse(0.6))
+    light = b.rectangle(to_world=scenes.trs(translate=(0, 4, -1), axis=(1, 0, 0), angle=80))
+    b.area_light(light, b.bitmap(rng.integers(0, 256, (4, 8, 3)).astype(np.float32) / np.float32(255)))
+    b.skybox(b.bitmap(rng.integers(0, 256, (8, 16, 3)).astype(np.float32) / np.float32(255)))
+    b.set_camera(32, 24, 60, origin=(0, 2.5, -7), target=(0, 0.8, 0), far=1000)
+    b.set_render("path", spp=8, max_depth=5, rr_depth=3, seed=4)
+    path = b.write_json(str(tmp_path))
+    h = C.c_void_p()
+    hip.check(hip.lib().rad_scene_load_json(path.encode(), C.byref(h)))
+    try:
+        d = hip.lib().rad_loaded_scene_desc(h).contents
+        rd = hip.lib().rad_loaded_scene_render(h).contents
+        fb1 = hip.Scene(hip_ctx, d).render(rd)[0]
+        fb2 = hip.Scene.from_builder(hip_ctx, b).render(b.render_desc())[0]
+        assert np.array_equal(fb1, fb2)
+    finally:
+        hip.lib().rad_loaded_scene_free(h)

@@ -499,10 +499,18 @@ def test_texture_nodes_and_defaults(tmp_path):
             _load_images(tmp_path, {}, assets, [bad])
         assert e.value.status == code
     sc = dict(BASE)
-    sc["scene"] = [{"shape": {"type": "rectangle"}, "light": {"type": "area", "radiance": {"type": "chessboard"}}}]
-    with pytest.raises(hip.RadError) as e:
+    sc["scene"] = [{"shape": {"type": "rectangle"}, "light": {"type": "area", "radiance": {"type": "chessboard"}}},
+                   {"light": {"type": "projection", "irradiance": {"type": "chessboard"}}}]
+    with pytest.raises(hip.RadError) as e:  # a projection light's image is not carried
         load(_write_scene(tmp_path, sc))
     assert e.value.status == 4
+    sc["scene"] = sc["scene"][:1]
+    h = load(_write_scene(tmp_path, sc))
+    try:
+        d = desc_of(h)
+        assert d.lights[0].tex == 1 and d.textures[0].type == F.RAD_TEXTURE_CHESSBOARD
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
 
 
 def test_textured_scene_round_trip_renders_identically(tmp_path):
