@@ -160,12 +160,14 @@ enum {
 typedef struct rad_light_desc {
   int32_t type;            /* RAD_LIGHT_* */
   int32_t shape;           /* area: emitting shape index (set by the loader) */
-  float radiance[3];       /* area / skybox radiance; point intensity; projection irradiance x scale */
+  float radiance[3];       /* area / skybox radiance; point intensity; projection irradiance x scale,
+                              or only the scale when the irradiance is a texture */
   float to_world[16];      /* skybox orientation; point: world position in the translation column;
                               projection: the light's frame (looks along +z) */
   float fov;               /* projection field of view in degrees (default 90) */
-  int32_t tex;             /* area / skybox: 1 + index of an RGB texture for the radiance, 0 = the constant
-                              (light/area.cpp:20-41, light/skybox.cpp:24-49) */
+  int32_t tex;             /* area / skybox / projection: 1 + index of an RGB texture for the radiance
+                              (irradiance), 0 = the constant (light/area.cpp:20-41, light/skybox.cpp:24-49,
+                              light/projection.cpp:21-26; the projection's aspect is the texture's) */
 } rad_light_desc;
 
 enum { RAD_MEDIUM_HOMOGENEOUS = 0, RAD_MEDIUM_HETEROGENEOUS = 1 };

@@ -578,13 +578,16 @@ struct PointLight final : Light {  // light/point.cpp:10-45
   Float PdfDirection(const Interaction&, const DirectionSampleResult&) const override { return 0; }
 };
 
-// light/projection.cpp:12-63 with a constant irradiance (aspect 1)
+// light/projection.cpp:12-63.  The irradiance is a constant (stored already
+// multiplied by the scale) or a texture; a texture's width / height sets the
+// aspect of the projected image (a chessboard counts as 1 x 1).
 struct ProjectionLight final : Light {
   Transform toWorld, toClip;
-  Spectrum irradiance;  // irradiance x scale
-  void Init(const float* tw, Float fov) {
+  Spectrum irradiance;  // constant: irradiance x scale; textured: the scale
+  const Texture* tex = nullptr;
+  void Init(const float* tw, Float fov, uint32_t texW, uint32_t texH) {
     toWorld = Transform(Mat4::FromRowMajor(tw));
-    Float aspect = 1, farC = 10000, nearC = Float(0.0001);
+    Float aspect = Float(texW) / Float(texH), farC = 10000, nearC = Float(0.0001);
     Float recip = 1 / (farC - nearC);
     Float cot = 1 / std::tan(Radian(fov / 2));
     Float P[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, farC * recip, -nearC * farC * recip, 0, 0, 1, 0};
@@ -608,7 +611,9 @@ struct ProjectionLight final : Light {
     Float sq = Dot(dsr.Dir, dsr.Dir);
     dsr.Dist = std::sqrt(sq);
     dsr.Dir = dsr.Dir * Rcp(dsr.Dist);
-    return {dsr, irradiance * PI / (Sqr(local.z) * Dot(-dsr.Dir, dsr.N))};
+    // SurfaceInteraction(dsr) carries the uv and no differentials (interaction.cpp:32-43)
+    Spectrum irr = tex ? Mul(tex->Eval(dsr.UV, V2(), V2()), irradiance) : irradiance;
+    return {dsr, irr * PI / (Sqr(local.z) * Dot(-dsr.Dir, dsr.N))};
   }
   Float PdfDirection(const Interaction&, const DirectionSampleResult&) const override { return 0; }
 };
@@ -734,7 +739,8 @@ void Scene::Create(const rad_scene_desc& desc, bool q) {
     } else if (ld.type == RAD_LIGHT_PROJECTION) {
       auto l = std::make_unique<ProjectionLight>();
       l->irradiance = Spectrum(ld.radiance[0], ld.radiance[1], ld.radiance[2]);
-      l->Init(ld.to_world, ld.fov);
+      l->tex = texOf(ld.tex);
+      l->Init(ld.to_world, ld.fov, tw, th);
       lights.push_back(std::move(l));
     } else {
       auto l = std::make_unique<SkyboxLight>();

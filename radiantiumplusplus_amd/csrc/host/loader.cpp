@@ -535,8 +535,7 @@ struct Builder {
     Vec<3>(*v, out3, key);
     return 0;
   }
-  // Point intensity and projection irradiance: a projection light's image
-  // texture is not carried by this build.
+  // Point intensity and a constant projection irradiance.
   void LightColor(const Value& node, const char* key, float* out3, float def) {
     const Value* v = node.find(key);
     out3[0] = out3[1] = out3[2] = def;
@@ -836,9 +835,15 @@ struct Builder {
         } else if (lt == "projection") {  // projection.cpp:14-36
           l.type = RAD_LIGHT_PROJECTION;
           float irr[3], sc[3];
-          LightColor(*lightNode, "irradiance", irr, 1.f);
           VecOr<3>(*lightNode, "scale", sc, {1, 1, 1});
-          for (int c = 0; c < 3; c++) l.radiance[c] = irr[c] * sc[c];
+          const Value* iv = lightNode->find("irradiance");
+          if (iv && iv->is_object()) {  // an irradiance image: the device multiplies texels by the scale
+            l.tex = MakeTexture(*iv, 3);
+            for (int c = 0; c < 3; c++) l.radiance[c] = sc[c];
+          } else {
+            LightColor(*lightNode, "irradiance", irr, 1.f);
+            for (int c = 0; c < 3; c++) l.radiance[c] = irr[c] * sc[c];
+          }
           l.fov = FloatOr(*lightNode, "fov", 90.f);
         } else {
           throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + lt);

@@ -499,15 +499,18 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
     r.radiance = mk(l.radiance[0], l.radiance[1], l.radiance[2]);
     r.to_world = M4::From(l.to_world).ToAffine();
     r.tex = -1;
-    if (l.tex != 0) {  // light/area.cpp:20-41, light/skybox.cpp:24-49
-      if (l.type != RAD_LIGHT_AREA && l.type != RAD_LIGHT_SKYBOX) throw std::runtime_error("only area and skybox radiance may be textured");
+    uint32_t texW = 1, texH = 1;  // Texture::Width / Height (texture.h:30-31); a chessboard is 1 x 1
+    if (l.tex != 0) {  // light/area.cpp:20-41, light/skybox.cpp:24-49, light/projection.cpp:21-26
+      if (l.type == RAD_LIGHT_POINT) throw std::runtime_error("a point light's intensity cannot be textured");
       if (l.tex < 0 || uint32_t(l.tex) > out.textures.size()) throw std::runtime_error("light: texture index out of range");
       const TexRec& t = out.textures[l.tex - 1];
       if (t.channels != 3) throw std::runtime_error("light: radiance texture must have 3 channels");
       r.tex = l.tex - 1;
       const uint32_t w = t.type == TEX_CHESSBOARD ? 1 : t.w[0], hgt = t.type == TEX_CHESSBOARD ? 1 : t.h[0];
+      texW = w;
+      texH = hgt;
       r.tex_uniform = w == 1 && hgt == 1;
-      if (!r.tex_uniform) {
+      if (!r.tex_uniform && l.type != RAD_LIGHT_PROJECTION) {
         std::vector<float> lum(size_t(w) * hgt);
         for (uint32_t y = 0; y < hgt; y++) {
           const float sinTheta = l.type == RAD_LIGHT_SKYBOX ? std::sin((float(y) + 0.5f) / float(hgt) * kPi) : 1.f;
@@ -522,10 +525,10 @@ void AssembleScene(const rad_scene_desc& desc, HostScene& out) {
       }
     }
     if (l.type == RAD_LIGHT_SKYBOX) r.to_local = M4::From(l.to_world).Inverse().ToAffine();
-    if (l.type == RAD_LIGHT_PROJECTION) {  // projection.cpp:14-36 with a constant irradiance (aspect 1)
+    if (l.type == RAD_LIGHT_PROJECTION) {  // projection.cpp:18-45; the aspect is the irradiance image's
       M4 tw = M4::From(l.to_world);
       r.to_local = tw.Inverse().ToAffine();
-      const float aspect = 1.f, farC = 10000.f, nearC = 0.0001f;
+      const float aspect = float(texW) / float(texH), farC = 10000.f, nearC = 0.0001f;
       const float recip = 1.f / (farC - nearC);
       const float cot = 1.f / std::tan(l.fov * 0.5f * (3.14159265358979323846f / 180.f));
       const float P[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, farC * recip, -nearC * farC * recip, 0, 0, 1, 0};

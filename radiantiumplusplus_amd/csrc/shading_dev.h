@@ -223,10 +223,12 @@ __device__ inline LightSample sample_area_shape(const DevScene& s, const ShapeRe
 }
 
 // Scene::SampleLightDirection (scene.cpp:87-96)
-// Point and projection lights (light/point.cpp:24-38, projection.cpp:40-60
-// with a constant irradiance), kept out of line: rare, and inlined they
-// raise the register count of every shading kernel.
-__device__ __noinline__ LightSample sample_delta_light(const LightRec& L, f3 refp) {
+// Point and projection lights (light/point.cpp:24-38, projection.cpp:40-60),
+// kept out of line: rare, and inlined they raise the register count of every
+// shading kernel.  A textured projection reads its irradiance image at the
+// projected uv with no differentials (interaction.cpp:32-43); `radiance` then
+// holds the scale.
+__device__ __noinline__ LightSample sample_delta_light(const DevScene& s, const LightRec& L, f3 refp) {
   LightSample ls;
   ls.delta = true;
   ls.pdf = 1.f;
@@ -260,7 +262,8 @@ __device__ __noinline__ LightSample sample_delta_light(const LightRec& L, f3 ref
   ls.n = L.axis;
   ls.dist = sqrtf(dot(d, d));
   ls.dir = d * (1.f / ls.dist);
-  ls.li = L.radiance * kPi / (sqr(local.z) * dot(-ls.dir, ls.n));
+  const f3 irr = L.tex < 0 ? L.radiance : mul(tex_eval(s, uint32_t(L.tex), TexCoord{u, v, 0.f, 0.f, 0.f, 0.f}), L.radiance);
+  ls.li = irr * kPi / (sqr(local.z) * dot(-ls.dir, ls.n));
   return ls;
 }
 
@@ -416,7 +419,7 @@ __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul,
       ls.li = emitter_radiance(s, L, tu, tv);
     }
   } else if (L.type == LIGHT_POINT || L.type == LIGHT_PROJECTION) {
-    ls = sample_delta_light(L, refp);
+    ls = sample_delta_light(s, L, refp);
   } else if (L.tex >= 0 && !L.tex_uniform) {
     ls = sample_area_textured(s, L, refp, u1, u2, quirks);
   } else {  // light/area.cpp:50-75 (a one-texel texture is read at the rectangle's uv = xi)

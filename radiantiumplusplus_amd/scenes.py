@@ -407,12 +407,17 @@ class SceneBuilder:
         return len(self.lights) - 1
 
     def projection_light(self, to_world, irradiance=1.0, scale=1.0, fov=90.0):
-        """`projection` light (light/projection.cpp) with a constant irradiance,
-        placed by to_world and projecting along its +z."""
-        irr, sc = _f3(irradiance), _f3(scale)
-        self.lights.append({"type": "projection", "radiance": [float(np.float32(a) * np.float32(b)) for a, b in zip(irr, sc)],
-                            "irradiance": irr, "scale": sc, "fov": float(fov), "shape": -1,
-                            "to_world": np.asarray(to_world, np.float32)})
+        """`projection` light (light/projection.cpp) placed by to_world and
+        projecting along its +z.  `irradiance` is a constant or an RGB texture
+        (`bitmap` / `chessboard` dict), whose width / height sets the aspect."""
+        sc = _f3(scale)
+        if isinstance(irradiance, dict):
+            irr, rad = irradiance, irradiance
+        else:
+            irr = _f3(irradiance)
+            rad = [float(np.float32(a) * np.float32(b)) for a, b in zip(irr, sc)]
+        self.lights.append({"type": "projection", "radiance": rad, "irradiance": irr, "scale": sc,
+                            "fov": float(fov), "shape": -1, "to_world": np.asarray(to_world, np.float32)})
         return len(self.lights) - 1
 
     def skybox(self, radiance, to_world=IDENTITY):
@@ -607,9 +612,9 @@ class SceneBuilder:
                               "projection": F.RAD_LIGHT_PROJECTION}[l["type"]]
             lights[i].fov = l.get("fov", 90.0)
             lights[i].shape = l.get("shape", -1)
-            if isinstance(l["radiance"], dict):  # a radiance texture (area / skybox)
+            if isinstance(l["radiance"], dict):  # a radiance (irradiance) texture; a projection keeps its scale
                 lights[i].tex = self._texture(l["radiance"], True, tex_list)
-                lights[i].radiance[:] = [1.0, 1.0, 1.0]
+                lights[i].radiance[:] = l["scale"] if l["type"] == "projection" else [1.0, 1.0, 1.0]
             else:
                 lights[i].radiance[:] = l["radiance"]
             lights[i].to_world[:] = [float(x) for x in np.asarray(l.get("to_world", IDENTITY), np.float32).reshape(-1)]
@@ -704,7 +709,9 @@ class SceneBuilder:
                 entities.append({"light": {"type": "point", "position": l["position"], "intensity": l["radiance"]}})
             elif l["type"] == "projection":
                 entities.append({"to_world": fmt(l["to_world"]),
-                                 "light": {"type": "projection", "irradiance": l["irradiance"], "scale": l["scale"],
+                                 "light": {"type": "projection",
+                                           "irradiance": self._json_value(l["irradiance"], directory, assets),
+                                           "scale": l["scale"],
                                            "fov": l["fov"]}})
         cam = self.camera
         camj = {"type": "perspective", "fov": cam["fov"], "near": cam["near"], "far": cam["far"],

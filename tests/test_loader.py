@@ -500,15 +500,18 @@ def test_texture_nodes_and_defaults(tmp_path):
         assert e.value.status == code
     sc = dict(BASE)
     sc["scene"] = [{"shape": {"type": "rectangle"}, "light": {"type": "area", "radiance": {"type": "chessboard"}}},
-                   {"light": {"type": "projection", "irradiance": {"type": "chessboard"}}}]
-    with pytest.raises(hip.RadError) as e:  # a projection light's image is not carried
+                   {"light": {"type": "projection", "irradiance": {"type": "chessboard"}, "scale": [1, 2, 3]}},
+                   {"light": {"type": "point", "intensity": {"type": "chessboard"}}}]
+    with pytest.raises(hip.RadError) as e:  # a point light's intensity is a colour only (point.cpp:24)
         load(_write_scene(tmp_path, sc))
     assert e.value.status == 4
-    sc["scene"] = sc["scene"][:1]
+    sc["scene"] = sc["scene"][:2]
     h = load(_write_scene(tmp_path, sc))
     try:
         d = desc_of(h)
         assert d.lights[0].tex == 1 and d.textures[0].type == F.RAD_TEXTURE_CHESSBOARD
+        # a projection's irradiance image: the texture, with the scale in radiance
+        assert d.lights[1].tex == 2 and list(d.lights[1].radiance) == [1, 2, 3]
     finally:
         hip.lib().rad_loaded_scene_free(h)
 

@@ -486,3 +486,53 @@ def test_projection_light_closed_form():
     expect = 0.5 * np.array([2.0, 1.0, 0.5]) * 1.5 / 9
     assert np.allclose(fb[inside], expect, rtol=1e-3)
     assert np.all(fb[outside] == 0)
+
+
+def projector_plane(irradiance, res=64):
+    """The plane of light_plane_scene at res x res, lit only by a projection
+    light 3 above it looking down (+z -> -y, light-frame y -> world z), fov 40,
+    scale 1.5."""
+    tw = np.array([[1, 0, 0, 0], [0, 0, -1, 3], [0, 1, 0, 0], [0, 0, 0, 1]], np.float32)
+    b = light_plane_scene(lambda b: b.projection_light(tw, irradiance=irradiance, scale=1.5, fov=40.0))
+    b.set_camera(res, res, 60, origin=(0, 5, 0.001), target=(0, 0, 0), up=(0, 0, 1), far=1000)
+    o = O.OracleScene(b)
+    fb = o.render(spp=16, seed=3)[0]
+    pts = np.zeros((res, res, 3))
+    for y in range(res):
+        for x in range(res):
+            r = o.camera_ray(x + 0.5, y + 0.5)
+            pts[y, x] = r["o"] - r["o"][1] / r["d"][1] * r["d"]
+    # image-plane coordinates of the plane points (projection.cpp:27-36):
+    # u = 1/2 - cot x / (2 z), v = 1/2 - aspect cot y / (2 z), light frame x = world x, y = world z, z = 3
+    return fb, pts, 5.78 / res  # pixel footprint on the plane
+
+
+def test_textured_projection_closed_form():
+    """A textured irradiance image (light/projection.cpp:21-26, 50-60): the
+    image's width / height sets the aspect, so a constant 4 x 2 bitmap lights
+    a frustum half as tall as it is wide with L = rho E s / z^2; a chessboard
+    (1 x 1, aspect 1) lights the cells its (u, v) falls in."""
+    cot = 1 / math.tan(math.radians(20))
+    E = np.array([0.6, 0.4, 0.8], np.float32)  # multiples of 1/5: exact through any filter weights
+    fb, p, px = projector_plane(scenes.SceneBuilder.bitmap(np.tile(E, (2, 4, 1)), filter="bilinear"))
+    half = 3 / cot
+    inside = (np.abs(p[..., 0]) < half - 2 * px) & (np.abs(p[..., 2]) < half / 2 - 2 * px)
+    outside = (np.abs(p[..., 0]) > half + 2 * px) | (np.abs(p[..., 2]) > half / 2 + 2 * px)
+    assert inside.sum() > 100 and (outside & (np.abs(p[..., 2]) < half)).sum() > 200
+    assert np.allclose(fb[inside], 0.5 * E * 1.5 / 9, rtol=1e-3)
+    assert np.all(fb[outside] == 0)
+
+    A, B = np.array([2.0, 1.0, 0.5]), np.array([0.25, 0.5, 1.0])
+    to_uv = np.diag([2.0, 2.0, 1.0, 1.0]).astype(np.float32)
+    fb, p, px = projector_plane(scenes.SceneBuilder.chessboard(tuple(A), tuple(B), to_uv=to_uv), res=128)
+    u = 0.5 - cot * p[..., 0] / 6
+    v = 0.5 - cot * p[..., 2] / 6
+    tu, tv = 2 * u, 2 * v
+    edge = lambda t: np.abs(t * 2 - np.rint(t * 2)) / 2  # distance to the nearest multiple of 1/2
+    margin = 1.5 * px * 2 * cot / 6  # 1.5 pixels in (tu, tv) units
+    clear = (u > 0) & (u < 1) & (v > 0) & (v < 1) & (edge(tu) > margin) & (edge(tv) > margin)
+    assert clear.sum() > 500
+    right, up = (tu - np.floor(tu)) > 0.5, (tv - np.floor(tv)) > 0.5
+    expect = np.where((right == up)[..., None], A, B) * 0.5 * 1.5 / 9
+    assert np.allclose(fb[clear], expect[clear], rtol=1e-3)
+    assert ((right == up) & clear).sum() > 150 and ((right != up) & clear).sum() > 150
