@@ -304,5 +304,205 @@ ImageF ReadHdr(const std::string& path, bool flipY) {
   return out;
 }
 
+namespace {
+
+uint32_t LE32(const uint8_t* p) { return uint32_t(p[0]) | (uint32_t(p[1]) << 8) | (uint32_t(p[2]) << 16) | (uint32_t(p[3]) << 24); }
+uint64_t LE64(const uint8_t* p) { return uint64_t(LE32(p)) | (uint64_t(LE32(p + 4)) << 32); }
+
+// IEEE 754 binary16 -> binary32 (exact).
+float HalfToFloat(uint16_t h) {
+  const uint32_t s = uint32_t(h >> 15) << 31, e = (h >> 10) & 31u, m = h & 1023u;
+  uint32_t bits;
+  if (e == 0) {
+    const float v = std::ldexp(float(m), -24);  // zero or subnormal
+    return s ? -v : v;
+  }
+  bits = e == 31 ? (s | 0x7f800000u | (m << 13)) : (s | ((e + 112u) << 23) | (m << 13));
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+// binary32 -> binary16, round to nearest, ties to even.
+uint16_t FloatToHalf(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint16_t s = uint16_t((x >> 16) & 0x8000u);
+  const uint32_t ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return uint16_t(s | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));  // inf / nan
+  if (ax >= 0x477ff000u) return uint16_t(s | 0x7c00u);  // >= 65520 rounds to inf
+  if (ax < 0x38800000u) {  // below 2^-14: a multiple of 2^-24, exact after scaling
+    float a;
+    std::memcpy(&a, &ax, 4);
+    return uint16_t(s | uint16_t(std::nearbyint(a * 16777216.0f)));
+  }
+  uint32_t h = (((ax >> 23) - 112u) << 10) | ((ax & 0x7fffffu) >> 13);
+  const uint32_t rem = ax & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return uint16_t(s | h);
+}
+
+// OpenEXR's ZIP / RLE post-processing: a byte-wise delta predictor over data
+// whose even and odd bytes were split into two halves.
+void ExrUnpredict(std::vector<uint8_t>& t, std::vector<uint8_t>& out) {
+  for (size_t i = 1; i < t.size(); i++) t[i] = uint8_t(t[i - 1] + t[i] - 128);
+  out.resize(t.size());
+  const size_t half = (t.size() + 1) / 2;
+  for (size_t k = 0; k < t.size(); k++) out[k] = (k & 1) ? t[half + k / 2] : t[k / 2];
+}
+
+}  // namespace
+
+ImageF ReadExr(const std::string& path) {
+  const std::vector<uint8_t> f = ReadFile(path);
+  if (f.size() < 8 || LE32(f.data()) != 20000630u) Bad(path, "not an OpenEXR file");
+  const uint32_t version = LE32(f.data() + 4);
+  if ((version & 0xffu) != 2) Bad(path, "unknown OpenEXR version");
+  if (version & 0x200u) throw ImageUnsupported("image " + path + ": tiled OpenEXR files are not supported");
+  if (version & 0x1800u) throw ImageUnsupported("image " + path + ": deep or multi-part OpenEXR files are not supported");
+  size_t pos = 8;
+  auto cstr = [&]() {
+    std::string r;
+    while (pos < f.size() && f[pos]) r.push_back(char(f[pos++]));
+    if (pos >= f.size()) Bad(path, "truncated header");
+    pos++;
+    return r;
+  };
+  struct Chan {
+    std::string name;
+    int32_t type;
+  };
+  std::vector<Chan> chans;
+  int32_t comp = -1, x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+  bool haveWindow = false;
+  for (;;) {
+    const std::string name = cstr();
+    if (name.empty()) break;
+    const std::string type = cstr();
+    if (pos + 4 > f.size()) Bad(path, "truncated header");
+    const uint32_t size = LE32(f.data() + pos);
+    pos += 4;
+    if (size > f.size() - pos) Bad(path, "truncated header");
+    const uint8_t* v = f.data() + pos;
+    if (name == "channels" && type == "chlist") {
+      size_t q = 0;
+      while (q < size && v[q]) {
+        std::string cn;
+        while (q < size && v[q]) cn.push_back(char(v[q++]));
+        q++;
+        if (q + 16 > size) Bad(path, "bad channel list");
+        const int32_t pt = int32_t(LE32(v + q)), xs = int32_t(LE32(v + q + 8)), ys = int32_t(LE32(v + q + 12));
+        if (pt < 0 || pt > 2) Bad(path, "bad pixel type");
+        if (xs != 1 || ys != 1) throw ImageUnsupported("image " + path + ": subsampled OpenEXR channels are not supported");
+        chans.push_back({cn, pt});
+        q += 16;
+      }
+    } else if (name == "compression" && size >= 1) {
+      comp = v[0];
+    } else if (name == "dataWindow" && size == 16) {
+      x0 = int32_t(LE32(v)), y0 = int32_t(LE32(v + 4)), x1 = int32_t(LE32(v + 8)), y1 = int32_t(LE32(v + 12));
+      haveWindow = true;
+    } else if (name == "tiles") {
+      throw ImageUnsupported("image " + path + ": tiled OpenEXR files are not supported");
+    }
+    pos += size;
+  }
+  if (chans.empty() || comp < 0 || !haveWindow || x1 < x0 || y1 < y0) Bad(path, "missing channels, compression or data window");
+  const int64_t W = int64_t(x1) - x0 + 1, H = int64_t(y1) - y0 + 1;
+  if (W > (1 << 20) || H > (1 << 20) || W * H > (int64_t(1) << 28)) Bad(path, "image too large");
+  int32_t linesPerBlock;
+  switch (comp) {
+    case 0: case 1: case 2: linesPerBlock = 1; break;  // NONE, RLE, ZIPS
+    case 3: linesPerBlock = 16; break;                  // ZIP
+    default: throw ImageUnsupported("image " + path + ": OpenEXR compression " + std::to_string(comp) + " is not supported");
+  }
+  // R / G / B by the lower-cased suffix after the last '.', first occurrence wins
+  int rgbChan[3] = {-1, -1, -1};
+  for (size_t c = 0; c < chans.size(); c++) {
+    std::string n = chans[c].name;
+    const size_t dot = n.rfind('.');
+    if (dot != std::string::npos) n = n.substr(dot + 1);
+    for (char& ch : n) ch = char(std::tolower(static_cast<unsigned char>(ch)));
+    const int k = n == "r" ? 0 : n == "g" ? 1 : n == "b" ? 2 : -1;
+    if (k >= 0 && rgbChan[k] < 0) rgbChan[k] = int(c);
+  }
+  const int32_t pixelType = chans[0].type;  // the frame buffer's type for every slice
+  if (pixelType == 0) throw ImageUnsupported("image " + path + ": UINT OpenEXR pixels are not supported");
+  for (int k = 0; k < 3; k++)
+    if (rgbChan[k] >= 0 && chans[size_t(rgbChan[k])].type == 0)
+      throw ImageUnsupported("image " + path + ": UINT OpenEXR channels are not supported");
+  std::vector<size_t> chanOff(chans.size());
+  size_t lineBytes = 0;
+  for (size_t c = 0; c < chans.size(); c++) {
+    chanOff[c] = lineBytes;
+    lineBytes += size_t(W) * (chans[c].type == 1 ? 2 : 4);
+  }
+  const int64_t blocks = (H + linesPerBlock - 1) / linesPerBlock;
+  if (pos + size_t(blocks) * 8 > f.size()) Bad(path, "truncated offset table");
+  ImageF img;
+  img.width = uint32_t(W);
+  img.height = uint32_t(H);
+  img.channels = 3;
+  img.px.assign(size_t(W) * size_t(H) * 3, 0.f);
+  std::vector<uint8_t> tmp, data;
+  for (int64_t b = 0; b < blocks; b++) {
+    const uint64_t off = LE64(f.data() + pos + size_t(b) * 8);
+    if (off > f.size() || f.size() - off < 8) Bad(path, "bad chunk offset");
+    const int32_t y = int32_t(LE32(f.data() + off));
+    const uint32_t packed = LE32(f.data() + off + 4);
+    if (packed > f.size() - off - 8) Bad(path, "truncated chunk");
+    if (y < y0 || y > y1 || (int64_t(y) - y0) % linesPerBlock != 0) Bad(path, "bad chunk line");
+    const int64_t lines = std::min<int64_t>(linesPerBlock, int64_t(y1) - y + 1);
+    const size_t want = size_t(lines) * lineBytes;
+    const uint8_t* src = f.data() + off + 8;
+    if (comp == 0 || packed >= want) {  // stored as is when compression does not pay
+      if (packed != want) Bad(path, "bad chunk size");
+      data.assign(src, src + want);
+    } else if (comp == 1) {  // RLE
+      tmp.clear();
+      for (size_t i = 0; i < packed;) {
+        const int8_t c = int8_t(src[i++]);
+        if (c < 0) {
+          if (i + size_t(-c) > packed) Bad(path, "bad RLE run");
+          tmp.insert(tmp.end(), src + i, src + i + size_t(-c));
+          i += size_t(-c);
+        } else {
+          if (i >= packed) Bad(path, "bad RLE run");
+          tmp.insert(tmp.end(), size_t(c) + 1, src[i++]);
+        }
+        if (tmp.size() > want) Bad(path, "RLE overrun");
+      }
+      if (tmp.size() != want) Bad(path, "bad RLE size");
+      ExrUnpredict(tmp, data);
+    } else {  // ZIPS / ZIP
+      tmp.resize(want);
+      uLongf n = uLongf(want);
+      if (uncompress(tmp.data(), &n, src, uLong(packed)) != Z_OK || n != want) Bad(path, "bad zlib chunk");
+      ExrUnpredict(tmp, data);
+    }
+    for (int64_t l = 0; l < lines; l++) {
+      const uint8_t* line = data.data() + size_t(l) * lineBytes;
+      float* row = img.px.data() + size_t(int64_t(y) - y0 + l) * size_t(W) * 3;
+      for (int k = 0; k < 3; k++) {
+        if (rgbChan[k] < 0) continue;
+        const size_t c = size_t(rgbChan[k]);
+        const uint8_t* p = line + chanOff[c];
+        for (int64_t x = 0; x < W; x++) {
+          float v;
+          if (chans[c].type == 1) {
+            v = HalfToFloat(uint16_t(p[2 * x] | (p[2 * x + 1] << 8)));
+          } else {
+            const uint32_t bits = LE32(p + 4 * x);
+            std::memcpy(&v, &bits, 4);
+            if (pixelType == 1) v = HalfToFloat(FloatToHalf(v));  // a FLOAT channel read into a HALF slice
+          }
+          row[3 * size_t(x) + size_t(k)] = v;
+        }
+      }
+    }
+  }
+  return img;
+}
+
 }  // namespace host
 }  // namespace rad

@@ -10,6 +10,7 @@
 #include <deque>
 #include <fstream>
 #include <map>
+#include <set>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -393,10 +394,12 @@ struct Builder {
   std::map<std::string, const ObjFile*> objAssets;
   std::map<std::string, std::unique_ptr<ObjFile>> objStore;
   std::map<std::string, const Grid*> volAssets;
+  std::set<std::string> vdbAssets;
   struct ImageAsset {
     const std::vector<float>* texels;
     uint32_t width, height;
     int channels;
+    bool rgbOnly = false;  // image_exr: no single-channel view (image_exr.cpp:20-28)
   };
   std::map<std::string, ImageAsset> imageAssets;
   std::map<std::string, int> bsdfVars;
@@ -447,8 +450,19 @@ struct Builder {
         }
         out.images.push_back(std::move(img.px));
         imageAssets[name] = ImageAsset{&out.images.back(), img.width, img.height, 3};
-      } else if (type == "image_exr" || type == "volume_data_vdb") {
-        // readers this build does not carry: rejected where a texture or medium uses them
+      } else if (type == "image_exr") {  // asset/image_exr.cpp:31-97: RGB, no flip, no colour conversion
+        rad::host::ImageF img;
+        try {
+          img = rad::host::ReadExr(Resolve(loc));
+        } catch (const rad::host::ImageUnsupported& e) {
+          throw LoadError(RAD_ERR_UNSUPPORTED, e.what());
+        } catch (const std::runtime_error& e) {
+          throw LoadError(RAD_ERR_IO, e.what());
+        }
+        out.images.push_back(std::move(img.px));
+        imageAssets[name] = ImageAsset{&out.images.back(), img.width, img.height, 3, true};
+      } else if (type == "volume_data_vdb") {  // OpenVDB grids are not carried: rejected where a medium uses them
+        vdbAssets.insert(name);
       } else {
         throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + type);
       }
@@ -467,6 +481,8 @@ struct Builder {
       std::string asset = Str(n, "asset_name");
       auto it = imageAssets.find(asset);
       if (it == imageAssets.end()) throw LoadError(RAD_ERR_INVALID_ARG, "unknown image asset: " + asset);
+      if (it->second.rgbOnly && channels != 3)
+        throw LoadError(RAD_ERR_UNSUPPORTED, "exr image " + asset + " has no single-channel view");
       if (it->second.channels != channels)
         throw LoadError(RAD_ERR_INVALID_ARG, "loaded image " + asset + (channels == 3 ? " is mono" : " is rgb"));
       t.type = RAD_TEXTURE_BITMAP;
@@ -716,6 +732,8 @@ struct Builder {
         if (Str(v, "type") != "volume_grid") throw LoadError(RAD_ERR_INVALID_ARG, "cannot find factory: " + Str(v, "type"));
         std::string asset = Str(v, "asset_name");
         auto it = volAssets.find(asset);
+        if (it == volAssets.end() && vdbAssets.count(asset))
+          throw LoadError(RAD_ERR_UNSUPPORTED, "OpenVDB volume " + asset + " is not supported");
         if (it == volAssets.end()) throw LoadError(RAD_ERR_INVALID_ARG, "unknown asset: " + asset);
         const Grid* g = it->second;
         m.grid_size[0] = g->sx;

@@ -550,3 +550,142 @@ def test_textured_scene_round_trip_renders_identically(tmp_path):
         assert np.array_equal(fb1, fb2)
     finally:
         hip.lib().rad_loaded_scene_free(h)
+
+
+# ---------------------------------------------------------------- OpenEXR input
+def _exr_bytes(chans, W, H, comp, x0=0, y0=0, tiled=False):
+    """An OpenEXR scanline file written from the format's published layout:
+    chans = {name: (pixel type 0 UINT / 1 HALF / 2 FLOAT, H x W array)}; comp
+    0 NONE, 1 RLE, 2 ZIPS, 3 ZIP, others written raw under that code."""
+    import struct
+    import zlib
+
+    names = sorted(chans)
+
+    def attr(name, typ, val):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(val)) + val
+
+    chl = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", chans[n][0], 0, 1, 1) for n in names) + b"\0"
+    hdr = struct.pack("<ii", 20000630, 2 | (0x200 if tiled else 0))
+    hdr += attr("channels", "chlist", chl) + attr("compression", "compression", bytes([comp]))
+    win = struct.pack("<iiii", x0, y0, x0 + W - 1, y0 + H - 1)
+    hdr += attr("dataWindow", "box2i", win) + attr("displayWindow", "box2i", win)
+    hdr += attr("lineOrder", "lineOrder", b"\0") + attr("pixelAspectRatio", "float", struct.pack("<f", 1))
+    hdr += attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0)) + attr("screenWindowWidth", "float", struct.pack("<f", 1))
+    hdr += b"\0"
+    dt = {0: "<u4", 1: "<f2", 2: "<f4"}
+
+    def line(y):
+        return b"".join(np.ascontiguousarray(chans[n][1][y], dt[chans[n][0]]).tobytes() for n in names)
+
+    def predict(raw):  # split even / odd bytes, then byte deltas + 128
+        t = np.frombuffer(raw[0::2] + raw[1::2], np.uint8).astype(np.int32)
+        out = t.copy()
+        out[1:] = (t[1:] - t[:-1] + 128) & 255
+        return out.astype(np.uint8).tobytes()
+
+    def rle(b):
+        out, i = bytearray(), 0
+        while i < len(b):
+            run = 1
+            while i + run < len(b) and b[i + run] == b[i] and run < 128:
+                run += 1
+            if run >= 3:
+                out += bytes([run - 1, b[i]])
+                i += run
+            else:
+                n = 1
+                while i + n < len(b) and n < 127 and not (i + n + 2 < len(b) and b[i + n] == b[i + n + 1] == b[i + n + 2]):
+                    n += 1
+                out += bytes([256 - n]) + b[i:i + n]
+                i += n
+        return bytes(out)
+
+    per = 16 if comp == 3 else 1
+    chunks = []
+    for y in range(0, H, per):
+        raw = b"".join(line(yy) for yy in range(y, min(H, y + per)))
+        if comp == 1:
+            data = rle(predict(raw))
+        elif comp in (2, 3):
+            data = zlib.compress(predict(raw))
+        else:
+            data = raw
+        if len(data) >= len(raw):
+            data = raw
+        chunks.append(struct.pack("<ii", y0 + y, len(data)) + data)
+    off = len(hdr) + 8 * len(chunks)
+    table = b""
+    for c in chunks:
+        table += struct.pack("<Q", off)
+        off += len(c)
+    return hdr + table + b"".join(chunks)
+
+
+def _exr_values(rng, H, W):
+    v = rng.normal(0, 3, (H, W)).astype(np.float32)
+    v[0, :4] = [0.0, -0.0, 6e-6, 70000.0]  # zero, signed zero, a half subnormal, beyond half's range
+    v[1, 2:9] = 1.25  # runs for RLE
+    return v
+
+
+@pytest.mark.filterwarnings("ignore:overflow encountered in cast")
+@pytest.mark.parametrize("comp", [0, 1, 2, 3])
+def test_openexr_images(tmp_path, comp):
+    """asset/image_exr.cpp + ImageReader::ReadExr: R / G / B by name suffix
+    (first in the sorted list), converted to the first channel's pixel type,
+    absent channels 0, the data window's rows top first, no flip."""
+    rng = np.random.default_rng(30 + comp)
+    W, H = 13, 37
+    f = [_exr_values(rng, H, W) for _ in range(6)]
+    h16 = lambda a: a.astype(np.float16)
+    # image a: HALF R, G, B plus a FLOAT depth channel, offset data window
+    a = {"B": (1, h16(f[0])), "G": (1, h16(f[1])), "R": (1, h16(f[2])), "Z": (2, f[3])}
+    # image b: first channel HALF (alpha), FLOAT colour read through HALF;
+    # "diffuse.G" sorts before "g" and wins; "R" is also matched
+    b = {"A": (1, h16(f[4])), "B": (2, f[0]), "R": (2, f[1]), "diffuse.G": (2, f[2]), "g": (2, f[5])}
+    # image c: FLOAT B and R only
+    c = {"B": (2, f[3]), "R": (2, f[4])}
+    files = {"a.exr": _exr_bytes(a, W, H, comp, x0=3, y0=-2), "b.exr": _exr_bytes(b, W, H, comp),
+             "c.exr": _exr_bytes(c, W, H, comp)}
+    assets = [{"type": "image_exr", "name": n[0], "location": n} for n in files]
+    h = _load_images(tmp_path, files, assets, [{"reflectance": {"type": "bitmap", "asset_name": n[0]}} for n in files])
+    try:
+        want = [np.stack([f[2], f[1], f[0]], -1).astype(np.float16).astype(np.float32),
+                np.stack([f[1], f[2], f[0]], -1).astype(np.float16).astype(np.float32),
+                np.stack([f[4], np.zeros_like(f[4]), f[3]], -1)]
+        for i in range(3):
+            t, tex = _texels_of(h, i)
+            assert (t.width, t.height) == (W, H)
+            assert np.array_equal(tex, want[i], equal_nan=True), i
+        assert np.isinf(_texels_of(h, 1)[1][0, 3, 0])  # 70000 read through a HALF slice
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
+
+
+def test_openexr_rejections_and_round_trip(tmp_path):
+    rng = np.random.default_rng(40)
+    W, H = 6, 5
+    v = {"B": (2, _exr_values(rng, H, W)), "G": (2, _exr_values(rng, H, W)), "R": (2, _exr_values(rng, H, W))}
+    cases = [(_exr_bytes(v, W, H, 4), 4),  # PIZ -> RAD_ERR_UNSUPPORTED (4); malformed -> RAD_ERR_IO (5)
+             (_exr_bytes(v, W, H, 0, tiled=True), 4),
+             (_exr_bytes({"A": (0, np.ones((H, W), np.uint32)), **v}, W, H, 0), 4),  # UINT first
+             (_exr_bytes(v, W, H, 3)[:-7], 5)]
+    for data, code in cases:
+        with pytest.raises(hip.RadError) as e:
+            _load_images(tmp_path, {"x.exr": data}, [{"type": "image_exr", "name": "x", "location": "x.exr"}],
+                         [{"reflectance": {"type": "bitmap", "asset_name": "x"}}])
+        assert e.value.status == code
+    with pytest.raises(hip.RadError) as e:  # no single-channel view (image_exr.cpp:20-28)
+        _load_images(tmp_path, {"x.exr": _exr_bytes(v, W, H, 0)}, [{"type": "image_exr", "name": "x", "location": "x.exr"}],
+                     [{"alpha": {"type": "bitmap", "asset_name": "x"}}])
+    assert e.value.status == 4
+    # the renderer's own EXR output reads back unchanged
+    img = np.ascontiguousarray(rng.uniform(0, 5, (H, W, 3)).astype(np.float32))
+    assert hip.lib().rad_write_exr(str(tmp_path / "o.exr").encode(), img.ctypes.data, W, H) == 0
+    h = _load_images(tmp_path, {}, [{"type": "image_exr", "name": "o", "location": "o.exr"}],
+                     [{"reflectance": {"type": "bitmap", "asset_name": "o"}}])
+    try:
+        assert np.array_equal(_texels_of(h, 0)[1], img)
+    finally:
+        hip.lib().rad_loaded_scene_free(h)
