@@ -199,6 +199,7 @@ struct DevScene {
   uint32_t light_count;
   int32_t env_light;
   float light_pdf;
+  uint32_t tex_lights;  // 1: some light's radiance (irradiance) is a texture
   const float* light_cdf;
   const MediumRec* media;
   int32_t global_medium;

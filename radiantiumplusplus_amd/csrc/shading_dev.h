@@ -222,12 +222,23 @@ __device__ inline LightSample sample_area_shape(const DevScene& s, const ShapeRe
   return ls;
 }
 
+// A radiance texel lookup without differentials, out of line: inlined into
+// every shading kernel the filter code costs registers on the common path.
+__device__ __noinline__ f3 emitter_texel(const DevScene& s, int32_t tex, float u, float v) {
+  return tex_eval(s, uint32_t(tex), TexCoord{u, v, 0.f, 0.f, 0.f, 0.f});
+}
+
 // Scene::SampleLightDirection (scene.cpp:87-96)
 // Point and projection lights (light/point.cpp:24-38, projection.cpp:40-60),
 // kept out of line: rare, and inlined they raise the register count of every
 // shading kernel.  A textured projection reads its irradiance image at the
 // projected uv with no differentials (interaction.cpp:32-43); `radiance` then
 // holds the scale.
+//
+// TL (here and below): the scene has textured emitters.  With TL false every
+// emitter is a constant and the texture code, whose out-of-line calls would
+// raise the shading kernels' register counts, is not compiled in.
+template <bool TL = true>
 __device__ __noinline__ LightSample sample_delta_light(const DevScene& s, const LightRec& L, f3 refp) {
   LightSample ls;
   ls.delta = true;
@@ -262,14 +273,18 @@ __device__ __noinline__ LightSample sample_delta_light(const DevScene& s, const 
   ls.n = L.axis;
   ls.dist = sqrtf(dot(d, d));
   ls.dir = d * (1.f / ls.dist);
-  const f3 irr = L.tex < 0 ? L.radiance : mul(tex_eval(s, uint32_t(L.tex), TexCoord{u, v, 0.f, 0.f, 0.f, 0.f}), L.radiance);
+  f3 irr = L.radiance;
+  if constexpr (TL)
+    if (L.tex >= 0) irr = mul(emitter_texel(s, L.tex, u, v), L.radiance);
   ls.li = irr * kPi / (sqr(local.z) * dot(-ls.dir, ls.n));
   return ls;
 }
 
 // ---- textured emitters (light/area.cpp:20-148, light/skybox.cpp:24-234)
+template <bool TL = true>
 __device__ inline f3 emitter_radiance(const DevScene& s, const LightRec& L, float u, float v) {
-  return L.tex < 0 ? L.radiance : tex_eval(s, uint32_t(L.tex), TexCoord{u, v, 0.f, 0.f, 0.f, 0.f});
+  if constexpr (!TL) return L.radiance;
+  return L.tex < 0 ? L.radiance : emitter_texel(s, L.tex, u, v);
 }
 // lat-long coordinates of a direction in the skybox frame (skybox.cpp:53-63)
 __device__ inline void sky_uv(f3 v, float& u, float& w) {
@@ -278,8 +293,9 @@ __device__ inline void sky_uv(f3 v, float& u, float& w) {
   u = phi / (2.f * kPi);
   w = acosf(fminf(fmaxf(v.y, -1.f), 1.f)) / kPi;
 }
+template <bool TL = true>
 __device__ inline f3 skybox_radiance(const DevScene& s, const LightRec& L, f3 worldDir) {
-  if (L.tex < 0) return L.radiance;
+  if (!TL || L.tex < 0) return L.radiance;
   float u, v;
   sky_uv(L.to_local.vec(worldDir), u, v);
   return emitter_radiance(s, L, u, v);
@@ -388,6 +404,7 @@ __device__ __noinline__ LightSample sample_skybox_textured(const DevScene& s, co
   return ls;
 }
 
+template <bool TL = true>
 __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul, float u1, float u2, bool quirks) {
   uint32_t index = 0;
   float lpdf = 1.f;
@@ -397,8 +414,8 @@ __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul,
   }
   const LightRec& L = s.lights[index];
   LightSample ls;
-  if (L.type == LIGHT_SKYBOX && L.tex >= 0 && !L.tex_uniform) {
-    ls = sample_skybox_textured(s, L, refp, u1, u2, quirks);
+  if (TL && L.type == LIGHT_SKYBOX && L.tex >= 0 && !L.tex_uniform) {
+    if constexpr (TL) ls = sample_skybox_textured(s, L, refp, u1, u2, quirks);
   } else if (L.type == LIGHT_SKYBOX) {  // skybox.cpp:65-88 (one-texel map)
     float radius = fmaxf(s.world_radius, norm(refp - s.world_center));
     float dist = 2.f * radius;
@@ -411,20 +428,20 @@ __device__ inline LightSample sample_light(const DevScene& s, f3 refp, float ul,
     ls.dist = dist;
     ls.pdf = 1.f / (4.f * kPi);
     ls.delta = false;
-    if (L.tex < 0) {
+    if (!TL || L.tex < 0) {
       ls.li = L.radiance;
     } else {
       float tu, tv;
       sky_uv(dir, tu, tv);
-      ls.li = emitter_radiance(s, L, tu, tv);
+      ls.li = emitter_radiance<TL>(s, L, tu, tv);
     }
   } else if (L.type == LIGHT_POINT || L.type == LIGHT_PROJECTION) {
-    ls = sample_delta_light(s, L, refp);
-  } else if (L.tex >= 0 && !L.tex_uniform) {
-    ls = sample_area_textured(s, L, refp, u1, u2, quirks);
+    ls = sample_delta_light<TL>(s, L, refp);
+  } else if (TL && L.tex >= 0 && !L.tex_uniform) {
+    if constexpr (TL) ls = sample_area_textured(s, L, refp, u1, u2, quirks);
   } else {  // light/area.cpp:50-75 (a one-texel texture is read at the rectangle's uv = xi)
     ls = sample_area_shape(s, s.shapes[L.shape], refp, u1, u2);
-    ls.li = dot(-ls.dir, ls.n) >= 0.f ? emitter_radiance(s, L, u1, u2) : splat(0.f);
+    ls.li = dot(-ls.dir, ls.n) >= 0.f ? emitter_radiance<TL>(s, L, u1, u2) : splat(0.f);
   }
   ls.pdf *= lpdf;
   return ls;
@@ -450,10 +467,12 @@ __device__ __noinline__ float textured_pdf_dir(const DevScene& s, const LightRec
 
 // Light::PdfDirection * 1/N for a light hit from `refp` (scene.cpp:98-103);
 // (hu, hv): the hit's texture coordinates on a textured rectangle emitter.
+template <bool TL = true>
 __device__ inline float light_pdf_dir(const DevScene& s, int32_t light, f3 refp, f3 p, f3 n_shading, f3 dir,
                                       float dist, float hu, float hv, bool quirks) {
   const LightRec& L = s.lights[light];
-  if (L.tex >= 0 && !L.tex_uniform) return textured_pdf_dir(s, L, n_shading, dir, dist, hu, hv, quirks) * s.light_pdf;
+  if constexpr (TL)
+    if (L.tex >= 0 && !L.tex_uniform) return textured_pdf_dir(s, L, n_shading, dir, dist, hu, hv, quirks) * s.light_pdf;
   if (L.type == LIGHT_SKYBOX) return (1.f / (4.f * kPi)) * s.light_pdf;
   if (!(dot(-dir, n_shading) >= 0.f)) return 0.f;  // area.cpp:77-80
   const ShapeRec& sr = s.shapes[L.shape];
