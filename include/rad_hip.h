@@ -269,7 +269,15 @@ enum {
   RAD_RENDER_PROFILE = 1u << 2,
   /* Shade every path with one kernel instead of sorting the queue by
    * material into specialised kernels (for comparison). */
-  RAD_RENDER_UNSORTED = 1u << 3
+  RAD_RENDER_UNSORTED = 1u << 3,
+  /* path, material-sorted: after every material classification a check
+   * verifies that the material queues and the pass-through (suspended) part of
+   * the next extension queue hold each extension-queue entry exactly once, in
+   * the queue its BSDF type selects; counts go to rad_job_stats.queue_* */
+  RAD_RENDER_CHECK_QUEUES = 1u << 4,
+  /* with RAD_RENDER_CHECK_QUEUES: count every extension-queue entry twice, so
+   * that every entry fails the check (shows that the check can fail) */
+  RAD_RENDER_CHECK_SELFTEST = 1u << 5
 };
 
 enum { /* rad_job_stats.kernel_ms slots */
@@ -316,6 +324,8 @@ typedef struct rad_job_stats {
   uint32_t blocks_per_cu, trace_blocks_per_cu; /* launch configuration used */
   uint64_t suspended;      /* extension traversals cut off at the per-launch node-visit budget and resumed
                               by the next wavefront iteration (results do not depend on it) */
+  uint64_t queue_checked;  /* RAD_RENDER_CHECK_QUEUES: extension-queue entries checked */
+  uint64_t queue_errors;   /* RAD_RENDER_CHECK_QUEUES: slots missing, duplicated or in the wrong queue */
 } rad_job_stats;
 
 /* Launch-configuration autotuner.  Times short renders of `probe` (its spp is

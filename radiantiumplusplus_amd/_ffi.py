@@ -229,11 +229,15 @@ class JobStats(C.Structure):
         ("blocks_per_cu", C.c_uint32),
         ("trace_blocks_per_cu", C.c_uint32),
         ("suspended", C.c_uint64),
+        ("queue_checked", C.c_uint64),
+        ("queue_errors", C.c_uint64),
     ]
 
 
 RAD_RENDER_PROFILE = 1 << 2
 RAD_RENDER_UNSORTED = 1 << 3
+RAD_RENDER_CHECK_QUEUES = 1 << 4
+RAD_RENDER_CHECK_SELFTEST = 1 << 5
 RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_SHADE, RAD_KERNEL_SHADOW = 0, 1, 2, 3
 RAD_KERNEL_GUIDE, RAD_KERNEL_TRAIN = 4, 5
 RAD_BVH_DEVICE, RAD_BVH_HOST = 0, 1

@@ -504,6 +504,16 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
 // rough glass and Disney get queues of their own (misses join the diffuse
 // queue); every other type shares the last queue and its run-time-dispatch
 // kernel.
+// Material queue of a finished extension: misses and shapes without a BSDF
+// (emitters) go to the always-launched diffuse kernel, textured BSDFs to the
+// generic one.
+__device__ __forceinline__ int material_key(const DevScene& scene, uint32_t geom) {
+  const int32_t bi = geom == 0xffffffffu ? -1 : scene.shapes[geom].bsdf;
+  if (bi < 0) return BSDF_DIFFUSE;
+  const BsdfRec& br = scene.bsdfs[bi];
+  return br.tex_any ? kMatOther : min(br.type, kMatOther);
+}
+
 template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[40];
@@ -520,13 +530,7 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
       held = w.susp && w.susp[s] != kNoSusp;  // traversal suspended: straight to the next extend queue
       if (!held) {
         uint32_t geom = w.hit_geom[s];
-        const int32_t bi = geom == 0xffffffffu ? -1 : scene.shapes[geom].bsdf;
-        if (bi < 0) {  // misses and shapes without a BSDF (emitters): the always-launched diffuse kernel
-          key = BSDF_DIFFUSE;
-        } else {
-          const BsdfRec& br = scene.bsdfs[bi];
-          key = br.tex_any ? kMatOther : min(br.type, kMatOther);
-        }
+        key = material_key(scene, geom);
         if (GUIDE) gd = guided_hit(w, scene, geom, quirks);
       }
     }
@@ -553,6 +557,48 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
     }
     if (GUIDE && gd) w.gq[r7[6]] = s;
   }
+}
+
+// Queue check (RAD_RENDER_CHECK_QUEUES).  The extension queue adds `add` to
+// each of its slots' tally words, every material queue and the pass-through
+// entries of the next extension queue subtract 1, and any word left nonzero
+// is a slot lost, duplicated or misplaced; an entry in the wrong material
+// queue, or passed through without a suspended traversal, counts as well.
+__global__ void k_queue_tally(Wavefront w, int in, uint32_t* tally, uint32_t add) {
+  const uint32_t n = w.cnt[in];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&tally[w.queue[in][i]], add);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&w.stats[ST_QCHECK], (unsigned long long)n);
+}
+__global__ void k_queue_drain(Wavefront w, DevScene scene, int in, uint32_t* tally) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t bad = 0;
+  for (int m = 0; m <= kMatQueues; m++) {
+    const bool pass = m == kMatQueues;  // the suspended slots classify passed through
+    const uint32_t n = pass ? w.cnt[1 - in] : w.cnt[CNT_MAT + m];
+    const uint32_t* q = pass ? w.queue[1 - in] : w.mq[m];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const uint32_t s = q[i];
+      if (s >= w.Q) {
+        bad++;
+        continue;
+      }
+      const bool held = w.susp && w.susp[s] != kNoSusp;
+      if (pass ? !held : (held || material_key(scene, w.hit_geom[s]) != m)) bad++;
+      atomicSub(&tally[s], 1u);
+    }
+  }
+  if (bad) atomicAdd(&w.stats[ST_QERR], (unsigned long long)bad);
+}
+__global__ void k_queue_verify(Wavefront w, uint32_t* tally) {
+  uint32_t bad = 0;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.Q; s += gridDim.x * blockDim.x) {
+    if (tally[s] != 0u) {
+      bad++;
+      tally[s] = 0u;
+    }
+  }
+  if (bad) atomicAdd(&w.stats[ST_QERR], (unsigned long long)bad);
 }
 
 struct ShadowJob {
@@ -747,6 +793,12 @@ static void LaunchMaterials(const Wavefront& w, const DevScene& scene, int in, u
     hipLaunchKernelGGL((k_shade_path<G, kMatOther, TL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 
+void LaunchQueueCheck(const Wavefront& w, const DevScene& scene, int in, uint32_t* tally, bool selftest,
+                      const LaunchCfg& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_queue_tally, dim3(c.blocks), dim3(256), 0, s, w, in, tally, selftest ? 2u : 1u);
+  hipLaunchKernelGGL(k_queue_drain, dim3(c.blocks), dim3(256), 0, s, w, scene, in, tally);
+  hipLaunchKernelGGL(k_queue_verify, dim3(c.blocks), dim3(256), 0, s, w, tally);
+}
 void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   if (w.flags & WF_GUIDE)
     hipLaunchKernelGGL(k_classify<true>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);

@@ -45,6 +45,8 @@ struct rad_job {
   bool sorted = false;        // material-sorted shading
   uint32_t blocks_per_cu = 0, trace_blocks_per_cu = 0;
   uint32_t mat_mask = 0;      // BSDF types present in the scene
+  uint32_t* tally = nullptr;  // RAD_RENDER_CHECK_QUEUES: per-slot tally words (inside the arena)
+  bool tally_selftest = false;
   bool joined = false;
 };
 
@@ -130,6 +132,7 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * (6 + kMatQueues) + Align(pass * 4) * 3 + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
   if (j.resume_cap) j.bytes += Align(Q * 4) + 2 * Align(uint64_t(j.resume_cap) * sizeof(ResumeState));
+  if (desc.flags & RAD_RENDER_CHECK_QUEUES) j.bytes += Align(Q * 4);
   if (j.guided)
     j.bytes += Align(Q * 4) * 3 + Align(Q * 2 * guide::kOut) +
                Align(uint64_t(kRecordCap) * sizeof(guide::TrainRecord)) + Align(kTrainBatch * 4) + 256 +
@@ -249,8 +252,10 @@ void RunJob(rad_job* job) {
         if (prof) HipCheck(hipEventRecord(ev[2], st), "event");
         const bool vol = w.integrator == RAD_INTEGRATOR_VOL_PATH;
         const bool guided = (w.flags & WF_GUIDE) != 0;
-        if (!vol && job->sorted)
+        if (!vol && job->sorted) {
           LaunchClassify(w, scene, in, cfg, st);  // with guiding, also the guide queue
+          if (job->tally) LaunchQueueCheck(w, scene, in, job->tally, job->tally_selftest, cfg, st);
+        }
         else if (!vol && guided)
           LaunchGuidePrep(w, scene, in, cfg, st);
         if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
@@ -427,6 +432,11 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
     w.resume_cap = js.resume_cap;
     w.visit_budget = VisitBudget();
   }
+  if (desc.flags & RAD_RENDER_CHECK_QUEUES) {
+    job->tally = (uint32_t*)take(u4);
+    job->tally_selftest = (desc.flags & RAD_RENDER_CHECK_SELFTEST) != 0;
+    HipCheck(hipMemset(job->tally, 0, u4), "queue check tally");
+  }
   if (js.guided) {
     w.gq = (uint32_t*)take(u4);
     w.gpend = (uint32_t*)take(u4);
@@ -501,6 +511,8 @@ void WaitJob(rad_job* job, float* fb, float* sumsq, rad_job_stats* stats) {
     stats->samples = job->stats[ST_SAMPLES];
     stats->segments = job->segments;
     stats->suspended = job->suspended;
+    stats->queue_checked = job->stats[ST_QCHECK];
+    stats->queue_errors = job->stats[ST_QERR];
     stats->shadow_rays = job->shadow_rays;
     stats->medium_steps = job->stats[ST_MEDIUM];
     stats->iterations = job->iterations;

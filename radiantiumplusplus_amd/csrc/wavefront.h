@@ -38,7 +38,7 @@ enum : uint32_t {
   CNT_SUSP = 14,      // 2 words: traversals suspended into resume pool 0 / 1
   CNT_WORDS = 16      // even, so the 64-bit statistics that follow stay aligned
 };
-enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_WORDS = 4 };
+enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_QCHECK = 2, ST_QERR = 3, ST_WORDS = 4 };
 enum : uint32_t { WF_QUIRKS = 1u, WF_GUIDE = 2u, WF_RECORD = 4u };
 enum : uint32_t { G_QUEUE = 0, G_RECORDS = 1, G_WSUM = 2, G_WORDS = 4 };
 enum : uint32_t { MED_UNSET = 0xffffffffu };
@@ -176,6 +176,9 @@ void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const La
 // (with WF_GUIDE it also fills the guide queue), then one specialised shade
 // kernel per type present (mat_mask bit = BSDF type).
 void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
+// Queue check after a classify pass (tally: Q zeroed words, left zeroed).
+void LaunchQueueCheck(const Wavefront& w, const DevScene& scene, int in, uint32_t* tally, bool selftest,
+                      const LaunchCfg& c, hipStream_t s);
 void LaunchShadeMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mat_mask, const LaunchCfg& c,
                           hipStream_t s);
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);

@@ -237,3 +237,31 @@ def test_suspended_traversals_leave_images_unchanged(hip_ctx, monkeypatch, budge
             assert st.suspended > 0
             assert st.segments == st0.segments
             assert np.array_equal(img, ref)
+
+
+@pytest.mark.parametrize("budget", ["0", "2"])
+def test_material_queues_are_a_permutation(hip_ctx, monkeypatch, budget):
+    """After every material classification the five material queues and the
+    pass-through (suspended) entries of the next extension queue hold each
+    extension-queue slot exactly once, each in the queue its BSDF type
+    selects (a device-side tally; budget 2 makes suspended traversals pass
+    through).  The check leaves the image unchanged, and the self-test mode,
+    which counts every entry twice, shows that it can fail."""
+    monkeypatch.setenv("RAD_VISIT_BUDGET", budget)
+    b = remaining_types_scene()
+    pos, _, _, idx = scenes.create_sphere(0.6, 48)  # a 2 k-triangle mesh: traversals deep enough to suspend
+    b.mesh(pos, idx, to_world=scenes.trs(translate=(2.2, 0.6, -1.5)), bsdf=b.rough_metal(alpha=0.2))
+    s = hip.Scene.from_builder(hip_ctx, b)
+    ref, _, st0 = s.render(b.render_desc(spp=8, seed=5))
+    img, _, st = s.render(b.render_desc(spp=8, seed=5, flags=F.RAD_RENDER_CHECK_QUEUES))
+    print(f"budget {budget}: checked {st.queue_checked} errors {st.queue_errors} "
+          f"segments {st.segments} suspended {st.suspended}")
+    assert st.queue_errors == 0
+    assert st.queue_checked == st.segments + st.suspended and st.segments > 0
+    assert (st.suspended > 0) == (budget != "0")
+    assert np.array_equal(img, ref)
+    _, _, bad = s.render(b.render_desc(spp=8, seed=5, flags=F.RAD_RENDER_CHECK_QUEUES | F.RAD_RENDER_CHECK_SELFTEST))
+    assert bad.queue_errors == bad.queue_checked > 0
+    # the unsorted kernel has no material queues: nothing is checked
+    _, _, un = s.render(b.render_desc(spp=8, seed=5, flags=F.RAD_RENDER_CHECK_QUEUES | F.RAD_RENDER_UNSORTED))
+    assert un.queue_checked == 0
