@@ -81,6 +81,8 @@ void UploadScene(rad_scene* s) {
   d.light_pdf = 1.0f / float(h.lights.size());
   d.tex_lights = 0;
   for (const LightRec& l : h.lights) d.tex_lights |= l.tex >= 0 ? 1u : 0u;
+  d.tex_bsdfs = 0;
+  for (const BsdfRec& b : h.bsdfs) d.tex_bsdfs |= b.tex_any ? 1u : 0u;
   d.light_cdf = s->d_cdf.as<float>();
   d.media = s->d_media.as<MediumRec>();
   d.global_medium = h.global_medium;

@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_extend(Wavefront w, DevScene sc
 // (probability alpha) or the BSDF, and weighted by the one-sample MIS
 // density alpha q + (1 - alpha) p_bsdf; NEE uses the same density in its MIS
 // weight; training records of the vertices are kept for 1 in rec_stride slots.
-template <bool GUIDE, int MAT = -1, bool TL = true>
+template <bool GUIDE, int MAT = -1, bool TX = true>
 __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
   const uint32_t n = MAT >= 0 ? w.cnt[CNT_MAT + MAT] : w.cnt[in];
@@ -250,8 +250,8 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
         // miss: environment emission (scene.cpp:105-113, skybox.cpp:53-63)
         if (scene.env_light >= 0) {
           float wgt = 1.f;
-          if (!spec) wgt = mis_weight(prevPdf, light_pdf_dir<TL>(scene, scene.env_light, prevP, rd, rd, rd, 0.f, 0.f, 0.f, quirks));
-          L += mul(T, skybox_radiance<TL>(scene, scene.lights[scene.env_light], rd)) * wgt;
+          if (!spec) wgt = mis_weight(prevPdf, light_pdf_dir<TX>(scene, scene.env_light, prevP, rd, rd, rd, 0.f, 0.f, 0.f, quirks));
+          L += mul(T, skybox_radiance<TX>(scene, scene.lights[scene.env_light], rd)) * wgt;
         }
       } else {
         SurfaceHit si = surface_hit(scene, ro, rd, hh.x, hh.y, hh.z, __float_as_uint(hh.w), geom);
@@ -262,10 +262,10 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             f3 rel = si.p - prevP;
             float dist = norm(rel);
             f3 dir = rel / dist;
-            wgt = mis_weight(prevPdf, light_pdf_dir<TL>(scene, sr.light, prevP, si.p, si.sh.n, dir, dist, hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f, quirks));
+            wgt = mis_weight(prevPdf, light_pdf_dir<TX>(scene, sr.light, prevP, si.p, si.sh.n, dir, dist, hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f, quirks));
           }
           f3 le = dot(si.sh.to_world(si.wi), si.n) >= 0.f
-                    ? emitter_radiance<TL>(scene, scene.lights[sr.light], hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f)
+                    ? emitter_radiance<TX>(scene, scene.lights[sr.light], hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f)
                     : splat(0.f);
           L += mul(T, le) * wgt;
         }
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           const BsdfRec* all = scene.bsdfs;
           const BsdfRec* bp0 = &scene.bsdfs[sr.bsdf];
           BsdfRec chain[3];
-          if constexpr (MAT < 0 || MAT == kMatOther) {  // textured BSDFs are classified into the generic queue
+          if constexpr (TX && (MAT < 0 || MAT == kMatOther)) {  // textured BSDFs are classified into the generic queue
             if (bp0->tex_any) {
               // only the camera ray carries differentials (path.cpp:172; SpawnRay drops them)
               const uint32_t W = uint32_t(scene.cam.width);
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           if (b.lobes & LOBE_NO_DELTA) {  // NEE (path.cpp:173-191)
             float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
             dim += 3;
-            LightSample ls = sample_light<TL>(scene, si.p, ul, u1, u2, quirks);
+            LightSample ls = sample_light<TX>(scene, si.p, ul, u1, u2, quirks);
             if (ls.pdf > 0.f) {
               f3 wo = si.sh.to_local(ls.dir);
               float bp;
@@ -767,30 +767,33 @@ void LaunchReset(const Wavefront& w, int out, hipStream_t s) {
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_extend, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene, in);
 }
-template <bool G, bool TL>
-static void LaunchOne(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL((k_shade_path<G, -1, TL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+// TX: texture code compiled in.  The four specialised material kernels need
+// it only for textured emitters (textured BSDFs are queued as "others"); the
+// generic and single kernels need it for textured BSDFs as well.
+template <bool G, int MAT>
+static void LaunchMat(bool tx, const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
+  if (tx)
+    hipLaunchKernelGGL((k_shade_path<G, MAT, true>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  else
+    hipLaunchKernelGGL((k_shade_path<G, MAT, false>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  const bool g = (w.flags & (WF_GUIDE | WF_RECORD)) != 0;
-  if (scene.tex_lights)
-    g ? LaunchOne<true, true>(w, scene, in, c, s) : LaunchOne<false, true>(w, scene, in, c, s);
+  const bool tx = scene.tex_lights || scene.tex_bsdfs;
+  if (w.flags & (WF_GUIDE | WF_RECORD))
+    LaunchMat<true, -1>(tx, w, scene, in, c, s);
   else
-    g ? LaunchOne<true, false>(w, scene, in, c, s) : LaunchOne<false, false>(w, scene, in, c, s);
+    LaunchMat<false, -1>(tx, w, scene, in, c, s);
 }
-template <bool G, bool TL>
+template <bool G>
 static void LaunchMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
                             hipStream_t s) {
+  const bool tl = scene.tex_lights != 0, tx = tl || scene.tex_bsdfs;
   // the diffuse kernel also takes the misses, so it always runs
-  hipLaunchKernelGGL((k_shade_path<G, BSDF_DIFFUSE, TL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
-  if (mask & (1u << BSDF_ROUGH_METAL))
-    hipLaunchKernelGGL((k_shade_path<G, BSDF_ROUGH_METAL, TL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
-  if (mask & (1u << BSDF_ROUGH_GLASS))
-    hipLaunchKernelGGL((k_shade_path<G, BSDF_ROUGH_GLASS, TL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
-  if (mask & (1u << BSDF_DISNEY))
-    hipLaunchKernelGGL((k_shade_path<G, BSDF_DISNEY, TL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
-  if (mask & (1u << kMatOther))
-    hipLaunchKernelGGL((k_shade_path<G, kMatOther, TL>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  LaunchMat<G, BSDF_DIFFUSE>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_ROUGH_METAL)) LaunchMat<G, BSDF_ROUGH_METAL>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_ROUGH_GLASS)) LaunchMat<G, BSDF_ROUGH_GLASS>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_DISNEY)) LaunchMat<G, BSDF_DISNEY>(tl, w, scene, in, c, s);
+  if (mask & (1u << kMatOther)) LaunchMat<G, kMatOther>(tx, w, scene, in, c, s);
 }
 
 void LaunchQueueCheck(const Wavefront& w, const DevScene& scene, int in, uint32_t* tally, bool selftest,
@@ -808,11 +811,10 @@ void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const Lau
 
 void LaunchShadeMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
                           hipStream_t s) {
-  const bool g = (w.flags & (WF_GUIDE | WF_RECORD)) != 0;
-  if (scene.tex_lights)
-    g ? LaunchMaterials<true, true>(w, scene, in, mask, c, s) : LaunchMaterials<false, true>(w, scene, in, mask, c, s);
+  if (w.flags & (WF_GUIDE | WF_RECORD))
+    LaunchMaterials<true>(w, scene, in, mask, c, s);
   else
-    g ? LaunchMaterials<true, false>(w, scene, in, mask, c, s) : LaunchMaterials<false, false>(w, scene, in, mask, c, s);
+    LaunchMaterials<false>(w, scene, in, mask, c, s);
 }
 
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {

@@ -200,6 +200,7 @@ struct DevScene {
   int32_t env_light;
   float light_pdf;
   uint32_t tex_lights;  // 1: some light's radiance (irradiance) is a texture
+  uint32_t tex_bsdfs;   // 1: some BSDF has a textured parameter
   const float* light_cdf;
   const MediumRec* media;
   int32_t global_medium;

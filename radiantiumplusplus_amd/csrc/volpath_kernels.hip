@@ -34,6 +34,8 @@ __device__ __forceinline__ uint32_t shadow_key(uint32_t key, uint32_t dim) { ret
 // scattering with phase-function NEE and sampling, and the surface branch
 // (emission with MIS, pass-through medium boundaries that are not counted,
 // BSDF NEE and sampling).  NEE contributions are finished by k_vol_shadow.
+// TX: texture code compiled in (the scene has textured emitters or BSDFs).
+template <bool TX>
 __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
           float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
           uint32_t sdim = dim;
           dim += 3;
-          LightSample ls = sample_light(scene, mei.p, ul, u1, u2, quirks);
+          LightSample ls = sample_light<TX>(scene, mei.p, ul, u1, u2, quirks);
           if (ls.pdf > 0.f) {
             float ph = phase_eval(M, wi, ls.dir);
             float wgt = ls.delta ? 1.f : mis_weight(ls.pdf, ph);
@@ -154,8 +156,8 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
           if (geom == 0xffffffffu) {  // escaped: environment emission, then stop
             if (scene.env_light >= 0) {
               float wgt = 1.f;
-              if (!spec) wgt = mis_weight(prevPdf, light_pdf_dir(scene, scene.env_light, prevP, rd, rd, rd, 0.f, 0.f, 0.f, quirks));
-              L += mul(T, skybox_radiance(scene, scene.lights[scene.env_light], rd)) * wgt;
+              if (!spec) wgt = mis_weight(prevPdf, light_pdf_dir<TX>(scene, scene.env_light, prevP, rd, rd, rd, 0.f, 0.f, 0.f, quirks));
+              L += mul(T, skybox_radiance<TX>(scene, scene.lights[scene.env_light], rd)) * wgt;
             }
           } else {
             SurfaceHit si = surface_hit(scene, ro, rd, hh.x, hh.y, hh.z, __float_as_uint(hh.w), geom);
@@ -166,10 +168,10 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
                 f3 rel = si.p - prevP;
                 float dist = norm(rel);
                 f3 dir = rel / dist;
-                wgt = mis_weight(prevPdf, light_pdf_dir(scene, sr.light, prevP, si.p, si.sh.n, dir, dist, hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f, quirks));
+                wgt = mis_weight(prevPdf, light_pdf_dir<TX>(scene, sr.light, prevP, si.p, si.sh.n, dir, dist, hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f, quirks));
               }
               f3 le = dot(si.sh.to_world(si.wi), si.n) >= 0.f
-                    ? emitter_radiance(scene, scene.lights[sr.light], hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f)
+                    ? emitter_radiance<TX>(scene, scene.lights[sr.light], hh.y * 0.5f + 0.5f, hh.z * 0.5f + 0.5f)
                     : splat(0.f);
               L += mul(T, le) * wgt;
             } else if (sr.bsdf < 0) {  // medium boundary without a BSDF: pass through (vol_path.cpp:171-176)
@@ -183,18 +185,20 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
               const BsdfRec* all = scene.bsdfs;
               const BsdfRec* bp0 = &scene.bsdfs[sr.bsdf];
               BsdfRec chain[3];
-              if (bp0->tex_any) {  // vol_path.cpp:28 drops the camera ray's differentials
-                const TexCoord tc = hit_texcoord(scene, si, hh, geom, false, 0.f, 0.f);
-                resolve_chain(scene, *bp0, tc, chain);
-                all = chain;
-                bp0 = &chain[0];
+              if constexpr (TX) {
+                if (bp0->tex_any) {  // vol_path.cpp:28 drops the camera ray's differentials
+                  const TexCoord tc = hit_texcoord(scene, si, hh, geom, false, 0.f, 0.f);
+                  resolve_chain(scene, *bp0, tc, chain);
+                  all = chain;
+                  bp0 = &chain[0];
+                }
               }
               const BsdfRec& b = *bp0;
               if (b.lobes & LOBE_NO_DELTA) {  // NEE through media (vol_path.cpp:178-190)
                 float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
                 uint32_t sdim = dim;
                 dim += 3;
-                LightSample ls = sample_light(scene, si.p, ul, u1, u2, quirks);
+                LightSample ls = sample_light<TX>(scene, si.p, ul, u1, u2, quirks);
                 if (ls.pdf > 0.f) {
                   f3 wo = si.sh.to_local(ls.dir);
                   float bp;
@@ -353,7 +357,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_vol_shadow(Wavefront w, DevScen
 }
 
 void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade_vol, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  if (scene.tex_lights || scene.tex_bsdfs)
+    hipLaunchKernelGGL(k_shade_vol<true>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  else
+    hipLaunchKernelGGL(k_shade_vol<false>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 void LaunchVolShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_vol_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);
