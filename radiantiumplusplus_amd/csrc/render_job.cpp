@@ -86,6 +86,13 @@ float GuideMinRough() {
   const char* e = std::getenv("RAD_GUIDE_MINROUGH");
   return e ? float(std::atof(e)) : kGuideMinRough;
 }
+// Share of the samples spent in training passes, 1 / RAD_GUIDE_TRAIN_DIV
+// (default 8: on the door-gap room 1/4 gave the same error at 2 % more time,
+// 1/16 a higher error).
+uint32_t TrainDiv() {
+  const char* e = std::getenv("RAD_GUIDE_TRAIN_DIV");
+  return e ? uint32_t(std::max(1, std::atoi(e))) : 8u;
+}
 // RAD_VISIT_BUDGET overrides kVisitBudget; 0 lets every traversal finish in
 // the launch that started it.
 uint32_t VisitBudget() {
@@ -143,13 +150,13 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
 // Sample passes: without training, as few passes as the result planes allow,
 // of equal size; with online training, first passes of r, 2r, 4r, ... rows
 // (each followed by training on the records it produced; r rows hold about
-// 2M paths, enough records for a useful first step) until a quarter of the
-// samples are spent, then the rest.
+// 2M paths, enough records for a useful first step) until an eighth of the
+// samples (1 / RAD_GUIDE_TRAIN_DIV) are spent, then the rest.
 void PlanPasses(rad_job* job, uint32_t npix, uint32_t maxRows) {
   const uint32_t spp = job->desc.spp;
   uint32_t done = 0;
   if (job->desc.guiding == 1) {
-    const uint32_t budget = std::max(1u, spp / 4);
+    const uint32_t budget = std::max(1u, spp / TrainDiv());
     uint32_t r = std::max(1u, std::min(budget, (kFirstTrainPaths + npix - 1) / npix));
     while (done + r <= budget && r <= maxRows) {
       job->pass_rows.push_back(r);
