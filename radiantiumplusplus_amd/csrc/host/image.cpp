@@ -437,6 +437,7 @@ ImageF ReadExr(const std::string& path) {
     chanOff[c] = lineBytes;
     lineBytes += size_t(W) * (chans[c].type == 1 ? 2 : 4);
   }
+  if (lineBytes > (size_t(1) << 26)) Bad(path, "scanline too large");
   const int64_t blocks = (H + linesPerBlock - 1) / linesPerBlock;
   if (pos + size_t(blocks) * 8 > f.size()) Bad(path, "truncated offset table");
   ImageF img;
