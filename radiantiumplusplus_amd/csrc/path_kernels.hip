@@ -64,14 +64,13 @@ __device__ __forceinline__ void finish_records(const Wavefront& w, uint32_t s, f
   w.glast[s] = w.gpend[s] = 0xffffffffu;
 }
 
-// Store the sample a dead slot finished into the result planes and start the
+// Store the sample a dead slot finished into the result entries and start the
 // next camera sample of the pass on it (sample_renderer.cpp:97-99).
 __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[CNT_DEAD];
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t pass = w.rows * w.npix;
-  const size_t plane = size_t(pass);
   const bool more = w.cnt[CNT_NEXT] < pass;  // stale is fine: only skips atomics once exhausted
   const uint32_t W = uint32_t(scene.cam.width);
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -84,10 +83,8 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
       uint4 m = w.meta[s];
       if (m.y != SAMPLE_NONE) {
         float4 l = w.lsum[s];
-        size_t r = size_t(m.y - w.row0) * w.npix + m.x;
-        w.res[r] = l.x;
-        w.res[r + plane] = l.y;
-        w.res[r + 2 * plane] = l.z;
+        // one 16-byte entry per sample: a single line written per finished sample
+        w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(l.x, l.y, l.z, 0.f);
         finished = true;
         if (w.flags & WF_RECORD) finish_records(w, s, mk(l.x, l.y, l.z));
       }
@@ -635,11 +632,10 @@ __global__ void k_accum(Wavefront w) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= w.npix) return;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
-  const size_t plane = size_t(w.rows) * w.npix;
   float4 a = w.pacc[p], a2 = w.pacc2[p];
   for (uint32_t r = 0; r < w.rows; r++) {
-    size_t k = size_t(r) * w.npix + p;
-    float x = w.res[k], y = w.res[k + plane], z = w.res[k + 2 * plane];
+    const float4 v = w.res[size_t(r) * w.npix + p];
+    const float x = v.x, y = v.y, z = v.z;
     bool bad = isnan(x) || isnan(y) || isnan(z);
     if (quirks)
       bad = bad || isinf(x) || x < 0.f;

@@ -55,7 +55,7 @@ namespace rad {
 namespace {
 
 constexpr uint64_t kDefaultSlots = 4u << 20;     // paths in flight
-constexpr uint64_t kResultBudget = 2ull << 30;   // bytes of result planes per pass
+constexpr uint64_t kResultBudget = 3ull << 30;   // bytes of per-sample results per pass
 constexpr uint32_t kRecordCap = 2u << 20;        // guide training records per pass
 constexpr uint32_t kRecordSlots = 512u << 10;    // slots that record training data
 constexpr uint32_t kTrainBatch = 8192;           // records per training step
@@ -125,7 +125,7 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   ImageSize(scene, desc, W, H);
   j.npix = uint32_t(W) * uint32_t(H);
   const uint64_t spp = std::max(1u, desc.spp);
-  uint64_t rows = kResultBudget / (uint64_t(j.npix) * 12);
+  uint64_t rows = kResultBudget / (uint64_t(j.npix) * 16);
   rows = std::min<uint64_t>(std::max<uint64_t>(rows, 1), spp);
   rows = std::min<uint64_t>(rows, 0x7fffffffull / j.npix);
   if (rows == 0) throw StatusError(RAD_ERR_INVALID_ARG, "image too large");
@@ -136,7 +136,7 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
   j.resume_cap = desc.integrator == RAD_INTEGRATOR_PATH && VisitBudget() > 0 ? uint32_t(std::min<uint64_t>(Q, kResumeCap)) : 0;
-  j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * (6 + kMatQueues) + Align(pass * 4) * 3 + Align(uint64_t(j.npix) * 16) * 2 +
+  j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * (6 + kMatQueues) + Align(pass * 16) + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
   if (j.resume_cap) j.bytes += Align(Q * 4) + 2 * Align(uint64_t(j.resume_cap) * sizeof(ResumeState));
   if (desc.flags & RAD_RENDER_CHECK_QUEUES) j.bytes += Align(Q * 4);
@@ -147,7 +147,7 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   return j;
 }
 
-// Sample passes: without training, as few passes as the result planes allow,
+// Sample passes: without training, as few passes as the result entries allow,
 // of equal size; with online training, first passes of r, 2r, 4r, ... rows
 // (each followed by training on the records it produced; r rows hold about
 // 2M paths, enough records for a useful first step) until an eighth of the
@@ -427,7 +427,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.queue[1] = (uint32_t*)take(u4);
   w.dead = (uint32_t*)take(u4);
   for (int k = 0; k < kMatQueues; k++) w.mq[k] = (uint32_t*)take(u4);
-  w.res = (float*)take(size_t(js.rows) * npix * 4 * 3);
+  w.res = (float4*)take(size_t(js.rows) * npix * 16);
   w.pacc = (float4*)take(size_t(npix) * 16);
   w.pacc2 = (float4*)take(size_t(npix) * 16);
   w.cnt = (uint32_t*)take(CNT_WORDS * 4 + ST_WORDS * 8);

@@ -1,15 +1,15 @@
 // Wavefront state of a render job: structure-of-arrays path records in HBM,
-// compacted work queues and the per-sample result planes.
+// compacted work queues and the per-sample result entries.
 //
 // A job owns Q path slots that are not tied to pixels.  Camera samples are
 // handed out from a global counter in sample-major order (id -> pixel
 // id % pixels, sample row0 + id / pixels), so every slot stays busy until
 // the pass runs out of samples and the primary rays of a wave are adjacent
-// pixels.  A finished sample's radiance goes to its own entry of the result
-// planes; after the pass, k_accum adds the entries of each pixel in sample
+// pixels.  A finished sample's radiance goes to its own 16-byte result
+// entry; after the pass, k_accum adds the entries of each pixel in sample
 // order (the order of sample_renderer.cpp:97-105), so the image does not
 // depend on which slot ran which sample.  Each iteration runs
-//   regen   (finished samples -> result planes, dead slots -> next sample)
+//   regen   (finished samples -> result entries, dead slots -> next sample)
 //   extend  (closest hit for every queued path)
 //   shade   (emission + MIS, NEE shadow-ray emission, BSDF sampling, RR)
 //   shadow  (occlusion test, add unoccluded NEE contributions)
@@ -73,7 +73,7 @@ struct Wavefront {
   float4* sh_o;    // shadow origin, tmax
   float4* sh_d;    // shadow direction, slot (bits)
   float4* sh_c;    // shadow contribution
-  float* res;      // result planes: R, G, B of rows * npix samples each
+  float4* res;     // result entries: RGB (+ padding) of rows * npix samples
   float4* pacc;    // per-pixel radiance sum
   float4* pacc2;   // per-pixel squared-radiance sum
   uint32_t* cnt;   // CNT_* words
