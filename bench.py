@@ -332,7 +332,7 @@ def main():
     launch = [int(x) for x in args.launch.split(",")] if args.launch else None
     if launch is None and not args.no_autotune:
         # launch configuration picked by timing short renders (images do not depend on it)
-        rep = scene.autotune(b.render_desc(spp=max(1, min(spp, (32 << 20) // (W * H))), seed=1))
+        rep = scene.autotune(b.render_desc(spp=max(1, min(spp, (128 << 20) // (W * H))), seed=1))
         tune = {"path_slots": rep.path_slots, "blocks_per_cu": rep.blocks_per_cu,
                 "trace_blocks_per_cu": rep.trace_blocks_per_cu,
                 "trials": [[t.path_slots, t.blocks_per_cu, t.trace_blocks_per_cu, round(t.ms, 3)]

@@ -253,7 +253,7 @@ rad_status rad_hip_autotune(rad_scene* scene, const rad_render_desc* probe, rad_
     trial(4u << 20, 8, 8);  // warm-up (first-use costs)
     uint32_t bestSlots = 4u << 20, bestB = 8, bestT = 8;
     float best = 3.4e38f;
-    for (uint32_t slots : {1u << 20, 2u << 20, 4u << 20, 8u << 20, 16u << 20}) {
+    for (uint32_t slots : {1u << 20, 2u << 20, 4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20}) {
       if (slots > 2 * samples && slots != (1u << 20)) continue;
       float ms = trial(slots, bestB, bestT);
       if (ms < best) best = ms, bestSlots = slots;
