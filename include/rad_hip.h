@@ -373,6 +373,8 @@ typedef struct rad_scene_info {
   float world_max[3];
   uint32_t bvh_depth;
   float sah_cost;
+  uint32_t bvh_stack_need; /* worst-case traversal stack entries of the tree (at most 64) */
+  uint32_t reserved;
 } rad_scene_info;
 
 typedef struct rad_ctx rad_ctx;

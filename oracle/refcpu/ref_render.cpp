@@ -501,7 +501,9 @@ int ref_intersect_closest(const ref_scene* s, const rad_ray* rays, uint32_t n, i
     Ray r{V3(rays[i].o[0], rays[i].o[1], rays[i].o[2]), V3(rays[i].d[0], rays[i].d[1], rays[i].d[2]), rays[i].tmin,
           rays[i].tmax};
     HitShapeRecord rec;
-    bool h = brute ? s->scene.accel.ClosestBrute(r, rec) : s->scene.accel.Closest(r, rec);
+    bool h = brute == 2 ? s->scene.accel.ClosestCulled(r, rec)
+             : brute    ? s->scene.accel.ClosestBrute(r, rec)
+                        : s->scene.accel.Closest(r, rec);
     hits[i].t = h ? rec.T : r.MaxT;
     hits[i].u = h ? rec.PrimitiveUV.x : 0;
     hits[i].v = h ? rec.PrimitiveUV.y : 0;
@@ -516,7 +518,11 @@ int ref_intersect_any(const ref_scene* s, const rad_ray* rays, uint32_t n, int b
   for (int64_t i = 0; i < (int64_t)n; i++) {
     Ray r{V3(rays[i].o[0], rays[i].o[1], rays[i].o[2]), V3(rays[i].d[0], rays[i].d[1], rays[i].d[2]), rays[i].tmin,
           rays[i].tmax};
-    occ[i] = (brute ? s->scene.accel.OccludedBrute(r) : s->scene.accel.Occluded(r)) ? 1 : 0;
+    occ[i] = (brute == 2 ? s->scene.accel.OccludedCulled(r)
+              : brute    ? s->scene.accel.OccludedBrute(r)
+                         : s->scene.accel.Occluded(r))
+                 ? 1
+                 : 0;
   }
   return 0;
 }

@@ -406,6 +406,14 @@ struct Accel {
   bool Occluded(const Ray& ray) const;
   bool ClosestBrute(const Ray& ray, HitShapeRecord& rec) const;
   bool OccludedBrute(const Ray& ray) const;
+  // Brute force over every primitive in the order of `prims`, skipping runs of
+  // kCullChunk consecutive primitives whose padded box the ray segment misses
+  // (a primitive hit at t lies inside its own box, so the result equals
+  // ClosestBrute / OccludedBrute; tests check that bit for bit).
+  static constexpr uint32_t kCullChunk = 64;
+  std::vector<Bounds> chunks;
+  bool ClosestCulled(const Ray& ray, HitShapeRecord& rec) const;
+  bool OccludedCulled(const Ray& ray) const;
   int BuildRecursive(std::vector<PrimRef>& refs, std::vector<Bounds>& pb, std::vector<V3>& pc, uint32_t begin, uint32_t end, int depth);
 };
 

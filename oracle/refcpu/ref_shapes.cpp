@@ -1009,6 +1009,13 @@ void Accel::Build(const std::vector<std::unique_ptr<Shape>>& s) {
   nodes.reserve(2 * refs.size() / 2 + 16);
   if (!refs.empty()) BuildRecursive(refs, pb, pc, 0, (uint32_t)refs.size(), 0);
   prims = std::move(refs);
+  chunks.clear();
+  for (uint32_t c = 0; c < prims.size(); c += kCullChunk) {
+    Bounds b;
+    for (uint32_t i = c; i < std::min<uint32_t>(c + kCullChunk, (uint32_t)prims.size()); i++)
+      b.Extend(s[prims[i].shape]->PrimBounds(prims[i].prim));
+    chunks.push_back(b);
+  }
   buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1089,6 +1096,62 @@ bool Accel::OccludedBrute(const Ray& ray) const {
   for (const auto& p : prims) {
     HitShapeRecord r;
     if ((*shapes)[p.shape]->IntersectPrim(p.prim, ray, ray.MinT, ray.MaxT, r)) return true;
+  }
+  return false;
+}
+
+// Conservative segment-box test in double precision: the box is padded by
+// 1e-4 of its extent plus magnitude and the segment by 1e-5 relative, far
+// beyond the float rounding of any primitive test, so a primitive that reports
+// a hit in [tmin, tmax] never sits in a rejected chunk.
+static bool SegmentMayHit(const Bounds& b, const Ray& ray, Float tmax) {
+  double t0 = double(ray.MinT) - 1e-5 * (1.0 + std::fabs(double(ray.MinT)));
+  double t1 = double(tmax) + 1e-5 * (1.0 + std::fabs(double(tmax)));
+  for (int a = 0; a < 3; a++) {
+    double lo = b.lo[a], hi = b.hi[a];
+    double pad = 1e-4 * ((hi - lo) + std::fabs(lo) + std::fabs(hi) + 1.0);
+    lo -= pad;
+    hi += pad;
+    double o = ray.O[a], d = ray.D[a];
+    if (d == 0.0) {
+      if (o < lo || o > hi) return false;
+      continue;
+    }
+    double tn = (lo - o) / d, tf = (hi - o) / d;
+    if (tn > tf) std::swap(tn, tf);
+    t0 = std::max(t0, tn);
+    t1 = std::min(t1, tf);
+    if (!(t0 <= t1)) return false;
+  }
+  return true;
+}
+
+bool Accel::ClosestCulled(const Ray& ray, HitShapeRecord& rec) const {
+  Float tfar = ray.MaxT;
+  bool hit = false;
+  for (uint32_t c = 0; c < chunks.size(); c++) {
+    if (!SegmentMayHit(chunks[c], ray, tfar)) continue;
+    const uint32_t end = std::min<uint32_t>((c + 1) * kCullChunk, (uint32_t)prims.size());
+    for (uint32_t i = c * kCullChunk; i < end; i++) {
+      HitShapeRecord r;
+      if ((*shapes)[prims[i].shape]->IntersectPrim(prims[i].prim, ray, ray.MinT, tfar, r)) {
+        rec = r;
+        tfar = r.T;
+        hit = true;
+      }
+    }
+  }
+  return hit;
+}
+
+bool Accel::OccludedCulled(const Ray& ray) const {
+  for (uint32_t c = 0; c < chunks.size(); c++) {
+    if (!SegmentMayHit(chunks[c], ray, ray.MaxT)) continue;
+    const uint32_t end = std::min<uint32_t>((c + 1) * kCullChunk, (uint32_t)prims.size());
+    for (uint32_t i = c * kCullChunk; i < end; i++) {
+      HitShapeRecord r;
+      if ((*shapes)[prims[i].shape]->IntersectPrim(prims[i].prim, ray, ray.MinT, ray.MaxT, r)) return true;
+    }
   }
   return false;
 }

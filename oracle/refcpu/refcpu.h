@@ -29,6 +29,8 @@ ref_scene* ref_scene_create(const rad_scene_desc* desc, int quirks);
 void ref_scene_destroy(ref_scene* s);
 double ref_scene_build_ms(const ref_scene* s);
 void ref_scene_bounds(const ref_scene* s, float* lo, float* hi);
+/* brute: 0 = the oracle's BVH, 1 = every primitive, 2 = every primitive with
+   exact chunk culling (same result as 1) */
 int ref_intersect_closest(const ref_scene* s, const rad_ray* rays, uint32_t n, int brute, rad_hit* hits);
 int ref_intersect_any(const ref_scene* s, const rad_ray* rays, uint32_t n, int brute, uint8_t* occluded);
 float ref_intersect_prim(const ref_scene* s, const rad_ray* ray, uint32_t geom, uint32_t prim);

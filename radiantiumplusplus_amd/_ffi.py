@@ -272,6 +272,8 @@ class SceneInfo(C.Structure):
         ("world_max", F3),
         ("bvh_depth", C.c_uint32),
         ("sah_cost", C.c_float),
+        ("bvh_stack_need", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

@@ -284,6 +284,7 @@ void BuildBvh4Host(const float* bounds, uint32_t count, int max_leaf, BvhBuildRe
   }
   out.order = std::move(c.ref);
   out.depth = BvhDepth(out.nodes);
+  out.stack_need = BvhStackNeed(out.nodes);
   out.sah_cost = BvhSahCost(out.nodes);
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -336,6 +337,32 @@ uint32_t BvhDepth(const std::vector<BvhNode4>& nodes) {
       if (nodes[n].child[i] >= 0 && nodes[n].count[i] == 0) st.push_back({nodes[n].child[i], k + 1});
   }
   return d;
+}
+
+uint32_t BvhStackNeed(const std::vector<BvhNode4>& nodes) {
+  if (nodes.empty()) return 0;
+  // post-order over the inner nodes with an explicit stack
+  std::vector<uint32_t> need(nodes.size(), 0);
+  std::vector<std::pair<uint32_t, bool>> st{{0u, false}};
+  while (!st.empty()) {
+    auto [n, expanded] = st.back();
+    st.pop_back();
+    const BvhNode4& nd = nodes[n];
+    if (!expanded) {
+      st.push_back({n, true});
+      for (int i = 0; i < 4; i++)
+        if (nd.child[i] >= 0 && nd.count[i] == 0) st.push_back({uint32_t(nd.child[i]), false});
+      continue;
+    }
+    uint32_t inner = 0, deepest = 0;
+    for (int i = 0; i < 4; i++)
+      if (nd.child[i] >= 0 && nd.count[i] == 0) {
+        inner++;
+        deepest = std::max(deepest, need[nd.child[i]]);
+      }
+    need[n] = inner ? inner - 1 + deepest : 0;
+  }
+  return need[0];
 }
 
 }  // namespace rad

@@ -14,6 +14,7 @@ struct BvhBuildResult {
   std::vector<BvhNode4> nodes;
   std::vector<uint32_t> order;  // leaf-order position -> input primitive index
   uint32_t depth = 0;
+  uint32_t stack_need = 0;  // traversal stack entries the tree can need (kTraceStackEntries)
   float sah_cost = 0;
   double build_ms = 0;
 };
@@ -25,5 +26,7 @@ void BuildBvh4Host(const float* bounds, uint32_t count, int max_leaf, BvhBuildRe
 // SAH cost of a finished tree (traversal 1, intersection 1, relative to root area).
 float BvhSahCost(const std::vector<BvhNode4>& nodes);
 uint32_t BvhDepth(const std::vector<BvhNode4>& nodes);
+// Worst-case traversal stack entries of the tree (see kTraceStackEntries).
+uint32_t BvhStackNeed(const std::vector<BvhNode4>& nodes);
 
 }  // namespace rad

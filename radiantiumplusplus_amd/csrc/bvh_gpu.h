@@ -11,6 +11,7 @@ namespace rad {
 struct DeviceBvhStats {
   uint32_t node_count = 0;  // 4-wide nodes
   uint32_t depth = 0;       // 4-wide levels
+  uint32_t stack_need = 0;  // traversal stack entries the tree can need (kTraceStackEntries)
   uint32_t levels = 0;      // binary build levels
   float sah_cost = 0;
   double build_ms = 0;

@@ -660,6 +660,21 @@ __global__ void k_collapse(const Job* jobs, uint32_t nJobs, const N2* n2, BvhNod
   out[jb.n4] = o;
 }
 
+// Worst-case traversal stack entries of the nodes [begin, end) of one level,
+// from those of the next level (launched deepest level first).
+__global__ void k_stack_need(const BvhNode4* nodes, uint32_t begin, uint32_t end, uint32_t* need) {
+  const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  const BvhNode4& nd = nodes[i];
+  uint32_t inner = 0, deepest = 0;
+  for (int s = 0; s < 4; s++)
+    if (nd.child[s] >= 0 && nd.count[s] == 0) {
+      inner++;
+      deepest = max(deepest, need[nd.child[s]]);
+    }
+  need[i] = inner ? inner - 1 + deepest : 0u;
+}
+
 __global__ void k_gather_prims(const float4* rec, const uint32_t* ids, uint32_t n, float4* out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -930,8 +945,10 @@ void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int
   uint32_t nJobs = 1, depth = 0;
   Job* jcur = dJobs0.as<Job>();
   Job* jnxt = dJobs1.as<Job>();
+  std::vector<uint32_t> levelStart{0u};  // the nodes of level k are [levelStart[k], levelStart[k + 1])
   while (nJobs > 0) {
     depth++;
+    levelStart.push_back(jc[0]);
     jc[1] = 0;
     HipCheck(hipMemcpyAsync(dJc.ptr, jc, 8, hipMemcpyHostToDevice, st), "collapse counters");
     hipLaunchKernelGGL(k_collapse, dim3(Grid(nJobs, 64)), dim3(64), 0, st, jcur, nJobs, dN2.as<N2>(),
@@ -942,6 +959,19 @@ void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int
     std::swap(jcur, jnxt);
   }
   const uint32_t n4 = jc[0];
+  levelStart.push_back(n4);
+  uint32_t stackNeed = 0;
+  {
+    DeviceBuffer dNeed;
+    dNeed.alloc(size_t(n4) * sizeof(uint32_t));
+    for (size_t k = levelStart.size() - 1; k-- > 0;) {
+      const uint32_t b = levelStart[k], e = levelStart[k + 1];
+      if (e > b) hipLaunchKernelGGL(k_stack_need, dim3(Grid(e - b)), dim3(256), 0, st, nodesOut.as<BvhNode4>(), b, e,
+                                    dNeed.as<uint32_t>());
+    }
+    HipCheck(hipMemcpyAsync(&stackNeed, dNeed.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "stack need");
+    HipCheck(hipStreamSynchronize(st), "stack need");
+  }
   primsOut.alloc(size_t(n) * 48);
   hipLaunchKernelGGL(k_gather_prims, dim3(Grid(n)), dim3(256), 0, st, dRec.as<float4>(), sorted, n, primsOut.as<float4>());
   // statistics: SAH cost relative to the root box
@@ -970,6 +1000,7 @@ void BuildBvh4Device(const float* bounds, const float4* records, uint32_t n, int
   HipCheck(hipGetLastError(), "BVH kernels");
   stats.node_count = n4;
   stats.depth = depth;
+  stats.stack_need = stackNeed;
   stats.levels = levels;
   stats.sah_cost = float(sah);
   stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -184,7 +184,7 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
     HostScene& h = scene->host;
     uint32_t n = uint32_t(h.prim_bounds.size() / 6);
     auto t0 = std::chrono::steady_clock::now();
-    uint32_t nodeCount = 0, depth = 0;
+    uint32_t nodeCount = 0, depth = 0, stackNeed = 0;
     float sah = 0.f;
     if (builder == RAD_BVH_DEVICE) {
       DeviceBvhStats ds;
@@ -192,6 +192,7 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
                       scene->d_prims, ds);
       nodeCount = ds.node_count;
       depth = ds.depth;
+      stackNeed = ds.stack_need;
       sah = ds.sah_cost;
     } else {
       BvhBuildResult r;
@@ -204,7 +205,17 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
       HipCheck(hipStreamSynchronize(scene->ctx->stream), "BVH upload");
       nodeCount = uint32_t(r.nodes.size());
       depth = r.depth;
+      stackNeed = r.stack_need;
       sah = r.sah_cost;
+    }
+    // The traversal keeps kTraceStackEntries stack entries per ray; a deeper
+    // need would drop subtrees (missed hits, light leaks), so such a tree is
+    // refused.  The scene keeps no half-built tree.
+    if (stackNeed > kTraceStackEntries) {
+      scene->bvh_ready = false;
+      throw StatusError(RAD_ERR_UNSUPPORTED, "BVH needs " + std::to_string(stackNeed) +
+                                                 " traversal stack entries; the traversal holds " +
+                                                 std::to_string(kTraceStackEntries));
     }
     QuantizeBvh4(scene->d_nodes, nodeCount, scene->ctx->stream, scene->d_qnodes);
     scene->builder = builder;
@@ -215,6 +226,7 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
     scene->dev.prim_count = n;
     scene->dev.node_count = nodeCount;
     scene->bvh_depth = depth;
+    scene->bvh_stack_need = stackNeed;
     scene->sah_cost = sah;
     scene->bvh_ready = true;
   });
@@ -392,6 +404,7 @@ rad_status rad_hip_scene_info(const rad_scene* scene, rad_scene_info* info) {
   }
   info->bvh_depth = scene->bvh_depth;
   info->sah_cost = scene->sah_cost;
+  info->bvh_stack_need = scene->bvh_stack_need;
   return RAD_OK;
 }
 

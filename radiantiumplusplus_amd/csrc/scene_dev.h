@@ -27,6 +27,13 @@ struct alignas(128) BvhNode4 {
 };
 static_assert(sizeof(BvhNode4) == 128, "BVH node must be one 128-byte line");
 
+// Entries of a traversal stack (node, entry distance).  A node visit pushes
+// all but the nearest of its hit inner children, so a tree needs at most
+// need(n) = (inner children of n - 1) + max over inner children c of need(c)
+// entries (0 for a node whose children are all leaves); the builders compute
+// need(root) and a tree that needs more than this is refused.
+constexpr uint32_t kTraceStackEntries = 64;
+
 // The traversal's copy of a BvhNode4 in 64 bytes: the node box origin, a
 // power-of-two step per axis, and every child's bounds as 8-bit multiples of
 // the step, rounded outward, so each child box contains the exact one.  A

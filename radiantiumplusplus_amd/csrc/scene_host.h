@@ -92,6 +92,7 @@ struct rad_scene {
   bool bvh_ready = false;
   double bvh_ms = 0;
   uint32_t bvh_depth = 0;
+  uint32_t bvh_stack_need = 0;
   int32_t builder = -1;
   float sah_cost = 0;
   std::shared_ptr<rad::guide::GuideState> guide;  // created on first guided render

@@ -134,7 +134,7 @@ __device__ __forceinline__ bool isect_prim(const DevScene& s, uint32_t p, f3 o, 
 // newest kLdsStack entries in LDS (a ring, lane-interleaved so a wave's
 // accesses hit distinct banks), older ones spilled to a private overflow
 // array that only deep traversals touch.
-constexpr int kStackDepth = 64;
+constexpr int kStackDepth = int(kTraceStackEntries);
 constexpr int kLdsStack = 16;
 constexpr int kTraceBlock = 256;  // threads per block of every traversal kernel
 
@@ -146,7 +146,7 @@ struct TraceStack {
   int sp, lo;  // entries [0, lo) live in the overflow array, [lo, sp) in LDS
 
   __device__ __forceinline__ void push(uint32_t n, float d) {
-    if (sp >= kStackDepth) return;  // unreachable for trees of depth <= kStackDepth / 3
+    if (sp >= kStackDepth) return;  // unreachable: the builders refuse trees that need more entries
     if (sp - lo == kLdsStack) {
       int k = (lo & (kLdsStack - 1)) * kTraceBlock;
       ov_node[lo] = node[k];

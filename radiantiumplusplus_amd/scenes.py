@@ -157,28 +157,37 @@ def fbm_heightfield(m, n, octaves, amplitude, seed):
 
 
 def perlin3d(res, octaves, seed):
-    """Seeded gradient (Perlin) noise fBm on a res^3 lattice, x fastest in memory."""
+    """Seeded gradient (Perlin) noise fBm on a res^3 lattice, x fastest in memory.
+
+    Per octave the noise is a sum over the 8 lattice corners of quintic-fade
+    weights times gradient . offset; with the per-axis weights written as
+    sparse res x (freq + 1) matrices (W: fade weight, P: fade weight times the
+    offset along that axis) each gradient component contracts separably,
+    n[z, y, x] = sum_c sum_kji Mz[z, k] My[y, j] Mx[x, i] g[k, j, i, c]."""
     rng = np.random.default_rng(seed)
     coords = (np.arange(res, dtype=np.float64) + 0.5) / res
     out = np.zeros((res, res, res), np.float64)  # [z, y, x]
     amp, freq = 1.0, 4
-    z, y, x = np.meshgrid(coords, coords, coords, indexing="ij")
+    ar = np.arange(res)
+
+    def contract(G, Mz, My, Mx):
+        T = np.tensordot(G, Mx, axes=([2], [1]))  # (k, j, x)
+        T = np.tensordot(T, My, axes=([1], [1]))  # (k, x, y)
+        return np.tensordot(Mz, T, axes=([1], [0])).transpose(0, 2, 1)  # (z, y, x)
+
     for _ in range(octaves):
         g = rng.normal(size=(freq + 1, freq + 1, freq + 1, 3))
         g /= np.linalg.norm(g, axis=-1, keepdims=True)
-        fx, fy, fz = x * freq, y * freq, z * freq
-        ix, iy, iz = (np.minimum(a.astype(np.int64), freq - 1) for a in (fx, fy, fz))
-        tx, ty, tz = fx - ix, fy - iy, fz - iz
-        sx, sy, sz = (t * t * t * (t * (t * 6 - 15) + 10) for t in (tx, ty, tz))
-        acc = 0.0
-        for dz in (0, 1):
-            for dy in (0, 1):
-                for dx in (0, 1):
-                    gr = g[iz + dz, iy + dy, ix + dx]
-                    dot = gr[..., 0] * (tx - dx) + gr[..., 1] * (ty - dy) + gr[..., 2] * (tz - dz)
-                    w = (sx if dx else 1 - sx) * (sy if dy else 1 - sy) * (sz if dz else 1 - sz)
-                    acc = acc + w * dot
-        out += amp * acc
+        f = coords * freq
+        i = np.minimum(f.astype(np.int64), freq - 1)
+        t = f - i
+        s = t * t * t * (t * (t * 6 - 15) + 10)
+        W = np.zeros((res, freq + 1))
+        W[ar, i], W[ar, i + 1] = 1 - s, s
+        P = np.zeros((res, freq + 1))
+        P[ar, i], P[ar, i + 1] = (1 - s) * t, s * (t - 1)
+        # g[..., 0] pairs with the x offset, 1 with y, 2 with z
+        out += amp * (contract(g[..., 0], W, W, P) + contract(g[..., 1], W, P, W) + contract(g[..., 2], P, W, W))
         amp *= 0.5
         freq *= 2
     return out.astype(np.float32)

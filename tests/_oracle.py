@@ -89,16 +89,22 @@ class OracleScene:
         lib().ref_scene_bounds(self.h, lo, hi)
         return np.array(lo[:]), np.array(hi[:])
 
+    @staticmethod
+    def _mode(brute):
+        # False: the oracle's BVH; True: every primitive; "culled": every
+        # primitive with exact chunk culling (same result as True)
+        return 2 if brute == "culled" else (1 if brute else 0)
+
     def closest(self, rays, brute=False):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.zeros(len(rays), HIT_DTYPE)
-        lib().ref_intersect_closest(self.h, rays.ctypes.data, len(rays), 1 if brute else 0, hits.ctypes.data)
+        lib().ref_intersect_closest(self.h, rays.ctypes.data, len(rays), self._mode(brute), hits.ctypes.data)
         return hits
 
     def occluded(self, rays, brute=False):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         out = np.zeros(len(rays), np.uint8)
-        lib().ref_intersect_any(self.h, rays.ctypes.data, len(rays), 1 if brute else 0, out.ctypes.data)
+        lib().ref_intersect_any(self.h, rays.ctypes.data, len(rays), self._mode(brute), out.ctypes.data)
         return out
 
     def prim_t(self, ray, geom, prim):

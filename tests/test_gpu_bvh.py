@@ -95,3 +95,61 @@ def test_device_bvh_render_matches_host_bvh(hip_ctx):
     c = gh.render(b.render_desc(spp=64, seed=3))[0]
     # same closest hits up to distance ties: identical samples almost everywhere
     assert np.mean(a != c) < 1e-3
+
+
+def nested_clumps(levels, seed=0):
+    """Clumps of six small triangles at geometrically shrinking scales: at
+    each scale three clumps sit around a smaller copy of the rest, so every
+    4-wide level can hold three inner siblings and the traversal stack need
+    grows with the number of levels."""
+    rng = np.random.default_rng(seed)
+    pts, centres = [], []
+    for i in range(levels):
+        s = 0.45 ** i
+        for axis in range(3):
+            c = np.zeros(3)
+            c[axis] = 0.8 * s
+            centres.append(c)
+            for _ in range(6):
+                p0 = c + rng.normal(0, 0.04 * s, 3)
+                pts += [p0, p0 + rng.normal(0, 0.03 * s, 3), p0 + rng.normal(0, 0.03 * s, 3)]
+    b = scenes.SceneBuilder()
+    b.mesh(np.array(pts, np.float32), np.arange(len(pts), dtype=np.uint32), bsdf=b.diffuse(0.5))
+    b.skybox(1.0)
+    b.set_camera(8, 8, 45, origin=(0, 0, -5), target=(0, 0, 0), far=100)
+    return b, np.array(centres)
+
+
+@pytest.mark.parametrize("levels", [6, 14, 30])
+def test_deep_trees_are_exact_or_refused(hip_ctx, levels):
+    """A tree whose worst-case traversal stack exceeds the 64 entries the
+    traversal keeps is refused at build time (a silent overflow would drop
+    subtrees); every tree that builds matches the brute force on rays aimed
+    at the clumps of every scale."""
+    b, centres = nested_clumps(levels)
+    for builder in (F.RAD_BVH_DEVICE, F.RAD_BVH_HOST):
+        try:
+            g = hip.Scene.from_builder(hip_ctx, b, builder=builder)
+        except hip.RadError as e:
+            print(f"levels {levels} builder {builder}: refused ({e})", flush=True)
+            assert e.status == 4 and "stack" in str(e)  # RAD_ERR_UNSUPPORTED
+            continue
+        info = g.info()
+        print(f"levels {levels} builder {builder}: depth {info.bvh_depth}, stack need {info.bvh_stack_need}",
+              flush=True)
+        assert info.bvh_stack_need <= 64
+        if levels <= 6:
+            assert info.bvh_stack_need <= 3 * (info.bvh_depth - 1)
+        o = O.OracleScene(b)
+        rng = np.random.default_rng(levels)
+        n = 6000
+        rays = np.zeros(n, hip.RAY_DTYPE)
+        org = rng.normal(size=(n, 3))
+        org = 3.0 * org / np.linalg.norm(org, axis=1, keepdims=True)
+        tgt = centres[rng.integers(len(centres), size=n)] + rng.normal(0, 1e-3, (n, 3))
+        d = tgt - org
+        rays["o"] = org.astype(np.float32)
+        rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        rays["tmax"] = 3e38
+        check_parity(o, rays, g.closest(rays), o.closest(rays, brute=True))
+        assert np.array_equal(g.occluded(rays).astype(bool), o.occluded(rays, brute=True).astype(bool))

@@ -96,6 +96,29 @@ def test_bvh_matches_brute_force():
     assert (hb["geom"] != 0xFFFFFFFF).mean() > 0.2
 
 
+@pytest.mark.parametrize("kind", ["balls", "terrain"])
+def test_culled_brute_force_equals_brute_force(kind):
+    """The chunk-culled brute force (used on scenes too large for the plain
+    one) tests the same primitives in the same order wherever a hit is
+    possible, so its hits are bit-identical to the plain brute force."""
+    if kind == "balls":
+        b = scenes.many_balls(n_spheres=24, slices=16, width=8, height=8)
+    else:
+        b = scenes.large_mesh(grid=64, n_spheres=12, width=8, height=8)
+    s = O.OracleScene(b)
+    lo, hi = s.bounds()
+    if kind == "terrain":  # the bounds reach out to the distant sun: stay over the terrain
+        lo, hi = np.array([-20.0, -1.0, -20.0]), np.array([20.0, 4.0, 20.0])
+    rays = random_rays(lo, hi, 3000)
+    # short segments and grazing terrain rays as well as unbounded ones
+    rays["tmax"][::5] = RNG.uniform(0.05, 2.0, len(rays[::5]))
+    rays["d"][1::7] = unit(rays["d"][1::7] * np.array([1.0, 0.01, 1.0], np.float32))
+    hb, hc = s.closest(rays, brute=True), s.closest(rays, brute="culled")
+    assert hb.tobytes() == hc.tobytes()
+    assert np.array_equal(s.occluded(rays, brute=True), s.occluded(rays, brute="culled"))
+    assert (hb["geom"] != 0xFFFFFFFF).mean() > 0.2
+
+
 # ------------------------------------------------------------------ camera
 def test_camera_horizontal_fov_and_orientation():
     """fov is horizontal; screen x = 0 looks along +left, y = 0 along +up
