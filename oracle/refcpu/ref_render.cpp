@@ -722,4 +722,55 @@ float ref_medium_transmittance(const ref_scene* s, int medium, const rad_ray* ra
   return 0.f;
 }
 
+int ref_phase_eval(int type, float g, const float* in, uint32_t n, float* out) {
+  HomogeneousMedium m;
+  m.phaseType = type;
+  m.g = g;
+  for (uint32_t i = 0; i < n; i++) {
+    MediumInteraction mi{};
+    mi.Wi = V3(in[6 * i], in[6 * i + 1], in[6 * i + 2]);
+    mi.Shading = Frame(mi.Wi);
+    out[i] = m.PhaseEval(mi, V3(in[6 * i + 3], in[6 * i + 4], in[6 * i + 5]));
+  }
+  return 0;
+}
+
+int ref_phase_sample(int type, float g, const float* in, uint32_t n, float* out) {
+  HomogeneousMedium m;
+  m.phaseType = type;
+  m.g = g;
+  for (uint32_t i = 0; i < n; i++) {
+    MediumInteraction mi{};
+    mi.Wi = V3(in[5 * i], in[5 * i + 1], in[5 * i + 2]);
+    mi.Shading = Frame(mi.Wi);
+    auto [wo, pdf] = m.PhaseSample(mi, V2(in[5 * i + 3], in[5 * i + 4]));
+    float* o = out + 4 * i;
+    o[0] = wo.x; o[1] = wo.y; o[2] = wo.z;
+    o[3] = pdf;
+  }
+  return 0;
+}
+
+int ref_light_sample(const ref_scene* s, uint32_t light, const float* in, uint32_t n, float* out) {
+  if (light >= s->scene.lights.size()) {
+    g_err = "light index out of range";
+    return 1;
+  }
+  const Light* L = s->scene.lights[light].get();
+  for (uint32_t i = 0; i < n; i++) {
+    Interaction ref{};
+    ref.P = V3(in[5 * i], in[5 * i + 1], in[5 * i + 2]);
+    auto [dsr, li] = L->SampleDirection(ref, V2(in[5 * i + 3], in[5 * i + 4]));
+    float* o = out + 15 * i;
+    o[0] = dsr.Dir.x; o[1] = dsr.Dir.y; o[2] = dsr.Dir.z;
+    o[3] = dsr.Dist;
+    o[4] = dsr.Pdf;
+    o[5] = dsr.P.x; o[6] = dsr.P.y; o[7] = dsr.P.z;
+    o[8] = dsr.N.x; o[9] = dsr.N.y; o[10] = dsr.N.z;
+    o[11] = li.x; o[12] = li.y; o[13] = li.z;
+    o[14] = L->PdfDirection(ref, dsr);
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -46,6 +46,14 @@ int ref_texture_eval(const rad_texture_desc* t, uint32_t count, uint32_t index, 
 /* first-hit uv and d(uv)/dx, d(uv)/dy of camera rays: in = n x {sx, sy}, out = n x 6 */
 int ref_camera_footprint(const ref_scene* s, const float* in, uint32_t n, float* out);
 float ref_medium_transmittance(const ref_scene* s, int medium, const rad_ray* ray, uint32_t channel, uint32_t seed);
+/* phase functions (RAD_PHASE_*, g): eval in = n x {wi[3] (= -ray direction), wo[3]}, out = n;
+   sample in = n x {wi[3], xi[2]}, out = n x {wo[3], pdf} */
+int ref_phase_eval(int type, float g, const float* in, uint32_t n, float* out);
+int ref_phase_sample(int type, float g, const float* in, uint32_t n, float* out);
+/* Light::SampleDirection of scene light `light` from points: in = n x {p[3], xi[2]},
+   out = n x {dir[3], dist, pdf, p_light[3], n_light[3], li[3], pdf_direction}, where the
+   last is Light::PdfDirection evaluated on the returned sample */
+int ref_light_sample(const ref_scene* s, uint32_t light, const float* in, uint32_t n, float* out);
 
 #ifdef __cplusplus
 }

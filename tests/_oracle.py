@@ -62,6 +62,9 @@ def lib():
         L.ref_erfinv.argtypes = [C.c_float]
         L.ref_medium_transmittance.restype = C.c_float
         L.ref_medium_transmittance.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32, C.c_uint32]
+        for name in ("ref_phase_eval", "ref_phase_sample"):
+            getattr(L, name).argtypes = [C.c_int, C.c_float, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ref_light_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]
         _lib = L
     return _lib
 
@@ -106,6 +109,21 @@ class OracleScene:
         out = np.zeros(len(rays), np.uint8)
         lib().ref_intersect_any(self.h, rays.ctypes.data, len(rays), self._mode(brute), out.ctypes.data)
         return out
+
+    def transmittance(self, medium, ray, channel, seeds):
+        """Delta-tracking transmittance estimates (0 or 1) of one ray, one per seed."""
+        r = np.ascontiguousarray(np.array([ray], RAY_DTYPE))
+        return np.array([lib().ref_medium_transmittance(self.h, medium, r.ctypes.data, channel, int(s)) for s in seeds])
+
+    def light_sample(self, light, points, xi):
+        """Light::SampleDirection from each point: a record array with dir, dist,
+        pdf, p, n, li and pdf_direction (PdfDirection of the same sample)."""
+        inp = np.ascontiguousarray(np.concatenate([points, xi], 1), np.float32)
+        out = np.zeros((len(inp), 15), np.float32)
+        if lib().ref_light_sample(self.h, light, inp.ctypes.data, len(inp), out.ctypes.data):
+            raise RuntimeError(lib().ref_last_error().decode())
+        return {"dir": out[:, 0:3], "dist": out[:, 3], "pdf": out[:, 4], "p": out[:, 5:8], "n": out[:, 8:11],
+                "li": out[:, 11:14], "pdf_direction": out[:, 14]}
 
     def prim_t(self, ray, geom, prim):
         r = np.ascontiguousarray(np.array([ray], RAY_DTYPE))
@@ -169,6 +187,21 @@ def bsdf_sample(desc, wi, lobe, xi, quirks=True):
     if lib().ref_bsdf_sample(arr, cnt, 1 if quirks else 0, inp.ctypes.data, n, out.ctypes.data):
         raise RuntimeError(lib().ref_last_error().decode())
     return out
+
+
+def phase_eval(kind, g, wi, wo):
+    """wi: -ray direction (the reference's MediumInteraction::Wi)."""
+    inp = np.ascontiguousarray(np.concatenate([wi, wo], 1), np.float32)
+    out = np.zeros(len(inp), np.float32)
+    lib().ref_phase_eval(kind, g, inp.ctypes.data, len(inp), out.ctypes.data)
+    return out
+
+
+def phase_sample(kind, g, wi, xi):
+    inp = np.ascontiguousarray(np.concatenate([wi, xi], 1), np.float32)
+    out = np.zeros((len(inp), 4), np.float32)
+    lib().ref_phase_sample(kind, g, inp.ctypes.data, len(inp), out.ctypes.data)
+    return out[:, :3], out[:, 3]
 
 
 def grid_eval(medium_desc, pts):
