@@ -191,11 +191,23 @@ def _coprime_step(target, tiles_x):
     return s
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(b, seconds):
     """Time the oracle, all host threads, on an even subsample of the frame's
     16x16 tiles at full spp.  The tile stride is coprime to the tiles per
     row, so the subsample covers every row and column band of the image; it
-    is calibrated so that the measured run takes about `seconds`."""
+    is calibrated so that the measured run takes about `seconds`.  Returns
+    the baseline record and the rendered tiles (mean and sum of squares)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
@@ -210,19 +222,52 @@ def cpu_baseline(b, seconds):
     # calibration: enough tiles to keep every thread busy
     step = _coprime_step(max(1, ntiles // (8 * threads)), tiles_x)
     for _ in range(4):
-        _, _, t = o.render(threads=threads, want_sumsq=False, tile_step=step)
+        fb, sq, t = o.render(threads=threads, want_sumsq=True, tile_step=step)
         if t >= 0.5 * seconds or step == 1:
             break
         step = _coprime_step(max(1.0, step * t / seconds), tiles_x)
     samples = _tile_pixels(W, H, step) * spp
-    return {
+    rec = {
         "value": samples / t,
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
+        "label": "vs refcpu (Embree unavailable)",
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "threads": threads,
+        "accel": "binned-SAH BVH collapsed to 4-wide nodes, SSE slab tests, x86-64-v3 build (AVX2/FMA)"
+                 if os.path.basename(O.LIB_PATH) == "librefcpu.so" else "4-wide SSE BVH, x86-64-v2 build",
         "sample": f"oracle {b.renderer['type']} integrator, 1 in {step} of the {ntiles} 16x16 tiles of the "
                   f"{W}x{H} frame at {spp} spp ({samples} samples, {t:.1f} s; BVH build {build_s:.1f} s excluded)",
     }
+    return rec, fb, sq, step
+
+
+def image_match(g_fb, g_sq, o_fb, o_sq, spp, step):
+    """Per-tile z-test of the GPU frame against the oracle's tiles (the same
+    16x16 tiles the CPU baseline rendered, both at the frame's spp) and the
+    relMSE of their tile means."""
+    H, W = g_fb.shape[:2]
+    tiles_x = (W + TILE - 1) // TILE
+    ntiles = tiles_x * ((H + TILE - 1) // TILE)
+    zs, rels = [], []
+    for t in range(0, ntiles, step):
+        x0, y0 = (t % tiles_x) * TILE, (t // tiles_x) * TILE
+        sl = (slice(y0, min(y0 + TILE, H)), slice(x0, min(x0 + TILE, W)))
+        stats = []
+        for fb, sq in ((g_fb, g_sq), (o_fb, o_sq)):
+            m = fb[sl].astype(np.float64)
+            var = np.maximum(sq[sl].astype(np.float64) / spp - m ** 2, 0.0) / spp
+            stats.append((m.mean(axis=(0, 1)), var.sum(axis=(0, 1)) / m.shape[0] ** 2 / m.shape[1] ** 2))
+        (ma, va), (mb, vb) = stats
+        floor = (max(2e-6, 2 * spp * 2.0 ** -24) * np.maximum(np.abs(ma), np.abs(mb))) ** 2
+        zs.append(np.abs(ma - mb) / np.sqrt(va + vb + floor + 1e-20))
+        rels.append((ma - mb) ** 2 / (mb ** 2 + 1e-4))
+    z, rel = np.array(zs), np.array(rels)
+    return {"tiles": len(zs), "tile": f"{TILE}x{TILE}", "max_z": float(z.max()), "frac_z_over_4": float((z > 4).mean()),
+            "relmse": float(rel.mean()), "spp_each": spp,
+            "note": "GPU frame (another seed) vs the oracle's tiles of the CPU-baseline run, both at the frame's spp"}
 
 
 def rel_mse(x, ref):
@@ -376,6 +421,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "value_note": "value = samples/s of the plain GPU path on the workload below; the guided-vs-plain "
+                      "time to equal relMSE is under guiding, p50/p99 request latency under serving",
         "dtype": "f32",
         "data": "synthetic",
         "config": {
@@ -423,9 +470,13 @@ def main():
         print("# serving stream", file=sys.stderr, flush=True)
         out["serving"] = serving_measure(ctx, args.serve_requests)
     if not args.no_cpu:
-        cb = cpu_baseline(b, args.cpu_seconds)
+        cb, ofb, osq, step = cpu_baseline(b, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = out["value"] / cb["value"]
+        out["gpu_over_cpu_label"] = "vs refcpu (Embree unavailable)"
+        rd_m = b.render_desc(spp=spp, seed=999, flags=F.RAD_RENDER_WANT_SUMSQ)
+        gfb, gsq, _ = scene.render(rd_m, want_sumsq=True)
+        out["image_match"] = image_match(gfb, gsq, ofb, osq, spp, step)
     print(json.dumps(out), flush=True)
 
 

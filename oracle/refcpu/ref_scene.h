@@ -390,13 +390,26 @@ struct Sampler {
 struct PrimRef {
   uint32_t shape, prim;
 };
+// CPU accel: a binned-SAH binary tree collapsed into a 4-wide tree whose
+// nodes hold their children's boxes as SoA lanes, so one SSE slab test
+// covers a node's four children (the timed CPU path; Embree itself is not
+// available).  Leaf triangles are tested inline from a packed copy of their
+// vertices with the same arithmetic as MeshShape::IntersectPrim.
 struct Accel {
   struct Node {
     Bounds b;
     int32_t left = -1, right = -1;
     uint32_t start = 0, count = 0;
   };
-  std::vector<Node> nodes;
+  struct alignas(64) Node4 {
+    float lo[3][4], hi[3][4];  // child boxes, axis-major, one lane per child
+    int32_t child[4];          // inner: node index; leaf: first entry of `prims`; empty: -1
+    uint32_t count[4];         // 0: inner; > 0: leaf primitive count
+  };
+  std::vector<Node> nodes;       // binary build tree
+  std::vector<Node4> nodes4;     // traversed tree
+  std::vector<float> triVerts;   // 9 floats per entry of `prims` (mesh triangles)
+  std::vector<uint8_t> isTri;    // entry of `prims` is a mesh triangle
   std::vector<PrimRef> prims;
   const std::vector<std::unique_ptr<Shape>>* shapes = nullptr;
   Bounds world;
@@ -415,6 +428,8 @@ struct Accel {
   bool ClosestCulled(const Ray& ray, HitShapeRecord& rec) const;
   bool OccludedCulled(const Ray& ray) const;
   int BuildRecursive(std::vector<PrimRef>& refs, std::vector<Bounds>& pb, std::vector<V3>& pc, uint32_t begin, uint32_t end, int depth);
+  int Collapse(int binaryNode);
+  bool IntersectEntry(uint32_t i, const Ray& ray, Float tnear, Float tfar, HitShapeRecord& rec) const;
 };
 
 // ---- scene (scene.cpp:11-122) ----

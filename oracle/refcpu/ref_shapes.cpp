@@ -1,5 +1,7 @@
 // ORACLE / TEST INFRASTRUCTURE ONLY (see ref_math.h).
 // Shapes, lights, camera, scene assembly and the CPU accel of the restatement.
+#include <immintrin.h>
+
 #include <chrono>
 #include <functional>
 #include <future>
@@ -894,20 +896,6 @@ std::tuple<const Light*, DirectionSampleResult, Spectrum> Scene::SampleLightDire
 }
 
 // ---------------------------------------------------------------- accel
-static inline bool SlabHit(const Bounds& b, const V3& o, const V3& invD, Float tmin, Float tmax) {
-  Float t0 = tmin, t1 = tmax;
-  for (int a = 0; a < 3; a++) {
-    Float tn = (b.lo[a] - o[a]) * invD[a];
-    Float tf = (b.hi[a] - o[a]) * invD[a];
-    if (tn > tf) std::swap(tn, tf);
-    tf *= 1 + 2 * 1.2e-7f * 3;  // robust slab test (Ize 2013)
-    t0 = tn > t0 ? tn : t0;
-    t1 = tf < t1 ? tf : t1;
-    if (!(t0 <= t1)) return false;
-  }
-  return true;
-}
-
 int Accel::BuildRecursive(std::vector<PrimRef>& refs, std::vector<Bounds>& pb, std::vector<V3>& pc, uint32_t begin,
                           uint32_t end, int depth) {
   Node node;
@@ -1009,6 +997,21 @@ void Accel::Build(const std::vector<std::unique_ptr<Shape>>& s) {
   nodes.reserve(2 * refs.size() / 2 + 16);
   if (!refs.empty()) BuildRecursive(refs, pb, pc, 0, (uint32_t)refs.size(), 0);
   prims = std::move(refs);
+  nodes4.clear();
+  if (!nodes.empty()) Collapse(0);
+  triVerts.assign(9 * prims.size(), 0.f);
+  isTri.assign(prims.size(), 0);
+  for (size_t i = 0; i < prims.size(); i++) {
+    const auto* mesh = dynamic_cast<const MeshShape*>(s[prims[i].shape].get());
+    if (!mesh) continue;
+    isTri[i] = 1;
+    for (int k = 0; k < 3; k++) {
+      const V3& p = mesh->position[mesh->indices[prims[i].prim * 3 + k]];
+      triVerts[9 * i + 3 * k] = p.x;
+      triVerts[9 * i + 3 * k + 1] = p.y;
+      triVerts[9 * i + 3 * k + 2] = p.z;
+    }
+  }
   chunks.clear();
   for (uint32_t c = 0; c < prims.size(); c += kCullChunk) {
     Bounds b;
@@ -1019,60 +1022,183 @@ void Accel::Build(const std::vector<std::unique_ptr<Shape>>& s) {
   buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Collapse the binary subtree at `b` into 4-wide nodes: open the inner child
+// of largest surface area until four children; returns the node index.
+int Accel::Collapse(int b) {
+  int kids[4];
+  int nk = 0;
+  if (nodes[b].count) {
+    kids[nk++] = b;
+  } else {
+    kids[nk++] = nodes[b].left;
+    kids[nk++] = nodes[b].right;
+  }
+  while (nk < 4) {
+    int best = -1;
+    Float bestArea = -1;
+    for (int k = 0; k < nk; k++) {
+      const Node& c = nodes[kids[k]];
+      if (c.count) continue;
+      Float a = c.b.Area();
+      if (a > bestArea) {
+        bestArea = a;
+        best = k;
+      }
+    }
+    if (best < 0) break;
+    const int open = kids[best];
+    kids[best] = nodes[open].left;
+    kids[nk++] = nodes[open].right;
+  }
+  const int idx = (int)nodes4.size();
+  nodes4.emplace_back();
+  Node4 q;
+  for (int k = 0; k < 4; k++) {
+    if (k >= nk) {
+      for (int a = 0; a < 3; a++) {
+        q.lo[a][k] = Inf;
+        q.hi[a][k] = -Inf;
+      }
+      q.child[k] = -1;
+      q.count[k] = 0;
+      continue;
+    }
+    const Node& c = nodes[kids[k]];
+    for (int a = 0; a < 3; a++) {
+      q.lo[a][k] = c.b.lo[a];
+      q.hi[a][k] = c.b.hi[a];
+    }
+    if (c.count) {
+      q.child[k] = (int32_t)c.start;
+      q.count[k] = c.count;
+    } else {
+      q.child[k] = Collapse(kids[k]);
+      q.count[k] = 0;
+    }
+  }
+  nodes4[idx] = q;
+  return idx;
+}
+
+bool Accel::IntersectEntry(uint32_t i, const Ray& ray, Float tnear, Float tfar, HitShapeRecord& rec) const {
+  if (!isTri[i]) return (*shapes)[prims[i].shape]->IntersectPrim(prims[i].prim, ray, tnear, tfar, rec);
+  const float* v = &triVerts[9 * size_t(i)];
+  const V3 p0(v[0], v[1], v[2]), p1(v[3], v[4], v[5]), p2(v[6], v[7], v[8]);
+  Float t, u, w;
+  if (!IntersectTriangleMT(p0, p1, p2, ray, tnear, tfar, t, u, w)) return false;
+  rec.T = t;
+  rec.PrimitiveUV = V2(u, w);
+  rec.PrimitiveIndex = prims[i].prim;
+  rec.ShapeIndex = prims[i].shape;
+  rec.GeometryNormal = Cross(p1 - p0, p2 - p0);
+  return true;
+}
+
+namespace {
+// Entry and exit distances of a ray against the four child boxes of a node.
+// A lane whose slab product is 0 * inf (a ray parallel to a face plane that
+// holds its origin) keeps that axis open, as the scalar slab test does.
+struct RaySimd {
+  __m128 o[3], inv[3], tmin;
+};
+
+inline RaySimd LoadRay(const Ray& ray) {
+  RaySimd r;
+  const float inv[3] = {1 / ray.D.x, 1 / ray.D.y, 1 / ray.D.z};
+  for (int a = 0; a < 3; a++) {
+    r.o[a] = _mm_set1_ps(ray.O[a]);
+    r.inv[a] = _mm_set1_ps(inv[a]);
+  }
+  r.tmin = _mm_set1_ps(ray.MinT);
+  return r;
+}
+
+inline int SlabTest4(const Accel::Node4& n, const RaySimd& r, float tfar, __m128& tnear) {
+  const __m128 ninf = _mm_set1_ps(-Inf), pinf = _mm_set1_ps(Inf);
+  __m128 tn = r.tmin, tf = pinf;
+  for (int a = 0; a < 3; a++) {
+    const __m128 t0 = _mm_mul_ps(_mm_sub_ps(_mm_load_ps(n.lo[a]), r.o[a]), r.inv[a]);
+    const __m128 t1 = _mm_mul_ps(_mm_sub_ps(_mm_load_ps(n.hi[a]), r.o[a]), r.inv[a]);
+    const __m128 nan = _mm_cmpunord_ps(t0, t1);
+    tn = _mm_max_ps(tn, _mm_blendv_ps(_mm_min_ps(t0, t1), ninf, nan));
+    tf = _mm_min_ps(tf, _mm_blendv_ps(_mm_max_ps(t0, t1), pinf, nan));
+  }
+  // robust slab test (Ize 2013): widen the exit by 3 * 2 ulp
+  tf = _mm_min_ps(_mm_mul_ps(tf, _mm_set1_ps(1 + 2 * 1.2e-7f * 3)), _mm_set1_ps(tfar));
+  tnear = tn;
+  return _mm_movemask_ps(_mm_cmple_ps(tn, tf));
+}
+}  // namespace
+
 bool Accel::Closest(const Ray& ray, HitShapeRecord& rec) const {
-  if (nodes.empty()) return false;
-  V3 invD(1 / ray.D.x, 1 / ray.D.y, 1 / ray.D.z);
+  if (nodes4.empty()) return false;
+  const RaySimd r = LoadRay(ray);
   Float tfar = ray.MaxT;
   bool hit = false;
-  int stack[128];
+  struct Entry {
+    int32_t node;
+    float t;
+  } stack[256];
   int sp = 0;
-  stack[sp++] = 0;
+  stack[sp++] = {0, ray.MinT};
   while (sp) {
-    const Node& n = nodes[stack[--sp]];
-    if (!SlabHit(n.b, ray.O, invD, ray.MinT, tfar)) continue;
-    if (n.count) {
-      for (uint32_t i = n.start; i < n.start + n.count; i++) {
-        HitShapeRecord r;
-        if ((*shapes)[prims[i].shape]->IntersectPrim(prims[i].prim, ray, ray.MinT, tfar, r)) {
-          rec = r;
-          tfar = r.T;
+    const Entry e = stack[--sp];
+    if (e.t > tfar) continue;
+    const Node4& n = nodes4[e.node];
+    __m128 tn4;
+    int mask = SlabTest4(n, r, tfar, tn4);
+    alignas(16) float tn[4];
+    _mm_store_ps(tn, tn4);
+    // leaves first: they can only shorten tfar
+    for (int k = 0; k < 4; k++) {
+      if (!(mask >> k & 1) || n.child[k] < 0 || n.count[k] == 0) continue;
+      for (uint32_t i = uint32_t(n.child[k]), end = i + n.count[k]; i < end; i++) {
+        HitShapeRecord h;
+        if (IntersectEntry(i, ray, ray.MinT, tfar, h)) {
+          rec = h;
+          tfar = h.T;
           hit = true;
         }
       }
-    } else {
-      // visit the nearer child first
-      const Node& L = nodes[n.left];
-      const Node& R = nodes[n.right];
-      Float dl = Dot(L.b.Center() - ray.O, ray.D), dr = Dot(R.b.Center() - ray.O, ray.D);
-      if (dl < dr) {
-        stack[sp++] = n.right;
-        stack[sp++] = n.left;
-      } else {
-        stack[sp++] = n.left;
-        stack[sp++] = n.right;
-      }
     }
+    // inner children still in front of the closest hit, nearest popped first
+    Entry in[4];
+    int m = 0;
+    for (int k = 0; k < 4; k++) {
+      if (!(mask >> k & 1) || n.child[k] < 0 || n.count[k] != 0 || tn[k] > tfar) continue;
+      Entry x{n.child[k], tn[k]};
+      int j = m++;
+      while (j > 0 && in[j - 1].t < x.t) {
+        in[j] = in[j - 1];
+        j--;
+      }
+      in[j] = x;  // descending distance
+    }
+    for (int k = 0; k < m; k++) stack[sp++] = in[k];
   }
   return hit;
 }
 
 bool Accel::Occluded(const Ray& ray) const {
-  if (nodes.empty()) return false;
-  V3 invD(1 / ray.D.x, 1 / ray.D.y, 1 / ray.D.z);
-  int stack[128];
+  if (nodes4.empty()) return false;
+  const RaySimd r = LoadRay(ray);
+  int32_t stack[256];
   int sp = 0;
   stack[sp++] = 0;
   while (sp) {
-    const Node& n = nodes[stack[--sp]];
-    if (!SlabHit(n.b, ray.O, invD, ray.MinT, ray.MaxT)) continue;
-    if (n.count) {
-      for (uint32_t i = n.start; i < n.start + n.count; i++) {
-        HitShapeRecord r;
-        if ((*shapes)[prims[i].shape]->IntersectPrim(prims[i].prim, ray, ray.MinT, ray.MaxT, r)) return true;
+    const Node4& n = nodes4[stack[--sp]];
+    __m128 tn4;
+    const int mask = SlabTest4(n, r, ray.MaxT, tn4);
+    for (int k = 0; k < 4; k++) {
+      if (!(mask >> k & 1) || n.child[k] < 0) continue;
+      if (n.count[k] == 0) {
+        stack[sp++] = n.child[k];
+        continue;
       }
-    } else {
-      stack[sp++] = n.left;
-      stack[sp++] = n.right;
+      for (uint32_t i = uint32_t(n.child[k]), end = i + n.count[k]; i < end; i++) {
+        HitShapeRecord h;
+        if (IntersectEntry(i, ray, ray.MinT, ray.MaxT, h)) return true;
+      }
     }
   }
   return false;

@@ -10,7 +10,22 @@ import numpy as np
 from radiantiumplusplus_amd import _ffi as F
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "librefcpu.so")
+
+
+def _host_has_avx2_fma():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = line.split()
+                    return "avx2" in flags and "fma" in flags
+    except OSError:
+        pass
+    return False
+
+
+# the x86-64-v3 build where the host has AVX2 and FMA, else the x86-64-v2 one
+LIB_PATH = os.path.join(ROOT, "oracle", "librefcpu.so" if _host_has_avx2_fma() else "librefcpu_v2.so")
 
 
 class RenderParams(C.Structure):
