@@ -135,14 +135,27 @@ __device__ __forceinline__ bool isect_prim(const DevScene& s, uint32_t p, f3 o, 
 // accesses hit distinct banks), older ones spilled to a private overflow
 // array that only deep traversals touch.
 constexpr int kStackDepth = int(kTraceStackEntries);
-constexpr int kLdsStack = 16;
+constexpr int kLdsStack = 8;
 constexpr int kTraceBlock = 256;  // threads per block of every traversal kernel
 
+// LDS pointers spelled with their address space, so stack accesses compile
+// to ds_read / ds_write rather than flat accesses.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) float lds_f32;
+
+// Per-lane private overflow of a traversal stack (entries below the LDS ring).
+struct TraceOverflow {
+  uint32_t node[kStackDepth - kLdsStack];
+  float dist[kStackDepth - kLdsStack];
+};
+
+// The struct holds no arrays, so its fields stay in registers; only the
+// overflow array, indexed by the stack depth, lives in private memory.
 struct TraceStack {
-  uint32_t* node;  // LDS, entry k of this lane at node[k * kTraceBlock]
-  float* dist;
-  uint32_t ov_node[kStackDepth - kLdsStack];
-  float ov_dist[kStackDepth - kLdsStack];
+  lds_u32* node;  // LDS, entry k of this lane at node[k * kTraceBlock]
+  lds_f32* dist;
+  uint32_t* ov_node;  // TraceOverflow of this lane
+  float* ov_dist;
   int sp, lo;  // entries [0, lo) live in the overflow array, [lo, sp) in LDS
 
   __device__ __forceinline__ void push(uint32_t n, float d) {
@@ -354,9 +367,12 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
   constexpr uint32_t kChunk = 256;
   const uint32_t lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
+  TraceOverflow ov;
   TraceStack st;
-  st.node = lds_node + threadIdx.x;
-  st.dist = lds_dist + threadIdx.x;
+  st.node = (lds_u32*)(lds_node + threadIdx.x);
+  st.dist = (lds_f32*)(lds_dist + threadIdx.x);
+  st.ov_node = ov.node;
+  st.ov_dist = ov.dist;
   TraceRay r;
   bool active = false, exhausted = false;
   uint32_t idx = 0, tag = 0, visits = 0;
@@ -441,9 +457,12 @@ __device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin,
   __shared__ uint32_t lds_node1[kLdsStack * kTraceBlock];
   __shared__ float lds_dist1[kLdsStack * kTraceBlock];
   TraceRay r;
+  TraceOverflow ov;
   TraceStack st;
-  st.node = lds_node1 + threadIdx.x;
-  st.dist = lds_dist1 + threadIdx.x;
+  st.node = (lds_u32*)(lds_node1 + threadIdx.x);
+  st.dist = (lds_f32*)(lds_dist1 + threadIdx.x);
+  st.ov_node = ov.node;
+  st.ov_dist = ov.dist;
   trace_begin(r, st, o, d, tmin, tmax);
   while (!trace_step<ANY>(s, r, st)) {
   }
