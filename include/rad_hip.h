@@ -408,6 +408,14 @@ void rad_hip_scene_destroy(rad_scene* scene);
 rad_status rad_hip_intersect_closest(rad_scene* scene, const rad_ray* rays, uint32_t n, rad_hit* hits);
 /* Occlusion for n rays (Embree::RayIntersect, embree.cpp:79-85): 1 = occluded. */
 rad_status rad_hip_intersect_any(rad_scene* scene, const rad_ray* rays, uint32_t n, uint8_t* occluded);
+/* rad_hip_intersect_closest without any device allocation: rays and hits are
+ * staged in a static device array of the library (262,144 rays per call at
+ * most, 13 MiB per GPU, held until the process ends); more rays fail with
+ * RAD_ERR_INVALID_ARG.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU race. */
+rad_status rad_hip_intersect_closest_noalloc(rad_scene* scene, const rad_ray* rays, uint32_t n, rad_hit* hits);
 /* Device-pointer variants on a caller stream (hipStream_t as void*, NULL = default). */
 rad_status rad_hip_intersect_closest_device(rad_scene* scene, const rad_ray* d_rays, uint32_t n,
                                             rad_hit* d_hits, void* stream);

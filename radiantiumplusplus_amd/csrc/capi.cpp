@@ -17,6 +17,15 @@
 
 using namespace rad;
 
+// Static staging for rad_hip_intersect_closest_noalloc: the rays and hits of
+// one call, kNoAllocMaxRays at most (8 MiB of rays + 5 MiB of hits).  Each
+// device holds its own copy until the process ends; its device address comes
+// from hipGetSymbolAddress, resolved once per device by rad_hip_init.
+constexpr uint32_t kNoAllocMaxRays = 1u << 18;
+constexpr size_t kScratchRayBytes = size_t(kNoAllocMaxRays) * sizeof(rad_ray);
+constexpr size_t kScratchBytes = kScratchRayBytes + size_t(kNoAllocMaxRays) * sizeof(rad_hit);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 thread_local std::string g_last_error;
 
@@ -133,6 +142,24 @@ size_t rad_abi_sizeof(const char* name) {
   return 0;
 }
 
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device ordinal: the address of `scratch`, null until resolved
+
+// With `device` current: the device address of its copy of `scratch`.
+void* ResolveScratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (size_t(device) >= g_scratch_ptr.size()) g_scratch_ptr.resize(size_t(device) + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    HipCheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw StatusError(RAD_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 rad_status rad_hip_init(int32_t device_ordinal, rad_ctx** out) {
   if (!out) return Fail(RAD_ERR_INVALID_ARG, "out is NULL");
   *out = nullptr;
@@ -149,6 +176,7 @@ rad_status rad_hip_init(int32_t device_ordinal, rad_ctx** out) {
     c->arch = prop.gcnArchName;
     c->cu_count = prop.multiProcessorCount;
     HipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    c->scratch = ResolveScratch(device_ordinal);
     *out = c;
   });
 }
@@ -488,6 +516,31 @@ rad_status rad_hip_intersect_any(rad_scene* scene, const rad_ray* rays, uint32_t
     HipCheck(hipGetLastError(), "occlusion launch");
     if (n) HipCheck(hipMemcpyAsync(occluded, scene->d_occ.ptr, size_t(n), hipMemcpyDeviceToHost, st), "copy results");
     HipCheck(hipStreamSynchronize(st), "occlusion");
+  });
+}
+
+// rad_hip_intersect_closest staged through the static array instead of
+// device buffers the library allocates: no hipMalloc / hipFree on any path.
+rad_status rad_hip_intersect_closest_noalloc(rad_scene* scene, const rad_ray* rays, uint32_t n, rad_hit* hits) {
+  if (!scene || (n && (!rays || !hits))) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
+  if (n > kNoAllocMaxRays)
+    return Fail(RAD_ERR_INVALID_ARG, "rad_hip_intersect_closest_noalloc: " + std::to_string(n) +
+                                         " rays exceed its static staging capacity of " +
+                                         std::to_string(kNoAllocMaxRays) + " rays per call");
+  return Guard([&] {
+    std::lock_guard<std::mutex> lk(scene->mu);
+    RequireBvh(scene);
+    HipCheck(hipSetDevice(scene->ctx->device), "hipSetDevice");
+    char* base = static_cast<char*>(scene->ctx->scratch);
+    if (!base) throw StatusError(RAD_ERR_STATE, "static staging of this device was not resolved");
+    rad_ray* d_rays = reinterpret_cast<rad_ray*>(base);
+    rad_hit* d_hits = reinterpret_cast<rad_hit*>(base + kScratchRayBytes);
+    hipStream_t st = scene->ctx->stream;
+    if (n) HipCheck(hipMemcpyAsync(d_rays, rays, sizeof(rad_ray) * size_t(n), hipMemcpyHostToDevice, st), "copy rays");
+    LaunchClosest(scene->dev, d_rays, n, d_hits, st);
+    HipCheck(hipGetLastError(), "closest-hit launch");
+    if (n) HipCheck(hipMemcpyAsync(hits, d_hits, sizeof(rad_hit) * size_t(n), hipMemcpyDeviceToHost, st), "copy hits");
+    HipCheck(hipStreamSynchronize(st), "closest-hit");
   });
 }
 

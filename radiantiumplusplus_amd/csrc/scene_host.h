@@ -16,6 +16,7 @@
 struct rad_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // device address of this device's static staging array (capi.cpp)
   std::string arch;
   int cu_count = 0;
 };
