@@ -54,18 +54,13 @@ int main(int argc, char** argv) {
   int rc = 0;
   try {
     Check(rad_hip_init(device, &ctx), "init");
-    auto scene = std::make_shared<DeviceScene>(ctx, p.string());
-    std::printf("[info] build done. %.1f ms\n", scene->bvh_build_ms());
-    FactoryManager factories = FactoryManager::Default();
-    const RendererFactory& f = factories.GetFactory(scene->renderer_type());
-    std::unique_ptr<Renderer> renderer = f.Create(scene);
-    if (spp > 0 || guiding >= 0) {
-      rad_render_desc d = scene->render_desc();
-      d.integrator = scene->renderer_type() == "vol_path" ? RAD_INTEGRATOR_VOL_PATH : RAD_INTEGRATOR_PATH;
-      if (spp > 0) d.spp = uint32_t(spp);
-      if (guiding >= 0) d.guiding = guiding;
-      renderer = std::make_unique<Renderer>(scene, d);
-    }
+    BuildContext build(ctx);
+    build.SetFromJson(p.string());
+    build.SetDefaultFactoryManager();
+    if (spp > 0) build.SetSampleCount(uint32_t(spp));
+    if (guiding >= 0) build.SetGuiding(guiding);
+    std::unique_ptr<Renderer> renderer = build.Build();
+    std::printf("[info] build done. %.1f ms\n", build.LastBvhBuildMs());
     std::printf("[info] start rendering...\n");
     std::fflush(stdout);
     renderer->Start();

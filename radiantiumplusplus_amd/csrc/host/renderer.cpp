@@ -42,8 +42,8 @@ double DeviceScene::bvh_build_ms() const {
   return info.bvh_build_ms;
 }
 
-Renderer::Renderer(std::shared_ptr<DeviceScene> scene, const rad_render_desc& desc)
-    : scene_(std::move(scene)), desc_(desc) {}
+Renderer::Renderer(BuildContext*, std::unique_ptr<DeviceScene> scene, const rad_render_desc& cfg)
+    : scene_(std::move(scene)), desc_(cfg) {}
 
 Renderer::~Renderer() {
   if (job_) rad_hip_job_destroy(job_);
@@ -102,40 +102,61 @@ void Renderer::SaveResult(const std::string& directory, const std::string& name)
 
 namespace {
 
-class HipFactory : public RendererFactory {
+class HipRendererFactory : public RendererFactory {
  public:
-  HipFactory(const char* name, int32_t integrator) : name_(name), integrator_(integrator) {}
-  std::string UniqueId() const override { return name_; }
-  std::unique_ptr<Renderer> Create(std::shared_ptr<DeviceScene> scene) const override {
-    rad_render_desc d = scene->render_desc();
+  HipRendererFactory(const char* name, int32_t integrator) : RendererFactory(name), integrator_(integrator) {}
+  std::unique_ptr<Renderer> Create(BuildContext* ctx, std::unique_ptr<DeviceScene> scene,
+                                   const rad_render_desc& cfg) const override {
+    rad_render_desc d = cfg;
     d.integrator = integrator_;
-    return std::make_unique<Renderer>(std::move(scene), d);
+    return std::make_unique<Renderer>(ctx, std::move(scene), d);
   }
 
  private:
-  std::string name_;
   int32_t integrator_;
 };
 
 }  // namespace
 
-void FactoryManager::RegisterFactory(std::unique_ptr<RendererFactory> f) {
-  std::string id = f->UniqueId();
-  if (factories_.count(id)) throw RadException(RAD_ERR_INVALID_ARG, "factory already registered: " + id);
-  factories_[id] = std::move(f);
+void FactoryManager::RegisterFactory(std::unique_ptr<Factory> f) {
+  const std::string name = f->GetName();
+  if (factories_.count(name)) throw RadException(RAD_ERR_INVALID_ARG, "factory already registered: " + name);
+  factories_[name] = std::move(f);
 }
 
-const RendererFactory& FactoryManager::GetFactory(const std::string& name) const {
+Factory* FactoryManager::GetFactory(const std::string& name) const {
   auto it = factories_.find(name);
-  if (it == factories_.end()) throw RadException(RAD_ERR_UNSUPPORTED, "no renderer factory named " + name);
-  return *it->second;
+  if (it == factories_.end()) throw RadException(RAD_ERR_UNSUPPORTED, "no factory named " + name);
+  return it->second.get();
 }
 
-FactoryManager FactoryManager::Default() {
-  FactoryManager m;
-  m.RegisterFactory(std::make_unique<HipFactory>("path", RAD_INTEGRATOR_PATH));
-  m.RegisterFactory(std::make_unique<HipFactory>("vol_path", RAD_INTEGRATOR_VOL_PATH));
-  return m;
+void RegisterHipRenderers(FactoryManager& m) {
+  m.RegisterFactory(std::make_unique<HipRendererFactory>("path", RAD_INTEGRATOR_PATH));
+  m.RegisterFactory(std::make_unique<HipRendererFactory>("vol_path", RAD_INTEGRATOR_VOL_PATH));
+}
+
+void BuildContext::SetDefaultFactoryManager() {
+  default_ = std::make_unique<FactoryManager>();
+  RegisterHipRenderers(*default_);
+  mgr_ = default_.get();
+}
+
+const FactoryManager& BuildContext::GetFactoryManager() const {
+  if (!mgr_) throw RadException(RAD_ERR_STATE, "no factory manager set");
+  return *mgr_;
+}
+
+std::unique_ptr<Renderer> BuildContext::Build() {
+  if (path_.empty()) throw RadException(RAD_ERR_STATE, "no scene set (SetFromJson)");
+  auto scene = std::make_unique<DeviceScene>(ctx_, path_);
+  bvh_ms_ = scene->bvh_build_ms();
+  rad_render_desc cfg = scene->render_desc();
+  if (spp_ > 0) cfg.spp = spp_;
+  if (guiding_ >= 0) cfg.guiding = guiding_;
+  const std::string type = scene->renderer_type();
+  auto* factory = GetFactoryManager().GetFactory<RendererFactory>(type);
+  if (!factory) throw RadException(RAD_ERR_UNSUPPORTED, "factory " + type + " is not a renderer factory");
+  return factory->Create(this, std::move(scene), cfg);
 }
 
 }  // namespace offline

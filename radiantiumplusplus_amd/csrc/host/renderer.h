@@ -1,14 +1,17 @@
 // C++ host interface over the C ABI, shaped like the reference's renderer
 // plugin API so a reference-side caller can switch to the GPU path:
-//   Renderer       ~ Rad::Renderer (render/renderer.h:19-48): Start (async),
+//   Factory / FactoryManager ~ core/factory.h:13-61, core/factory.cpp:5-19: a
+//                    factory carries its name from construction; duplicate
+//                    and unknown names throw
+//   RendererFactory ~ build/factory.h:84-89: Create(BuildContext*, scene, config)
+//   BuildContext    ~ build/build_context.h:20-58: SetFromJson, the default
+//                    factory manager, Build() -> the renderer named by
+//                    renderer.type (build_context.cpp:88-321)
+//   Renderer        ~ render/renderer.h:19-48: owns its scene; Start (async),
 //                    Wait, Stop, IsComplete, AllTaskCount, CompletedTaskCount,
 //                    ElapsedTime (ms), SaveResult (<name>.exr)
-//   RendererFactory / FactoryManager ~ build/factory.h:84-89 and
-//                    core/factory.cpp:5-19 (duplicate names and unknown
-//                    names throw)
-// The factories "path" and "vol_path" build GPU renderers for the scenes of
-// the reference's JSON format (renderer.type); renderer.device "hip" is the
-// default and the only device this library provides.
+// The factories "path" and "vol_path" build GPU renderers; renderer.device
+// "hip" is the default and the only device this library provides.
 #pragma once
 
 #include <chrono>
@@ -31,7 +34,8 @@ struct RadException : std::runtime_error {
 };
 void Check(rad_status s, const char* what);
 
-// A scene loaded from JSON and resident on the device.
+// A scene loaded from JSON and resident on the device (the Scene that
+// BuildContext::Build assembles, with the BVH in place of the Embree accel).
 class DeviceScene {
  public:
   DeviceScene(rad_ctx* ctx, const std::string& jsonPath);
@@ -52,9 +56,11 @@ class DeviceScene {
   const rad_render_desc* render_ = nullptr;
 };
 
+class BuildContext;
+
 class Renderer {
  public:
-  Renderer(std::shared_ptr<DeviceScene> scene, const rad_render_desc& desc);
+  Renderer(BuildContext* ctx, std::unique_ptr<DeviceScene> scene, const rad_render_desc& cfg);
   virtual ~Renderer();
   Renderer(const Renderer&) = delete;
   Renderer& operator=(const Renderer&) = delete;
@@ -73,7 +79,7 @@ class Renderer {
   const rad_job_stats& Stats() const { return stats_; }
 
  protected:
-  std::shared_ptr<DeviceScene> scene_;
+  std::unique_ptr<DeviceScene> scene_;
   rad_render_desc desc_;
   rad_job* job_ = nullptr;
   std::vector<float> image_;
@@ -82,22 +88,65 @@ class Renderer {
   bool waited_ = false;
 };
 
-class RendererFactory {
+class Factory {
  public:
-  virtual ~RendererFactory() = default;
-  virtual std::string UniqueId() const = 0;
-  virtual std::unique_ptr<Renderer> Create(std::shared_ptr<DeviceScene> scene) const = 0;
+  explicit Factory(std::string name) : name_(std::move(name)) {}
+  virtual ~Factory() = default;
+  const std::string& GetName() const { return name_; }
+
+ protected:
+  std::string name_;
+};
+
+class RendererFactory : public Factory {
+ public:
+  explicit RendererFactory(std::string name) : Factory(std::move(name)) {}
+  // cfg: the renderer settings of the scene (renderer.* and sampler.*)
+  virtual std::unique_ptr<Renderer> Create(BuildContext* ctx, std::unique_ptr<DeviceScene> scene,
+                                           const rad_render_desc& cfg) const = 0;
 };
 
 class FactoryManager {
  public:
-  void RegisterFactory(std::unique_ptr<RendererFactory> f);
-  const RendererFactory& GetFactory(const std::string& name) const;
-  // "path" and "vol_path" on the HIP device.
-  static FactoryManager Default();
+  void RegisterFactory(std::unique_ptr<Factory> f);
+  Factory* GetFactory(const std::string& name) const;
+  template <typename T>
+  T* GetFactory(const std::string& name) const {
+    return dynamic_cast<T*>(GetFactory(name));
+  }
 
  private:
-  std::map<std::string, std::unique_ptr<RendererFactory>> factories_;
+  std::map<std::string, std::unique_ptr<Factory>> factories_;
+};
+
+// "path" and "vol_path" on the HIP device (GetRadOfflineFactories, build/factory.cpp:39-75).
+void RegisterHipRenderers(FactoryManager& m);
+
+class BuildContext {
+ public:
+  explicit BuildContext(rad_ctx* ctx) : ctx_(ctx) {}
+  void SetFactoryManager(FactoryManager& mgr) { mgr_ = &mgr; }
+  void SetDefaultFactoryManager();
+  const FactoryManager& GetFactoryManager() const;
+  // The scene file (build_context.cpp:47-76 reads the same keys).
+  void SetFromJson(const std::string& path) { path_ = path; }
+  // Optional overrides of sampler.sample_count and of the guiding mode.
+  void SetSampleCount(uint32_t spp) { spp_ = spp; }
+  void SetGuiding(int32_t mode) { guiding_ = mode; }
+  // Loads the scene onto the device and creates the renderer named by
+  // renderer.type (build_context.cpp:317-320).
+  std::unique_ptr<Renderer> Build();
+  // BVH build time of the last Build (embree.cpp:38-42 logs the same).
+  double LastBvhBuildMs() const { return bvh_ms_; }
+
+ private:
+  rad_ctx* ctx_;
+  FactoryManager* mgr_ = nullptr;
+  std::unique_ptr<FactoryManager> default_;
+  std::string path_;
+  uint32_t spp_ = 0;
+  int32_t guiding_ = -1;
+  double bvh_ms_ = 0;
 };
 
 }  // namespace offline
