@@ -309,6 +309,7 @@ rad_status rad_hip_autotune(rad_scene* scene, const rad_render_desc* probe, rad_
     scene->tuned_slots = bestSlots;
     scene->tuned_blocks_per_cu = bestB;
     scene->tuned_trace_blocks_per_cu = bestT;
+    TakeSpareArena(scene).reset();  // trial arenas reach ~16 GiB at 64 M slots: not kept
     rep->path_slots = bestSlots;
     rep->blocks_per_cu = bestB;
     rep->trace_blocks_per_cu = bestT;
@@ -569,7 +570,9 @@ rad_status rad_hip_render_start(rad_scene* scene, const rad_render_desc* desc, r
   *out = nullptr;
   return Guard([&] {
     RequireBvh(scene);
-    *out = StartJob(scene, *desc, TakeSpareArena(scene));
+    rad_render_desc d = *desc;
+    ResolveLaunch(scene, d);
+    *out = StartJob(scene, d, TakeSpareArena(scene));
   });
 }
 

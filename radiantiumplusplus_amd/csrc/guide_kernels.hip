@@ -314,6 +314,8 @@ GuideState::GuideState() {
   m2.alloc(kParams * 4);
   wh.alloc(kWeights * 2);
   bias.alloc(kBiases * 4);
+  pub_wh.alloc(kWeights * 2);
+  pub_bias.alloc(kBiases * 4);
   scal.alloc(64);
   std::vector<float> p;
   InitParams(p, 1234u);
@@ -330,7 +332,22 @@ void GuideState::Upload(const std::vector<float>& p) {
                      bias.as<float>());
   HipCheck(hipGetLastError(), "guide publish");
   HipCheck(hipDeviceSynchronize(), "guide publish");
+  Publish(nullptr);
   steps = 0;
+}
+
+void GuideState::Publish(hipStream_t s) {
+  HipCheck(hipStreamSynchronize(s), "guide training");
+  std::lock_guard<std::mutex> lk(pub_mu);
+  HipCheck(hipMemcpyAsync(pub_wh.ptr, wh.ptr, kWeights * 2, hipMemcpyDeviceToDevice, s), "publish guide weights");
+  HipCheck(hipMemcpyAsync(pub_bias.ptr, bias.ptr, kBiases * 4, hipMemcpyDeviceToDevice, s), "publish guide biases");
+  HipCheck(hipStreamSynchronize(s), "publish guide");
+}
+
+void GuideState::Snapshot(void* wh_dst, void* bias_dst) {
+  std::lock_guard<std::mutex> lk(pub_mu);
+  HipCheck(hipMemcpy(wh_dst, pub_wh.ptr, kWeights * 2, hipMemcpyDeviceToDevice), "guide snapshot");
+  HipCheck(hipMemcpy(bias_dst, pub_bias.ptr, kBiases * 4, hipMemcpyDeviceToDevice), "guide snapshot");
 }
 
 GuideNet GuideState::Net(const float lo[3], const float hi[3], float alpha, float min_rough) const {

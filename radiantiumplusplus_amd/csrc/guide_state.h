@@ -2,6 +2,7 @@
 // moments, the FP16 / bias copies the kernels read, and the launchers.
 #pragma once
 
+#include <mutex>
 #include <vector>
 
 #include "guide.h"
@@ -12,10 +13,17 @@ namespace guide {
 
 struct GuideState {
   DeviceBuffer params, grad, m1, m2, wh, bias, scal;
+  // The FP16 weights and biases as of the last finished training pass (or
+  // upload): renders that only read the guide copy these under pub_mu, so a
+  // training render on another stream never hands them a half-updated net.
+  DeviceBuffer pub_wh, pub_bias;
+  std::mutex pub_mu;
   uint32_t steps = 0;
   uint64_t records_used = 0;
   GuideState();
   void Upload(const std::vector<float>& p);  // replace the weights, reset Adam
+  void Publish(hipStream_t s);               // wh / bias -> pub_* once the work on s is done
+  void Snapshot(void* wh_dst, void* bias_dst);  // the published copy -> device buffers of a render
   GuideNet Net(const float lo[3], const float hi[3], float alpha, float min_rough = 0.f) const;
   void Step(float lr, float gscale, hipStream_t s);  // Adam on the accumulated gradient
 };

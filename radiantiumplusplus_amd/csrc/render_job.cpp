@@ -131,7 +131,8 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   if (rows == 0) throw StatusError(RAD_ERR_INVALID_ARG, "image too large");
   j.rows = uint32_t(rows);
   const uint64_t pass = rows * j.npix;
-  uint64_t Q = desc.path_slots > 0 ? desc.path_slots : scene->tuned_slots ? scene->tuned_slots : kDefaultSlots;
+  const uint32_t tuned = scene->tuned_slots.load();
+  uint64_t Q = desc.path_slots > 0 ? desc.path_slots : tuned ? tuned : kDefaultSlots;
   Q = std::max<uint64_t>(1, std::min(Q, pass));
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
@@ -201,6 +202,7 @@ void Train(rad_job* job, hipStream_t st) {
     HipCheck(hipStreamSynchronize(st), "guide training");
     std::memcpy(&loss, &job->h_cnt[CNT_WORDS + 2 * ST_WORDS + G_WORDS], 4);
     job->guide_loss = loss / float(kTrainBatch);
+    gs.Publish(st);
   }
   HipCheck(hipMemsetAsync(w.gcnt, 0, G_WORDS * 4, st), "clear guide counters");
 }
@@ -214,10 +216,9 @@ void RunJob(rad_job* job) {
     Wavefront& w = job->w;
     hipStream_t st = job->stream;
     const int cus = job->scene->ctx->cu_count;
-    const int bpc = job->desc.blocks_per_cu > 0 ? int(job->desc.blocks_per_cu)
-                    : job->scene->tuned_blocks_per_cu ? int(job->scene->tuned_blocks_per_cu) : 8;
-    const int tpc = job->desc.trace_blocks_per_cu > 0 ? int(job->desc.trace_blocks_per_cu)
-                    : job->scene->tuned_trace_blocks_per_cu ? int(job->scene->tuned_trace_blocks_per_cu) : 8;
+    rad_render_desc launch = job->desc;  // resolved at submission; defaults for direct callers
+    ResolveLaunch(job->scene, launch);
+    const int bpc = int(launch.blocks_per_cu), tpc = int(launch.trace_blocks_per_cu);
     LaunchCfg cfg{std::max(64, cus * bpc), std::max(64, cus * tpc)};
     job->blocks_per_cu = uint32_t(bpc);
     job->trace_blocks_per_cu = uint32_t(tpc);
@@ -354,6 +355,21 @@ void RunJob(rad_job* job) {
 
 }  // namespace
 
+void ResolveLaunch(const rad_scene* scene, rad_render_desc& d) {
+  if (d.path_slots == 0) {
+    const uint32_t t = scene->tuned_slots.load();
+    d.path_slots = t ? t : uint32_t(kDefaultSlots);
+  }
+  if (d.blocks_per_cu == 0) {
+    const uint32_t t = scene->tuned_blocks_per_cu.load();
+    d.blocks_per_cu = t ? t : 8u;
+  }
+  if (d.trace_blocks_per_cu == 0) {
+    const uint32_t t = scene->tuned_trace_blocks_per_cu.load();
+    d.trace_blocks_per_cu = t ? t : 8u;
+  }
+}
+
 uint64_t JobFootprint(const rad_scene* scene, const rad_render_desc& desc, uint32_t* slots) {
   JobShape j = Shape(scene, desc);
   if (slots) *slots = j.Q;
@@ -461,9 +477,8 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
       if (!scene->guide) scene->guide = std::make_shared<guide::GuideState>();
       job->guide = scene->guide;
     }
-    if (desc.guiding == 2) {  // snapshot: concurrent training of the scene's guide cannot touch this render
-      HipCheck(hipMemcpy(wh, job->guide->wh.ptr, guide::kWeights * 2, hipMemcpyDeviceToDevice), "guide snapshot");
-      HipCheck(hipMemcpy(bias, job->guide->bias.ptr, guide::kBiases * 4, hipMemcpyDeviceToDevice), "guide snapshot");
+    if (desc.guiding == 2) {  // snapshot of the published weights: concurrent training cannot touch this render
+      job->guide->Snapshot(wh, bias);
       w.net = job->guide->Net(scene->host.world_lo, scene->host.world_hi, GuideAlpha(), GuideMinRough());
       w.net.w = wh;
       w.net.b = bias;

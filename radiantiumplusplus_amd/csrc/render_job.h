@@ -18,6 +18,10 @@ struct StatusError : std::runtime_error {
 };
 
 // `reuse`: an arena from an earlier job, taken when large enough.
+// Fill path_slots / blocks_per_cu / trace_blocks_per_cu left 0 with the
+// scene's tuned values (or the defaults), so that a request's footprint and
+// its job see one launch configuration even if the autotuner runs between.
+void ResolveLaunch(const rad_scene* scene, rad_render_desc& desc);
 rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr<DeviceBuffer> reuse = nullptr);
 bool JobComplete(const rad_job* job);
 // Detach the job's arena (joins its thread first) so a later job can reuse it.

@@ -4,6 +4,7 @@
 // the shape / light / medium constructors it calls.
 #pragma once
 
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -98,8 +99,9 @@ struct rad_scene {
   float sah_cost = 0;
   std::shared_ptr<rad::guide::GuideState> guide;  // created on first guided render
   std::mutex guide_mu;  // held by a render that trains the guide
-  // launch configuration chosen by rad_hip_autotune (0 = library default)
-  uint32_t tuned_slots = 0, tuned_blocks_per_cu = 0, tuned_trace_blocks_per_cu = 0;
+  // launch configuration chosen by rad_hip_autotune (0 = library default);
+  // renders resolve it once, when they are submitted (ResolveLaunch)
+  std::atomic<uint32_t> tuned_slots{0}, tuned_blocks_per_cu{0}, tuned_trace_blocks_per_cu{0};
   // The arena of the last finished render job, kept for the next one: fresh
   // device allocations are cleared by the driver in the background, which
   // competed with the first renders after each allocation.

@@ -218,14 +218,15 @@ rad_status rad_hip_server_submit(rad_server* s, rad_scene* scene, const rad_rend
   if (!s || !scene || !desc || !request) return Fail(RAD_ERR_INVALID_ARG, "NULL argument");
   if (!scene->bvh_ready) return Fail(RAD_ERR_STATE, "scene has no BVH (rad_hip_bvh_build)");
   auto r = std::make_unique<Request>();
+  r->desc = *desc;
+  ResolveLaunch(scene, r->desc);
   try {
-    r->footprint = JobFootprint(scene, *desc, nullptr);
+    r->footprint = JobFootprint(scene, r->desc, nullptr);
   } catch (const StatusError& e) {
     return Fail(e.code, e.what());
   }
   if (r->footprint > s->budget) return Fail(RAD_ERR_OUT_OF_MEMORY, "request footprint exceeds the server's HBM budget");
   r->scene = scene;
-  r->desc = *desc;
   r->width = uint32_t(desc->width > 0 ? desc->width : scene->host.cam.width);
   r->height = uint32_t(desc->height > 0 ? desc->height : scene->host.cam.height);
   std::lock_guard<std::mutex> lk(s->mu);
