@@ -264,96 +264,167 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
 
 // The transmittance loop of VolPath::SampleLight (vol_path.cpp:224-342):
 // ratio tracking on the path's channel through media, passing through
-// surfaces without a BSDF, blocked by any surface with one.
+// surfaces without a BSDF, blocked by any surface with one.  The reference
+// intersects the scene again at every medium collision; here each straight
+// segment is traced once and the surface distance is carried across null
+// collisions (si.T - mei.T, as vol_path.cpp:300 keeps it), which changes
+// nothing but the cost.  Lanes are refilled from the shadow queue as their
+// rays finish, so one long transmittance walk does not hold its wave.
+struct VolShadowRay {
+  f3 o, d, oTrace;  // origin, direction, origin the cached hit distance h.t refers to
+  f3 tr;
+  float dist, totalDist, siT;
+  uint32_t s, skey, sdim, ch;
+  int32_t medium;
+  bool needsIntersection, hitValid;
+  Hit h;
+  float4 c;  // contribution
+};
+
+__device__ __forceinline__ void vol_shadow_begin(const Wavefront& w, uint32_t i, VolShadowRay& r) {
+  const float4 o4 = w.sh_o[i], d4 = w.sh_d[i];
+  r.c = w.sh_c[i];
+  r.s = __float_as_uint(d4.w);
+  const uint32_t pk = __float_as_uint(r.c.w);
+  r.medium = int32_t(pk & 0xffu) - 1;
+  r.ch = (pk >> 8) & 3u;
+  r.skey = shadow_key(w.key[r.s], pk >> 10);
+  r.sdim = 0;
+  r.o = mk(o4.x, o4.y, o4.z);
+  r.d = mk(d4.x, d4.y, d4.z);
+  r.oTrace = r.o;
+  r.dist = o4.w;
+  r.totalDist = 0.f;
+  r.tr = splat(1.f);
+  r.needsIntersection = true;
+  r.hitValid = false;
+  r.siT = kInf;
+}
+
+// One pass of the reference's while loop; returns false once the ray is done.
+__device__ __forceinline__ bool vol_shadow_step(const DevScene& scene, VolShadowRay& r, uint32_t& steps) {
+  const float remaining = r.dist * (1.f - kShadowEps) - r.totalDist;
+  if (!(remaining > 0.f)) return false;
+  bool escaped = false;
+  bool activeMedium = r.medium >= 0;
+  bool activeSurface = !activeMedium;
+  if (activeMedium) {
+    const MediumRec& M = scene.media[r.medium];
+    MediumSample mei = medium_sample(scene, M, r.o, r.d, remaining, rng_u(r.skey, r.sdim++), r.ch);
+    steps++;
+    if (r.needsIntersection) {  // once per straight segment
+      r.hitValid = trace<false>(scene, r.o, r.d, 0.f, remaining, r.h);
+      r.siT = r.hitValid ? r.h.t : kInf;
+      r.oTrace = r.o;
+      r.needsIntersection = false;
+    }
+    if (r.siT < mei.T) mei.T = kInf;
+    float t = fminf(remaining, fminf(mei.T, r.siT)) - mei.minT;
+    if (t >= 0.f) {
+      f3 e = mk(expf(mei.comb.x * -t), expf(mei.comb.y * -t), expf(mei.comb.z * -t));
+      f3 ffp = (r.siT < mei.T || mei.T > remaining) ? e : mul(e, mei.comb);
+      float trPdf = chan(ffp, r.ch);
+      f3 mTr = trPdf > 0.f ? e / trPdf : splat(0.f);
+      r.tr = mul(r.tr, mTr);
+    }
+    if (mei.T > remaining && mei.T != kInf) r.totalDist = r.dist;
+    if (mei.T > remaining) mei.T = kInf;
+    escaped = mei.T == kInf;
+    activeMedium = mei.T != kInf;
+    if (activeMedium) {
+      r.totalDist += mei.T;
+      r.o = mei.p;
+      r.siT = r.siT - mei.T;
+      r.tr = mul(r.tr, mei.sig_n);
+    }
+  }
+  bool intersect = activeSurface && r.needsIntersection;
+  if (intersect) {
+    r.hitValid = trace<false>(scene, r.o, r.d, 0.f, remaining, r.h);
+    r.siT = r.hitValid ? r.h.t : kInf;
+    r.oTrace = r.o;
+  }
+  r.needsIntersection = r.needsIntersection && !intersect;
+  activeSurface = activeSurface || escaped;
+  if (activeSurface) r.totalDist += r.siT;
+  activeSurface = activeSurface && r.hitValid && r.siT != kInf && !activeMedium;
+  int32_t nextMedium = r.medium;
+  if (activeSurface) {
+    const ShapeRec& sr = scene.shapes[r.h.geom];
+    if (sr.bsdf >= 0) r.tr = splat(0.f);
+    // h.t was measured from the origin of the trace that found the hit
+    SurfaceHit si = surface_hit(scene, r.oTrace, r.d, r.h.t, r.h.u, r.h.v, r.h.prim, r.h.geom);
+    r.o = offset_p(si.p, si.n, r.d);
+    if (sr.med_in >= 0 || sr.med_out >= 0) nextMedium = dot(r.d, si.n) > 0.f ? sr.med_out : sr.med_in;
+  }
+  r.needsIntersection = r.needsIntersection || activeSurface;
+  r.medium = nextMedium;
+  return (activeMedium || activeSurface) && !is_zero(r.tr);
+}
+
 __global__ void __launch_bounds__(kTraceBlock) k_vol_shadow(Wavefront w, DevScene scene) {
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[CNT_SHADOW];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t steps = 0;
-    if (i < n) {
-      float4 o4 = w.sh_o[i], d4 = w.sh_d[i], c4 = w.sh_c[i];
-      const uint32_t s = __float_as_uint(d4.w);
-      const uint32_t pk = __float_as_uint(c4.w);
-      int32_t medium = int32_t(pk & 0xffu) - 1;
-      const uint32_t ch = (pk >> 8) & 3u;
-      const uint32_t skey = shadow_key(w.key[s], pk >> 10);
-      uint32_t sdim = 0;
-      f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
-      const float dist = o4.w;
-      float totalDist = 0.f;
-      f3 tr = splat(1.f);
-      bool active = true, needsIntersection = true;
-      Hit h;
-      bool hitValid = false;
-      float siT = kInf;
-      for (int guard = 0; active && guard < 100000; guard++) {
-        float remaining = dist * (1.f - kShadowEps) - totalDist;
-        active = active && remaining > 0.f;
-        if (!active) break;
-        bool escaped = false;
-        bool activeMedium = active && medium >= 0;
-        bool activeSurface = active && !activeMedium;
-        if (activeMedium) {
-          const MediumRec& M = scene.media[medium];
-          MediumSample mei = medium_sample(scene, M, o, d, remaining, rng_u(skey, sdim++), ch);
-          steps++;
-          float traceMax = remaining;
-          if (M.type == RAD_MEDIUM_HOMOGENEOUS && mei.T != kInf) traceMax = fminf(mei.T, remaining);
-          if (needsIntersection) {
-            hitValid = trace<false>(scene, o, d, 0.f, traceMax, h);
-            siT = hitValid ? h.t : kInf;
-          }
-          if (siT < mei.T) mei.T = kInf;
-          float t = fminf(remaining, fminf(mei.T, siT)) - mei.minT;
-          if (t >= 0.f) {
-            f3 e = mk(expf(mei.comb.x * -t), expf(mei.comb.y * -t), expf(mei.comb.z * -t));
-            f3 ffp = (siT < mei.T || mei.T > remaining) ? e : mul(e, mei.comb);
-            float trPdf = chan(ffp, ch);
-            f3 mTr = trPdf > 0.f ? e / trPdf : splat(0.f);
-            tr = mul(tr, mTr);
-          }
-          if (mei.T > remaining && mei.T != kInf) totalDist = dist;
-          if (mei.T > remaining) mei.T = kInf;
-          escaped = mei.T == kInf;
-          activeMedium = mei.T != kInf;
-          if (activeMedium) {
-            totalDist += mei.T;
-            o = mei.p;
-            siT = siT - mei.T;
-            tr = mul(tr, mei.sig_n);
-          }
+  uint32_t* fetch = &w.cnt[CNT_FETCH_SH];
+  constexpr uint32_t kChunk = 256;  // queue entries a wave takes with one atomic
+  const uint32_t lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  VolShadowRay r;
+  bool active = false, exhausted = false;
+  uint32_t cbase = 0, cleft = 0, steps = 0, guard = 0;
+  for (;;) {
+    const unsigned long long need = __ballot(!active);
+    if (need && (!exhausted || cleft > 0)) {
+      const uint32_t want = (uint32_t)__popcll(need);
+      const uint32_t rank = (uint32_t)__popcll(need & below);
+      uint32_t nbase = 0, nleft = 0;
+      if (cleft < want && !exhausted) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(fetch, kChunk);
+        b = __shfl(b, 0);
+        if (b >= n) {
+          exhausted = true;
+        } else {
+          nbase = b;
+          nleft = min(kChunk, n - b);
+          if (b + kChunk >= n) exhausted = true;
         }
-        bool intersect = activeSurface && needsIntersection;
-        if (intersect) {
-          hitValid = trace<false>(scene, o, d, 0.f, remaining, h);
-          siT = hitValid ? h.t : kInf;
-        }
-        needsIntersection = needsIntersection && !intersect;
-        activeSurface = activeSurface || escaped;
-        if (activeSurface) totalDist += siT;
-        activeSurface = activeSurface && hitValid && siT != kInf && active && !activeMedium;
-        int32_t nextMedium = medium;
-        if (activeSurface) {
-          const ShapeRec& sr = scene.shapes[h.geom];
-          if (sr.bsdf >= 0) tr = splat(0.f);
-          // the hit distance was measured from the current origin along d
-          SurfaceHit si = surface_hit(scene, o, d, h.t, h.u, h.v, h.prim, h.geom);
-          o = offset_p(si.p, si.n, d);
-          if (sr.med_in >= 0 || sr.med_out >= 0) nextMedium = dot(d, si.n) > 0.f ? sr.med_out : sr.med_in;
-        }
-        needsIntersection = needsIntersection || activeSurface;
-        active = active && (activeMedium || activeSurface) && !is_zero(tr);
-        medium = nextMedium;
       }
-      if (!is_zero(tr)) {
-        float4 l = w.lsum[s];
-        w.lsum[s] = make_float4(l.x + c4.x * tr.x, l.y + c4.y * tr.y, l.z + c4.z * tr.z, l.w);
+      uint32_t i = 0xffffffffu;
+      if (rank < cleft)
+        i = cbase + rank;
+      else if (rank - cleft < nleft)
+        i = nbase + (rank - cleft);
+      const uint32_t fromOld = min(want, cleft);
+      cbase += fromOld;
+      cleft -= fromOld;
+      if (nleft) {
+        const uint32_t fromNew = min(want - fromOld, nleft);
+        cbase = nbase + fromNew;
+        cleft = nleft - fromNew;
+      }
+      if (!active && i < n) {
+        vol_shadow_begin(w, i, r);
+        active = true;
+        guard = 0;
       }
     }
-    block_count(&w.stats[ST_MEDIUM], steps, lds);
+    if (__ballot(active) == 0ull) {
+      if (exhausted && cleft == 0) break;
+      continue;
+    }
+    if (active) {
+      // the guard bounds a walk that no longer moves (finite distances end)
+      if (!vol_shadow_step(scene, r, steps) || ++guard >= 100000) {
+        if (!is_zero(r.tr)) {
+          const float4 l = w.lsum[r.s];
+          w.lsum[r.s] = make_float4(l.x + r.c.x * r.tr.x, l.y + r.c.y * r.tr.y, l.z + r.c.z * r.tr.z, l.w);
+        }
+        active = false;
+      }
+    }
   }
+  block_count(&w.stats[ST_MEDIUM], steps, lds);
 }
 
 void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
