@@ -65,7 +65,7 @@ constexpr float kGuideAlpha = 0.5f;
 constexpr float kTrainWmax = 64.f;               // cap of a record's normalised weight
 constexpr uint32_t kFirstTrainPaths = 2u << 20;   // paths in the first training pass
 constexpr float kGuideMinRough = 0.f;            // glossy BSDFs below this alpha are not guided
-constexpr uint32_t kVisitBudget = 512;           // node visits of a path ray per extend launch
+constexpr uint32_t kVisitBudget = 1024;          // node visits of a path ray per extend launch
 constexpr uint32_t kResumeCap = 16384;           // suspended traversals per extend launch
 
 // Training hyper-parameters; RAD_GUIDE_STEPS / RAD_GUIDE_LR / RAD_GUIDE_ALPHA

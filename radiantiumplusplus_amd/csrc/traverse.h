@@ -364,7 +364,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
   __shared__ uint32_t lds_node[kLdsStack * kTraceBlock];
   __shared__ float lds_dist[kLdsStack * kTraceBlock];
   constexpr int kRefillBelow = 48;  // refill once fewer lanes than this are busy
-  constexpr uint32_t kChunk = 256;
+  constexpr uint32_t kChunk = 128;
   const uint32_t lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   TraceOverflow ov;
