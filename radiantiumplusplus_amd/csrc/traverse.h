@@ -297,17 +297,21 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
     m[i] = cc[i] >= 0 && tnear <= tfb;
     tn[i] = tnear;
   }
-  // leaf children first: they can only shorten tfar
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    if (m[i] && cnt[i] > 0) {
-      uint32_t first = uint32_t(cc[i]);
-      for (uint32_t k = 0; k < cnt[i]; k++) {
-        if (isect_prim(s, first + k, r.o, r.d, r.tmin, r.tfar, r.h)) {
-          r.hit = true;
-          if (ANY) return true;
-        }
-      }
+  // leaf children first: they can only shorten tfar.  The primitives of this
+  // lane's hit leaves form one list (children in order, primitives in
+  // order), so lanes whose leaves sit in different child slots test together
+  // instead of one slot after another.
+  const uint32_t l0 = (m[0] && cnt[0] > 0) ? cnt[0] : 0u, l1 = (m[1] && cnt[1] > 0) ? cnt[1] : 0u;
+  const uint32_t l2 = (m[2] && cnt[2] > 0) ? cnt[2] : 0u, l3 = (m[3] && cnt[3] > 0) ? cnt[3] : 0u;
+  const uint32_t e0 = l0, e1 = e0 + l1, e2 = e1 + l2, total = e2 + l3;
+  for (uint32_t j = 0; j < total; j++) {
+    const uint32_t prim = j < e0   ? uint32_t(cc[0]) + j
+                          : j < e1 ? uint32_t(cc[1]) + (j - e0)
+                          : j < e2 ? uint32_t(cc[2]) + (j - e1)
+                                   : uint32_t(cc[3]) + (j - e2);
+    if (isect_prim(s, prim, r.o, r.d, r.tmin, r.tfar, r.h)) {
+      r.hit = true;
+      if (ANY) return true;
     }
   }
   // inner children still in front of the closest hit, sorted by a fixed network
