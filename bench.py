@@ -279,7 +279,7 @@ def guiding_measure(ctx, scale=1.0, budget=256, ref_spp=4096, quirks=False):
     training inside each render's own budget, from a fresh network) to reach
     the relMSE of the plain GPU tracer at `budget` spp, against a plain
     render at `ref_spp`.  Guided renders are timed at budgets on a grid of
-    16 spp (binary search for the smallest whose relMSE is at most plain's);
+    8 spp (binary search for the smallest whose relMSE is at most plain's);
     both errors are taken against the same reference, so its own noise
     enters both alike.  With the reference's quirks on, visible-normal GGX
     vertices (the glossy boxes) are not guided: their sample density is not
@@ -308,12 +308,12 @@ def guiding_measure(ctx, scale=1.0, budget=256, ref_spp=4096, quirks=False):
 
     runs = {}
 
-    def run(k):  # k units of 16 spp
-        r = guided(16 * k)
-        runs[16 * k] = r
+    def run(k):  # k units of 8 spp
+        r = guided(8 * k)
+        runs[8 * k] = r
         return r
 
-    kb = budget // 16
+    kb = budget // 8
     best = run(kb)
     e_g_budget = best[1]
     lo, hi, reached = 0, kb, best[1] <= e_p
@@ -329,7 +329,7 @@ def guiding_measure(ctx, scale=1.0, budget=256, ref_spp=4096, quirks=False):
         else:
             lo = mid
     t_eq, e_eq, st_eq = best
-    spp_eq = 16 * hi
+    spp_eq = 8 * hi
     return {
         "workload": f"guiding_room_{W}x{H}_quirks_{'on' if quirks else 'off'}",
         "plain_spp": budget,
@@ -341,7 +341,7 @@ def guiding_measure(ctx, scale=1.0, budget=256, ref_spp=4096, quirks=False):
         "guided_relmse_there": e_eq,
         "guided_s_to_plain_relmse": t_eq if reached else None,
         "time_ratio_guided_over_plain": (t_eq / t_plain) if reached else None,
-        "method": "timed guided renders from a fresh network on a 16-spp grid (binary search), smallest budget "
+        "method": "timed guided renders from a fresh network on an 8-spp grid (binary search), smallest budget "
                   "whose relMSE is at most the plain render's; no extrapolation",
         "trials": {str(k): [round(v[0], 3), v[1]] for k, v in sorted(runs.items())},  # spp: [s, relMSE]
         "reference": f"plain GPU render at {ref_spp} spp (the room's plain GPU render is tested against the oracle)",

@@ -63,6 +63,7 @@ struct GuideNet {
   float lo[3], inv_ext[3];  // scene bounds for the position encoding
   float alpha;        // probability of sampling the guide (vs the BSDF)
   float min_rough;    // glossy BSDFs with a smaller GGX/Beckmann alpha are not guided
+  uint32_t max_depth; // vertices deeper than this many bounces are not guided
 };
 
 // Host-owned state of a scene's guide (weights, Adam moments, buffers).

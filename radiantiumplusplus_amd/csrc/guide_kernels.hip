@@ -361,6 +361,7 @@ GuideNet GuideState::Net(const float lo[3], const float hi[3], float alpha, floa
   }
   n.alpha = alpha;
   n.min_rough = min_rough;
+  n.max_depth = 0xffffffffu;
   return n;
 }
 

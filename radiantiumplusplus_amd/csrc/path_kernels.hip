@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             }
           }
           const BsdfRec& b = *bp0;
-          const bool guided = GUIDE && guide::guidable(b, quirks, w.net.min_rough);
+          const bool guided = GUIDE && guide::guidable(b, quirks, w.net.min_rough) && depth <= w.net.max_depth;
           guide::Mixture mx;
           if (guided && guideOn) mx = guide::decode(w.graw + size_t(s) * guide::kOut);
           const float a = (guided && guideOn) ? alpha : 0.f;
@@ -409,8 +409,10 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
 
 // Guided vertices of this iteration (surface hits whose BSDF the guide may
 // sample) -> the guide queue, for the unsorted shading path.
-__device__ __forceinline__ bool guided_hit(const Wavefront& w, const DevScene& scene, uint32_t geom, bool quirks) {
+__device__ __forceinline__ bool guided_hit(const Wavefront& w, const DevScene& scene, uint32_t geom, bool quirks,
+                                           uint32_t s) {
   if (geom == 0xffffffffu) return false;
+  if (w.net.max_depth != 0xffffffffu && w.meta[s].z > w.net.max_depth) return false;
   const ShapeRec& sr = scene.shapes[geom];
   return sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks, w.net.min_rough);
 }
@@ -426,7 +428,7 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
     uint32_t s = 0;
     if (i < n) {
       s = w.queue[in][i];
-      g = !(w.susp && w.susp[s] != kNoSusp) && guided_hit(w, scene, w.hit_geom[s], quirks);
+      g = !(w.susp && w.susp[s] != kNoSusp) && guided_hit(w, scene, w.hit_geom[s], quirks, s);
     }
     uint32_t* c1[1] = {&w.gcnt[G_QUEUE]};
     bool p1[1] = {g};
@@ -528,7 +530,7 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
       if (!held) {
         uint32_t geom = w.hit_geom[s];
         key = material_key(scene, geom);
-        if (GUIDE) gd = guided_hit(w, scene, geom, quirks);
+        if (GUIDE) gd = guided_hit(w, scene, geom, quirks, s);
       }
     }
     uint32_t r7[7] = {0, 0, 0, 0, 0, 0, 0};
