@@ -43,6 +43,10 @@ def parse():
     p.add_argument("--grid", type=int, default=2048)
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     p.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
+    p.add_argument("--summarize-counters", nargs="+", metavar="PMC_DIR",
+                   help="print per-kernel counter sums of rocprofv3 --pmc output directories and exit")
+    p.add_argument("--guiding-only", action="store_true",
+                   help="only the guided-vs-plain measurement (quirks off); prints its record")
     p.add_argument("--unsorted", action="store_true", help="shade with one kernel (no material sort)")
     p.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     p.add_argument("--launch", default=None, metavar="SLOTS,BPC,TPC",
@@ -51,8 +55,10 @@ def parse():
     p.add_argument("--no-serving", action="store_true", help="skip the serving-latency measurement")
     p.add_argument("--guide-scale", type=float, default=1.0, help="image scale of the guiding workload")
     p.add_argument("--serve-requests", type=int, default=40)
-    p.add_argument("--summarize", nargs=6, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "PMC_JSON"),
-                   help="print a text summary of rocprofv3 csv output directories; PMC_JSON: the bench line of "
+    p.add_argument("--summarize", nargs="+", metavar="DIR",
+                   help="TRACE [FETCH WRITE COPY_FETCH COPY_WRITE PMC_JSON]: print a text summary of rocprofv3 "
+                        "csv output directories (the kernel trace alone, or with the traffic counters); "
+                        "PMC_JSON: the bench line of "
                         "the counter runs (their segment count gives the traffic per ray)")
     return p.parse_args()
 
@@ -79,7 +85,9 @@ def summarize(dirs, copy_mib=1024):
     """Kernel-trace statistics plus FETCH_SIZE / WRITE_SIZE per dispatch,
     scaled by the readings of the plain copy of `copy_mib` MiB, and the
     closest-hit kernel's traffic per traced ray."""
-    trace, fetch, write, cfetch, cwrite, pmc_json = dirs
+    if len(dirs) not in (1, 6):
+        raise SystemExit("--summarize takes TRACE, or TRACE FETCH WRITE COPY_FETCH COPY_WRITE PMC_JSON")
+    trace = dirs[0]
     W = 44
     out = [f"{'kernel':{W}s} calls   total_ms    avg_us    min_us    max_us   vgpr  scratch"]
     stats = {}
@@ -93,6 +101,9 @@ def summarize(dirs, copy_mib=1024):
         out.append(f"{k[:W]:{W}s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:10.2f} "
                    f"{float(r['AverageNs']) / 1e3:9.1f} {float(r['MinNs']) / 1e3:9.1f} {float(r['MaxNs']) / 1e3:9.1f} "
                    f"{v:>6s} {sc:>8s}")
+    if len(dirs) == 1:
+        return "\n".join(out)
+    fetch, write, cfetch, cwrite, pmc_json = dirs[1:]
 
     def per_kernel(d):
         acc = {}
@@ -126,6 +137,49 @@ def summarize(dirs, copy_mib=1024):
         out.append("")
         out.append(f"k_extend: {rays:.0f} rays traced in the counter runs, {total / 1e9:.2f} GB read + written beyond L2")
         out.append(f"{TRAFFIC_TAG}{total / rays:.1f}")
+    return "\n".join(out)
+
+
+def summarize_counters(dirs):
+    """Per-kernel sums of the counters in rocprofv3 --pmc output directories,
+    with the derived rates the traversal analysis uses: VALU lane utilisation
+    (active lanes per VALU instruction / 64), the fractions of wave cycles spent
+    waiting and issuing, L2 hit rate, the mean L1 -> L2 read latency and the
+    F16 MFMA work."""
+    acc, launches = {}, {}
+    for d in dirs:
+        for r in _csv_rows(d, "counter_collection.csv"):
+            k = _short(r["Kernel_Name"])
+            c = acc.setdefault(k, {})
+            c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            launches.setdefault((d, k), set()).add(r["Dispatch_Id"])
+    out = []
+    for k in sorted(acc):
+        c = acc[k]
+        n = max(len(v) for (d, kk), v in launches.items() if kk == k)
+        out.append(f"{k}  ({n} dispatches per counter pass)")
+        for name in sorted(c):
+            out.append(f"  {name:32s} {c[name]:.4g}")
+        dv = []
+        if c.get("SQ_ACTIVE_INST_VALU") and "SQ_THREAD_CYCLES_VALU" in c:
+            dv.append(f"VALU lane utilisation {c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.2f}")
+        if c.get("SQ_WAVE_CYCLES"):
+            for name, lab in (("SQ_WAIT_ANY", "waiting"), ("SQ_ACTIVE_INST_ANY", "issuing")):
+                if name in c:
+                    dv.append(f"{lab} {c[name] / c['SQ_WAVE_CYCLES']:.2f} of wave cycles")
+        if c.get("SQ_WAVES"):
+            for name, lab in (("SQ_INSTS_VALU", "VALU"), ("SQ_INSTS_VMEM_RD", "VMEM rd"), ("SQ_INSTS_LDS", "LDS"),
+                              ("SQ_INSTS_SALU", "SALU")):
+                if name in c:
+                    dv.append(f"{lab}/wave {c[name] / c['SQ_WAVES']:.0f}")
+        if "TCC_HIT_sum" in c and c["TCC_HIT_sum"] + c.get("TCC_MISS_sum", 0):
+            dv.append(f"L2 hit {c['TCC_HIT_sum'] / (c['TCC_HIT_sum'] + c['TCC_MISS_sum']):.3f}")
+        if c.get("TCP_TCC_READ_REQ_sum") and "TCP_TCC_READ_REQ_LATENCY_sum" in c:
+            dv.append(f"L1->L2 read latency {c['TCP_TCC_READ_REQ_LATENCY_sum'] / c['TCP_TCC_READ_REQ_sum']:.0f} cycles")
+        if c.get("SQ_INSTS_VALU_MFMA_MOPS_F16"):  # counts F16 MFMA work in units of 512 FLOP
+            dv.append(f"F16 MFMA {c['SQ_INSTS_VALU_MFMA_MOPS_F16'] * 512 / 1e12:.3f} TFLOP")
+        if dv:
+            out.append("  -> " + ", ".join(dv))
     return "\n".join(out)
 
 
@@ -389,6 +443,14 @@ def main():
         return copy_probe(args.copy_probe)
     if args.summarize:
         print(summarize(args.summarize))
+        return
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters))
+        return
+    if args.guiding_only:
+        from radiantiumplusplus_amd import hip
+
+        print(json.dumps(guiding_measure(hip.Context(0), args.guide_scale)), flush=True)
         return
     from radiantiumplusplus_amd import _ffi as F
     from radiantiumplusplus_amd import hip
