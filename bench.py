@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     p.add_argument("--summarize-counters", nargs="+", metavar="PMC_DIR",
                    help="print per-kernel counter sums of rocprofv3 --pmc output directories and exit")
+    p.add_argument("--summarize-overlap", metavar="TRACE",
+                   help="print how much kernels of different streams overlap in a rocprofv3 kernel trace and exit")
+    p.add_argument("--serving-only", action="store_true", help="only the serving stream measurement; prints its record")
     p.add_argument("--guiding-only", action="store_true",
                    help="only the guided-vs-plain measurement (quirks off); prints its record")
     p.add_argument("--unsorted", action="store_true", help="shade with one kernel (no material sort)")
@@ -181,6 +184,41 @@ def summarize_counters(dirs):
         if dv:
             out.append("  -> " + ", ".join(dv))
     return "\n".join(out)
+
+
+def summarize_overlap(trace):
+    """How much the kernels of different HIP streams overlap in a kernel
+    trace: the time during which at least one kernel runs, the part of it
+    with kernels of two or more streams in flight, and the time-weighted mean
+    number of streams with a kernel in flight."""
+    ev = []
+    streams = set()
+    for r in _csv_rows(trace, "kernel_trace.csv"):
+        t0, t1, sid = int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Stream_Id"]
+        if t1 > t0:
+            ev += [(t0, 1, sid), (t1, -1, sid)]
+            streams.add(sid)
+    ev.sort(key=lambda e: (e[0], e[1]))
+    live = {}
+    busy = multi = weighted = 0
+    prev = None
+    for t, d, sid in ev:
+        if prev is not None:
+            n = len(live)
+            dt = t - prev
+            if n >= 1:
+                busy += dt
+                weighted += n * dt
+            if n >= 2:
+                multi += dt
+        live[sid] = live.get(sid, 0) + d
+        if live[sid] == 0:
+            del live[sid]
+        prev = t
+    return (f"streams with kernels: {len(streams)}\n"
+            f"GPU busy (some kernel in flight): {busy / 1e6:.1f} ms\n"
+            f"two or more streams in flight: {multi / 1e6:.1f} ms ({multi / max(busy, 1):.3f} of busy time)\n"
+            f"mean streams in flight while busy: {weighted / max(busy, 1):.2f}")
 
 
 def traffic_per_ray():
@@ -436,7 +474,7 @@ def serving_measure(ctx, n_requests=40, rate=20.0):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not supported (sample-space sharding is a design only)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.copy_probe:
@@ -446,6 +484,14 @@ def main():
         return
     if args.summarize_counters:
         print(summarize_counters(args.summarize_counters))
+        return
+    if args.summarize_overlap:
+        print(summarize_overlap(args.summarize_overlap))
+        return
+    if args.serving_only:
+        from radiantiumplusplus_amd import hip
+
+        print(json.dumps(serving_measure(hip.Context(0))), flush=True)
         return
     if args.guiding_only:
         from radiantiumplusplus_amd import hip
@@ -550,6 +596,8 @@ def main():
             "kernel_ms_per_step": {k: sum(s.kernel_ms[i] for s in stats) / args.steps
                                    for i, k in enumerate(["regen", "extend", "shade", "shadow", "guide", "train"])},
             "bvh_build_ms": info.bvh_build_ms,
+            "bvh_depth": int(info.bvh_depth),
+            "bvh_stack_need": int(info.bvh_stack_need),
             "bvh_builder": args.builder,
             "material_sorted": not args.unsorted,
             "autotune": tune,
