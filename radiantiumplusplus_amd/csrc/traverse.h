@@ -26,8 +26,9 @@ __device__ __forceinline__ f3 crossx(f3 a, f3 b) {
 // Moller-Trumbore in the form Embree 3 applies to its TriangleM primitives:
 // e1 = v0 - v1, e2 = v2 - v0, Ng = e2 x e1, C = v0 - O, R = C x D,
 // accept den != 0, U, V >= 0, U + V <= |den|, |den|*tnear < T <= |den|*tfar.
-__device__ __forceinline__ bool isect_tri(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float tnear, float tfar, float& t,
-                                          float& u, float& v) {
+// Traversal keeps only t = T / |den| of the closest hit; tri_uv recomputes
+// U, V and |den| with the same operations once the ray is finished.
+__device__ __forceinline__ bool isect_tri(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float tnear, float tfar, float& t) {
   f3 e1 = v0 - v1;
   f3 e2 = v2 - v0;
   f3 ng = crossx(e2, e1);
@@ -46,9 +47,25 @@ __device__ __forceinline__ bool isect_tri(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float
   if (den < 0.f) T = -T;
   if (!(absDen * tnear < T && T <= absDen * tfar)) return false;
   t = T / absDen;
+  return true;
+}
+
+__device__ __forceinline__ void tri_uv(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float& u, float& v) {
+  f3 e1 = v0 - v1;
+  f3 e2 = v2 - v0;
+  f3 ng = crossx(e2, e1);
+  f3 c = v0 - o;
+  f3 r = crossx(c, d);
+  float den = dotx(ng, d);
+  float absDen = fabsf(den);
+  float U = dotx(r, e2);
+  float V = dotx(r, e1);
+  if (den < 0.f) {
+    U = -U;
+    V = -V;
+  }
   u = U / absDen;
   v = V / absDen;
-  return true;
 }
 
 // Sphere user geometry (sphere.cpp:40-54 with SolveQuadratic, math_ext.h:79-97).
@@ -83,7 +100,7 @@ __device__ __forceinline__ bool isect_sphere(f3 center, float radius, f3 ro, f3 
 // Rectangle user geometry (rectangle.cpp:34-46): object-space plane z = 0,
 // |x|, |y| <= 1, with an unnormalised object-space direction.
 __device__ __forceinline__ bool isect_rect(const Affine& toObj, f3 ro, f3 rd, float near, float far, float& t,
-                                           float& u, float& v) {
+                                           float& u, float& v) {  // u, v: the local hit point
   const float* m = toObj.m;
   f3 o = mk(fmaf(m[0], ro.x, fmaf(m[1], ro.y, fmaf(m[2], ro.z, m[3]))),
             fmaf(m[4], ro.x, fmaf(m[5], ro.y, fmaf(m[6], ro.z, m[7]))),
@@ -101,32 +118,50 @@ __device__ __forceinline__ bool isect_rect(const Affine& toObj, f3 ro, f3 rd, fl
 
 __device__ __forceinline__ f3 xyz(float4 a) { return mk(a.x, a.y, a.z); }
 
-// Test one primitive record; on a hit narrows `tfar` and fills `h`.
-__device__ __forceinline__ bool isect_prim(const DevScene& s, uint32_t p, f3 o, f3 d, float tnear, float& tfar,
-                                           Hit& h) {
+// Test one primitive record; on a hit narrows `tfar` to its distance.
+__device__ __forceinline__ bool isect_prim(const DevScene& s, uint32_t p, f3 o, f3 d, float tnear, float& tfar) {
   const float4* rec = s.prims + 3 * size_t(p);
-  float4 a = rec[0], b = rec[1], c = rec[2];
-  uint32_t kind = __float_as_uint(c.w);
-  float t, u = 0.f, v = 0.f;
+  const float4 c = rec[2];
+  const uint32_t kind = __float_as_uint(c.w);
+  float t;
   if (kind == PRIM_TRI) {
-    if (!isect_tri(xyz(a), xyz(b), xyz(c), o, d, tnear, tfar, t, u, v)) return false;
+    const float4 a = rec[0], b = rec[1];
+    if (!isect_tri(xyz(a), xyz(b), xyz(c), o, d, tnear, tfar, t)) return false;
+  } else if (kind == PRIM_SPHERE) {
+    const float4 a = rec[0];
+    if (!isect_sphere(xyz(a), a.w, o, d, tnear, tfar, t)) return false;
+  } else {
+    float u, v;
+    if (!isect_rect(s.shapes[__float_as_uint(rec[1].w)].to_local, o, d, tnear, tfar, t, u, v)) return false;
+  }
+  tfar = t;
+  return true;
+}
+
+// The hit record of primitive p at distance t (the closest hit the
+// traversal kept): geometry and primitive ids from the record, u, v
+// recomputed with the arithmetic of the test that accepted it.
+__device__ __forceinline__ void prim_hit(const DevScene& s, uint32_t p, f3 o, f3 d, float t, Hit& h) {
+  const float4* rec = s.prims + 3 * size_t(p);
+  const float4 a = rec[0], b = rec[1], c = rec[2];
+  const uint32_t kind = __float_as_uint(c.w);
+  h.t = t;
+  h.u = 0.f;
+  h.v = 0.f;
+  if (kind == PRIM_TRI) {
+    tri_uv(xyz(a), xyz(b), xyz(c), o, d, h.u, h.v);
     h.geom = __float_as_uint(a.w);
     h.prim = __float_as_uint(b.w);
   } else if (kind == PRIM_SPHERE) {
-    if (!isect_sphere(xyz(a), a.w, o, d, tnear, tfar, t)) return false;
     h.geom = __float_as_uint(b.w);
     h.prim = 0;
   } else {
-    uint32_t g = __float_as_uint(b.w);
-    if (!isect_rect(s.shapes[g].to_local, o, d, tnear, tfar, t, u, v)) return false;
+    const uint32_t g = __float_as_uint(b.w);
+    float tt;
+    isect_rect(s.shapes[g].to_local, o, d, -kInf, kInf, tt, h.u, h.v);
     h.geom = g;
     h.prim = 0;
   }
-  h.t = t;
-  h.u = u;
-  h.v = v;
-  tfar = t;
-  return true;
 }
 
 // ---------------------------------------------------------------- traversal
@@ -186,20 +221,35 @@ struct TraceStack {
 };
 
 struct TraceRay {
-  f3 o, d, inv, oinv;
-  float tmin, tfar;
+  f3 o, d;
+  f3 inv;           // 1 / d, components below 1e-18 in magnitude raised to it (sign kept)
+  bool nx, ny, nz;  // direction components negative: the near plane is the box's high bound
+  float tmin, tfar;  // tfar: the closest hit's distance once hit is set
   uint32_t node;
+  uint32_t hitp;  // primitive record of the closest hit
   bool hit;
-  Hit h;
+  Hit h;  // filled by trace_finish
 };
+
+// Fills the hit record of a finished closest-hit traversal.
+__device__ __forceinline__ void trace_finish(const DevScene& s, TraceRay& r) {
+  if (r.hit) {
+    prim_hit(s, r.hitp, r.o, r.d, r.tfar, r.h);
+  } else {
+    r.h.t = r.tfar;
+    r.h.u = r.h.v = 0.f;
+    r.h.prim = 0;
+    r.h.geom = 0xffffffffu;
+  }
+}
 
 // A closest-hit traversal interrupted at a launch's visit budget: the node
 // to visit next, the pending stack, the closest hit so far.  Resuming it
 // replays nothing, so the result is the one an uninterrupted traversal gives.
 struct ResumeState {
-  uint32_t node, sp, hit, prim, geom;
-  float tfar, t, u, v;
-  uint32_t pad[3];
+  uint32_t node, sp, hit, hitp;
+  float tfar;
+  uint32_t pad[7];
   uint32_t stack_node[kStackDepth];
   float stack_dist[kStackDepth];
 };
@@ -208,12 +258,8 @@ __device__ inline void save_trace(ResumeState& rs, const TraceRay& r, const Trac
   rs.node = r.node;
   rs.sp = uint32_t(st.sp);
   rs.hit = r.hit ? 1u : 0u;
-  rs.prim = r.h.prim;
-  rs.geom = r.h.geom;
+  rs.hitp = r.hitp;
   rs.tfar = r.tfar;
-  rs.t = r.h.t;
-  rs.u = r.h.u;
-  rs.v = r.h.v;
   for (int k = 0; k < st.sp; k++) {
     if (k < st.lo) {
       rs.stack_node[k] = st.ov_node[k];
@@ -230,12 +276,8 @@ __device__ inline void save_trace(ResumeState& rs, const TraceRay& r, const Trac
 __device__ inline void restore_trace(const ResumeState& rs, TraceRay& r, TraceStack& st) {
   r.node = rs.node;
   r.hit = rs.hit != 0;
-  r.h.prim = rs.prim;
-  r.h.geom = rs.geom;
+  r.hitp = rs.hitp;
   r.tfar = rs.tfar;
-  r.h.t = rs.t;
-  r.h.u = rs.u;
-  r.h.v = rs.v;
   const int n = min(int(rs.sp), kStackDepth);
   for (int k = 0; k < n; k++) st.push(rs.stack_node[k], rs.stack_dist[k]);
 }
@@ -243,13 +285,18 @@ __device__ inline void restore_trace(const ResumeState& rs, TraceRay& r, TraceSt
 __device__ __forceinline__ void trace_begin(TraceRay& r, TraceStack& st, f3 o, f3 d, float tmin, float tmax) {
   r.o = o;
   r.d = d;
-  r.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  r.oinv = mk(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
+  // A zero direction component would make the slab arithmetic produce
+  // inf - inf; a tiny one of the same sign keeps every slab finite.
+  const auto safe = [](float x) { return fabsf(x) < 1e-18f ? copysignf(1e-18f, x) : x; };
+  r.inv = mk(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
+  r.nx = r.inv.x < 0.f;
+  r.ny = r.inv.y < 0.f;
+  r.nz = r.inv.z < 0.f;
   r.tmin = tmin;
   r.tfar = tmax;
   r.node = 0;
   r.hit = false;
-  r.h.geom = 0xffffffffu;
+  r.hitp = 0xffffffffu;
   st.sp = 0;
   st.lo = 0;
 }
@@ -273,8 +320,17 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
   const uint4* q = reinterpret_cast<const uint4*>(s.qnodes) + 4 * size_t(r.node);
   const uint4 hdr = q[0], bxy = q[1], bz = q[2];
   const int4 ch = *reinterpret_cast<const int4*>(q + 3);
-  const float ox = __uint_as_float(hdr.x), oy = __uint_as_float(hdr.y), oz = __uint_as_float(hdr.z);
-  const float sx = quant_step(hdr.w), sy = quant_step(hdr.w >> 8), sz = quant_step(hdr.w >> 16);
+  // A child plane at 8-bit offset q along x lies at origin + q * step; its ray
+  // parameter is q * (step / d) + (origin - o) / d.  step is a power of two,
+  // so the first factor is exact, and the near plane of each axis is chosen
+  // once per node by the direction's sign.
+  const float ax = quant_step(hdr.w) * r.inv.x, ay = quant_step(hdr.w >> 8) * r.inv.y,
+              az = quant_step(hdr.w >> 16) * r.inv.z;
+  const float bx = (__uint_as_float(hdr.x) - r.o.x) * r.inv.x, by = (__uint_as_float(hdr.y) - r.o.y) * r.inv.y,
+              bz0 = (__uint_as_float(hdr.z) - r.o.z) * r.inv.z;
+  const uint32_t nearx = r.nx ? bxy.y : bxy.x, farx = r.nx ? bxy.x : bxy.y;
+  const uint32_t neary = r.ny ? bxy.w : bxy.z, fary = r.ny ? bxy.z : bxy.w;
+  const uint32_t nearz = r.nz ? bz.y : bz.x, farz = r.nz ? bz.x : bz.y;
   const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
   uint32_t cnt[4];
   float tn[4];
@@ -283,17 +339,14 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
   for (int i = 0; i < 4; i++) {
     const int sh = 8 * i;
     cnt[i] = (bz.z >> sh) & 0xffu;
-    const float lo_x = dequant((bxy.x >> sh) & 0xffu, sx, ox), hi_x = dequant((bxy.y >> sh) & 0xffu, sx, ox);
-    const float lo_y = dequant((bxy.z >> sh) & 0xffu, sy, oy), hi_y = dequant((bxy.w >> sh) & 0xffu, sy, oy);
-    const float lo_z = dequant((bz.x >> sh) & 0xffu, sz, oz), hi_z = dequant((bz.y >> sh) & 0xffu, sz, oz);
-    float t0x = fmaf(lo_x, r.inv.x, -r.oinv.x), t1x = fmaf(hi_x, r.inv.x, -r.oinv.x);
-    float t0y = fmaf(lo_y, r.inv.y, -r.oinv.y), t1y = fmaf(hi_y, r.inv.y, -r.oinv.y);
-    float t0z = fmaf(lo_z, r.inv.z, -r.oinv.z), t1z = fmaf(hi_z, r.inv.z, -r.oinv.z);
-    float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-    float tfb = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), r.tfar));
-    // slack for the rounding of the fused slab arithmetic
-    tnear = tnear - fabsf(tnear) * 4.8e-7f;
-    tfb = tfb + fabsf(tfb) * 4.8e-7f;
+    const float tnx = fmaf(float((nearx >> sh) & 0xffu), ax, bx), tfx = fmaf(float((farx >> sh) & 0xffu), ax, bx);
+    const float tny = fmaf(float((neary >> sh) & 0xffu), ay, by), tfy = fmaf(float((fary >> sh) & 0xffu), ay, by);
+    const float tnz = fmaf(float((nearz >> sh) & 0xffu), az, bz0), tfz = fmaf(float((farz >> sh) & 0xffu), az, bz0);
+    float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+    float tfb = fminf(fminf(tfx, tfy), fminf(tfz, r.tfar));
+    // slack for the rounding of the slab arithmetic
+    tnear = fmaf(-fabsf(tnear), 4.8e-7f, tnear);
+    tfb = fmaf(fabsf(tfb), 4.8e-7f, tfb);
     m[i] = cc[i] >= 0 && tnear <= tfb;
     tn[i] = tnear;
   }
@@ -305,11 +358,13 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
   const uint32_t l2 = (m[2] && cnt[2] > 0) ? cnt[2] : 0u, l3 = (m[3] && cnt[3] > 0) ? cnt[3] : 0u;
   const uint32_t e0 = l0, e1 = e0 + l1, e2 = e1 + l2, total = e2 + l3;
   for (uint32_t j = 0; j < total; j++) {
-    const uint32_t prim = j < e0   ? uint32_t(cc[0]) + j
-                          : j < e1 ? uint32_t(cc[1]) + (j - e0)
-                          : j < e2 ? uint32_t(cc[2]) + (j - e1)
-                                   : uint32_t(cc[3]) + (j - e2);
-    if (isect_prim(s, prim, r.o, r.d, r.tmin, r.tfar, r.h)) {
+    // child slot of list entry j, chosen by selects rather than branches
+    const uint32_t base = j >= e2 ? uint32_t(cc[3]) - e2
+                          : j >= e1 ? uint32_t(cc[2]) - e1
+                          : j >= e0 ? uint32_t(cc[1]) - e0
+                                    : uint32_t(cc[0]);
+    if (isect_prim(s, base + j, r.o, r.d, r.tmin, r.tfar)) {
+      r.hitp = base + j;
       r.hit = true;
       if (ANY) return true;
     }
@@ -438,6 +493,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
     for (int k = 0; k < 256; k++) {
       if (active) {
         if (trace_step<ANY>(s, r, st)) {
+          if (!ANY) trace_finish(s, r);
           job.finish(idx, tag, r);
           active = false;
         } else if constexpr (Job::kSuspend) {
@@ -470,6 +526,7 @@ __device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin,
   trace_begin(r, st, o, d, tmin, tmax);
   while (!trace_step<ANY>(s, r, st)) {
   }
+  if (!ANY) trace_finish(s, r);
   h = r.h;
   return r.hit;
 }

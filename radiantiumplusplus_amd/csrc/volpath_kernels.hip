@@ -308,16 +308,19 @@ __device__ __forceinline__ bool vol_shadow_step(const DevScene& scene, VolShadow
   bool escaped = false;
   bool activeMedium = r.medium >= 0;
   bool activeSurface = !activeMedium;
+  // The reference intersects in the medium branch (vol_path.cpp:258-263) or
+  // the surface branch (:316-320), whichever is active, from the same origin
+  // and only when a new straight segment starts; one call site serves both.
+  if (r.needsIntersection) {
+    r.hitValid = trace<false>(scene, r.o, r.d, 0.f, remaining, r.h);
+    r.siT = r.hitValid ? r.h.t : kInf;
+    r.oTrace = r.o;
+    r.needsIntersection = false;
+  }
   if (activeMedium) {
     const MediumRec& M = scene.media[r.medium];
     MediumSample mei = medium_sample(scene, M, r.o, r.d, remaining, rng_u(r.skey, r.sdim++), r.ch);
     steps++;
-    if (r.needsIntersection) {  // once per straight segment
-      r.hitValid = trace<false>(scene, r.o, r.d, 0.f, remaining, r.h);
-      r.siT = r.hitValid ? r.h.t : kInf;
-      r.oTrace = r.o;
-      r.needsIntersection = false;
-    }
     if (r.siT < mei.T) mei.T = kInf;
     float t = fminf(remaining, fminf(mei.T, r.siT)) - mei.minT;
     if (t >= 0.f) {
@@ -338,13 +341,6 @@ __device__ __forceinline__ bool vol_shadow_step(const DevScene& scene, VolShadow
       r.tr = mul(r.tr, mei.sig_n);
     }
   }
-  bool intersect = activeSurface && r.needsIntersection;
-  if (intersect) {
-    r.hitValid = trace<false>(scene, r.o, r.d, 0.f, remaining, r.h);
-    r.siT = r.hitValid ? r.h.t : kInf;
-    r.oTrace = r.o;
-  }
-  r.needsIntersection = r.needsIntersection && !intersect;
   activeSurface = activeSurface || escaped;
   if (activeSurface) r.totalDist += r.siT;
   activeSurface = activeSurface && r.hitValid && r.siT != kInf && !activeMedium;
