@@ -516,48 +516,63 @@ __device__ __forceinline__ int material_key(const DevScene& scene, uint32_t geom
   return br.tex_any ? kMatOther : min(br.type, kMatOther);
 }
 
+// Each thread classifies kClassifyPerThread entries per block iteration
+// (entries base + e * 256 + thread): their slot gathers are in flight
+// together and one set of block atomics covers them all.
+constexpr int kClassifyPerThread = 4;
+
 template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
-  __shared__ uint32_t lds[40];
+  constexpr int E = kClassifyPerThread;
+  constexpr int N = GUIDE ? 7 : 6;  // five material queues, pass-through, (guide queue)
+  __shared__ uint32_t lds[N * 5];
   const uint32_t n = w.cnt[in];
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t stride = gridDim.x * blockDim.x * E;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
-    int key = -1;
-    uint32_t s = 0;
-    bool gd = false, held = false;
-    if (i < n) {
-      s = w.queue[in][i];
-      held = w.susp && w.susp[s] != kNoSusp;  // traversal suspended: straight to the next extend queue
-      if (!held) {
-        uint32_t geom = w.hit_geom[s];
-        key = material_key(scene, geom);
-        if (GUIDE) gd = guided_hit(w, scene, geom, quirks, s);
+  uint32_t* ctr[N];
+#pragma unroll
+  for (int k = 0; k < kMatQueues; k++) ctr[k] = &w.cnt[CNT_MAT + k];
+  ctr[kMatQueues] = &w.cnt[1 - in];
+  if (GUIDE) ctr[N - 1] = &w.gcnt[G_QUEUE];
+  for (uint32_t base = blockIdx.x * blockDim.x * E; base < n; base += stride) {
+    uint32_t s[E];
+    int cls[E];
+    bool gd[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const uint32_t i = base + e * blockDim.x + threadIdx.x;
+      s[e] = 0;
+      cls[e] = -1;
+      gd[e] = false;
+      if (i < n) {
+        s[e] = w.queue[in][i];
+        // traversal suspended: straight to the next extend queue
+        const bool held = w.susp && w.susp[s[e]] != kNoSusp;
+        if (held) {
+          cls[e] = kMatQueues;
+        } else {
+          const uint32_t geom = w.hit_geom[s[e]];
+          cls[e] = material_key(scene, geom);
+          if (GUIDE) gd[e] = guided_hit(w, scene, geom, quirks, s[e]);
+        }
       }
     }
-    uint32_t r7[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (GUIDE) {  // lds: 7 counters x (4 waves + 1) = 35 words
-      uint32_t* c7[7] = {&w.cnt[CNT_MAT],     &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
-                         &w.cnt[CNT_MAT + 4], &w.cnt[1 - in],      &w.gcnt[G_QUEUE]};
-      bool p7[7] = {key == 0, key == 1, key == 2, key == 3, key == 4, held, gd};
-      block_append<7>(c7, p7, r7, lds);
-    } else {
-      uint32_t* c6[6] = {&w.cnt[CNT_MAT], &w.cnt[CNT_MAT + 1], &w.cnt[CNT_MAT + 2], &w.cnt[CNT_MAT + 3],
-                         &w.cnt[CNT_MAT + 4], &w.cnt[1 - in]};
-      bool p6[6] = {key == 0, key == 1, key == 2, key == 3, key == 4, held};
-      uint32_t r6[6];
-      block_append<6>(c6, p6, r6, lds);
+    uint32_t pos[E], posg[E];
+    block_append_n<N, E, GUIDE>(ctr, cls, gd, pos, posg, lds);
 #pragma unroll
-      for (int k = 0; k < 6; k++) r7[k] = r6[k];
+    for (int e = 0; e < E; e++) {
+      const int c = cls[e];
+      if (c >= 0) {
+        uint32_t* q = c == kMatQueues ? w.queue[1 - in]
+                      : c == 0        ? w.mq[0]
+                      : c == 1        ? w.mq[1]
+                      : c == 2        ? w.mq[2]
+                      : c == 3        ? w.mq[3]
+                                      : w.mq[4];
+        q[pos[e]] = s[e];
+      }
+      if (GUIDE && gd[e]) w.gq[posg[e]] = s[e];
     }
-    if (held) w.queue[1 - in][r7[5]] = s;
-    if (key >= 0) {
-      uint32_t* q = key == 0 ? w.mq[0] : key == 1 ? w.mq[1] : key == 2 ? w.mq[2] : key == 3 ? w.mq[3] : w.mq[4];
-      uint32_t r = key == 0 ? r7[0] : key == 1 ? r7[1] : key == 2 ? r7[2] : key == 3 ? r7[3] : r7[4];
-      q[r] = s;
-    }
-    if (GUIDE && gd) w.gq[r7[6]] = s;
   }
 }
 

@@ -147,6 +147,56 @@ __device__ __forceinline__ void block_append(uint32_t* const (&ctr)[N], const bo
   __syncthreads();
 }
 
+// Block-wide append of E entries per thread.  Entry e goes to one of the
+// first N - EXTRA counters (cls[e], or -1 for none) and, with EXTRA, also to
+// counter N - 1 when ext[e]; one atomic per counter covers all E entries of
+// every thread, so the block's load -> count -> atomic -> store chain is paid
+// once per E * blockDim entries.  pos / pos_ext: each entry's position in its
+// queue(s).  lds: N * (waves + 1) words.  Every thread of the block calls it.
+template <int N, int E, bool EXTRA>
+__device__ __forceinline__ void block_append_n(uint32_t* const (&ctr)[N], const int (&cls)[E], const bool (&ext)[E],
+                                               uint32_t (&pos)[E], uint32_t (&pos_ext)[E], uint32_t* lds) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const bool isExt = EXTRA && k == N - 1;
+    uint32_t acc = 0;  // wave-uniform running count of counter k
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const bool p = isExt ? ext[e] : cls[e] == k;
+      const unsigned long long m = __ballot(p);
+      const uint32_t local = acc + (uint32_t)__popcll(m & below);
+      if (p) {
+        if (isExt)
+          pos_ext[e] = local;
+        else
+          pos[e] = local;
+      }
+      acc += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) lds[k * W + wave] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < uint32_t(N)) {
+    const int k = int(threadIdx.x);
+    uint32_t total = 0;
+    for (uint32_t v = 0; v < W; v++) {
+      uint32_t c = lds[k * W + v];
+      lds[k * W + v] = total;
+      total += c;
+    }
+    lds[N * W + k] = total ? atomicAdd(ctr[k], total) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    if (cls[e] >= 0) pos[e] += lds[N * W + cls[e]] + lds[cls[e] * W + wave];
+    if (EXTRA && ext[e]) pos_ext[e] += lds[N * W + N - 1] + lds[(N - 1) * W + wave];
+  }
+  __syncthreads();
+}
+
 // Block-wide sum into a 64-bit statistics counter (one atomic per block).
 __device__ __forceinline__ void block_count(unsigned long long* counter, uint32_t v, uint32_t* lds) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
