@@ -311,11 +311,33 @@ __device__ __forceinline__ void cswap(float& da, uint32_t& na, float& db, uint32
   na = tn;
 }
 
-// Visit one node: test its four child boxes, intersect the primitives of hit
-// leaf children, descend into the nearest hit inner child and push the
-// others (far first).  Returns true when the ray is finished.
+// Per-wave LDS of the pooled primitive tests: a counter, kPoolSize
+// (primitive, owner lane) entries and one closest-hit key per lane.
+constexpr uint32_t kPoolSize = 64;
+struct PrimPool {
+  lds_u32* ctr;    // entries posted in this node step
+  lds_u32* prim;   // [kPoolSize]
+  lds_u32* owner;  // [kPoolSize]
+  __attribute__((address_space(3))) unsigned long long* key;  // this wave's 64 lanes
+};
+
+// Orders the wave's LDS accesses around the pooled tests: LDS executes one
+// wave's instructions in order, so only the compiler needs holding back.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Visit one node: test its four child boxes, push its hit inner children
+// (far first) and descend into the nearest, and intersect the primitives of
+// its hit leaf children.  The lanes of a wave test their leaf primitives as
+// one pool: each lane posts its (primitive, lane) entries to LDS, every lane
+// taking part in this step tests entries of the pool against the owner's ray,
+// and a 64-bit LDS minimum of (t, primitive) returns each owner its closest
+// hit.  Returns true when the ray is finished.  Every lane whose ray is in
+// flight calls it together (the pool reads other lanes' registers).
 template <bool ANY>
-__device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, TraceStack& st) {
+__device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, TraceStack& st, const PrimPool& pool) {
   // one 64-byte quantised node: origin + steps, 8-bit child bounds, counts, children
   const uint4* q = reinterpret_cast<const uint4*>(s.qnodes) + 4 * size_t(r.node);
   const uint4 hdr = q[0], bxy = q[1], bz = q[2];
@@ -350,26 +372,10 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
     m[i] = cc[i] >= 0 && tnear <= tfb;
     tn[i] = tnear;
   }
-  // leaf children first: they can only shorten tfar.  The primitives of this
-  // lane's hit leaves form one list (children in order, primitives in
-  // order), so lanes whose leaves sit in different child slots test together
-  // instead of one slot after another.
-  const uint32_t l0 = (m[0] && cnt[0] > 0) ? cnt[0] : 0u, l1 = (m[1] && cnt[1] > 0) ? cnt[1] : 0u;
-  const uint32_t l2 = (m[2] && cnt[2] > 0) ? cnt[2] : 0u, l3 = (m[3] && cnt[3] > 0) ? cnt[3] : 0u;
-  const uint32_t e0 = l0, e1 = e0 + l1, e2 = e1 + l2, total = e2 + l3;
-  for (uint32_t j = 0; j < total; j++) {
-    // child slot of list entry j, chosen by selects rather than branches
-    const uint32_t base = j >= e2 ? uint32_t(cc[3]) - e2
-                          : j >= e1 ? uint32_t(cc[2]) - e1
-                          : j >= e0 ? uint32_t(cc[1]) - e0
-                                    : uint32_t(cc[0]);
-    if (isect_prim(s, base + j, r.o, r.d, r.tmin, r.tfar)) {
-      r.hitp = base + j;
-      r.hit = true;
-      if (ANY) return true;
-    }
-  }
-  // inner children still in front of the closest hit, sorted by a fixed network
+  // inner children in front of the current closest hit, sorted by a fixed
+  // network: the nearest is visited next, the others pushed far first.  A
+  // leaf hit below can only move tfar closer; entries then behind it are
+  // skipped when popped.
   float d0 = kInf, d1 = kInf, d2 = kInf, d3 = kInf;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
   if (m[0] && cnt[0] == 0 && tn[0] <= r.tfar) { d0 = tn[0]; n0 = uint32_t(cc[0]); }
@@ -381,10 +387,69 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
   cswap(d0, n0, d2, n2);
   cswap(d1, n1, d3, n3);
   cswap(d1, n1, d2, n2);
-  if (d0 != kInf) {
-    if (d3 != kInf) st.push(n3, d3);
-    if (d2 != kInf) st.push(n2, d2);
-    if (d1 != kInf) st.push(n1, d1);
+  if (d3 != kInf) st.push(n3, d3);
+  if (d2 != kInf) st.push(n2, d2);
+  if (d1 != kInf) st.push(n1, d1);
+  // this lane's leaf primitives as one list (children in order)
+  const uint32_t l0 = (m[0] && cnt[0] > 0) ? cnt[0] : 0u, l1 = (m[1] && cnt[1] > 0) ? cnt[1] : 0u;
+  const uint32_t l2 = (m[2] && cnt[2] > 0) ? cnt[2] : 0u, l3 = (m[3] && cnt[3] > 0) ? cnt[3] : 0u;
+  const uint32_t e0 = l0, e1 = e0 + l1, e2 = e1 + l2, total = e2 + l3;
+  const uint32_t b0 = uint32_t(cc[0]), b1 = uint32_t(cc[1]) - e0, b2 = uint32_t(cc[2]) - e1, b3 = uint32_t(cc[3]) - e2;
+  const uint32_t lane = __lane_id();
+  const uint32_t pre =  // this lane's first pool index
+      total ? __hip_atomic_fetch_add(pool.ctr, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0u;
+  wave_lds_fence();
+  const uint32_t sum = *pool.ctr;  // wave-uniform
+  bool leafHit = false;
+  if (sum > 0) {
+    if (total) pool.key[lane] = ~0ull;
+    const unsigned long long act = __ballot(true);
+    const uint32_t nact = (uint32_t)__popcll(act);
+    const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+    for (uint32_t rb = 0; rb < sum; rb += kPoolSize) {
+      // post this lane's entries with pool index in [rb, rb + kPoolSize)
+      const uint32_t k0 = rb > pre ? rb - pre : 0u;
+      const uint32_t k1 = min(total, rb + kPoolSize > pre ? rb + kPoolSize - pre : 0u);
+      for (uint32_t k = k0; k < k1; k++) {
+        const uint32_t prim = k >= e2 ? b3 + k : k >= e1 ? b2 + k : k >= e0 ? b1 + k : b0 + k;
+        pool.prim[pre + k - rb] = prim;
+        pool.owner[pre + k - rb] = lane;
+      }
+      wave_lds_fence();
+      const uint32_t cnt_r = min(kPoolSize, sum - rb);
+      for (uint32_t jb = 0; jb < cnt_r; jb += nact) {
+        const uint32_t j = jb + rank;
+        const bool valid = j < cnt_r;
+        const uint32_t own = valid ? pool.owner[j] : lane;
+        // the owner's ray, read from its registers (all lanes of the step take part)
+        const f3 o = mk(__shfl(r.o.x, own), __shfl(r.o.y, own), __shfl(r.o.z, own));
+        const f3 d = mk(__shfl(r.d.x, own), __shfl(r.d.y, own), __shfl(r.d.z, own));
+        const float tmin = __shfl(r.tmin, own);
+        float tfar = __shfl(r.tfar, own);
+        if (valid) {
+          const uint32_t prim = pool.prim[j];
+          if (isect_prim(s, prim, o, d, tmin, tfar))
+            __hip_atomic_fetch_min(&pool.key[own], (uint64_t(__float_as_uint(tfar)) << 32) | prim, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+      }
+      wave_lds_fence();
+    }
+    if (total) {
+      const unsigned long long k = pool.key[lane];
+      if (k != ~0ull) {
+        r.tfar = __uint_as_float(uint32_t(k >> 32));
+        r.hitp = uint32_t(k);
+        r.hit = true;
+        leafHit = true;
+      }
+    }
+    wave_lds_fence();
+    if (rank == 0) *pool.ctr = 0u;  // ready for the next step
+    wave_lds_fence();
+  }
+  if (ANY && leafHit) return true;
+  if (d0 != kInf && d0 <= r.tfar) {
     r.node = n0;
     return false;
   }
@@ -422,6 +487,9 @@ template <bool ANY, class Job>
 __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint32_t* fetch, Job& job) {
   __shared__ uint32_t lds_node[kLdsStack * kTraceBlock];
   __shared__ float lds_dist[kLdsStack * kTraceBlock];
+  __shared__ uint32_t pool_ctr[kTraceBlock / 64], pool_prim[kTraceBlock / 64][kPoolSize],
+      pool_owner[kTraceBlock / 64][kPoolSize];
+  __shared__ unsigned long long pool_key[kTraceBlock];
   constexpr int kRefillBelow = 48;  // refill once fewer lanes than this are busy
   constexpr uint32_t kChunk = 128;
   const uint32_t lane = __lane_id();
@@ -432,6 +500,11 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
   st.dist = (lds_f32*)(lds_dist + threadIdx.x);
   st.ov_node = ov.node;
   st.ov_dist = ov.dist;
+  const uint32_t wv = threadIdx.x >> 6;
+  const PrimPool pool{(lds_u32*)&pool_ctr[wv], (lds_u32*)pool_prim[wv], (lds_u32*)pool_owner[wv],
+                      (__attribute__((address_space(3))) unsigned long long*)&pool_key[wv * 64]};
+  if (lane == 0) *pool.ctr = 0u;
+  wave_lds_fence();
   TraceRay r;
   bool active = false, exhausted = false;
   uint32_t idx = 0, tag = 0, visits = 0;
@@ -492,7 +565,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
     }
     for (int k = 0; k < 256; k++) {
       if (active) {
-        if (trace_step<ANY>(s, r, st)) {
+        if (trace_step<ANY>(s, r, st, pool)) {
           if (!ANY) trace_finish(s, r);
           job.finish(idx, tag, r);
           active = false;
@@ -516,6 +589,9 @@ template <bool ANY>
 __device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin, float tmax, Hit& h) {
   __shared__ uint32_t lds_node1[kLdsStack * kTraceBlock];
   __shared__ float lds_dist1[kLdsStack * kTraceBlock];
+  __shared__ uint32_t pool_ctr1[kTraceBlock / 64], pool_prim1[kTraceBlock / 64][kPoolSize],
+      pool_owner1[kTraceBlock / 64][kPoolSize];
+  __shared__ unsigned long long pool_key1[kTraceBlock];
   TraceRay r;
   TraceOverflow ov;
   TraceStack st;
@@ -523,8 +599,14 @@ __device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin,
   st.dist = (lds_f32*)(lds_dist1 + threadIdx.x);
   st.ov_node = ov.node;
   st.ov_dist = ov.dist;
+  const uint32_t wv = threadIdx.x >> 6;
+  const PrimPool pool{(lds_u32*)&pool_ctr1[wv], (lds_u32*)pool_prim1[wv], (lds_u32*)pool_owner1[wv],
+                      (__attribute__((address_space(3))) unsigned long long*)&pool_key1[wv * 64]};
+  // the counter is zero between node steps; clear it once for this launch
+  if (__lane_id() == uint32_t(__ffsll((long long)__ballot(true)) - 1)) *pool.ctr = 0u;
+  wave_lds_fence();
   trace_begin(r, st, o, d, tmin, tmax);
-  while (!trace_step<ANY>(s, r, st)) {
+  while (!trace_step<ANY>(s, r, st, pool)) {
   }
   if (!ANY) trace_finish(s, r);
   h = r.h;
