@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -253,6 +254,14 @@ rad_status rad_hip_bvh_build(rad_scene* scene, const rad_bvh_opts* opts) {
     scene->dev.prims = scene->d_prims.as<float4>();
     scene->dev.prim_count = n;
     scene->dev.node_count = nodeCount;
+    {
+      // RAD_TRACE_POOL (1..64) shrinks the pooled primitive tests so that
+      // tests can drive the path for entries beyond the pool; images do not
+      // depend on it.
+      const char* e = std::getenv("RAD_TRACE_POOL");
+      const long v = e ? std::strtol(e, nullptr, 10) : long(kTracePool);
+      scene->dev.pool_cap = uint32_t(std::min<long>(std::max<long>(v, 1), long(kTracePool)));
+    }
     scene->bvh_depth = depth;
     scene->bvh_stack_need = stackNeed;
     scene->sah_cost = sah;

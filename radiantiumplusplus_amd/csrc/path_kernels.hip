@@ -183,11 +183,11 @@ struct ExtendJob {
   }
 };
 
-// Capped at 96 VGPRs (5 waves per SIMD): traversal waits on node and
-// primitive loads, and resident waves hide that wait, but the pooled
-// primitive tests need registers; per 1080p large-mesh frame 84.3 ms at 5
-// waves, 93.4 ms at 6 (36 registers spill), 180 ms at 7 (72 spill).
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(5))) k_extend(Wavefront w, DevScene scene, int in) {
+// Capped at 80 VGPRs (6 waves per SIMD; four registers spill): traversal
+// waits on node and primitive loads, and resident waves hide that wait, but
+// the pooled primitive tests need registers; per 1080p large-mesh frame
+// 78.2 ms at 6 waves, 84.8 ms at 5.
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))) k_extend(Wavefront w, DevScene scene, int in) {
   ExtendJob job{w.queue[in], w.ray_o, w.ray_d, w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
                 &w.cnt[CNT_SUSP + 1 - in], w.resume_cap, w.susp ? w.visit_budget : 0xffffffffu};
   trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
@@ -640,9 +640,9 @@ struct ShadowJob {
   }
 };
 
-// Capped at 80 VGPRs (6 waves per SIMD; five registers spill): per 1080p
-// large-mesh frame 34.0 ms at 6 waves, 36.5 ms at 5, 39.9 ms at 7.
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))) k_shadow(Wavefront w, DevScene scene) {
+// Capped at 72 VGPRs (7 waves per SIMD; five registers spill): per 1080p
+// large-mesh frame 32.1 ms at 7 waves, 33.6 ms at 6.
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_shadow(Wavefront w, DevScene scene) {
   ShadowJob job{w.sh_o, w.sh_d, w.sh_c, w.lsum};
   trace_stream<true>(scene, w.cnt[CNT_SHADOW], &w.cnt[CNT_FETCH_SH], job);
 }

@@ -33,6 +33,9 @@ static_assert(sizeof(BvhNode4) == 128, "BVH node must be one 128-byte line");
 // entries (0 for a node whose children are all leaves); the builders compute
 // need(root) and a tree that needs more than this is refused.
 constexpr uint32_t kTraceStackEntries = 64;
+// Entries of a wave's pooled primitive tests per node step (traverse.h); a
+// lane whose entries fall beyond the pool tests them itself.
+constexpr uint32_t kTracePool = 64;
 
 // The traversal's copy of a BvhNode4 in 64 bytes: the node box origin, a
 // power-of-two step per axis, and every child's bounds as 8-bit multiples of
@@ -219,6 +222,7 @@ struct DevScene {
   f3 world_center;
   float world_radius;
   uint32_t quirks;
+  uint32_t pool_cap;  // pooled primitive tests per wave step, 1..kTracePool (smaller only to test the overflow path)
 };
 
 }  // namespace rad

@@ -313,7 +313,7 @@ __device__ __forceinline__ void cswap(float& da, uint32_t& na, float& db, uint32
 
 // Per-wave LDS of the pooled primitive tests: a counter, kPoolSize
 // (primitive, owner lane) entries and one closest-hit key per lane.
-constexpr uint32_t kPoolSize = 64;
+constexpr uint32_t kPoolSize = kTracePool;
 struct PrimPool {
   lds_u32* ctr;    // entries posted in this node step
   lds_u32* prim;   // [kPoolSize]
@@ -402,39 +402,47 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
   const uint32_t sum = *pool.ctr;  // wave-uniform
   bool leafHit = false;
   if (sum > 0) {
-    if (total) pool.key[lane] = ~0ull;
+    // Post the entries that fit the pool; a lane whose entries run past it
+    // (the wave has more than kPoolSize, which is rare) tests those itself.
+    const uint32_t cap = s.pool_cap;
+    const uint32_t fit = pre < cap ? min(total, cap - pre) : 0u;
+    unsigned long long best = ~0ull;
+    for (uint32_t k = 0; k < total; k++) {
+      const uint32_t prim = k >= e2 ? b3 + k : k >= e1 ? b2 + k : k >= e0 ? b1 + k : b0 + k;
+      if (k < fit) {
+        pool.prim[pre + k] = prim;
+        pool.owner[pre + k] = lane;
+      } else {
+        float tf = r.tfar;
+        if (isect_prim(s, prim, r.o, r.d, r.tmin, tf)) {
+          const unsigned long long key = (uint64_t(__float_as_uint(tf)) << 32) | prim;
+          if (key < best) best = key;  // unsigned order: (t, primitive)
+        }
+      }
+    }
+    if (total) pool.key[lane] = best;
+    wave_lds_fence();
     const unsigned long long act = __ballot(true);
     const uint32_t nact = (uint32_t)__popcll(act);
     const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
-    for (uint32_t rb = 0; rb < sum; rb += kPoolSize) {
-      // post this lane's entries with pool index in [rb, rb + kPoolSize)
-      const uint32_t k0 = rb > pre ? rb - pre : 0u;
-      const uint32_t k1 = min(total, rb + kPoolSize > pre ? rb + kPoolSize - pre : 0u);
-      for (uint32_t k = k0; k < k1; k++) {
-        const uint32_t prim = k >= e2 ? b3 + k : k >= e1 ? b2 + k : k >= e0 ? b1 + k : b0 + k;
-        pool.prim[pre + k - rb] = prim;
-        pool.owner[pre + k - rb] = lane;
+    const uint32_t cnt_r = min(cap, sum);
+    for (uint32_t jb = 0; jb < cnt_r; jb += nact) {
+      const uint32_t j = jb + rank;
+      const bool valid = j < cnt_r;
+      const uint32_t own = valid ? pool.owner[j] : lane;
+      // the owner's ray, read from its registers (all lanes of the step take part)
+      const f3 o = mk(__shfl(r.o.x, own), __shfl(r.o.y, own), __shfl(r.o.z, own));
+      const f3 d = mk(__shfl(r.d.x, own), __shfl(r.d.y, own), __shfl(r.d.z, own));
+      const float tmin = __shfl(r.tmin, own);
+      float tfar = __shfl(r.tfar, own);
+      if (valid) {
+        const uint32_t prim = pool.prim[j];
+        if (isect_prim(s, prim, o, d, tmin, tfar))
+          __hip_atomic_fetch_min(&pool.key[own], (uint64_t(__float_as_uint(tfar)) << 32) | prim, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
-      wave_lds_fence();
-      const uint32_t cnt_r = min(kPoolSize, sum - rb);
-      for (uint32_t jb = 0; jb < cnt_r; jb += nact) {
-        const uint32_t j = jb + rank;
-        const bool valid = j < cnt_r;
-        const uint32_t own = valid ? pool.owner[j] : lane;
-        // the owner's ray, read from its registers (all lanes of the step take part)
-        const f3 o = mk(__shfl(r.o.x, own), __shfl(r.o.y, own), __shfl(r.o.z, own));
-        const f3 d = mk(__shfl(r.d.x, own), __shfl(r.d.y, own), __shfl(r.d.z, own));
-        const float tmin = __shfl(r.tmin, own);
-        float tfar = __shfl(r.tfar, own);
-        if (valid) {
-          const uint32_t prim = pool.prim[j];
-          if (isect_prim(s, prim, o, d, tmin, tfar))
-            __hip_atomic_fetch_min(&pool.key[own], (uint64_t(__float_as_uint(tfar)) << 32) | prim, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
-        }
-      }
-      wave_lds_fence();
     }
+    wave_lds_fence();
     if (total) {
       const unsigned long long k = pool.key[lane];
       if (k != ~0ull) {
@@ -444,7 +452,6 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
         leafHit = true;
       }
     }
-    wave_lds_fence();
     if (rank == 0) *pool.ctr = 0u;  // ready for the next step
     wave_lds_fence();
   }

@@ -99,3 +99,38 @@ def test_empty_batch_and_bvh_info(hip_ctx):
     info = g.info()
     assert info.primitive_count == b.triangle_count + 2
     assert info.node_count > 0 and info.bvh_depth > 0
+
+
+@pytest.mark.parametrize("pool", [1, 5])
+def test_primitives_beyond_the_wave_pool(hip_ctx, monkeypatch, pool):
+    """A wave tests up to 64 leaf primitives per node step as one pool; a
+    lane whose entries fall beyond it tests them itself.  RAD_TRACE_POOL
+    shrinks the pool so that this path carries most primitives: hits must
+    still match the brute force, and images must equal the full pool's bit
+    for bit (the closest hit is the minimum of (t, primitive) either way)."""
+    b = scenes.many_balls(seed=6, n_spheres=64, slices=32, width=24, height=24)
+    o = O.OracleScene(b)
+    full = hip.Scene.from_builder(hip_ctx, b)
+    monkeypatch.setenv("RAD_TRACE_POOL", str(pool))
+    small = hip.Scene.from_builder(hip_ctx, b)
+    lo, hi = o.bounds()
+    rays = ray_batch(lo, hi, 20000, 13)
+    check_parity(o, rays, small.closest(rays), o.closest(rays, brute=True))
+    occ_b = o.occluded(rays, brute=True)
+    assert (small.occluded(rays) != occ_b).sum() <= 5
+    d = b.render_desc(spp=8, seed=3)
+    img_full = full.render(d)[0]
+    img_small = small.render(d)[0]
+    assert np.array_equal(img_full, img_small)
+
+
+def test_vol_path_image_independent_of_the_wave_pool(hip_ctx, monkeypatch):
+    """The transmittance rays of vol_path trace one ray per lane in a
+    data-dependent loop; their pooled tests must give the same image with a
+    one-entry pool."""
+    b = scenes.volume_box(width=24, height=24, res=16, spp=8)
+    full = hip.Scene.from_builder(hip_ctx, b)
+    monkeypatch.setenv("RAD_TRACE_POOL", "1")
+    small = hip.Scene.from_builder(hip_ctx, b)
+    d = b.render_desc(spp=8, seed=5)
+    assert np.array_equal(full.render(d)[0], small.render(d)[0])
