@@ -274,7 +274,7 @@ void RunJob(rad_job* job) {
         if (prof) HipCheck(hipEventRecord(ev[2], st), "event");
         const bool vol = w.integrator == RAD_INTEGRATOR_VOL_PATH;
         const bool guided = (w.flags & WF_GUIDE) != 0;
-        if (!vol && job->sorted) {
+        if (job->sorted) {
           LaunchClassify(w, scene, in, cfg, st);  // with guiding, also the guide queue
           if (job->tally) LaunchQueueCheck(w, scene, in, job->tally, job->tally_selftest, cfg, st);
         }
@@ -283,7 +283,9 @@ void RunJob(rad_job* job) {
         if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
         if (!vol && guided) LaunchGuideInfer(w, scene, inferBlocks, st);
         if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
-        if (vol)
+        if (vol && job->sorted)
+          LaunchShadeVolMaterials(w, scene, in, job->mat_mask, cfg, st);
+        else if (vol)
           LaunchShadeVol(w, scene, in, cfg, st);
         else if (job->sorted)
           LaunchShadeMaterials(w, scene, in, job->mat_mask, cfg, st);
@@ -512,7 +514,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.rr_depth = desc.rr_depth;
   w.flags = (desc.flags & RAD_RENDER_FIX_QUIRKS) ? 0u : WF_QUIRKS;
   w.integrator = desc.integrator;
-  job->sorted = desc.integrator == RAD_INTEGRATOR_PATH && !(desc.flags & RAD_RENDER_UNSORTED);
+  job->sorted = !(desc.flags & RAD_RENDER_UNSORTED);
   for (const BsdfRec& b : scene->host.bsdfs) job->mat_mask |= 1u << (b.tex_any ? kMatOther : std::min<int32_t>(b.type, kMatOther));
   HipCheck(hipHostMalloc((void**)&job->h_cnt, CNT_WORDS * 4 + ST_WORDS * 8 + 2 * G_WORDS * 4, hipHostMallocDefault),
            "hipHostMalloc");

@@ -330,13 +330,15 @@ __device__ __forceinline__ void wave_lds_fence() {
 
 // Visit one node: test its four child boxes, push its hit inner children
 // (far first) and descend into the nearest, and intersect the primitives of
-// its hit leaf children.  The lanes of a wave test their leaf primitives as
-// one pool: each lane posts its (primitive, lane) entries to LDS, every lane
-// taking part in this step tests entries of the pool against the owner's ray,
-// and a 64-bit LDS minimum of (t, primitive) returns each owner its closest
-// hit.  Returns true when the ray is finished.  Every lane whose ray is in
-// flight calls it together (the pool reads other lanes' registers).
-template <bool ANY>
+// its hit leaf children; the closest hit is the minimum of (t, primitive)
+// over the accepted primitives.  POOL: the lanes of a wave test their leaf
+// primitives as one pool: each lane posts its (primitive, lane) entries to
+// LDS, every lane taking part in this step tests entries of the pool against
+// the owner's ray, and a 64-bit LDS minimum returns each owner its closest
+// hit; every lane whose ray is in flight then calls it together (the pool
+// reads other lanes' registers).  Without POOL each lane tests its own
+// primitives.  Returns true when the ray is finished.
+template <bool ANY, bool POOL>
 __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, TraceStack& st, const PrimPool& pool) {
   // one 64-byte quantised node: origin + steps, 8-bit child bounds, counts, children
   const uint4* q = reinterpret_cast<const uint4*>(s.qnodes) + 4 * size_t(r.node);
@@ -395,12 +397,31 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
   const uint32_t l2 = (m[2] && cnt[2] > 0) ? cnt[2] : 0u, l3 = (m[3] && cnt[3] > 0) ? cnt[3] : 0u;
   const uint32_t e0 = l0, e1 = e0 + l1, e2 = e1 + l2, total = e2 + l3;
   const uint32_t b0 = uint32_t(cc[0]), b1 = uint32_t(cc[1]) - e0, b2 = uint32_t(cc[2]) - e1, b3 = uint32_t(cc[3]) - e2;
+  bool leafHit = false;
+  if constexpr (!POOL) {
+    // one lane on its own: its primitives in order, the same (t, primitive) minimum
+    unsigned long long best = ~0ull;
+    for (uint32_t k = 0; k < total; k++) {
+      const uint32_t prim = k >= e2 ? b3 + k : k >= e1 ? b2 + k : k >= e0 ? b1 + k : b0 + k;
+      float tf = r.tfar;
+      if (isect_prim(s, prim, r.o, r.d, r.tmin, tf)) {
+        const unsigned long long key = (uint64_t(__float_as_uint(tf)) << 32) | prim;
+        if (key < best) best = key;
+        if (ANY) break;
+      }
+    }
+    if (best != ~0ull) {
+      r.tfar = __uint_as_float(uint32_t(best >> 32));
+      r.hitp = uint32_t(best);
+      r.hit = true;
+      leafHit = true;
+    }
+  } else {
   const uint32_t lane = __lane_id();
   const uint32_t pre =  // this lane's first pool index
       total ? __hip_atomic_fetch_add(pool.ctr, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0u;
   wave_lds_fence();
   const uint32_t sum = *pool.ctr;  // wave-uniform
-  bool leafHit = false;
   if (sum > 0) {
     // Post the entries that fit the pool; a lane whose entries run past it
     // (the wave has more than kPoolSize, which is rare) tests those itself.
@@ -454,6 +475,7 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
     }
     if (rank == 0) *pool.ctr = 0u;  // ready for the next step
     wave_lds_fence();
+  }
   }
   if (ANY && leafHit) return true;
   if (d0 != kInf && d0 <= r.tfar) {
@@ -572,7 +594,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
     }
     for (int k = 0; k < 256; k++) {
       if (active) {
-        if (trace_step<ANY>(s, r, st, pool)) {
+        if (trace_step<ANY, true>(s, r, st, pool)) {
           if (!ANY) trace_finish(s, r);
           job.finish(idx, tag, r);
           active = false;
@@ -596,9 +618,7 @@ template <bool ANY>
 __device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin, float tmax, Hit& h) {
   __shared__ uint32_t lds_node1[kLdsStack * kTraceBlock];
   __shared__ float lds_dist1[kLdsStack * kTraceBlock];
-  __shared__ uint32_t pool_ctr1[kTraceBlock / 64], pool_prim1[kTraceBlock / 64][kPoolSize],
-      pool_owner1[kTraceBlock / 64][kPoolSize];
-  __shared__ unsigned long long pool_key1[kTraceBlock];
+
   TraceRay r;
   TraceOverflow ov;
   TraceStack st;
@@ -606,14 +626,9 @@ __device__ __forceinline__ bool trace(const DevScene& s, f3 o, f3 d, float tmin,
   st.dist = (lds_f32*)(lds_dist1 + threadIdx.x);
   st.ov_node = ov.node;
   st.ov_dist = ov.dist;
-  const uint32_t wv = threadIdx.x >> 6;
-  const PrimPool pool{(lds_u32*)&pool_ctr1[wv], (lds_u32*)pool_prim1[wv], (lds_u32*)pool_owner1[wv],
-                      (__attribute__((address_space(3))) unsigned long long*)&pool_key1[wv * 64]};
-  // the counter is zero between node steps; clear it once for this launch
-  if (__lane_id() == uint32_t(__ffsll((long long)__ballot(true)) - 1)) *pool.ctr = 0u;
-  wave_lds_fence();
   trace_begin(r, st, o, d, tmin, tmax);
-  while (!trace_step<ANY>(s, r, st, pool)) {
+  // lanes loop independently here, so primitives are tested per lane
+  while (!trace_step<ANY, false>(s, r, st, PrimPool{})) {
   }
   if (!ANY) trace_finish(s, r);
   h = r.h;

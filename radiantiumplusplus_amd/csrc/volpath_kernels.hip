@@ -35,10 +35,15 @@ __device__ __forceinline__ uint32_t shadow_key(uint32_t key, uint32_t dim) { ret
 // (emission with MIS, pass-through medium boundaries that are not counted,
 // BSDF NEE and sampling).  NEE contributions are finished by k_vol_shadow.
 // TX: texture code compiled in (the scene has textured emitters or BSDFs).
-template <bool TX>
+// MAT >= 0: the kernel shades the paths k_classify put in material queue MAT
+// (the BSDF of the surface the ray hit; medium events need no BSDF), with
+// that BSDF's code inlined; MAT < 0 shades the whole extend queue with every
+// BSDF type.  The generic kernel needs the register file of the largest BSDF
+// (one wave per SIMD); the specialised ones run several.
+template <bool TX, int MAT = -1>
 __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
-  const uint32_t n = w.cnt[in];
+  const uint32_t n = MAT >= 0 ? w.cnt[CNT_MAT + MAT] : w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
   const int out = 1 - in;
@@ -48,7 +53,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     if (i < n) {
-      s = w.queue[in][i];
+      s = MAT >= 0 ? w.mq[MAT][i] : w.queue[in][i];
       float4 ro4 = w.ray_o[s], rd4 = w.ray_d[s];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float tmin = ro4.w, tmax = rd4.w;
@@ -202,7 +207,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
                 if (ls.pdf > 0.f) {
                   f3 wo = si.sh.to_local(ls.dir);
                   float bp;
-                  f3 f = bsdf_eval_w(all, b, quirks, si.wi, wo, bp);
+                  f3 f = bsdf_eval_t<MAT>(all, b, quirks, si.wi, wo, bp);
                   float wgt = ls.delta ? 1.f : mis_weight(ls.pdf, bp);
                   f3 c = mul(mul(T, f), ls.li) * wgt / ls.pdf;
                   if (!is_zero(c)) {
@@ -217,7 +222,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
               float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
               dim += 3;
               BsdfSample bs;
-              f3 f = bsdf_sample_w(all, b, quirks, si.wi, ul, u1, u2, bs);
+              f3 f = bsdf_sample_t<MAT>(all, b, quirks, si.wi, ul, u1, u2, bs);
               if (bs.pdf > 0.f) {
                 T = mul(T, f) / bs.pdf;
                 eta *= bs.eta;
@@ -428,6 +433,26 @@ void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const Lau
     hipLaunchKernelGGL(k_shade_vol<true>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
   else
     hipLaunchKernelGGL(k_shade_vol<false>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+}
+
+template <int MAT>
+static void LaunchVolMat(bool tx, const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
+  if (tx)
+    hipLaunchKernelGGL((k_shade_vol<true, MAT>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+  else
+    hipLaunchKernelGGL((k_shade_vol<false, MAT>), dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+}
+
+// After k_classify: one kernel per material queue present (the diffuse queue
+// also holds misses and medium boundaries, so it always runs).
+void LaunchShadeVolMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
+                             hipStream_t s) {
+  const bool tl = scene.tex_lights != 0, tx = tl || scene.tex_bsdfs;
+  LaunchVolMat<BSDF_DIFFUSE>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_ROUGH_METAL)) LaunchVolMat<BSDF_ROUGH_METAL>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_ROUGH_GLASS)) LaunchVolMat<BSDF_ROUGH_GLASS>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_DISNEY)) LaunchVolMat<BSDF_DISNEY>(tl, w, scene, in, c, s);
+  if (mask & (1u << kMatOther)) LaunchVolMat<kMatOther>(tx, w, scene, in, c, s);
 }
 void LaunchVolShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_vol_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);

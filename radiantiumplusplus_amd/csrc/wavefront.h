@@ -237,6 +237,8 @@ void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, hip
 void LaunchAccum(const Wavefront& w, hipStream_t s);
 // vol_path (defined in volpath_kernels.hip).
 void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
+void LaunchShadeVolMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
+                             hipStream_t s);
 void LaunchVolShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
 void LaunchFinal(const Wavefront& w, float* fb, float* fb_sumsq, hipStream_t s);
 
