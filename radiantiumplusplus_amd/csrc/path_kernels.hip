@@ -106,8 +106,8 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
       f3 o, d;
       float tmin, tmax;
       camera_ray(scene.cam, float(px) + jx, float(py) + jy, o, d, tmin, tmax);
-      w.ray_o[s] = make_float4(o.x, o.y, o.z, tmin);
-      w.ray_d[s] = make_float4(d.x, d.y, d.z, tmax);
+      w.ray[2 * size_t(s)] = make_float4(o.x, o.y, o.z, tmin);
+      w.ray[2 * size_t(s) + 1] = make_float4(d.x, d.y, d.z, tmax);
       w.thr[s] = make_float4(1.f, 1.f, 1.f, 1.f);
       w.lsum[s] = make_float4(0.f, 0.f, 0.f, 1.f);
       w.prev[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
@@ -145,7 +145,7 @@ __global__ void k_reset(uint32_t* cnt, uint32_t* gcnt, int out) {
 struct ExtendJob {
   static constexpr bool kSuspend = true;
   const uint32_t* queue;
-  const float4 *ray_o, *ray_d;
+  const float4* ray;
   float4* hit;
   uint32_t* hit_geom;
   uint32_t* susp;                 // null: traversals run to the end
@@ -169,7 +169,7 @@ struct ExtendJob {
   }
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
     uint32_t s = queue[i];
-    float4 a = ray_o[s], b = ray_d[s];
+    float4 a = ray[2 * size_t(s)], b = ray[2 * size_t(s) + 1];
     o = mk(a.x, a.y, a.z);
     d = mk(b.x, b.y, b.z);
     tmin = a.w;
@@ -188,7 +188,7 @@ struct ExtendJob {
 // the pooled primitive tests need registers; per 1080p large-mesh frame
 // 78.2 ms at 6 waves, 84.8 ms at 5.
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))) k_extend(Wavefront w, DevScene scene, int in) {
-  ExtendJob job{w.queue[in], w.ray_o, w.ray_d, w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
+  ExtendJob job{w.queue[in], w.ray, w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
                 &w.cnt[CNT_SUSP + 1 - in], w.resume_cap, w.susp ? w.visit_budget : 0xffffffffu};
   trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
 }
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     const bool held = MAT < 0 && i < n && w.susp && w.susp[s] != kNoSusp;
     if (held) alive = true;
     if (i < n && !held) {
-      float4 ro4 = w.ray_o[s], rd4 = w.ray_d[s];
+      float4 ro4 = w.ray[2 * size_t(s)], rd4 = w.ray[2 * size_t(s) + 1];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float4 hh = w.hit[s];
       uint32_t geom = w.hit_geom[s];
@@ -361,8 +361,8 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             T = mul(T, f) / pdf;
             eta *= bs.eta;
             f3 no = offset_p(si.p, si.n, wo);
-            w.ray_o[s] = make_float4(no.x, no.y, no.z, 0.f);
-            w.ray_d[s] = make_float4(wo.x, wo.y, wo.z, kFloatMax);
+            w.ray[2 * size_t(s)] = make_float4(no.x, no.y, no.z, 0.f);
+            w.ray[2 * size_t(s) + 1] = make_float4(wo.x, wo.y, wo.z, kFloatMax);
             prevPdf = pdf;
             prevP = si.p;
             spec = (bs.lobes & LOBE_DELTA) != 0;
@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
     uint32_t s = 0;
     if (i < n) {
       s = w.gq[i];
-      float4 ro = w.ray_o[s], rd = w.ray_d[s], hh = w.hit[s];
+      float4 ro = w.ray[2 * size_t(s)], rd = w.ray[2 * size_t(s) + 1], hh = w.hit[s];
       f3 d = mk(rd.x, rd.y, rd.z);
       f3 p = d * hh.x + mk(ro.x, ro.y, ro.z);
       f3 gn = guide::geometric_normal(scene, p, __float_as_uint(hh.w), w.hit_geom[s]);

@@ -442,8 +442,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   };
   Wavefront& w = job->w;
   const size_t f4 = size_t(Q) * 16, u4 = size_t(Q) * 4;
-  w.ray_o = (float4*)take(f4);
-  w.ray_d = (float4*)take(f4);
+  w.ray = (float4*)take(2 * f4);
   w.thr = (float4*)take(f4);
   w.lsum = (float4*)take(f4);
   w.prev = (float4*)take(f4);

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     if (i < n) {
       s = MAT >= 0 ? w.mq[MAT][i] : w.queue[in][i];
-      float4 ro4 = w.ray_o[s], rd4 = w.ray_d[s];
+      float4 ro4 = w.ray[2 * size_t(s)], rd4 = w.ray[2 * size_t(s) + 1];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float tmin = ro4.w, tmax = rd4.w;
       float4 hh = w.hit[s];
@@ -242,8 +242,8 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
         }
       }
       if (alive) {
-        w.ray_o[s] = make_float4(ro.x, ro.y, ro.z, tmin);
-        w.ray_d[s] = make_float4(rd.x, rd.y, rd.z, tmax);
+        w.ray[2 * size_t(s)] = make_float4(ro.x, ro.y, ro.z, tmin);
+        w.ray[2 * size_t(s) + 1] = make_float4(rd.x, rd.y, rd.z, tmax);
         w.thr[s] = make_float4(T.x, T.y, T.z, eta);
         w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
         w.meta[s] = make_uint4(m.x, m.y, next_depth, dim);

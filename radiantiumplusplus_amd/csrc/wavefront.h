@@ -50,8 +50,9 @@ constexpr uint32_t kNoSusp = 0xffffffffu;
 constexpr int kMatQueues = 5;
 
 struct Wavefront {
-  float4* ray_o;   // origin, tmin
-  float4* ray_d;   // direction, tmax
+  // ray of slot s: ray[2 s] = origin, tmin; ray[2 s + 1] = direction, tmax
+  // (one 32-byte record, so a ray is one memory sector, not two)
+  float4* ray;
   float4* thr;     // throughput, eta
   float4* lsum;    // radiance of the sample in flight, previous BSDF pdf
   float4* prev;    // previous vertex position, path flags (bits)
