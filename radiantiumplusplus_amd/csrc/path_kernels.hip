@@ -622,7 +622,7 @@ __global__ void k_queue_verify(Wavefront w, uint32_t* tally) {
 struct ShadowJob {
   static constexpr bool kSuspend = false;
   const float4 *sh_o, *sh_d, *sh_c;
-  float4* lsum;
+  SlotField<float4> lsum;
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
     float4 a = sh_o[i], b = sh_d[i];
     o = mk(a.x, a.y, a.z);

@@ -443,10 +443,11 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   Wavefront& w = job->w;
   const size_t f4 = size_t(Q) * 16, u4 = size_t(Q) * 4;
   w.ray = (float4*)take(2 * f4);
-  w.thr = (float4*)take(f4);
-  w.lsum = (float4*)take(f4);
-  w.prev = (float4*)take(f4);
-  w.meta = (uint4*)take(f4);
+  float4* rec = (float4*)take(4 * f4);
+  w.thr.p = rec;
+  w.lsum.p = rec + 1;
+  w.prev.p = rec + 2;
+  w.meta.p = reinterpret_cast<uint4*>(rec + 3);
   w.hit = (float4*)take(f4);
   w.sh_o = (float4*)take(f4);
   w.sh_d = (float4*)take(f4);

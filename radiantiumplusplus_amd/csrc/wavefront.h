@@ -49,14 +49,23 @@ constexpr int kMatOther = 4;
 constexpr uint32_t kNoSusp = 0xffffffffu;
 constexpr int kMatQueues = 5;
 
+// Field k of a 64-byte per-slot record of four 16-byte fields: element s is
+// p[4 s], so the four fields of a slot share one memory sector.
+template <class T>
+struct SlotField {
+  T* p;
+  __host__ __device__ T& operator[](size_t s) const { return p[4 * s]; }
+};
+
 struct Wavefront {
   // ray of slot s: ray[2 s] = origin, tmin; ray[2 s + 1] = direction, tmax
   // (one 32-byte record, so a ray is one memory sector, not two)
   float4* ray;
-  float4* thr;     // throughput, eta
-  float4* lsum;    // radiance of the sample in flight, previous BSDF pdf
-  float4* prev;    // previous vertex position, path flags (bits)
-  uint4* meta;     // pixel, sample index (SAMPLE_NONE = idle), depth, rng dimension
+  // the path record of slot s: four fields in one 64-byte record
+  SlotField<float4> thr;   // throughput, eta
+  SlotField<float4> lsum;  // radiance of the sample in flight, previous BSDF pdf
+  SlotField<float4> prev;  // previous vertex position, path flags (bits)
+  SlotField<uint4> meta;   // pixel, sample index (SAMPLE_NONE = idle), depth, rng dimension
   uint32_t* key;   // rng key of the sample in flight
   uint32_t* medium;  // vol_path: MED_UNSET, or current medium + 1 (0 = none) | channel << 24
   float4* hit;     // t, u, v, prim (bits)
