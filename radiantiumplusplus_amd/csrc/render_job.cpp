@@ -149,7 +149,10 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
   j.resume_cap = desc.integrator == RAD_INTEGRATOR_PATH && VisitBudget() > 0 ? uint32_t(std::min<uint64_t>(Q, kResumeCap)) : 0;
-  j.bytes = Align(Q * 16) * 10 + Align(Q * 4) * (6 + kMatQueues) + Align(pass * 16) + Align(uint64_t(j.npix) * 16) * 2 +
+  // per slot: ray (32 B), path record (64 B), hit, shadow origin / direction
+  // / contribution (16 B each); key, medium, hit geometry and queues (4 B)
+  j.bytes = Align(Q * 32) + Align(Q * 64) + Align(Q * 16) * 4 + Align(Q * 4) * (6 + kMatQueues) +
+            Align(pass * 16) + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
   if (j.resume_cap) j.bytes += Align(Q * 4) + 2 * Align(uint64_t(j.resume_cap) * sizeof(ResumeState));
   if (desc.flags & RAD_RENDER_CHECK_QUEUES) j.bytes += Align(Q * 4);
@@ -435,9 +438,11 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
     job->arena->alloc(js.bytes);
   }
   char* p = static_cast<char*>(job->arena->ptr);
+  char* const end = p + job->arena->bytes;
   auto take = [&](size_t n) {
     char* r = p;
     p += Align(n);
+    if (p > end) throw StatusError(RAD_ERR_HIP, "render arena layout exceeds its size");
     return r;
   };
   Wavefront& w = job->w;
