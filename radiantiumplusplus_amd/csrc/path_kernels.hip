@@ -65,67 +65,89 @@ __device__ __forceinline__ void finish_records(const Wavefront& w, uint32_t s, f
 }
 
 // Store the sample a dead slot finished into the result entries and start the
-// next camera sample of the pass on it (sample_renderer.cpp:97-99).
+// next camera sample of the pass on it (sample_renderer.cpp:97-99).  Each
+// thread takes kRegenPerThread dead slots per block iteration (entries
+// base + e * 256 + thread), so one set of block atomics covers them all.
+constexpr int kRegenPerThread = 4;
+
 __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int in) {
+  constexpr int E = kRegenPerThread;
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[CNT_DEAD];
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t stride = gridDim.x * blockDim.x * E;
   const uint32_t pass = w.rows * w.npix;
   const bool more = w.cnt[CNT_NEXT] < pass;  // stale is fine: only skips atomics once exhausted
   const uint32_t W = uint32_t(scene.cam.width);
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    uint32_t i = base + threadIdx.x;
-    bool valid = i < n;
-    bool finished = false;
-    uint32_t s = 0;
-    if (valid) {
-      s = w.dead[i];
-      uint4 m = w.meta[s];
-      if (m.y != SAMPLE_NONE) {
-        float4 l = w.lsum[s];
-        // one 16-byte entry per sample: a single line written per finished sample
-        w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(l.x, l.y, l.z, 0.f);
-        finished = true;
-        if (w.flags & WF_RECORD) finish_records(w, s, mk(l.x, l.y, l.z));
+  for (uint32_t base = blockIdx.x * blockDim.x * E; base < n; base += stride) {
+    uint32_t s[E];
+    int cls[E];
+    bool none[E] = {};
+    uint32_t finished = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const uint32_t i = base + e * blockDim.x + threadIdx.x;
+      s[e] = 0;
+      cls[e] = -1;
+      if (i < n) {
+        s[e] = w.dead[i];
+        cls[e] = 0;
+        const uint4 m = w.meta[s[e]];
+        if (m.y != SAMPLE_NONE) {
+          const float4 l = w.lsum[s[e]];
+          // one 16-byte entry per sample: a single line written per finished sample
+          w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(l.x, l.y, l.z, 0.f);
+          finished++;
+          if (w.flags & WF_RECORD) finish_records(w, s[e], mk(l.x, l.y, l.z));
+        }
       }
     }
-    uint32_t id = pass;
+    uint32_t id[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) id[e] = pass;
     if (more) {
       uint32_t* c1[1] = {&w.cnt[CNT_NEXT]};
-      bool p1[1] = {valid};
-      uint32_t r1[1];
-      block_append<1>(c1, p1, r1, lds);
-      id = r1[0];
+      uint32_t pos[E], unused[E];
+      block_append_n<1, E, false>(c1, cls, none, pos, unused, lds);
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (cls[e] >= 0) id[e] = pos[e];
     }
-    bool start = valid && id < pass;
-    if (start) {
-      uint32_t pixel = id % w.npix, sample = w.row0 + id / w.npix;
-      uint32_t key = sample_key(w.seed, w.request, pixel, sample);
-      uint32_t px = pixel % W, py = pixel / W;
-      float jx = rng_u(key, 0), jy = rng_u(key, 1);
-      f3 o, d;
-      float tmin, tmax;
-      camera_ray(scene.cam, float(px) + jx, float(py) + jy, o, d, tmin, tmax);
-      w.ray[2 * size_t(s)] = make_float4(o.x, o.y, o.z, tmin);
-      w.ray[2 * size_t(s) + 1] = make_float4(d.x, d.y, d.z, tmax);
-      w.thr[s] = make_float4(1.f, 1.f, 1.f, 1.f);
-      w.lsum[s] = make_float4(0.f, 0.f, 0.f, 1.f);
-      w.prev[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
-      w.meta[s] = make_uint4(pixel, sample, 0u, 2u);
-      w.key[s] = key;
-      if (w.flags & WF_RECORD) w.glast[s] = w.gpend[s] = 0xffffffffu;
-      if (w.medium) w.medium[s] = 0xffffffffu;  // vol_path: medium / channel chosen at shade time
-    } else if (valid) {
-      w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);  // idle for the rest of the pass
+    int qcls[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const bool start = cls[e] >= 0 && id[e] < pass;
+      qcls[e] = start ? 0 : -1;
+      if (start) {
+        const uint32_t sl = s[e];
+        const uint32_t pixel = id[e] % w.npix, sample = w.row0 + id[e] / w.npix;
+        const uint32_t key = sample_key(w.seed, w.request, pixel, sample);
+        const uint32_t px = pixel % W, py = pixel / W;
+        const float jx = rng_u(key, 0), jy = rng_u(key, 1);
+        f3 o, d;
+        float tmin, tmax;
+        camera_ray(scene.cam, float(px) + jx, float(py) + jy, o, d, tmin, tmax);
+        w.ray[2 * size_t(sl)] = make_float4(o.x, o.y, o.z, tmin);
+        w.ray[2 * size_t(sl) + 1] = make_float4(d.x, d.y, d.z, tmax);
+        w.thr[sl] = make_float4(1.f, 1.f, 1.f, 1.f);
+        w.lsum[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
+        w.prev[sl] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
+        w.meta[sl] = make_uint4(pixel, sample, 0u, 2u);
+        w.key[sl] = key;
+        if (w.flags & WF_RECORD) w.glast[sl] = w.gpend[sl] = 0xffffffffu;
+        if (w.medium) w.medium[sl] = 0xffffffffu;  // vol_path: medium / channel chosen at shade time
+      } else if (cls[e] >= 0) {
+        w.meta[s[e]] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);  // idle for the rest of the pass
+      }
     }
     {
       uint32_t* c1[1] = {&w.cnt[in]};
-      bool p1[1] = {start};
-      uint32_t r1[1];
-      block_append<1>(c1, p1, r1, lds);
-      if (start) w.queue[in][r1[0]] = s;
+      uint32_t pos[E], unused[E];
+      block_append_n<1, E, false>(c1, qcls, none, pos, unused, lds);
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (qcls[e] >= 0) w.queue[in][pos[e]] = s[e];
     }
-    block_count(&w.stats[ST_SAMPLES], finished ? 1u : 0u, lds);
+    block_count(&w.stats[ST_SAMPLES], finished, lds);
   }
 }
 
