@@ -123,14 +123,3 @@ def test_primitives_beyond_the_wave_pool(hip_ctx, monkeypatch, pool):
     img_small = small.render(d)[0]
     assert np.array_equal(img_full, img_small)
 
-
-def test_vol_path_image_independent_of_the_wave_pool(hip_ctx, monkeypatch):
-    """The transmittance rays of vol_path trace one ray per lane in a
-    data-dependent loop; their pooled tests must give the same image with a
-    one-entry pool."""
-    b = scenes.volume_box(width=24, height=24, res=16, spp=8)
-    full = hip.Scene.from_builder(hip_ctx, b)
-    monkeypatch.setenv("RAD_TRACE_POOL", "1")
-    small = hip.Scene.from_builder(hip_ctx, b)
-    d = b.render_desc(spp=8, seed=5)
-    assert np.array_equal(full.render(d)[0], small.render(d)[0])
