@@ -145,10 +145,11 @@ def summarize(dirs, copy_mib=1024):
 
 def summarize_counters(dirs):
     """Per-kernel sums of the counters in rocprofv3 --pmc output directories,
-    with the derived rates the traversal analysis uses: VALU lane utilisation
-    (active lanes per VALU instruction / 64), the fractions of wave cycles spent
-    waiting and issuing, L2 hit rate, the mean L1 -> L2 read latency and the
-    F16 MFMA work."""
+    with the derived rates the traversal analysis uses: VALU busy (SIMD cycles
+    spent issuing VALU instructions over the SIMD cycles of the kernel), VALU
+    lane utilisation (active lanes per VALU instruction / 64), the fractions
+    of wave cycles spent waiting and issuing, L2 hit rate, the mean L1 -> L2
+    read latency and the F16 MFMA work."""
     acc, launches = {}, {}
     for d in dirs:
         for r in _csv_rows(d, "counter_collection.csv"):
@@ -156,6 +157,11 @@ def summarize_counters(dirs):
             c = acc.setdefault(k, {})
             c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
             launches.setdefault((d, k), set()).add(r["Dispatch_Id"])
+    simds = xcc = None
+    for d in dirs:
+        for r in _csv_rows(d, "agent_info.csv"):
+            if r.get("Name", "").startswith("gfx"):
+                simds, xcc = int(r["Simd_Count"]), int(r["Num_Xcc"])
     out = []
     for k in sorted(acc):
         c = acc[k]
@@ -166,6 +172,10 @@ def summarize_counters(dirs):
         dv = []
         if c.get("SQ_ACTIVE_INST_VALU") and "SQ_THREAD_CYCLES_VALU" in c:
             dv.append(f"VALU lane utilisation {c['SQ_THREAD_CYCLES_VALU'] / (64 * c['SQ_ACTIVE_INST_VALU']):.2f}")
+        if simds and c.get("GRBM_GUI_ACTIVE") and "SQ_ACTIVE_INST_VALU" in c:
+            # a wave64 VALU instruction holds its 16-lane SIMD for 4 cycles;
+            # GRBM_GUI_ACTIVE counts the kernel's cycles once per XCC
+            dv.append(f"VALU busy {4 * c['SQ_ACTIVE_INST_VALU'] / (simds * c['GRBM_GUI_ACTIVE'] / xcc):.2f} of SIMD cycles")
         if c.get("SQ_WAVE_CYCLES"):
             for name, lab in (("SQ_WAIT_ANY", "waiting"), ("SQ_ACTIVE_INST_ANY", "issuing")):
                 if name in c:
@@ -221,16 +231,23 @@ def summarize_overlap(trace):
             f"mean streams in flight while busy: {weighted / max(busy, 1):.2f}")
 
 
-def traffic_per_ray():
-    """The closest-hit kernel's measured bytes per ray from the committed
-    counter summary (None when absent)."""
+def _profile_tag(tag):
     try:
         for ln in open(PROFILE_SUMMARY):
-            if ln.startswith(TRAFFIC_TAG):
-                return float(ln[len(TRAFFIC_TAG):])
+            if ln.startswith(tag):
+                return float(ln[len(tag):])
     except OSError:
         pass
     return None
+
+
+def traffic_per_ray():
+    """The closest-hit kernel's measured bytes per ray from the committed
+    counter summary (None when absent)."""
+    return _profile_tag(TRAFFIC_TAG)
+
+
+VALU_TAG = "VALU k_extend busy="
 
 
 def copy_probe(mib):
@@ -584,6 +601,10 @@ def main():
                             "--pmc runs in profiles/k_extend_large_mesh.txt (scaled by a plain copy), times this "
                             "run's rays per launch",
             "avg_launch_ms": ext_ms / max(ext_launch, 1),
+            "valu_busy": _profile_tag(VALU_TAG),
+            "valu_note": "fraction of SIMD cycles issuing VALU instructions in k_extend (SQ_ACTIVE_INST_VALU, "
+                         "GRBM_GUI_ACTIVE in profiles/k_extend_large_mesh.txt): the kernel is bound by instruction "
+                         "issue, not by HBM bandwidth",
             "launch": {"path_slots": stats[0].slots, "blocks_per_cu": stats[0].blocks_per_cu,
                        "trace_blocks_per_cu": stats[0].trace_blocks_per_cu},
             "rays_per_s": segs / (ext_ms * 1e-3) if ext_ms > 0 else 0.0,

@@ -19,6 +19,10 @@ __global__ void k_init_pass(Wavefront w, int first) {
     w.pacc2[s] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   if (s < w.Q) {
+    // the whole 64-byte path record: a partial write of one field is slower
+    w.thr[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    w.lsum[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    w.prev[s] = make_float4(0.f, 0.f, 0.f, 0.f);
     w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);
     w.dead[s] = s;
     if (w.glast) w.glast[s] = w.gpend[s] = 0xffffffffu;
