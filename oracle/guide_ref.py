@@ -11,8 +11,11 @@ states the mathematics the device kernels implement:
   scalar s in bin i of k: max(0, 1 - |s k - (i + 1/2)|);
 * 5 dense layers 80-64-64-64-64-48, ReLU between, parameters flat per layer
   as W[out][in] then b[out];
-* outputs: 8 lobes x (logit, mean xyz, log kappa) -> softmax weights,
-  normalised means, kappa = exp(clamp(., -3, 6));
+* outputs: 8 lobes x (logit, target xyz, log kappa) -> softmax weights,
+  means pointing from the vertex x to the lobe's target point c + h t
+  (c the bounds' centre, h their half extent, t the three outputs; a lobe
+  probe without a vertex takes the outputs as the mean direction itself),
+  kappa = exp(clamp(., -3, 6));
 * vMF density kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)};
 * loss (1/n) sum_j min(w_j * wnorm, 64) * -log q(dir_j | x_j).
 
@@ -78,12 +81,21 @@ def forward(params, feats, lo, hi):
     return A, Z
 
 
-def mixture(raw):
+def lobe_frame(lo, hi):
+    """Centre and half extent of the lobe-target frame."""
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    return 0.5 * (lo + hi), np.maximum(0.5 * (hi - lo), 1e-6)
+
+
+def mixture(raw, pos=None, lo=None, hi=None):
     raw = np.asarray(raw, np.float64).reshape(-1, K_OUT)[:, :5 * LOBES].reshape(-1, LOBES, 5)
     a = raw[:, :, 0]
     pi = np.exp(a - a.max(axis=1, keepdims=True))
     pi /= pi.sum(axis=1, keepdims=True)
     m = raw[:, :, 1:4]
+    if pos is not None:
+        c, h = lobe_frame(lo, hi)
+        m = c + h * m - np.asarray(pos, np.float64).reshape(-1, 1, 3)
     ln = np.linalg.norm(m, axis=2, keepdims=True)
     mu = np.where(ln > 1e-6, m / np.where(ln > 1e-6, ln, 1.0), np.array([0.0, 1.0, 0.0]))
     kr = raw[:, :, 4]
@@ -112,7 +124,8 @@ def loss_and_grad(params, feats, dirs, w, wnorm, lo, hi):
     dirs = np.asarray(dirs, np.float64).reshape(-1, 3)
     wn = np.minimum(np.asarray(w, np.float64) * wnorm, MAX_WEIGHT)
     wn = np.where(np.asarray(w) > 0, wn, 0.0)
-    pi, mu, kappa, ln, kr = mixture(raw)
+    pos = np.asarray(feats, np.float64).reshape(-1, 9)[:, 0:3]
+    pi, mu, kappa, ln, kr = mixture(raw, pos, lo, hi)
     lv, c = log_vmf(mu, kappa, dirs)
     t = np.log(pi) + lv
     tm = t.max(axis=1, keepdims=True)
@@ -126,7 +139,7 @@ def loss_and_grad(params, feats, dirs, w, wnorm, lo, hi):
     inside = (kr > -3.0) & (kr < 6.0)
     d[:, :, 4] = np.where(inside, -wn[:, None] * r * dk * kappa, 0.0)
     gm = (dirs[:, None, :] - mu * c[:, :, None]) * (kappa / np.where(ln > 1e-6, ln, 1.0))[:, :, None]
-    gm = np.where((ln > 1e-6)[:, :, None], gm, 0.0)
+    gm = np.where((ln > 1e-6)[:, :, None], gm, 0.0) * lobe_frame(lo, hi)[1]
     d[:, :, 1:4] = -wn[:, None, None] * r[:, :, None] * gm
     dz = np.zeros((n, K_OUT))
     dz[:, :5 * LOBES] = d.reshape(n, -1)

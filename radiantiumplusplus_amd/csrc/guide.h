@@ -1,6 +1,11 @@
 // Learned path guiding: a 5-layer MLP (80 -> 64 -> 64 -> 64 -> 64 -> 48)
 // maps a surface vertex (position, normal, outgoing direction; one-blob
 // encoded) to an 8-lobe von Mises-Fisher mixture over world directions.
+// Each lobe's mean direction points from the vertex to a target point the
+// network outputs (parallax-aware lobes, as in Ruppert, Herholz and Lensch,
+// "Robust fitting of parallax-aware mixtures for path guiding", 2020): light
+// that arrives from a region of the scene, such as a lit spot, is one
+// target for every vertex instead of a direction that turns with position.
 // Inference and training run as FP16 MFMA kernels with FP32 accumulation;
 // FP32 master weights are updated with Adam.  The mixture is combined with
 // BSDF sampling by one-sample MIS, so guided renders stay unbiased.
@@ -61,6 +66,10 @@ struct GuideNet {
   const _Float16* w;  // kWeights halves, output-major per layer
   const float* b;     // kBiases
   float lo[3], inv_ext[3];  // scene bounds for the position encoding
+  // Lobe targets: a lobe's three mean outputs t place a point c + h t
+  // (c the bounds' centre, h their half extent per axis); the lobe points
+  // from the vertex to it.
+  float center[3], half[3];
   float alpha;        // probability of sampling the guide (vs the BSDF)
   float min_rough;    // glossy BSDFs with a smaller GGX/Beckmann alpha are not guided
   uint32_t max_depth; // vertices deeper than this many bounces are not guided

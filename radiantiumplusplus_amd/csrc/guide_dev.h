@@ -59,9 +59,17 @@ struct Mixture {
   float kappa[kLobes];
 };
 
-// raw: 40 network outputs (logit, mean xyz, log kappa per lobe).
+// Vector from vertex p to a lobe's target c + h * (its three mean outputs).
 template <class T>
-__device__ __forceinline__ Mixture decode(const T* raw) {
+__device__ __forceinline__ f3 lobe_vector(const T* lobe, f3 p, f3 cen, f3 half) {
+  return mk(fmaf(half.x, float(lobe[1]), cen.x) - p.x, fmaf(half.y, float(lobe[2]), cen.y) - p.y,
+            fmaf(half.z, float(lobe[3]), cen.z) - p.z);
+}
+
+// raw: 40 network outputs (logit, target xyz, log kappa per lobe) of the
+// vertex at p; cen, half: the lobe-target frame (GuideNet::center, half).
+template <class T>
+__device__ __forceinline__ Mixture decode(const T* raw, f3 p, f3 cen, f3 half) {
   Mixture m;
   float mx = -kInf;
 #pragma unroll
@@ -75,7 +83,7 @@ __device__ __forceinline__ Mixture decode(const T* raw) {
 #pragma unroll
   for (int l = 0; l < kLobes; l++) {
     m.pi[l] /= sum;
-    f3 v = mk(float(raw[kRawPerLobe * l + 1]), float(raw[kRawPerLobe * l + 2]), float(raw[kRawPerLobe * l + 3]));
+    f3 v = lobe_vector(raw + kRawPerLobe * l, p, cen, half);
     float len = sqrtf(sqnorm(v));
     m.mu[l] = len > 1e-6f ? v / len : mk(0.f, 1.f, 0.f);
     m.kappa[l] = expf(fminf(fmaxf(float(raw[kRawPerLobe * l + 4]), -3.f), 6.f));

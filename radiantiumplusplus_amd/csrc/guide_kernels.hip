@@ -59,7 +59,8 @@ __global__ void __launch_bounds__(256) k_guide_infer(GuideNet net, const float* 
 // gradient), so large sample weights stay inside the FP16 range.
 constexpr float kDzScale = 1.f / 32.f;
 
-__device__ inline float record_loss_grad(const float* raw, f3 dir, float wn, float* d) {
+// Lobe l points from the vertex p to the target c + h * (raw means).
+__device__ inline float record_loss_grad(const float* raw, f3 dir, float wn, f3 p, f3 cen, f3 half, float* d) {
   float a[kLobes], logv[kLobes], kap[kLobes], c[kLobes], len[kLobes];
   f3 mu[kLobes];
   bool kin[kLobes];
@@ -72,7 +73,7 @@ __device__ inline float record_loss_grad(const float* raw, f3 dir, float wn, flo
   }
   for (int l = 0; l < kLobes; l++) {
     a[l] /= se;  // pi_l
-    f3 m = mk(raw[kRawPerLobe * l + 1], raw[kRawPerLobe * l + 2], raw[kRawPerLobe * l + 3]);
+    f3 m = lobe_vector(raw + kRawPerLobe * l, p, cen, half);
     len[l] = sqrtf(sqnorm(m));
     mu[l] = len[l] > 1e-6f ? m / len[l] : mk(0.f, 1.f, 0.f);
     float kr = raw[kRawPerLobe * l + 4];
@@ -93,7 +94,8 @@ __device__ inline float record_loss_grad(const float* raw, f3 dir, float wn, flo
     float e = expf(-2.f * k);
     float dlogv_dk = 1.f / k - 2.f * e / (1.f - e) + (c[l] - 1.f);
     d[kRawPerLobe * l + 4] = kin[l] ? -wn * r * dlogv_dk * k : 0.f;
-    f3 gm = len[l] > 1e-6f ? (dir - mu[l] * c[l]) * (k / len[l]) : splat(0.f);
+    // d(log vMF) / d(target output) = (dir - mu c) kappa / |v| times h
+    f3 gm = len[l] > 1e-6f ? mul(dir - mu[l] * c[l], half) * (k / len[l]) : splat(0.f);
     d[kRawPerLobe * l + 1] = -wn * r * gm.x;
     d[kRawPerLobe * l + 2] = -wn * r * gm.y;
     d[kRawPerLobe * l + 3] = -wn * r * gm.z;
@@ -151,7 +153,9 @@ __global__ void __launch_bounds__(256) k_guide_train(GuideNet net, const TrainRe
     float l = 0.f;
     if (valid && rec.w > 0.f) {
       float wn = fminf(rec.w * wnorm, wmax);
-      l = record_loss_grad(sOut + rloc * kOut, mk(rec.dir[0], rec.dir[1], rec.dir[2]), wn, d);
+      l = record_loss_grad(sOut + rloc * kOut, mk(rec.dir[0], rec.dir[1], rec.dir[2]), wn,
+                           mk(rec.p[0], rec.p[1], rec.p[2]), mk(net.center[0], net.center[1], net.center[2]),
+                           mk(net.half[0], net.half[1], net.half[2]), d);
     }
     for (int k = 0; k < kOut; k++) sDz[dzOff(4) + k * 64 + rloc] = (_Float16)(d[k] * kDzScale);
     if (l != 0.f) atomicAdd(loss, l);
@@ -275,13 +279,13 @@ __global__ void k_guide_publish(const float* p, _Float16* wh, float* bias) {
 __global__ void k_guide_pdf(const float* raw, const float* dirs, uint32_t n, float* pdf) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Mixture m = decode(raw);
+  Mixture m = decode(raw, splat(0.f), splat(0.f), splat(1.f));  // the means taken as directions
   pdf[i] = mixture_pdf(m, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
 }
 __global__ void k_guide_sample(const float* raw, const float* xi, uint32_t n, float* out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Mixture m = decode(raw);
+  Mixture m = decode(raw, splat(0.f), splat(0.f), splat(1.f));
   f3 d = mixture_sample(m, xi[3 * i], xi[3 * i + 1], xi[3 * i + 2]);
   out[4 * i] = d.x;
   out[4 * i + 1] = d.y;
@@ -358,6 +362,10 @@ GuideNet GuideState::Net(const float lo[3], const float hi[3], float alpha, floa
     float e = hi[i] - lo[i];
     n.lo[i] = lo[i];
     n.inv_ext[i] = e > 0.f ? 1.f / e : 0.f;
+  }
+  for (int i = 0; i < 3; i++) {
+    n.center[i] = 0.5f * (lo[i] + hi[i]);
+    n.half[i] = std::max(0.5f * (hi[i] - lo[i]), 1e-6f);
   }
   n.alpha = alpha;
   n.min_rough = min_rough;

@@ -316,7 +316,9 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           const BsdfRec& b = *bp0;
           const bool guided = GUIDE && guide::guidable(b, quirks, w.net.min_rough) && depth <= w.net.max_depth;
           guide::Mixture mx;
-          if (guided && guideOn) mx = guide::decode(w.graw + size_t(s) * guide::kOut);
+          if (guided && guideOn)
+            mx = guide::decode(w.graw + size_t(s) * guide::kOut, si.p, mk(w.net.center[0], w.net.center[1], w.net.center[2]),
+                               mk(w.net.half[0], w.net.half[1], w.net.half[2]));
           const float a = (guided && guideOn) ? alpha : 0.f;
           if (b.lobes & LOBE_NO_DELTA) {  // NEE (path.cpp:173-191)
             float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);

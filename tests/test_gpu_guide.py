@@ -95,7 +95,10 @@ def grad_case(p, feats, dirs, w):
 def test_gradient_matches_oracle(hip_ctx):
     """Training batches whose target directions share a common region (the
     situation guiding learns from): the FP16 gradient of every layer within
-    1e-2 of the float64 oracle."""
+    1e-2 of the float64 oracle run on the same FP16-rounded weights, and
+    within 1.5e-2 of the oracle on the FP32 weights (lobe means point from the
+    vertex to a target, so the mean gradients scale with 1 / |target - vertex|
+    and vertices near a target weigh the FP16 rounding of the weights more)."""
     rng = np.random.default_rng(2)
     n = 1024
     feats = random_feats(rng, n)
@@ -104,9 +107,10 @@ def test_gradient_matches_oracle(hip_ctx):
     dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
     w = rng.exponential(1.0, n)
     w[rng.random(n) < 0.2] = 0.0
-    gl, rl, errs, _ = grad_case(random_params(3), feats, dirs, w)
+    gl, rl, errs, errs16 = grad_case(random_params(3), feats, dirs, w)
     assert abs(gl - rl) <= 1e-2 * abs(rl)
-    assert max(errs) < 1e-2, errs
+    assert max(errs16) < 1e-2, errs16
+    assert max(errs) < 1.5e-2, errs
 
 
 def test_gradient_with_cancelling_records(hip_ctx):

@@ -100,3 +100,22 @@ def test_one_blob_partition_of_unity():
         inner = (s * bins >= 0.5) & (s * bins <= bins - 0.5)
         assert np.allclose(tot[inner], 1.0)
         assert (tot <= 1.0 + 1e-12).all()
+
+
+def test_lobe_means_point_at_their_target():
+    """A lobe's mean is the unit vector from the vertex to its target point
+    c + h t (c, h: centre and half extent of the bounds, t: the lobe's three
+    outputs), so one output places a target that every vertex sees at its
+    own direction; without a vertex (probes) the outputs are the direction."""
+    lo, hi = np.array([-3.0, 0.0, -3.0]), np.array([3.0, 3.0, 3.0])
+    target = np.array([2.0, 1.5, -1.0])
+    c, h = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    raw = np.zeros(G.K_OUT)
+    raw[1:4] = (target - c) / h
+    verts = np.array([[0.0, 0.0, 0.0], [-2.5, 2.0, 2.0], [2.0, 0.2, -1.0]])
+    _, mu, _, ln, _ = G.mixture(np.tile(raw, (3, 1)), verts, lo, hi)
+    want = target - verts
+    assert np.allclose(ln[:, 0], np.linalg.norm(want, axis=1))
+    assert np.allclose(mu[:, 0], want / np.linalg.norm(want, axis=1, keepdims=True))
+    _, mu_dir, _, _, _ = G.mixture(raw[None, :])
+    assert np.allclose(mu_dir[0, 0], raw[1:4] / np.linalg.norm(raw[1:4]))
