@@ -57,6 +57,7 @@ struct Mixture {
   float pi[kLobes];
   f3 mu[kLobes];
   float kappa[kLobes];
+  float wnorm[kLobes];  // pi * kappa / (2 pi (1 - e^{-2 kappa})): the lobe's weighted vMF normalisation
 };
 
 // Vector from vertex p to a lobe's target c + h * (its three mean outputs).
@@ -87,6 +88,7 @@ __device__ __forceinline__ Mixture decode(const T* raw, f3 p, f3 cen, f3 half) {
     float len = sqrtf(sqnorm(v));
     m.mu[l] = len > 1e-6f ? v / len : mk(0.f, 1.f, 0.f);
     m.kappa[l] = expf(fminf(fmaxf(float(raw[kRawPerLobe * l + 4]), -3.f), 6.f));
+    m.wnorm[l] = m.pi[l] * (m.kappa[l] / ((2.f * kPi) * (-expm1f(-2.f * m.kappa[l]))));
   }
   return m;
 }
@@ -97,10 +99,12 @@ __device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
   return norm * expf(kappa * (dot(mu, w) - 1.f));
 }
 
+// The normalisations are taken from the decode, so an evaluation costs one
+// exponential per lobe.
 __device__ __forceinline__ float mixture_pdf(const Mixture& m, f3 w) {
   float p = 0.f;
 #pragma unroll
-  for (int l = 0; l < kLobes; l++) p += m.pi[l] * vmf_pdf(m.mu[l], m.kappa[l], w);
+  for (int l = 0; l < kLobes; l++) p += m.wnorm[l] * expf(m.kappa[l] * (dot(m.mu[l], w) - 1.f));
   return p;
 }
 
