@@ -68,6 +68,28 @@ void UploadScene(rad_scene* s) {
   s->d_nrm.upload(h.nrm.data(), h.nrm.size() * sizeof(float), st);
   s->d_uv.upload(h.uv.data(), h.uv.size() * sizeof(float), st);
   s->d_idx.upload(h.idx.data(), h.idx.size() * sizeof(uint32_t), st);
+  {
+    // the per-triangle shading records (DevScene::tris)
+    std::vector<float> tris(h.idx.size() / 3 * 24, 0.f);
+    for (const ShapeRec& sr : h.shapes) {
+      if (sr.type != SHAPE_MESH) continue;
+      for (uint32_t t = 0; t < sr.tri_count; t++) {
+        const size_t k = size_t(sr.tri_offset) + t;
+        float* o = tris.data() + 24 * k;
+        for (int j = 0; j < 3; j++) {
+          const size_t f = size_t(sr.vtx_offset) + h.idx[3 * k + j];
+          for (int a = 0; a < 3; a++) {
+            o[3 * j + a] = h.pos[3 * f + a];
+            o[9 + 3 * j + a] = h.nrm[3 * f + a];
+          }
+          o[18 + 2 * j] = h.uv[2 * f];
+          o[18 + 2 * j + 1] = h.uv[2 * f + 1];
+        }
+      }
+    }
+    s->d_tris.upload(tris.data(), tris.size() * sizeof(float), st);
+    HipCheck(hipStreamSynchronize(st), "triangle records upload");  // the host vector is released next
+  }
   s->d_bsdfs.upload(h.bsdfs.data(), h.bsdfs.size() * sizeof(BsdfRec), st);
   s->d_lights.upload(h.lights.data(), h.lights.size() * sizeof(LightRec), st);
   s->d_cdf.upload(h.light_cdf.data(), h.light_cdf.size() * sizeof(float), st);
@@ -84,6 +106,7 @@ void UploadScene(rad_scene* s) {
   d.nrm = s->d_nrm.as<float>();
   d.uv = s->d_uv.as<float>();
   d.idx = s->d_idx.as<uint32_t>();
+  d.tris = s->d_tris.as<float4>();
   d.bsdfs = s->d_bsdfs.as<BsdfRec>();
   d.lights = s->d_lights.as<LightRec>();
   d.light_count = uint32_t(h.lights.size());

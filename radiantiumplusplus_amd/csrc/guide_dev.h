@@ -153,9 +153,9 @@ __device__ __forceinline__ bool guidable(const BsdfRec& b, bool quirks, float mi
 __device__ inline f3 geometric_normal(const DevScene& s, f3 p, uint32_t prim, uint32_t geom) {
   const ShapeRec& sr = s.shapes[geom];
   if (sr.type == SHAPE_MESH) {
-    const uint32_t* I = s.idx + 3 * size_t(sr.tri_offset + prim);
-    f3 p0 = load3(s.pos, sr.vtx_offset + I[0]), p1 = load3(s.pos, sr.vtx_offset + I[1]),
-       p2 = load3(s.pos, sr.vtx_offset + I[2]);
+    const float4* T = s.tris + 6 * size_t(sr.tri_offset + prim);  // DevScene::tris
+    const float4 t0 = T[0], t1 = T[1], t2 = T[2];
+    f3 p0 = mk(t0.x, t0.y, t0.z), p1 = mk(t0.w, t1.x, t1.y), p2 = mk(t1.z, t1.w, t2.x);
     return normalize(cross(p1 - p0, p2 - p0));
   }
   if (sr.type == SHAPE_SPHERE) return normalize(p - sr.center);

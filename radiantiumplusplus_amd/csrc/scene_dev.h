@@ -204,6 +204,11 @@ struct DevScene {
   const float* nrm;        // world-space (unnormalised) normals, 3 floats per vertex
   const float* uv;         // 2 floats per vertex
   const uint32_t* idx;     // 3 indices per triangle, local to the shape's vertex range
+  // Per triangle (tri_offset + prim), 6 float4: its positions p0 p1 p2, normals
+  // n0 n1 n2 and uvs, packed in that order (24 floats; zeros where the mesh has
+  // none), so shading a hit reads one 96-byte record instead of the indices
+  // and nine scattered vertex attributes.
+  const float4* tris;
   const BsdfRec* bsdfs;
   const LightRec* lights;
   uint32_t light_count;

@@ -752,6 +752,7 @@ CameraRec MakeCameraRec(const rad_camera_desc& c) {
 
 uint64_t rad_scene::device_bytes() const {
   return d_nodes.bytes + d_qnodes.bytes + d_prims.bytes + d_shapes.bytes + d_pos.bytes + d_nrm.bytes + d_uv.bytes + d_idx.bytes +
+         d_tris.bytes +
          d_bsdfs.bytes + d_lights.bytes + d_cdf.bytes + d_media.bytes + d_grid.bytes + d_tex.bytes + d_texels.bytes +
          d_ldist.bytes + (spare_arena ? spare_arena->bytes : 0);
 }

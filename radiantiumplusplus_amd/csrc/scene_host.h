@@ -87,7 +87,7 @@ struct rad_scene {
   rad_ctx* ctx = nullptr;
   rad::HostScene host;
   rad::DevScene dev{};
-  rad::DeviceBuffer d_nodes, d_qnodes, d_prims, d_shapes, d_pos, d_nrm, d_uv, d_idx, d_bsdfs, d_lights, d_cdf, d_media, d_grid,
+  rad::DeviceBuffer d_nodes, d_qnodes, d_prims, d_shapes, d_pos, d_nrm, d_uv, d_idx, d_tris, d_bsdfs, d_lights, d_cdf, d_media, d_grid,
       d_tex, d_texels, d_ldist;
   rad::DeviceBuffer d_rays, d_hits, d_occ;  // staging for the host-pointer intersect entry points
   std::mutex mu;

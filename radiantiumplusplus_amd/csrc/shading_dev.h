@@ -62,17 +62,18 @@ __device__ inline SurfaceHit surface_hit(const DevScene& s, f3 ro, f3 rd, float 
   h.shape = int32_t(geom);
   f3 dpdu;
   if (sr.type == SHAPE_MESH) {  // mesh_base.cpp:38-82
-    const uint32_t* I = s.idx + 3 * size_t(sr.tri_offset + prim);
-    size_t f0 = sr.vtx_offset + I[0], f1 = sr.vtx_offset + I[1], f2 = sr.vtx_offset + I[2];
-    f3 p0 = load3(s.pos, f0), p1 = load3(s.pos, f1), p2 = load3(s.pos, f2);
+    const float4* T = s.tris + 6 * size_t(sr.tri_offset + prim);  // DevScene::tris
+    const float4 t0 = T[0], t1 = T[1], t2 = T[2];
+    f3 p0 = mk(t0.x, t0.y, t0.z), p1 = mk(t0.w, t1.x, t1.y), p2 = mk(t1.z, t1.w, t2.x);
     float b0 = 1.f - u - v;
     f3 dp0 = p1 - p0, dp1 = p2 - p0;
     h.p = p0 * b0 + (p1 * u + (p2 * v));
     h.n = normalize(cross(dp0, dp1));
     if (sr.flags & SHAPE_HAS_UVS) {
-      float u0 = s.uv[2 * f0], v0 = s.uv[2 * f0 + 1];
-      float u1 = s.uv[2 * f1], v1 = s.uv[2 * f1 + 1];
-      float u2 = s.uv[2 * f2], v2 = s.uv[2 * f2 + 1];
+      const float4 t4 = T[4], t5 = T[5];
+      float u0 = t4.z, v0 = t4.w;
+      float u1 = t5.x, v1 = t5.y;
+      float u2 = t5.z, v2 = t5.w;
       float du0 = u1 - u0, dv0 = v1 - v0, du1 = u2 - u0, dv1 = v2 - v0;
       float det = du0 * dv1 - (dv0 * du1);
       float invDet = det == 0.f ? 0.f : 1.f / det;
@@ -82,7 +83,8 @@ __device__ inline SurfaceHit surface_hit(const DevScene& s, f3 ro, f3 rd, float 
       coordinate_system(h.n, dpdu, y);
     }
     if (sr.flags & SHAPE_HAS_NORMALS) {
-      f3 n0 = load3(s.nrm, f0), n1 = load3(s.nrm, f1), n2 = load3(s.nrm, f2);
+      const float4 t3 = T[3], t4 = T[4];
+      f3 n0 = mk(t2.y, t2.z, t2.w), n1 = mk(t3.x, t3.y, t3.z), n2 = mk(t3.w, t4.x, t4.y);
       f3 sn = n0 * b0 + (n1 * u + (n2 * v));
       h.sh.n = sn * (1.f / sqrtf(sqnorm(sn)));
     } else {

@@ -22,13 +22,13 @@ __device__ inline HitUV hit_uv(const DevScene& s, const SurfaceHit& h, float bu,
   const ShapeRec& sr = s.shapes[geom];
   HitUV r;
   if (sr.type == SHAPE_MESH) {
-    const uint32_t* I = s.idx + 3 * size_t(sr.tri_offset + prim);
-    const size_t f0 = sr.vtx_offset + I[0], f1 = sr.vtx_offset + I[1], f2 = sr.vtx_offset + I[2];
+    const float4* T = s.tris + 6 * size_t(sr.tri_offset + prim);  // DevScene::tris
     if (sr.flags & SHAPE_HAS_UVS) {
-      const f3 p0 = load3(s.pos, f0), p1 = load3(s.pos, f1), p2 = load3(s.pos, f2);
+      const float4 t0 = T[0], t1 = T[1], t2 = T[2], t4 = T[4], t5 = T[5];
+      const f3 p0 = mk(t0.x, t0.y, t0.z), p1 = mk(t0.w, t1.x, t1.y), p2 = mk(t1.z, t1.w, t2.x);
       const float b0 = 1.f - bu - bv;
-      const float u0 = s.uv[2 * f0], v0 = s.uv[2 * f0 + 1], u1 = s.uv[2 * f1], v1 = s.uv[2 * f1 + 1];
-      const float u2 = s.uv[2 * f2], v2 = s.uv[2 * f2 + 1];
+      const float u0 = t4.z, v0 = t4.w, u1 = t5.x, v1 = t5.y;
+      const float u2 = t5.z, v2 = t5.w;
       r.u = u0 * b0 + (u1 * bu + (u2 * bv));
       r.v = v0 * b0 + (v1 * bu + (v2 * bv));
       const float du0 = u1 - u0, dv0 = v1 - v0, du1 = u2 - u0, dv1 = v2 - v0;
