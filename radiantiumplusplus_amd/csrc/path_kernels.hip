@@ -10,8 +10,9 @@
 
 namespace rad {
 
-// Start of a pass: every slot idle and on the dead list, so the first regen
-// hands each one a sample.
+// Start of a pass: every slot on the dead list, so the first regen hands
+// each one a sample (that regen is told that no slot holds a finished
+// sample, so the path records need no clearing here).
 __global__ void k_init_pass(Wavefront w, int first) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (first && s < w.npix) {
@@ -19,11 +20,6 @@ __global__ void k_init_pass(Wavefront w, int first) {
     w.pacc2[s] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   if (s < w.Q) {
-    // the whole 64-byte path record: a partial write of one field is slower
-    w.thr[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-    w.lsum[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-    w.prev[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-    w.meta[s] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);
     w.dead[s] = s;
     if (w.glast) w.glast[s] = w.gpend[s] = 0xffffffffu;
     if (w.susp) w.susp[s] = kNoSusp;
@@ -74,7 +70,8 @@ __device__ __forceinline__ void finish_records(const Wavefront& w, uint32_t s, f
 // base + e * 256 + thread), so one set of block atomics covers them all.
 constexpr int kRegenPerThread = 4;
 
-__global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int in) {
+// fresh: the first regen of a pass, whose dead slots hold no finished sample.
+__global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int in, int fresh) {
   constexpr int E = kRegenPerThread;
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[CNT_DEAD];
@@ -95,7 +92,7 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
       if (i < n) {
         s[e] = w.dead[i];
         cls[e] = 0;
-        const uint4 m = w.meta[s[e]];
+        const uint4 m = fresh ? make_uint4(0u, SAMPLE_NONE, 0u, 0u) : w.meta[s[e]];
         if (m.y != SAMPLE_NONE) {
           const float4 l = w.lsum[s[e]];
           // one 16-byte entry per sample: a single line written per finished sample
@@ -805,8 +802,8 @@ void LaunchInitPass(const Wavefront& w, bool first, hipStream_t s) {
   uint32_t n = first ? std::max(w.Q, w.npix) : w.Q;
   hipLaunchKernelGGL(k_init_pass, dim3(Blocks(n, 256)), dim3(256), 0, s, w, first ? 1 : 0);
 }
-void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  hipLaunchKernelGGL(k_regen, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
+void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, bool fresh, const LaunchCfg& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_regen, dim3(c.blocks), dim3(256), 0, s, w, scene, in, fresh ? 1 : 0);
 }
 void LaunchReset(const Wavefront& w, int out, hipStream_t s) {
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w.cnt, w.gcnt, out);

@@ -268,9 +268,11 @@ void RunJob(rad_job* job) {
       job->guide_alpha = (w.flags & WF_GUIDE) ? w.net.alpha : 0.f;
       LaunchInitPass(w, pi == 0, st);
       int in = 0;
+      bool fresh = true;  // the pass's first regen: no slot holds a finished sample
       for (;;) {
         if (prof) HipCheck(hipEventRecord(ev[0], st), "event");
-        LaunchRegen(w, scene, in, cfg, st);
+        LaunchRegen(w, scene, in, fresh, cfg, st);
+        fresh = false;
         LaunchReset(w, 1 - in, st);
         if (prof) HipCheck(hipEventRecord(ev[1], st), "event");
         LaunchExtend(w, scene, in, cfg, st);

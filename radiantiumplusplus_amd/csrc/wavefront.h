@@ -228,7 +228,7 @@ struct LaunchCfg {
 
 // Host-side launchers (defined in path_kernels.hip).
 void LaunchInitPass(const Wavefront& w, bool first, hipStream_t s);
-void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
+void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, bool fresh, const LaunchCfg& c, hipStream_t s);
 void LaunchReset(const Wavefront& w, int out, hipStream_t s);
 void LaunchExtend(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
 void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
