@@ -53,12 +53,28 @@ __device__ __forceinline__ float one_blob(const FeatScalars& fs, int c, int r) {
   return fmaxf(0.f, 1.f - fabsf(x));
 }
 
+// Hardware transcendentals (v_exp_f32, v_log_f32, v_rcp_f32, v_rsq_f32,
+// v_sin_f32, v_cos_f32; about 1 ulp) for the mixture's own arithmetic: its
+// density and its sampler agree to a relative ~1e-6, far below the noise of
+// the estimate, and shading evaluates the mixture several times per vertex.
+__device__ __forceinline__ float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+__device__ __forceinline__ float hw_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+
+// One lane's decoded mixture, held in LDS rather than registers (48 floats
+// per lane would cost the shading kernels a wave per SIMD): lobe l of the
+// lane is km[l * stride] (mean direction, kappa) and pw[l * stride]
+// (weight, weighted vMF normalisation pi kappa / (2 pi (1 - e^{-2 kappa}))).
+// A block of B lanes gives lane t the pointers km0 + t, pw0 + t and stride B,
+// so a lobe read by a wave is one contiguous LDS row.
 struct Mixture {
-  float pi[kLobes];
-  f3 mu[kLobes];
-  float kappa[kLobes];
-  float wnorm[kLobes];  // pi * kappa / (2 pi (1 - e^{-2 kappa})): the lobe's weighted vMF normalisation
+  float4* km;
+  float2* pw;
+  uint32_t stride;
 };
+
+__device__ __forceinline__ Mixture lane_mixture(float4* km0, float2* pw0, uint32_t lane, uint32_t lanes) {
+  return Mixture{km0 + lane, pw0 + lane, lanes};
+}
 
 // Vector from vertex p to a lobe's target c + h * (its three mean outputs).
 template <class T>
@@ -70,27 +86,29 @@ __device__ __forceinline__ f3 lobe_vector(const T* lobe, f3 p, f3 cen, f3 half) 
 // raw: 40 network outputs (logit, target xyz, log kappa per lobe) of the
 // vertex at p; cen, half: the lobe-target frame (GuideNet::center, half).
 template <class T>
-__device__ __forceinline__ Mixture decode(const T* raw, f3 p, f3 cen, f3 half) {
-  Mixture m;
+__device__ __forceinline__ void decode(const Mixture& m, const T* raw, f3 p, f3 cen, f3 half) {
+  float e[kLobes];
   float mx = -kInf;
 #pragma unroll
   for (int l = 0; l < kLobes; l++) mx = fmaxf(mx, float(raw[kRawPerLobe * l]));
   float sum = 0.f;
 #pragma unroll
   for (int l = 0; l < kLobes; l++) {
-    m.pi[l] = expf(float(raw[kRawPerLobe * l]) - mx);
-    sum += m.pi[l];
+    e[l] = hw_exp(float(raw[kRawPerLobe * l]) - mx);
+    sum += e[l];
   }
+  const float inv_sum = __builtin_amdgcn_rcpf(sum);
 #pragma unroll
   for (int l = 0; l < kLobes; l++) {
-    m.pi[l] /= sum;
+    const float pi = e[l] * inv_sum;
     f3 v = lobe_vector(raw + kRawPerLobe * l, p, cen, half);
-    float len = sqrtf(sqnorm(v));
-    m.mu[l] = len > 1e-6f ? v / len : mk(0.f, 1.f, 0.f);
-    m.kappa[l] = expf(fminf(fmaxf(float(raw[kRawPerLobe * l + 4]), -3.f), 6.f));
-    m.wnorm[l] = m.pi[l] * (m.kappa[l] / ((2.f * kPi) * (-expm1f(-2.f * m.kappa[l]))));
+    const float len2 = sqnorm(v);
+    f3 mu = len2 > 1e-12f ? v * __builtin_amdgcn_rsqf(len2) : mk(0.f, 1.f, 0.f);
+    float kappa = hw_exp(fminf(fmaxf(float(raw[kRawPerLobe * l + 4]), -3.f), 6.f));
+    m.km[l * m.stride] = make_float4(mu.x, mu.y, mu.z, kappa);
+    // kappa >= e^-3, so 1 - e^{-2 kappa} >= 0.095 needs no expm1
+    m.pw[l * m.stride] = make_float2(pi, pi * kappa * __builtin_amdgcn_rcpf((2.f * kPi) * (1.f - hw_exp(-2.f * kappa))));
   }
-  return m;
 }
 
 // vMF density on the sphere: kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}.
@@ -104,33 +122,32 @@ __device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
 __device__ __forceinline__ float mixture_pdf(const Mixture& m, f3 w) {
   float p = 0.f;
 #pragma unroll
-  for (int l = 0; l < kLobes; l++) p += m.wnorm[l] * expf(m.kappa[l] * (dot(m.mu[l], w) - 1.f));
+  for (int l = 0; l < kLobes; l++) {
+    const float4 a = m.km[l * m.stride];
+    p += m.pw[l * m.stride].y * hw_exp(a.w * (dot(mk(a.x, a.y, a.z), w) - 1.f));
+  }
   return p;
 }
 
-// Pick a lobe with u0 (selects only, no dynamic indexing), then sample it
-// by inverting the CDF of the cosine to the lobe axis.
+// Pick a lobe with u0 by its weights, then sample it by inverting the CDF of
+// the cosine to the lobe axis.
 __device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
-  f3 mu = m.mu[kLobes - 1];
-  float k = m.kappa[kLobes - 1];
+  uint32_t pick = kLobes - 1;
   float c = 0.f;
-  bool found = false;
 #pragma unroll
   for (int l = 0; l < kLobes - 1; l++) {
-    c += m.pi[l];
-    bool pick = !found && u0 < c;
-    // exact 0/1 blend: a select chain here is turned into a scratch lookup
-    float pf = pick ? 1.f : 0.f, qf = 1.f - pf;
-    mu = mk(fmaf(m.mu[l].x, pf, mu.x * qf), fmaf(m.mu[l].y, pf, mu.y * qf), fmaf(m.mu[l].z, pf, mu.z * qf));
-    k = fmaf(m.kappa[l], pf, k * qf);
-    found = found || pick;
+    c += m.pw[l * m.stride].x;
+    if (pick == kLobes - 1 && u0 < c) pick = l;
   }
-  float w = 1.f + logf(fmaxf(u1 + (1.f - u1) * expf(-2.f * k), 1e-30f)) / k;
+  const float4 a = m.km[pick * m.stride];
+  const f3 mu = mk(a.x, a.y, a.z);
+  const float k = a.w;
+  float w = 1.f + hw_log(fmaxf(u1 + (1.f - u1) * hw_exp(-2.f * k), 1e-30f)) * __builtin_amdgcn_rcpf(k);
   w = fminf(fmaxf(w, -1.f), 1.f);
   float r = safe_sqrt(1.f - w * w);
-  float a = 2.f * kPi * u2;
   Frame fr = frame_from_n(mu);
-  return normalize(fr.to_world(mk(r * cosf(a), r * sinf(a), w)));
+  // azimuth 2 pi u2: the hardware sine and cosine take revolutions
+  return normalize(fr.to_world(mk(r * __builtin_amdgcn_cosf(u2), r * __builtin_amdgcn_sinf(u2), w)));
 }
 
 // A vertex is guided when its BSDF is reflective only (a guided refraction

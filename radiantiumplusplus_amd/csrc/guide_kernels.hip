@@ -276,16 +276,23 @@ __global__ void k_guide_publish(const float* p, _Float16* wh, float* bias) {
 }
 
 // Mixture probes: pdf of directions, and samples, for raw output rows.
-__global__ void k_guide_pdf(const float* raw, const float* dirs, uint32_t n, float* pdf) {
+constexpr uint32_t kProbeBlock = 128;
+__global__ void __launch_bounds__(kProbeBlock) k_guide_pdf(const float* raw, const float* dirs, uint32_t n, float* pdf) {
+  __shared__ float4 km[kLobes * kProbeBlock];
+  __shared__ float2 pw[kLobes * kProbeBlock];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Mixture m = decode(raw, splat(0.f), splat(0.f), splat(1.f));  // the means taken as directions
+  const Mixture m = lane_mixture(km, pw, threadIdx.x, kProbeBlock);
+  decode(m, raw, splat(0.f), splat(0.f), splat(1.f));  // the means taken as directions
   pdf[i] = mixture_pdf(m, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
 }
-__global__ void k_guide_sample(const float* raw, const float* xi, uint32_t n, float* out) {
+__global__ void __launch_bounds__(kProbeBlock) k_guide_sample(const float* raw, const float* xi, uint32_t n, float* out) {
+  __shared__ float4 km[kLobes * kProbeBlock];
+  __shared__ float2 pw[kLobes * kProbeBlock];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Mixture m = decode(raw, splat(0.f), splat(0.f), splat(1.f));
+  const Mixture m = lane_mixture(km, pw, threadIdx.x, kProbeBlock);
+  decode(m, raw, splat(0.f), splat(0.f), splat(1.f));
   f3 d = mixture_sample(m, xi[3 * i], xi[3 * i + 1], xi[3 * i + 2]);
   out[4 * i] = d.x;
   out[4 * i + 1] = d.y;
@@ -396,10 +403,10 @@ void LaunchBatch(uint32_t* batch, uint32_t B, uint32_t n, uint32_t seed, hipStre
 }
 
 void LaunchPdf(const float* raw, const float* dirs, uint32_t n, float* pdf, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_guide_pdf, dim3((n + 127) / 128), dim3(128), 0, s, raw, dirs, n, pdf);
+  if (n) hipLaunchKernelGGL(k_guide_pdf, dim3((n + kProbeBlock - 1) / kProbeBlock), dim3(kProbeBlock), 0, s, raw, dirs, n, pdf);
 }
 void LaunchSample(const float* raw, const float* xi, uint32_t n, float* out, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_guide_sample, dim3((n + 127) / 128), dim3(128), 0, s, raw, xi, n, out);
+  if (n) hipLaunchKernelGGL(k_guide_sample, dim3((n + kProbeBlock - 1) / kProbeBlock), dim3(kProbeBlock), 0, s, raw, xi, n, out);
 }
 
 }  // namespace guide

@@ -159,7 +159,7 @@ __global__ void k_reset(uint32_t* cnt, uint32_t* gcnt, int out) {
     cnt[CNT_SHADOW] = 0;
     cnt[CNT_FETCH_EXT] = 0;
     cnt[CNT_FETCH_SH] = 0;
-    for (int k = 0; k < kMatQueues; k++) cnt[CNT_MAT + k] = 0;
+    for (int k = 0; k < kMatQueues; k++) cnt[CNT_MAT + k] = cnt[CNT_GMAT + k] = 0;
     cnt[CNT_SUSP + out] = 0;  // resume pool `out` was drained by the previous extend
     if (gcnt) gcnt[G_QUEUE] = 0;
   }
@@ -216,20 +216,38 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
   trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
 }
 
+// A vertex is guided when it is a surface hit at depth <= max_depth, on a
+// shape that is no emitter, whose BSDF (its record before textures) the
+// guide may sample.  The one decision for the vertex: classify queues it,
+// inference evaluates the network for it, and shading uses the mixture.
+__device__ __forceinline__ bool guided_vertex(const Wavefront& w, const DevScene& scene, uint32_t geom, uint32_t depth,
+                                              bool quirks) {
+  if (w.net.max_depth != 0xffffffffu && depth > w.net.max_depth) return false;
+  const ShapeRec& sr = scene.shapes[geom];
+  return sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks, w.net.min_rough);
+}
+
 // One bounce of Path::Li (path.cpp:141-225) for every queued path.  GUIDE:
 // at guided vertices the direction is drawn from the learned vMF mixture
 // (probability alpha) or the BSDF, and weighted by the one-sample MIS
 // density alpha q + (1 - alpha) p_bsdf; NEE uses the same density in its MIS
 // weight; training records of the vertices are kept for 1 in rec_stride slots.
+// Sorted (MAT >= 0), the guided kernels shade the material's guided vertices
+// and the plain ones the rest, so unguided vertices never pay for the
+// mixture's registers and LDS; unsorted (MAT < 0), one kernel shades all.
 template <bool GUIDE, int MAT = -1, bool TX = true>
 __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
-  const uint32_t n = MAT >= 0 ? w.cnt[CNT_MAT + MAT] : w.cnt[in];
+  // decoded guide mixtures, one per lane (48 KB: three blocks per CU)
+  __shared__ float4 gkm[GUIDE ? guide::kLobes * 256 : 1];
+  __shared__ float2 gpw[GUIDE ? guide::kLobes * 256 : 1];
+  constexpr bool kSplit = GUIDE && MAT >= 0;  // a guided material queue, read from the back
+  const uint32_t n = MAT >= 0 ? w.cnt[(GUIDE ? CNT_GMAT : CNT_MAT) + MAT] : w.cnt[in];
   const uint32_t* const queue = MAT >= 0 ? w.mq[MAT] : w.queue[in];
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
   const bool guideOn = GUIDE && (w.flags & WF_GUIDE) != 0;
-  const bool record = GUIDE && (w.flags & WF_RECORD) != 0;
+  const bool record = (w.flags & WF_RECORD) != 0;  // unguided vertices close their path's pending record too
   const float alpha = w.net.alpha;
   const int out = 1 - in;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -238,7 +256,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     guide::TrainRecord rec;
-    if (i < n) s = queue[i];
+    if (i < n) s = kSplit ? queue[w.Q - 1 - i] : queue[i];
     // a traversal suspended by this iteration's extend continues next iteration
     // (the material queues never hold one)
     const bool held = MAT < 0 && i < n && w.susp && w.susp[s] != kNoSusp;
@@ -311,11 +329,11 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             }
           }
           const BsdfRec& b = *bp0;
-          const bool guided = GUIDE && guide::guidable(b, quirks, w.net.min_rough) && depth <= w.net.max_depth;
-          guide::Mixture mx;
+          const bool guided = kSplit || (GUIDE && guided_vertex(w, scene, geom, depth, quirks));
+          const guide::Mixture mx = guide::lane_mixture(gkm, gpw, threadIdx.x, 256);
           if (guided && guideOn)
-            mx = guide::decode(w.graw + size_t(s) * guide::kOut, si.p, mk(w.net.center[0], w.net.center[1], w.net.center[2]),
-                               mk(w.net.half[0], w.net.half[1], w.net.half[2]));
+            guide::decode(mx, w.graw + size_t(s) * guide::kOut, si.p, mk(w.net.center[0], w.net.center[1], w.net.center[2]),
+                          mk(w.net.half[0], w.net.half[1], w.net.half[2]));
           const float a = (guided && guideOn) ? alpha : 0.f;
           if (b.lobes & LOBE_NO_DELTA) {  // NEE (path.cpp:173-191)
             float ul = rng_u(key, dim), u1 = rng_u(key, dim + 1), u2 = rng_u(key, dim + 2);
@@ -410,7 +428,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       w.meta[s] = make_uint4(m.x, m.y, depth + 1, dim);
     }
     bool valid = i < n;
-    if (record) {
+    if (GUIDE && record) {
       uint32_t* c1[1] = {&w.gcnt[G_RECORDS]};
       bool p1[1] = {want_rec};
       uint32_t r1[1];
@@ -436,15 +454,13 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   }
 }
 
-// Guided vertices of this iteration (surface hits whose BSDF the guide may
-// sample) -> the guide queue, for the unsorted shading path.
 __device__ __forceinline__ bool guided_hit(const Wavefront& w, const DevScene& scene, uint32_t geom, bool quirks,
                                            uint32_t s) {
-  if (geom == 0xffffffffu) return false;
-  if (w.net.max_depth != 0xffffffffu && w.meta[s].z > w.net.max_depth) return false;
-  const ShapeRec& sr = scene.shapes[geom];
-  return sr.light < 0 && sr.bsdf >= 0 && guide::guidable(scene.bsdfs[sr.bsdf], quirks, w.net.min_rough);
+  return geom != 0xffffffffu && guided_vertex(w, scene, geom, w.meta[s].z, quirks);
 }
+
+// Guided vertices of this iteration -> the guide queue, for the unsorted
+// shading path.
 
 __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
@@ -547,19 +563,28 @@ __device__ __forceinline__ int material_key(const DevScene& scene, uint32_t geom
 // together and one set of block atomics covers them all.
 constexpr int kClassifyPerThread = 4;
 
+// GUIDE (guiding or training records on): guided vertices go to their
+// material's guided queue (class kMatQueues + 1 + material) and, with
+// guiding on, to the guide queue that inference reads.
 template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
   constexpr int E = kClassifyPerThread;
-  constexpr int N = GUIDE ? 7 : 6;  // five material queues, pass-through, (guide queue)
+  // five material queues, pass-through, (five guided material queues, guide queue)
+  constexpr int N = GUIDE ? 2 * kMatQueues + 2 : kMatQueues + 1;
   __shared__ uint32_t lds[N * 5];
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x * E;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
+  const bool infer = (w.flags & WF_GUIDE) != 0;
   uint32_t* ctr[N];
 #pragma unroll
   for (int k = 0; k < kMatQueues; k++) ctr[k] = &w.cnt[CNT_MAT + k];
   ctr[kMatQueues] = &w.cnt[1 - in];
-  if (GUIDE) ctr[N - 1] = &w.gcnt[G_QUEUE];
+  if (GUIDE) {
+#pragma unroll
+    for (int k = 0; k < kMatQueues; k++) ctr[kMatQueues + 1 + k] = &w.cnt[CNT_GMAT + k];
+    ctr[N - 1] = &w.gcnt[G_QUEUE];
+  }
   for (uint32_t base = blockIdx.x * blockDim.x * E; base < n; base += stride) {
     uint32_t s[E];
     int cls[E];
@@ -579,7 +604,10 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
         } else {
           const uint32_t geom = w.hit_geom[s[e]];
           cls[e] = material_key(scene, geom);
-          if (GUIDE) gd[e] = guided_hit(w, scene, geom, quirks, s[e]);
+          if (GUIDE && cls[e] != BSDF_ROUGH_GLASS && guided_hit(w, scene, geom, quirks, s[e])) {
+            cls[e] += kMatQueues + 1;
+            gd[e] = infer;
+          }
         }
       }
     }
@@ -589,13 +617,15 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
     for (int e = 0; e < E; e++) {
       const int c = cls[e];
       if (c >= 0) {
-        uint32_t* q = c == kMatQueues ? w.queue[1 - in]
-                      : c == 0        ? w.mq[0]
-                      : c == 1        ? w.mq[1]
-                      : c == 2        ? w.mq[2]
-                      : c == 3        ? w.mq[3]
+        const bool g = GUIDE && c > kMatQueues;  // guided: from the back of the material's queue
+        const int k = g ? c - kMatQueues - 1 : c;
+        uint32_t* q = k == kMatQueues ? w.queue[1 - in]
+                      : k == 0        ? w.mq[0]
+                      : k == 1        ? w.mq[1]
+                      : k == 2        ? w.mq[2]
+                      : k == 3        ? w.mq[3]
                                       : w.mq[4];
-        q[pos[e]] = s[e];
+        q[g ? w.Q - 1 - pos[e] : pos[e]] = s[e];
       }
       if (GUIDE && gd[e]) w.gq[posg[e]] = s[e];
     }
@@ -616,18 +646,27 @@ __global__ void k_queue_tally(Wavefront w, int in, uint32_t* tally, uint32_t add
 __global__ void k_queue_drain(Wavefront w, DevScene scene, int in, uint32_t* tally) {
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t bad = 0;
-  for (int m = 0; m <= kMatQueues; m++) {
+  const bool split = (w.flags & (WF_GUIDE | WF_RECORD)) != 0;
+  const bool quirks = (w.flags & WF_QUIRKS) != 0;
+  for (int m = 0; m <= 2 * kMatQueues; m++) {
     const bool pass = m == kMatQueues;  // the suspended slots classify passed through
-    const uint32_t n = pass ? w.cnt[1 - in] : w.cnt[CNT_MAT + m];
-    const uint32_t* q = pass ? w.queue[1 - in] : w.mq[m];
+    const bool g = m > kMatQueues;      // a material's guided vertices, from the back of its queue
+    const int k = g ? m - kMatQueues - 1 : m;
+    const uint32_t n = pass ? w.cnt[1 - in] : w.cnt[(g ? CNT_GMAT : CNT_MAT) + k];
+    const uint32_t* q = pass ? w.queue[1 - in] : w.mq[k];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      const uint32_t s = q[i];
+      const uint32_t s = g ? q[w.Q - 1 - i] : q[i];
       if (s >= w.Q) {
         bad++;
         continue;
       }
       const bool held = w.susp && w.susp[s] != kNoSusp;
-      if (pass ? !held : (held || material_key(scene, w.hit_geom[s]) != m)) bad++;
+      if (pass) {
+        if (!held) bad++;
+      } else if (held || material_key(scene, w.hit_geom[s]) != k ||
+                 g != (split && k != BSDF_ROUGH_GLASS && guided_hit(w, scene, w.hit_geom[s], quirks, s))) {
+        bad++;
+      }
       atomicSub(&tally[s], 1u);
     }
   }
@@ -828,16 +867,23 @@ void LaunchShadePath(const Wavefront& w, const DevScene& scene, int in, const La
   else
     LaunchMat<false, -1>(tx, w, scene, in, c, s);
 }
-template <bool G>
-static void LaunchMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
-                            hipStream_t s) {
+// The plain kernels shade the unguided vertices of each material; with
+// guiding or training records on, the guided kernels then shade the guided
+// ones (rough glass is never guided: it transmits).
+static void LaunchMaterials(bool guided, const Wavefront& w, const DevScene& scene, int in, uint32_t mask,
+                            const LaunchCfg& c, hipStream_t s) {
   const bool tl = scene.tex_lights != 0, tx = tl || scene.tex_bsdfs;
   // the diffuse kernel also takes the misses, so it always runs
-  LaunchMat<G, BSDF_DIFFUSE>(tl, w, scene, in, c, s);
-  if (mask & (1u << BSDF_ROUGH_METAL)) LaunchMat<G, BSDF_ROUGH_METAL>(tl, w, scene, in, c, s);
-  if (mask & (1u << BSDF_ROUGH_GLASS)) LaunchMat<G, BSDF_ROUGH_GLASS>(tl, w, scene, in, c, s);
-  if (mask & (1u << BSDF_DISNEY)) LaunchMat<G, BSDF_DISNEY>(tl, w, scene, in, c, s);
-  if (mask & (1u << kMatOther)) LaunchMat<G, kMatOther>(tx, w, scene, in, c, s);
+  LaunchMat<false, BSDF_DIFFUSE>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_ROUGH_METAL)) LaunchMat<false, BSDF_ROUGH_METAL>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_ROUGH_GLASS)) LaunchMat<false, BSDF_ROUGH_GLASS>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_DISNEY)) LaunchMat<false, BSDF_DISNEY>(tl, w, scene, in, c, s);
+  if (mask & (1u << kMatOther)) LaunchMat<false, kMatOther>(tx, w, scene, in, c, s);
+  if (!guided) return;
+  if (mask & (1u << BSDF_DIFFUSE)) LaunchMat<true, BSDF_DIFFUSE>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_ROUGH_METAL)) LaunchMat<true, BSDF_ROUGH_METAL>(tl, w, scene, in, c, s);
+  if (mask & (1u << BSDF_DISNEY)) LaunchMat<true, BSDF_DISNEY>(tl, w, scene, in, c, s);
+  if (mask & (1u << kMatOther)) LaunchMat<true, kMatOther>(tx, w, scene, in, c, s);
 }
 
 void LaunchQueueCheck(const Wavefront& w, const DevScene& scene, int in, uint32_t* tally, bool selftest,
@@ -847,7 +893,7 @@ void LaunchQueueCheck(const Wavefront& w, const DevScene& scene, int in, uint32_
   hipLaunchKernelGGL(k_queue_verify, dim3(c.blocks), dim3(256), 0, s, w, tally);
 }
 void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {
-  if (w.flags & WF_GUIDE)
+  if (w.flags & (WF_GUIDE | WF_RECORD))
     hipLaunchKernelGGL(k_classify<true>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
   else
     hipLaunchKernelGGL(k_classify<false>, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
@@ -855,10 +901,7 @@ void LaunchClassify(const Wavefront& w, const DevScene& scene, int in, const Lau
 
 void LaunchShadeMaterials(const Wavefront& w, const DevScene& scene, int in, uint32_t mask, const LaunchCfg& c,
                           hipStream_t s) {
-  if (w.flags & (WF_GUIDE | WF_RECORD))
-    LaunchMaterials<true>(w, scene, in, mask, c, s);
-  else
-    LaunchMaterials<false>(w, scene, in, mask, c, s);
+  LaunchMaterials((w.flags & (WF_GUIDE | WF_RECORD)) != 0, w, scene, in, mask, c, s);
 }
 
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s) {

@@ -36,7 +36,8 @@ enum : uint32_t {
   CNT_FETCH_SH = 6,   // shadow-kernel work counter
   CNT_MAT = 8,        // 5 words: per-material queue counts (diffuse + misses, metal, glass, Disney, others)
   CNT_SUSP = 14,      // 2 words: traversals suspended into resume pool 0 / 1
-  CNT_WORDS = 16      // even, so the 64-bit statistics that follow stay aligned
+  CNT_GMAT = 16,      // 5 words: guided vertices per material (stored from the back of that material's queue)
+  CNT_WORDS = 24      // even, so the 64-bit statistics that follow stay aligned
 };
 enum : uint32_t { ST_SAMPLES = 0, ST_MEDIUM = 1, ST_QCHECK = 2, ST_QERR = 3, ST_WORDS = 4 };
 enum : uint32_t { WF_QUIRKS = 1u, WF_GUIDE = 2u, WF_RECORD = 4u };
@@ -78,7 +79,11 @@ struct Wavefront {
   uint32_t* susp;
   ResumeState* resume[2];
   uint32_t resume_cap, visit_budget;
-  uint32_t* mq[5];   // the extend queue partitioned by material (k_classify); [4]: every other type
+  // The extend queue partitioned by material (k_classify); [4]: every other
+  // type.  With guiding or training records on, entry j of a material's
+  // guided vertices is mq[k][Q - 1 - j] (count CNT_GMAT + k): both parts
+  // together never exceed the Q slots.
+  uint32_t* mq[5];
   uint32_t* dead;
   float4* sh_o;    // shadow origin, tmax
   float4* sh_d;    // shadow direction, slot (bits)

@@ -187,6 +187,27 @@ def test_guided_render_is_unbiased(hip_ctx):
     assert rel < 2e-3
 
 
+def test_guided_queues_are_a_permutation(hip_ctx):
+    """With guiding or training records on, classify puts each material's
+    guided vertices at the back of its queue and the rest at the front: the
+    device tally finds every extension-queue slot exactly once, each guided
+    entry where the guided-vertex test holds and no unguided entry where it
+    does, through training passes (records, no guide) and guided passes.
+    The check leaves a fixed-weight guided image unchanged."""
+    b = small_room()
+    s = hip.Scene.from_builder(hip_ctx, b)
+    rd = b.render_desc(spp=1024, seed=3, flags=F.RAD_RENDER_CHECK_QUEUES)
+    rd.guiding = 1
+    _, _, st = s.render(rd)
+    print(f"guide steps {st.guide_steps} checked {st.queue_checked} errors {st.queue_errors}")
+    assert st.guide_steps > 0 and st.guide_alpha > 0
+    assert st.queue_errors == 0 and st.queue_checked == st.segments + st.suspended > 0
+    ref, _, _ = render(hip_ctx, s, b, 64, seed=4, guiding=2)
+    img, _, st2 = render(hip_ctx, s, b, 64, seed=4, guiding=2, flags=F.RAD_RENDER_CHECK_QUEUES)
+    assert st2.queue_errors == 0 and st2.queue_checked == st2.segments + st2.suspended > 0
+    assert np.array_equal(img, ref)
+
+
 def test_trained_guide_reduces_error(hip_ctx):
     """At equal spp the trained guide's error against a converged unguided
     reference is well below that of BSDF sampling alone (quirks off, so the
