@@ -21,7 +21,7 @@ namespace guide {
 __global__ void __launch_bounds__(256) k_guide_infer(GuideNet net, const float* feats, const uint32_t* n_dev, uint32_t n_host,
                                                      const uint32_t* slot_of, _Float16* raw) {
   __shared__ __align__(16) _Float16 sW[kWeights];
-  __shared__ float sB[kBiases];
+  __shared__ __align__(16) _Float16 sB[kBiases];
   load_weights_lds_frag(sW, sB, net.w, net.b);
   __syncthreads();
   const uint32_t n = n_dev ? *n_dev : n_host;
@@ -37,18 +37,12 @@ __global__ void __launch_bounds__(256) k_guide_infer(GuideNet net, const float* 
       for (int k = 0; k < kFeat; k++) f9[k] = 0.f;
     }
     FeatScalars fs = feat_scalars(f9, net.lo, net.inv_ext);
-    h4 act[4][4];
-    f4v out[3];
-    forward_tile<true>(sW, sB, fs, act, out, nullptr);
+    h4 out[3];
+    infer_tile(sW, sB, fs, out);
     if (i < n) {
       size_t base = size_t(slot_of ? slot_of[i] : i) * kOut;
 #pragma unroll
-      for (int t = 0; t < 3; t++) {
-        h4 o;
-#pragma unroll
-        for (int j = 0; j < 4; j++) o[j] = (_Float16)out[t][j];
-        *reinterpret_cast<h4*>(raw + base + 16 * t + 4 * g) = o;
-      }
+      for (int t = 0; t < 3; t++) *reinterpret_cast<h4*>(raw + base + 16 * t + 4 * g) = out[t];
     }
   }
 }

@@ -35,7 +35,7 @@ __device__ __forceinline__ int frag_base(int l, int t, int c) {
   return HalfOffset(l) + (t * (LayerIn(l) / 16) + c) * 256;
 }
 
-__device__ __forceinline__ void load_weights_lds_frag(_Float16* sW, float* sB, const _Float16* w, const float* b) {
+__device__ __forceinline__ void load_weights_lds_frag(_Float16* sW, _Float16* sBh, const _Float16* w, const float* b) {
   const uint2* src = reinterpret_cast<const uint2*>(w);
   for (int i = threadIdx.x; i < kWeights / 4; i += blockDim.x) {
     const int h = 4 * i;
@@ -47,7 +47,7 @@ __device__ __forceinline__ void load_weights_lds_frag(_Float16* sW, float* sB, c
     const int lane = (o & 15) + 16 * ((in & 15) >> 2);
     *reinterpret_cast<uint2*>(sW + frag_base(l, o >> 4, in >> 4) + 4 * lane) = src[i];
   }
-  for (int i = threadIdx.x; i < kBiases; i += blockDim.x) sB[i] = b[i];
+  for (int i = threadIdx.x; i < kBiases; i += blockDim.x) sBh[i] = (_Float16)b[i];
 }
 
 // A operand (16 rows 16t.., 16 columns 16c..) of layer l for this lane.
@@ -122,6 +122,58 @@ __device__ __forceinline__ void forward_tile(const _Float16* sW, const float* sB
   for (int t = 0; t < 3; t++)
 #pragma unroll
     for (int j = 0; j < 4; j++) out[t][j] += sB[bias_base(4) + 16 * t + 4 * g + j];
+}
+
+// Inference forward pass of one 16-vertex tile from the fragment-major
+// weights (load_weights_lds_frag).  The epilogues run on packed FP16: an
+// accumulator quad is converted to halves, then bias and ReLU are two packed
+// adds and two packed maxes, a quarter of the per-value FP32 add / max /
+// convert sequence (the layer outputs are FP16 MFMA operands either way).
+// out[t]: raw outputs 16 t + 4 g .. 16 t + 4 g + 3 of vertex v.
+__device__ __forceinline__ void infer_tile(const _Float16* sW, const _Float16* sBh, const FeatScalars& fs, h4 out[3]) {
+  const int lane = __lane_id();
+  const int g = lane >> 4;
+  const h4 zero = h4{0, 0, 0, 0};
+  f4v acc[4];
+  h4 x[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < kIn / 16; c++) {
+    h4 b;
+#pragma unroll
+    for (int j = 0; j < 4; j++) b[j] = (_Float16)one_blob(fs, c, 4 * g + j);
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = mfma16(weight_operand<true>(sW, 0, t, c, lane), b, acc[t]);
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    if (l > 0) {
+      __builtin_amdgcn_sched_barrier(0);  // keep each layer's weight loads inside the layer
+#pragma unroll
+      for (int t = 0; t < 4; t++) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] = mfma16(weight_operand<true>(sW, l, t, c, lane), x[c], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const h4 bias = *reinterpret_cast<const h4*>(sBh + bias_base(l) + 16 * t + 4 * g);
+      x[t] = __builtin_elementwise_max(__builtin_convertvector(acc[t], h4) + bias, zero);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  f4v o[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) o[t] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int t = 0; t < 3; t++) o[t] = mfma16(weight_operand<true>(sW, 4, t, c, lane), x[c], o[t]);
+#pragma unroll
+  for (int t = 0; t < 3; t++)
+    out[t] = __builtin_convertvector(o[t], h4) + *reinterpret_cast<const h4*>(sBh + bias_base(4) + 16 * t + 4 * g);
 }
 
 }  // namespace guide

@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           const bool guided = kSplit || (GUIDE && guided_vertex(w, scene, geom, depth, quirks));
           const guide::Mixture mx = guide::lane_mixture(gkm, gpw, threadIdx.x, 256);
           if (guided && guideOn)
-            guide::decode(mx, w.graw + size_t(s) * guide::kOut, si.p, mk(w.net.center[0], w.net.center[1], w.net.center[2]),
+            guide::decode(mx, w.graw + size_t(kSplit ? guided_row(w.cnt, MAT, i) : s) * guide::kOut, si.p, mk(w.net.center[0], w.net.center[1], w.net.center[2]),
                           mk(w.net.half[0], w.net.half[1], w.net.half[2]));
           const float a = (guided && guideOn) ? alpha : 0.f;
           if (b.lobes & LOBE_NO_DELTA) {  // NEE (path.cpp:173-191)
@@ -461,7 +461,6 @@ __device__ __forceinline__ bool guided_hit(const Wavefront& w, const DevScene& s
 
 // Guided vertices of this iteration -> the guide queue, for the unsorted
 // shading path.
-
 __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
@@ -483,21 +482,31 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
   }
 }
 
-// Network inference for the guide queue: the inputs (hit position, geometric
-// normal, outgoing direction) are formed from the slot's ray and hit, and
-// the 48 raw outputs go to graw[slot].  A wave takes 64 queue entries at a
-// time, one per lane, so the chain of dependent loads behind the inputs
-// (queue -> slot -> ray and hit -> shape -> triangle) is walked for 64
-// vertices at once; the MLP then runs as four 16-vertex tiles whose inputs
-// are shuffled in from the lanes that gathered them.  The weights sit in LDS
-// once per block of kInferBlock threads.
+// Network inference for the guided vertices: the inputs (hit position,
+// geometric normal, outgoing direction) are formed from the slot's ray and
+// hit, and the 40 used raw outputs go to the vertex's graw row.  SORTED: the
+// vertices are the guided material queues' entries in row order (entry i of
+// the concatenation is row i); otherwise the guide queue, row = slot.  A
+// wave takes 64 entries at a time, one per lane, so the chain of dependent
+// loads behind the inputs (queue -> slot -> ray and hit -> shape ->
+// triangle) is walked for 64 vertices at once; the MLP then runs as four
+// 16-vertex tiles whose inputs are shuffled in from the lanes that gathered
+// them.  The weights sit in LDS once per block of kInferBlock threads.
 constexpr int kInferBlock = 512;
+template <bool SORTED>
 __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, DevScene scene) {
   __shared__ __align__(16) _Float16 sW[guide::kWeights];
-  __shared__ float sB[guide::kBiases];
+  __shared__ __align__(16) _Float16 sB[guide::kBiases];
   guide::load_weights_lds_frag(sW, sB, w.net.w, w.net.b);
   __syncthreads();
-  const uint32_t n = w.gcnt[G_QUEUE];
+  uint32_t end[kMatQueues];  // SORTED: end row of each material's guided entries
+  uint32_t n = 0;
+  if (SORTED) {
+#pragma unroll
+    for (int k = 0; k < kMatQueues; k++) end[k] = n += w.cnt[CNT_GMAT + k];
+  } else {
+    n = w.gcnt[G_QUEUE];
+  }
   const int lane = __lane_id();
   const int v = lane & 15, g = lane >> 4;
   const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
@@ -506,7 +515,17 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
     float f9[guide::kFeat];
     uint32_t s = 0;
     if (i < n) {
-      s = w.gq[i];
+      if (SORTED) {
+        int k = 0;
+        uint32_t j = i;
+#pragma unroll
+        for (int q = 0; q < kMatQueues - 1; q++)
+          if (i >= end[q]) k = q + 1, j = i - end[q];
+        const uint32_t* q = k == 0 ? w.mq[0] : k == 1 ? w.mq[1] : k == 2 ? w.mq[2] : k == 3 ? w.mq[3] : w.mq[4];
+        s = q[w.Q - 1 - j];
+      } else {
+        s = w.gq[i];
+      }
       float4 ro = w.ray[2 * size_t(s)], rd = w.ray[2 * size_t(s) + 1], hh = w.hit[s];
       f3 d = mk(rd.x, rd.y, rd.z);
       f3 p = d * hh.x + mk(ro.x, ro.y, ro.z);
@@ -524,19 +543,14 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
       guide::FeatScalars fs;
 #pragma unroll
       for (int k = 0; k < 7; k++) fs.s[k] = __shfl(mine.s[k], src);
-      const uint32_t sv = __shfl(s, src);
-      guide::h4 act[4][4];
-      guide::f4v out[3];
-      guide::forward_tile<true>(sW, sB, fs, act, out, nullptr);
+      const uint32_t row = SORTED ? base + src : __shfl(s, src);
+      guide::h4 out[3];
+      guide::infer_tile(sW, sB, fs, out);
       if (base + src < n) {
-        _Float16* dst = w.graw + size_t(sv) * guide::kOut + 4 * g;
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-          guide::h4 o;
-#pragma unroll
-          for (int j = 0; j < 4; j++) o[j] = (_Float16)out[t][j];
-          *reinterpret_cast<guide::h4*>(dst + 16 * t) = o;
-        }
+        _Float16* dst = w.graw + size_t(row) * guide::kOut + 4 * g;
+        *reinterpret_cast<guide::h4*>(dst) = out[0];
+        *reinterpret_cast<guide::h4*>(dst + 16) = out[1];
+        if (g < 2) *reinterpret_cast<guide::h4*>(dst + 32) = out[2];  // outputs 40..47 are padding
       }
     }
   }
@@ -564,18 +578,16 @@ __device__ __forceinline__ int material_key(const DevScene& scene, uint32_t geom
 constexpr int kClassifyPerThread = 4;
 
 // GUIDE (guiding or training records on): guided vertices go to their
-// material's guided queue (class kMatQueues + 1 + material) and, with
-// guiding on, to the guide queue that inference reads.
+// material's guided queue (class kMatQueues + 1 + material).
 template <bool GUIDE>
 __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, int in) {
   constexpr int E = kClassifyPerThread;
-  // five material queues, pass-through, (five guided material queues, guide queue)
-  constexpr int N = GUIDE ? 2 * kMatQueues + 2 : kMatQueues + 1;
+  // five material queues, pass-through, (five guided material queues)
+  constexpr int N = GUIDE ? 2 * kMatQueues + 1 : kMatQueues + 1;
   __shared__ uint32_t lds[N * 5];
   const uint32_t n = w.cnt[in];
   const uint32_t stride = gridDim.x * blockDim.x * E;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
-  const bool infer = (w.flags & WF_GUIDE) != 0;
   uint32_t* ctr[N];
 #pragma unroll
   for (int k = 0; k < kMatQueues; k++) ctr[k] = &w.cnt[CNT_MAT + k];
@@ -583,18 +595,16 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
   if (GUIDE) {
 #pragma unroll
     for (int k = 0; k < kMatQueues; k++) ctr[kMatQueues + 1 + k] = &w.cnt[CNT_GMAT + k];
-    ctr[N - 1] = &w.gcnt[G_QUEUE];
   }
   for (uint32_t base = blockIdx.x * blockDim.x * E; base < n; base += stride) {
     uint32_t s[E];
     int cls[E];
-    bool gd[E];
+    bool none[E] = {};
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const uint32_t i = base + e * blockDim.x + threadIdx.x;
       s[e] = 0;
       cls[e] = -1;
-      gd[e] = false;
       if (i < n) {
         s[e] = w.queue[in][i];
         // traversal suspended: straight to the next extend queue
@@ -604,15 +614,12 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
         } else {
           const uint32_t geom = w.hit_geom[s[e]];
           cls[e] = material_key(scene, geom);
-          if (GUIDE && cls[e] != BSDF_ROUGH_GLASS && guided_hit(w, scene, geom, quirks, s[e])) {
-            cls[e] += kMatQueues + 1;
-            gd[e] = infer;
-          }
+          if (GUIDE && cls[e] != BSDF_ROUGH_GLASS && guided_hit(w, scene, geom, quirks, s[e])) cls[e] += kMatQueues + 1;
         }
       }
     }
-    uint32_t pos[E], posg[E];
-    block_append_n<N, E, GUIDE>(ctr, cls, gd, pos, posg, lds);
+    uint32_t pos[E], unused[E];
+    block_append_n<N, E, false>(ctr, cls, none, pos, unused, lds);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int c = cls[e];
@@ -627,7 +634,6 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
                                       : w.mq[4];
         q[g ? w.Q - 1 - pos[e] : pos[e]] = s[e];
       }
-      if (GUIDE && gd[e]) w.gq[posg[e]] = s[e];
     }
   }
 }
@@ -908,8 +914,11 @@ void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const La
   hipLaunchKernelGGL(k_guide_prep, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 
-void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_guide_infer_hits, dim3(blocks), dim3(kInferBlock), 0, s, w, scene);
+void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, bool sorted, hipStream_t s) {
+  if (sorted)
+    hipLaunchKernelGGL(k_guide_infer_hits<true>, dim3(blocks), dim3(kInferBlock), 0, s, w, scene);
+  else
+    hipLaunchKernelGGL(k_guide_infer_hits<false>, dim3(blocks), dim3(kInferBlock), 0, s, w, scene);
 }
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);

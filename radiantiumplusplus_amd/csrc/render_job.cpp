@@ -280,13 +280,13 @@ void RunJob(rad_job* job) {
         const bool vol = w.integrator == RAD_INTEGRATOR_VOL_PATH;
         const bool guided = (w.flags & WF_GUIDE) != 0;
         if (job->sorted) {
-          LaunchClassify(w, scene, in, cfg, st);  // with guiding, also the guide queue
+          LaunchClassify(w, scene, in, cfg, st);  // with guiding or records, also the guided queues
           if (job->tally) LaunchQueueCheck(w, scene, in, job->tally, job->tally_selftest, cfg, st);
         }
         else if (!vol && guided)
           LaunchGuidePrep(w, scene, in, cfg, st);
         if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
-        if (!vol && guided) LaunchGuideInfer(w, scene, inferBlocks, st);
+        if (!vol && guided) LaunchGuideInfer(w, scene, inferBlocks, job->sorted, st);
         if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
         if (vol && job->sorted)
           LaunchShadeVolMaterials(w, scene, in, job->mat_mask, cfg, st);

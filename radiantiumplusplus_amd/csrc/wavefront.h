@@ -50,6 +50,14 @@ constexpr int kMatOther = 4;
 constexpr uint32_t kNoSusp = 0xffffffffu;
 constexpr int kMatQueues = 5;
 
+// Row of entry j of material k's guided queue in Wavefront::graw: the
+// materials' guided vertices one after another, in material order.
+__device__ __forceinline__ uint32_t guided_row(const uint32_t* cnt, int k, uint32_t j) {
+  uint32_t r = j;
+  for (int q = 0; q < k; q++) r += cnt[CNT_GMAT + q];
+  return r;
+}
+
 // Field k of a 64-byte per-slot record of four 16-byte fields: element s is
 // p[4 s], so the four fields of a slot share one memory sector.
 template <class T>
@@ -94,8 +102,11 @@ struct Wavefront {
   uint32_t* cnt;   // CNT_* words
   unsigned long long* stats;  // ST_* words
   // learned guiding (null when off)
-  uint32_t* gq;      // guide queue: slots whose vertex is guided this iteration
-  _Float16* graw;    // per slot: 48 raw network outputs
+  uint32_t* gq;      // unsorted shading: slots whose vertex is guided this iteration
+  // 48 raw network outputs (40 used) per guided vertex.  Sorted shading: row
+  // guided_row(cnt, k, j) for entry j of material k's guided queue, so
+  // inference writes and shading reads consecutive rows; unsorted: row = slot.
+  _Float16* graw;
   uint32_t* gpend;   // per slot: training record still waiting for its radiance baseline
   uint32_t* glast;   // per slot: newest training record of the path in flight
   guide::TrainRecord* recs;
@@ -248,7 +259,7 @@ void LaunchShadeMaterials(const Wavefront& w, const DevScene& scene, int in, uin
                           hipStream_t s);
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
-void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, hipStream_t s);
+void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, bool sorted, hipStream_t s);
 void LaunchAccum(const Wavefront& w, hipStream_t s);
 // vol_path (defined in volpath_kernels.hip).
 void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
