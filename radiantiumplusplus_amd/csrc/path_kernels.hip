@@ -90,9 +90,11 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
       s[e] = 0;
       cls[e] = -1;
       if (i < n) {
-        s[e] = w.dead[i];
+        const uint32_t d = w.dead[i];
+        s[e] = d & ~kDeadDone;
         cls[e] = 0;
-        const uint4 m = fresh ? make_uint4(0u, SAMPLE_NONE, 0u, 0u) : w.meta[s[e]];
+        if (d & kDeadDone) finished++;  // result written by the shading kernel
+        const uint4 m = fresh || (d & kDeadDone) ? make_uint4(0u, SAMPLE_NONE, 0u, 0u) : w.meta[s[e]];
         if (m.y != SAMPLE_NONE) {
           const float4 l = w.lsum[s[e]];
           // one 16-byte entry per sample: a single line written per finished sample
@@ -252,7 +254,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   const int out = 1 - in;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
-    bool alive = false, emit = false, want_rec = false;
+    bool alive = false, emit = false, want_rec = false, done = false;
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     guide::TrainRecord rec;
@@ -426,6 +428,13 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
       w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
       w.meta[s] = make_uint4(m.x, m.y, depth + 1, dim);
+      // A sample that ends here with no shadow ray still to add to it (and no
+      // training records to close) is final: its result entry is written now,
+      // and regen reads nothing for the slot.
+      if (!alive && !emit && !record) {
+        w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(L.x, L.y, L.z, 0.f);
+        done = true;
+      }
     }
     bool valid = i < n;
     if (GUIDE && record) {
@@ -445,7 +454,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     uint32_t r3[3];
     block_append<3>(c3, p3, r3, lds);
     if (valid && alive) w.queue[out][r3[0]] = s;
-    if (valid && !alive) w.dead[r3[1]] = s;
+    if (valid && !alive) w.dead[r3[1]] = s | (done ? kDeadDone : 0u);
     if (emit) {
       w.sh_o[r3[2]] = so;
       w.sh_d[r3[2]] = sdir;

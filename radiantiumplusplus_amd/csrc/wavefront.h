@@ -48,6 +48,9 @@ enum : uint32_t { MED_UNSET = 0xffffffffu };
 // Disney) have their own; kMatOther takes every other type.
 constexpr int kMatOther = 4;
 constexpr uint32_t kNoSusp = 0xffffffffu;
+// Dead-list flag: the shading kernel already wrote this slot's finished
+// sample to the result entries, so regen reads nothing for it.
+constexpr uint32_t kDeadDone = 0x80000000u;
 constexpr int kMatQueues = 5;
 
 // Row of entry j of material k's guided queue in Wavefront::graw: the
@@ -92,7 +95,7 @@ struct Wavefront {
   // guided vertices is mq[k][Q - 1 - j] (count CNT_GMAT + k): both parts
   // together never exceed the Q slots.
   uint32_t* mq[5];
-  uint32_t* dead;
+  uint32_t* dead;  // slots whose sample ended this iteration (| kDeadDone: result already written)
   float4* sh_o;    // shadow origin, tmax
   float4* sh_d;    // shadow direction, slot (bits)
   float4* sh_c;    // shadow contribution
