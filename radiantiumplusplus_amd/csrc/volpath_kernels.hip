@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
   const int out = 1 - in;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
-    bool alive = false, emit = false, msample = false;
+    bool alive = false, emit = false, msample = false, done = false;
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     if (i < n) {
@@ -250,6 +250,12 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
         w.medium[s] = uint32_t(medium + 1) | (ch << 24);
       }
       w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
+      // a sample that ends here with no shadow ray still to add to it is
+      // final: its result entry is written now, and regen reads nothing for it
+      if (!alive && !emit) {
+        w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(L.x, L.y, L.z, 0.f);
+        done = true;
+      }
     }
     bool valid = i < n;
     uint32_t* c3[3] = {&w.cnt[out], &w.cnt[CNT_DEAD], &w.cnt[CNT_SHADOW]};
@@ -257,7 +263,7 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     uint32_t r3[3];
     block_append<3>(c3, p3, r3, lds);
     if (valid && alive) w.queue[out][r3[0]] = s;
-    if (valid && !alive) w.dead[r3[1]] = s;
+    if (valid && !alive) w.dead[r3[1]] = s | (done ? kDeadDone : 0u);
     if (emit) {
       w.sh_o[r3[2]] = so;
       w.sh_d[r3[2]] = sdir;
