@@ -424,17 +424,19 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           }
         }
       }
-      w.thr[s] = make_float4(T.x, T.y, T.z, eta);
-      w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
-      w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
-      w.meta[s] = make_uint4(m.x, m.y, depth + 1, dim);
       // A sample that ends here with no shadow ray still to add to it (and no
       // training records to close) is final: its result entry is written now,
-      // and regen reads nothing for the slot.
-      if (!alive && !emit && !record) {
+      // and regen reads nothing for the slot.  Of an ended sample regen reads
+      // only the radiance and the pixel and sample, which stay as they are.
+      if (alive) {
+        w.thr[s] = make_float4(T.x, T.y, T.z, eta);
+        w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
+        w.meta[s] = make_uint4(m.x, m.y, depth + 1, dim);
+      } else if (!emit && !record) {
         w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(L.x, L.y, L.z, 0.f);
         done = true;
       }
+      if (!done) w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
     }
     bool valid = i < n;
     if (GUIDE && record) {

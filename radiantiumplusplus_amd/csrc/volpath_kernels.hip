@@ -249,12 +249,13 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
         w.meta[s] = make_uint4(m.x, m.y, next_depth, dim);
         w.medium[s] = uint32_t(medium + 1) | (ch << 24);
       }
-      w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
       // a sample that ends here with no shadow ray still to add to it is
       // final: its result entry is written now, and regen reads nothing for it
       if (!alive && !emit) {
         w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(L.x, L.y, L.z, 0.f);
         done = true;
+      } else {
+        w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
       }
     }
     bool valid = i < n;
