@@ -167,8 +167,10 @@ __global__ void k_reset(uint32_t* cnt, uint32_t* gcnt, int out) {
   }
 }
 
+// The tag of an extend ray is its queue position; its slot is queue[tag].
 struct ExtendJob {
   static constexpr bool kSuspend = true;
+  static constexpr bool kTagIsIndex = true;
   const uint32_t* queue;
   const float4* ray;
   float4* hit;
@@ -178,18 +180,19 @@ struct ExtendJob {
   ResumeState* rout;              // pool this launch suspends into
   uint32_t* rcount;               // entries of rout
   uint32_t cap, budget;
-  __device__ __forceinline__ void resume(uint32_t s, TraceRay& r, TraceStack& st) {
+  __device__ __forceinline__ void resume(uint32_t i, TraceRay& r, TraceStack& st) {
     if (!susp) return;
+    const uint32_t s = queue[i];
     const uint32_t j = susp[s];
     if (j == kNoSusp) return;
     restore_trace(rin[j], r, st);
     susp[s] = kNoSusp;
   }
-  __device__ __forceinline__ bool suspend(uint32_t s, const TraceRay& r, const TraceStack& st) {
+  __device__ __forceinline__ bool suspend(uint32_t i, const TraceRay& r, const TraceStack& st) {
     const uint32_t j = atomicAdd(rcount, 1u);
     if (j >= cap) return false;  // pool full: the ray finishes in this launch
     save_trace(rout[j], r, st);
-    susp[s] = j;
+    susp[queue[i]] = j;
     return true;
   }
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
@@ -199,12 +202,14 @@ struct ExtendJob {
     d = mk(b.x, b.y, b.z);
     tmin = a.w;
     tmax = b.w;
-    tag = s;
+    tag = i;
     return true;
   }
-  __device__ __forceinline__ void finish(uint32_t, uint32_t s, const TraceRay& r) {
-    hit[s] = make_float4(r.h.t, r.h.u, r.h.v, __uint_as_float(r.hit ? r.h.prim : 0u));
-    hit_geom[s] = r.hit ? r.h.geom : 0xffffffffu;
+  // hits are stored by extend-queue position: consecutive rays of the
+  // queue write consecutive entries, and classify reads them in order
+  __device__ __forceinline__ void finish(uint32_t i, uint32_t, const TraceRay& r) {
+    hit[i] = make_float4(r.h.t, r.h.u, r.h.v, __uint_as_float(r.hit ? r.h.prim : 0u));
+    hit_geom[i] = r.hit ? r.h.geom : 0xffffffffu;
   }
 };
 
@@ -255,10 +260,13 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool alive = false, emit = false, want_rec = false, done = false;
-    uint32_t s = 0;
+    uint32_t s = 0, p = 0;  // slot, and its position in the extend queue (which indexes the hits)
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     guide::TrainRecord rec;
-    if (i < n) s = kSplit ? queue[w.Q - 1 - i] : queue[i];
+    if (i < n) {
+      p = MAT < 0 ? i : kSplit ? queue[w.Q - 1 - i] : queue[i];
+      s = w.queue[in][p];
+    }
     // a traversal suspended by this iteration's extend continues next iteration
     // (the material queues never hold one)
     const bool held = MAT < 0 && i < n && w.susp && w.susp[s] != kNoSusp;
@@ -266,8 +274,8 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     if (i < n && !held) {
       float4 ro4 = w.ray[2 * size_t(s)], rd4 = w.ray[2 * size_t(s) + 1];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
-      float4 hh = w.hit[s];
-      uint32_t geom = w.hit_geom[s];
+      float4 hh = w.hit[p];
+      uint32_t geom = w.hit_geom[p];
       float4 t4 = w.thr[s];
       f3 T = mk(t4.x, t4.y, t4.z);
       float eta = t4.w;
@@ -470,8 +478,8 @@ __device__ __forceinline__ bool guided_hit(const Wavefront& w, const DevScene& s
   return geom != 0xffffffffu && guided_vertex(w, scene, geom, w.meta[s].z, quirks);
 }
 
-// Guided vertices of this iteration -> the guide queue, for the unsorted
-// shading path.
+// Guided vertices of this iteration -> the guide queue (their extend-queue
+// positions), for the unsorted shading path.
 __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene, int in) {
   __shared__ uint32_t lds[32];
   const uint32_t n = w.cnt[in];
@@ -483,13 +491,13 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
     uint32_t s = 0;
     if (i < n) {
       s = w.queue[in][i];
-      g = !(w.susp && w.susp[s] != kNoSusp) && guided_hit(w, scene, w.hit_geom[s], quirks, s);
+      g = !(w.susp && w.susp[s] != kNoSusp) && guided_hit(w, scene, w.hit_geom[i], quirks, s);
     }
     uint32_t* c1[1] = {&w.gcnt[G_QUEUE]};
     bool p1[1] = {g};
     uint32_t r1[1];
     block_append<1>(c1, p1, r1, lds);
-    if (g) w.gq[r1[0]] = s;
+    if (g) w.gq[r1[0]] = i;
   }
 }
 
@@ -497,15 +505,16 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
 // geometric normal, outgoing direction) are formed from the slot's ray and
 // hit, and the 40 used raw outputs go to the vertex's graw row.  SORTED: the
 // vertices are the guided material queues' entries in row order (entry i of
-// the concatenation is row i); otherwise the guide queue, row = slot.  A
-// wave takes 64 entries at a time, one per lane, so the chain of dependent
-// loads behind the inputs (queue -> slot -> ray and hit -> shape ->
-// triangle) is walked for 64 vertices at once; the MLP then runs as four
+// the concatenation is row i); otherwise the guide queue, row = slot.  Both
+// hold extend-queue positions, which index the hits.  A wave takes 64
+// entries at a time, one per lane, so the chain of dependent loads behind
+// the inputs (queue -> position -> slot and hit -> ray, shape -> triangle)
+// is walked for 64 vertices at once; the MLP then runs as four
 // 16-vertex tiles whose inputs are shuffled in from the lanes that gathered
 // them.  The weights sit in LDS once per block of kInferBlock threads.
 constexpr int kInferBlock = 512;
 template <bool SORTED>
-__global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, DevScene scene) {
+__global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, DevScene scene, int in) {
   __shared__ __align__(16) _Float16 sW[guide::kWeights];
   __shared__ __align__(16) _Float16 sB[guide::kBiases];
   guide::load_weights_lds_frag(sW, sB, w.net.w, w.net.b);
@@ -524,7 +533,7 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
   for (uint32_t base = (blockIdx.x * waves + wave) * 64; base < n; base += gridDim.x * waves * 64) {
     const uint32_t i = base + lane;
     float f9[guide::kFeat];
-    uint32_t s = 0;
+    uint32_t s = 0, pos = 0;
     if (i < n) {
       if (SORTED) {
         int k = 0;
@@ -533,14 +542,15 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
         for (int q = 0; q < kMatQueues - 1; q++)
           if (i >= end[q]) k = q + 1, j = i - end[q];
         const uint32_t* q = k == 0 ? w.mq[0] : k == 1 ? w.mq[1] : k == 2 ? w.mq[2] : k == 3 ? w.mq[3] : w.mq[4];
-        s = q[w.Q - 1 - j];
+        pos = q[w.Q - 1 - j];
       } else {
-        s = w.gq[i];
+        pos = w.gq[i];
       }
-      float4 ro = w.ray[2 * size_t(s)], rd = w.ray[2 * size_t(s) + 1], hh = w.hit[s];
+      s = w.queue[in][pos];
+      float4 ro = w.ray[2 * size_t(s)], rd = w.ray[2 * size_t(s) + 1], hh = w.hit[pos];
       f3 d = mk(rd.x, rd.y, rd.z);
       f3 p = d * hh.x + mk(ro.x, ro.y, ro.z);
-      f3 gn = guide::geometric_normal(scene, p, __float_as_uint(hh.w), w.hit_geom[s]);
+      f3 gn = guide::geometric_normal(scene, p, __float_as_uint(hh.w), w.hit_geom[pos]);
       f9[0] = p.x, f9[1] = p.y, f9[2] = p.z;
       f9[3] = gn.x, f9[4] = gn.y, f9[5] = gn.z;
       f9[6] = -d.x, f9[7] = -d.y, f9[8] = -d.z;
@@ -623,7 +633,7 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
         if (held) {
           cls[e] = kMatQueues;
         } else {
-          const uint32_t geom = w.hit_geom[s[e]];
+          const uint32_t geom = w.hit_geom[i];
           cls[e] = material_key(scene, geom);
           if (GUIDE && cls[e] != BSDF_ROUGH_GLASS && guided_hit(w, scene, geom, quirks, s[e])) cls[e] += kMatQueues + 1;
         }
@@ -643,7 +653,10 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
                       : k == 2        ? w.mq[2]
                       : k == 3        ? w.mq[3]
                                       : w.mq[4];
-        q[g ? w.Q - 1 - pos[e] : pos[e]] = s[e];
+        // material queues hold extend-queue positions (which index the hits);
+        // the next extend queue, slots
+        const uint32_t i = base + e * blockDim.x + threadIdx.x;
+        q[g ? w.Q - 1 - pos[e] : pos[e]] = k == kMatQueues ? s[e] : i;
       }
     }
   }
@@ -672,7 +685,13 @@ __global__ void k_queue_drain(Wavefront w, DevScene scene, int in, uint32_t* tal
     const uint32_t n = pass ? w.cnt[1 - in] : w.cnt[(g ? CNT_GMAT : CNT_MAT) + k];
     const uint32_t* q = pass ? w.queue[1 - in] : w.mq[k];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      const uint32_t s = g ? q[w.Q - 1 - i] : q[i];
+      // material entries are extend-queue positions, pass-through entries slots
+      const uint32_t e = g ? q[w.Q - 1 - i] : q[i];
+      if (pass ? e >= w.Q : e >= w.cnt[in]) {
+        bad++;
+        continue;
+      }
+      const uint32_t s = pass ? e : w.queue[in][e];
       if (s >= w.Q) {
         bad++;
         continue;
@@ -680,8 +699,8 @@ __global__ void k_queue_drain(Wavefront w, DevScene scene, int in, uint32_t* tal
       const bool held = w.susp && w.susp[s] != kNoSusp;
       if (pass) {
         if (!held) bad++;
-      } else if (held || material_key(scene, w.hit_geom[s]) != k ||
-                 g != (split && k != BSDF_ROUGH_GLASS && guided_hit(w, scene, w.hit_geom[s], quirks, s))) {
+      } else if (held || material_key(scene, w.hit_geom[e]) != k ||
+                 g != (split && k != BSDF_ROUGH_GLASS && guided_hit(w, scene, w.hit_geom[e], quirks, s))) {
         bad++;
       }
       atomicSub(&tally[s], 1u);
@@ -702,6 +721,7 @@ __global__ void k_queue_verify(Wavefront w, uint32_t* tally) {
 
 struct ShadowJob {
   static constexpr bool kSuspend = false;
+  static constexpr bool kTagIsIndex = false;
   const float4 *sh_o, *sh_d, *sh_c;
   SlotField<float4> lsum;
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
@@ -773,6 +793,7 @@ __global__ void k_final(Wavefront w, float* fb, float* sq) {
 
 struct ClosestJob {
   static constexpr bool kSuspend = false;
+  static constexpr bool kTagIsIndex = false;
   const float4* rays;
   float* hits;
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
@@ -796,6 +817,7 @@ struct ClosestJob {
 
 struct AnyJob {
   static constexpr bool kSuspend = false;
+  static constexpr bool kTagIsIndex = false;
   const float4* rays;
   uint8_t* occ;
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
@@ -925,11 +947,11 @@ void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const La
   hipLaunchKernelGGL(k_guide_prep, dim3(c.blocks), dim3(256), 0, s, w, scene, in);
 }
 
-void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, bool sorted, hipStream_t s) {
+void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int in, int blocks, bool sorted, hipStream_t s) {
   if (sorted)
-    hipLaunchKernelGGL(k_guide_infer_hits<true>, dim3(blocks), dim3(kInferBlock), 0, s, w, scene);
+    hipLaunchKernelGGL(k_guide_infer_hits<true>, dim3(blocks), dim3(kInferBlock), 0, s, w, scene, in);
   else
-    hipLaunchKernelGGL(k_guide_infer_hits<false>, dim3(blocks), dim3(kInferBlock), 0, s, w, scene);
+    hipLaunchKernelGGL(k_guide_infer_hits<false>, dim3(blocks), dim3(kInferBlock), 0, s, w, scene, in);
 }
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s) {
   hipLaunchKernelGGL(k_shadow, dim3(c.trace_blocks), dim3(kTraceBlock), 0, s, w, scene);

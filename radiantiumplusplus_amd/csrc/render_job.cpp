@@ -286,7 +286,7 @@ void RunJob(rad_job* job) {
         else if (!vol && guided)
           LaunchGuidePrep(w, scene, in, cfg, st);
         if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
-        if (!vol && guided) LaunchGuideInfer(w, scene, inferBlocks, job->sorted, st);
+        if (!vol && guided) LaunchGuideInfer(w, scene, in, inferBlocks, job->sorted, st);
         if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
         if (vol && job->sorted)
           LaunchShadeVolMaterials(w, scene, in, job->mat_mask, cfg, st);

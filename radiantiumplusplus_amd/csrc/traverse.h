@@ -507,6 +507,8 @@ __device__ __forceinline__ bool trace_step(const DevScene& s, TraceRay& r, Trace
 // each wave takes the 64 items at its own position once).
 // Job: bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag)
 //      void finish(uint32_t i, uint32_t tag, const TraceRay& r)
+//      static constexpr bool kTagIsIndex; with it the tag is the item index
+//      i itself, and the stream keeps one register for both
 //      static constexpr bool kSuspend; with it, a ray that has visited
 //      job.budget nodes in this launch is offered to the job, which may keep
 //      it for a later launch (per-ray costs are heavy-tailed: near-grazing
@@ -583,7 +585,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
           f3 o, d;
           float tmin, tmax;
           if (job.load(i, o, d, tmin, tmax, tag)) {
-            idx = i;
+            if constexpr (!Job::kTagIsIndex) idx = i;
             trace_begin(r, st, o, d, tmin, tmax);
             if constexpr (Job::kSuspend) job.resume(tag, r, st);
             visits = 0;
@@ -600,7 +602,7 @@ __device__ __forceinline__ void trace_stream(const DevScene& s, uint32_t n, uint
       if (active) {
         if (trace_step<ANY, true>(s, r, st, pool)) {
           if (!ANY) trace_finish(s, r);
-          job.finish(idx, tag, r);
+          job.finish(Job::kTagIsIndex ? tag : idx, tag, r);
           active = false;
         } else if constexpr (Job::kSuspend) {
           if (++visits >= job.budget) {

@@ -53,12 +53,14 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
     if (i < n) {
-      s = MAT >= 0 ? w.mq[MAT][i] : w.queue[in][i];
+      // material queues hold extend-queue positions, which index the hits
+      const uint32_t p = MAT >= 0 ? w.mq[MAT][i] : i;
+      s = w.queue[in][p];
       float4 ro4 = w.ray[2 * size_t(s)], rd4 = w.ray[2 * size_t(s) + 1];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float tmin = ro4.w, tmax = rd4.w;
-      float4 hh = w.hit[s];
-      uint32_t geom = w.hit_geom[s];
+      float4 hh = w.hit[p];
+      uint32_t geom = w.hit_geom[p];
       float4 t4 = w.thr[s];
       f3 T = mk(t4.x, t4.y, t4.z);
       float eta = t4.w;

@@ -80,6 +80,8 @@ struct Wavefront {
   SlotField<uint4> meta;   // pixel, sample index (SAMPLE_NONE = idle), depth, rng dimension
   uint32_t* key;   // rng key of the sample in flight
   uint32_t* medium;  // vol_path: MED_UNSET, or current medium + 1 (0 = none) | channel << 24
+  // hit of the ray at position i of the extend queue (not slot-indexed, so
+  // extend writes and classify reads consecutive entries)
   float4* hit;     // t, u, v, prim (bits)
   uint32_t* hit_geom;
   uint32_t* queue[2];
@@ -90,8 +92,8 @@ struct Wavefront {
   uint32_t* susp;
   ResumeState* resume[2];
   uint32_t resume_cap, visit_budget;
-  // The extend queue partitioned by material (k_classify); [4]: every other
-  // type.  With guiding or training records on, entry j of a material's
+  // The extend queue partitioned by material (k_classify), as extend-queue
+  // positions; [4]: every other type.  With guiding or training records on, entry j of a material's
   // guided vertices is mq[k][Q - 1 - j] (count CNT_GMAT + k): both parts
   // together never exceed the Q slots.
   uint32_t* mq[5];
@@ -105,7 +107,7 @@ struct Wavefront {
   uint32_t* cnt;   // CNT_* words
   unsigned long long* stats;  // ST_* words
   // learned guiding (null when off)
-  uint32_t* gq;      // unsorted shading: slots whose vertex is guided this iteration
+  uint32_t* gq;      // unsorted shading: extend-queue positions of this iteration's guided vertices
   // 48 raw network outputs (40 used) per guided vertex.  Sorted shading: row
   // guided_row(cnt, k, j) for entry j of material k's guided queue, so
   // inference writes and shading reads consecutive rows; unsorted: row = slot.
@@ -262,7 +264,7 @@ void LaunchShadeMaterials(const Wavefront& w, const DevScene& scene, int in, uin
                           hipStream_t s);
 void LaunchShadow(const Wavefront& w, const DevScene& scene, const LaunchCfg& c, hipStream_t s);
 void LaunchGuidePrep(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
-void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int blocks, bool sorted, hipStream_t s);
+void LaunchGuideInfer(const Wavefront& w, const DevScene& scene, int in, int blocks, bool sorted, hipStream_t s);
 void LaunchAccum(const Wavefront& w, hipStream_t s);
 // vol_path (defined in volpath_kernels.hip).
 void LaunchShadeVol(const Wavefront& w, const DevScene& scene, int in, const LaunchCfg& c, hipStream_t s);
