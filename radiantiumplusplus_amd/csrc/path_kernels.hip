@@ -124,13 +124,6 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
         const uint32_t sl = s[e];
         const uint32_t pixel = id[e] % w.npix, sample = w.row0 + id[e] / w.npix;
         const uint32_t key = sample_key(w.seed, w.request, pixel, sample);
-        const uint32_t px = pixel % W, py = pixel / W;
-        const float jx = rng_u(key, 0), jy = rng_u(key, 1);
-        f3 o, d;
-        float tmin, tmax;
-        camera_ray(scene.cam, float(px) + jx, float(py) + jy, o, d, tmin, tmax);
-        w.ray[2 * size_t(sl)] = make_float4(o.x, o.y, o.z, tmin);
-        w.ray[2 * size_t(sl) + 1] = make_float4(d.x, d.y, d.z, tmax);
         w.thr[sl] = make_float4(1.f, 1.f, 1.f, 1.f);
         w.lsum[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
         w.prev[sl] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
@@ -147,8 +140,20 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
       uint32_t pos[E], unused[E];
       block_append_n<1, E, false>(c1, qcls, none, pos, unused, lds);
 #pragma unroll
-      for (int e = 0; e < E; e++)
-        if (qcls[e] >= 0) w.queue[in][pos[e]] = s[e];
+      for (int e = 0; e < E; e++) {
+        if (qcls[e] < 0) continue;
+        w.queue[in][pos[e]] = s[e];
+        // the camera ray, stored at the slot's queue position
+        const uint32_t pixel = id[e] % w.npix, sample = w.row0 + id[e] / w.npix;
+        const uint32_t key = sample_key(w.seed, w.request, pixel, sample);
+        const uint32_t px = pixel % W, py = pixel / W;
+        const float jx = rng_u(key, 0), jy = rng_u(key, 1);
+        f3 o, d;
+        float tmin, tmax;
+        camera_ray(scene.cam, float(px) + jx, float(py) + jy, o, d, tmin, tmax);
+        w.ray[in][2 * size_t(pos[e])] = make_float4(o.x, o.y, o.z, tmin);
+        w.ray[in][2 * size_t(pos[e]) + 1] = make_float4(d.x, d.y, d.z, tmax);
+      }
     }
     block_count(&w.stats[ST_SAMPLES], finished, lds);
   }
@@ -196,8 +201,7 @@ struct ExtendJob {
     return true;
   }
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
-    uint32_t s = queue[i];
-    float4 a = ray[2 * size_t(s)], b = ray[2 * size_t(s) + 1];
+    float4 a = ray[2 * size_t(i)], b = ray[2 * size_t(i) + 1];
     o = mk(a.x, a.y, a.z);
     d = mk(b.x, b.y, b.z);
     tmin = a.w;
@@ -218,7 +222,7 @@ struct ExtendJob {
 // the pooled primitive tests need registers; per 1080p large-mesh frame
 // 78.2 ms at 6 waves, 84.8 ms at 5.
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))) k_extend(Wavefront w, DevScene scene, int in) {
-  ExtendJob job{w.queue[in], w.ray, w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
+  ExtendJob job{w.queue[in], w.ray[in], w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
                 &w.cnt[CNT_SUSP + 1 - in], w.resume_cap, w.susp ? w.visit_budget : 0xffffffffu};
   trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
 }
@@ -260,8 +264,9 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool alive = false, emit = false, want_rec = false, done = false;
-    uint32_t s = 0, p = 0;  // slot, and its position in the extend queue (which indexes the hits)
+    uint32_t s = 0, p = 0;  // slot, and its position in the extend queue (which indexes hits and rays)
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
+    float4 nr0 = make_float4(0, 0, 0, 0), nr1 = make_float4(0, 0, 0, 0);  // the continuing ray
     guide::TrainRecord rec;
     if (i < n) {
       p = MAT < 0 ? i : kSplit ? queue[w.Q - 1 - i] : queue[i];
@@ -270,9 +275,13 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     // a traversal suspended by this iteration's extend continues next iteration
     // (the material queues never hold one)
     const bool held = MAT < 0 && i < n && w.susp && w.susp[s] != kNoSusp;
-    if (held) alive = true;
+    if (held) {
+      alive = true;
+      nr0 = w.ray[in][2 * size_t(p)];
+      nr1 = w.ray[in][2 * size_t(p) + 1];
+    }
     if (i < n && !held) {
-      float4 ro4 = w.ray[2 * size_t(s)], rd4 = w.ray[2 * size_t(s) + 1];
+      float4 ro4 = w.ray[in][2 * size_t(p)], rd4 = w.ray[in][2 * size_t(p) + 1];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float4 hh = w.hit[p];
       uint32_t geom = w.hit_geom[p];
@@ -414,8 +423,8 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
             T = mul(T, f) / pdf;
             eta *= bs.eta;
             f3 no = offset_p(si.p, si.n, wo);
-            w.ray[2 * size_t(s)] = make_float4(no.x, no.y, no.z, 0.f);
-            w.ray[2 * size_t(s) + 1] = make_float4(wo.x, wo.y, wo.z, kFloatMax);
+            nr0 = make_float4(no.x, no.y, no.z, 0.f);
+            nr1 = make_float4(wo.x, wo.y, wo.z, kFloatMax);
             prevPdf = pdf;
             prevP = si.p;
             spec = (bs.lobes & LOBE_DELTA) != 0;
@@ -463,7 +472,11 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     bool p3[3] = {valid && alive, valid && !alive, emit};
     uint32_t r3[3];
     block_append<3>(c3, p3, r3, lds);
-    if (valid && alive) w.queue[out][r3[0]] = s;
+    if (valid && alive) {
+      w.queue[out][r3[0]] = s;
+      w.ray[out][2 * size_t(r3[0])] = nr0;
+      w.ray[out][2 * size_t(r3[0]) + 1] = nr1;
+    }
     if (valid && !alive) w.dead[r3[1]] = s | (done ? kDeadDone : 0u);
     if (emit) {
       w.sh_o[r3[2]] = so;
@@ -546,8 +559,8 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
       } else {
         pos = w.gq[i];
       }
-      s = w.queue[in][pos];
-      float4 ro = w.ray[2 * size_t(s)], rd = w.ray[2 * size_t(s) + 1], hh = w.hit[pos];
+      if (!SORTED) s = w.queue[in][pos];  // the unsorted rows are slots
+      float4 ro = w.ray[in][2 * size_t(pos)], rd = w.ray[in][2 * size_t(pos) + 1], hh = w.hit[pos];
       f3 d = mk(rd.x, rd.y, rd.z);
       f3 p = d * hh.x + mk(ro.x, ro.y, ro.z);
       f3 gn = guide::geometric_normal(scene, p, __float_as_uint(hh.w), w.hit_geom[pos]);
@@ -654,9 +667,14 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
                       : k == 3        ? w.mq[3]
                                       : w.mq[4];
         // material queues hold extend-queue positions (which index the hits);
-        // the next extend queue, slots
+        // the next extend queue, slots, and a passed-through ray moves to its
+        // new position there
         const uint32_t i = base + e * blockDim.x + threadIdx.x;
         q[g ? w.Q - 1 - pos[e] : pos[e]] = k == kMatQueues ? s[e] : i;
+        if (k == kMatQueues) {
+          w.ray[1 - in][2 * size_t(pos[e])] = w.ray[in][2 * size_t(i)];
+          w.ray[1 - in][2 * size_t(pos[e]) + 1] = w.ray[in][2 * size_t(i) + 1];
+        }
       }
     }
   }

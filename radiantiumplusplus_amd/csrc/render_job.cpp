@@ -149,9 +149,10 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
   j.resume_cap = desc.integrator == RAD_INTEGRATOR_PATH && VisitBudget() > 0 ? uint32_t(std::min<uint64_t>(Q, kResumeCap)) : 0;
-  // per slot: ray (32 B), path record (64 B), hit, shadow origin / direction
-  // / contribution (16 B each); key, medium, hit geometry and queues (4 B)
-  j.bytes = Align(Q * 32) + Align(Q * 64) + Align(Q * 16) * 4 + Align(Q * 4) * (6 + kMatQueues) +
+  // per slot: rays of the two extend queues (32 B each), path record (64 B),
+  // hit, shadow origin / direction / contribution (16 B each); key, medium,
+  // hit geometry and queues (4 B)
+  j.bytes = Align(Q * 32) * 2 + Align(Q * 64) + Align(Q * 16) * 4 + Align(Q * 4) * (6 + kMatQueues) +
             Align(pass * 16) + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
   if (j.resume_cap) j.bytes += Align(Q * 4) + 2 * Align(uint64_t(j.resume_cap) * sizeof(ResumeState));
@@ -449,7 +450,8 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   };
   Wavefront& w = job->w;
   const size_t f4 = size_t(Q) * 16, u4 = size_t(Q) * 4;
-  w.ray = (float4*)take(2 * f4);
+  w.ray[0] = (float4*)take(2 * f4);
+  w.ray[1] = (float4*)take(2 * f4);
   float4* rec = (float4*)take(4 * f4);
   w.thr.p = rec;
   w.lsum.p = rec + 1;

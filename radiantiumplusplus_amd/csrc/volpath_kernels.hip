@@ -52,11 +52,12 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     bool alive = false, emit = false, msample = false, done = false;
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
+    float4 nr0 = make_float4(0, 0, 0, 0), nr1 = make_float4(0, 0, 0, 0);  // the continuing ray
     if (i < n) {
-      // material queues hold extend-queue positions, which index the hits
+      // material queues hold extend-queue positions, which index hits and rays
       const uint32_t p = MAT >= 0 ? w.mq[MAT][i] : i;
       s = w.queue[in][p];
-      float4 ro4 = w.ray[2 * size_t(s)], rd4 = w.ray[2 * size_t(s) + 1];
+      float4 ro4 = w.ray[in][2 * size_t(p)], rd4 = w.ray[in][2 * size_t(p) + 1];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float tmin = ro4.w, tmax = rd4.w;
       float4 hh = w.hit[p];
@@ -244,8 +245,8 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
         }
       }
       if (alive) {
-        w.ray[2 * size_t(s)] = make_float4(ro.x, ro.y, ro.z, tmin);
-        w.ray[2 * size_t(s) + 1] = make_float4(rd.x, rd.y, rd.z, tmax);
+        nr0 = make_float4(ro.x, ro.y, ro.z, tmin);
+        nr1 = make_float4(rd.x, rd.y, rd.z, tmax);
         w.thr[s] = make_float4(T.x, T.y, T.z, eta);
         w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
         w.meta[s] = make_uint4(m.x, m.y, next_depth, dim);
@@ -265,7 +266,11 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     bool p3[3] = {valid && alive, valid && !alive, emit};
     uint32_t r3[3];
     block_append<3>(c3, p3, r3, lds);
-    if (valid && alive) w.queue[out][r3[0]] = s;
+    if (valid && alive) {
+      w.queue[out][r3[0]] = s;
+      w.ray[out][2 * size_t(r3[0])] = nr0;
+      w.ray[out][2 * size_t(r3[0]) + 1] = nr1;
+    }
     if (valid && !alive) w.dead[r3[1]] = s | (done ? kDeadDone : 0u);
     if (emit) {
       w.sh_o[r3[2]] = so;

@@ -70,9 +70,12 @@ struct SlotField {
 };
 
 struct Wavefront {
-  // ray of slot s: ray[2 s] = origin, tmin; ray[2 s + 1] = direction, tmax
-  // (one 32-byte record, so a ray is one memory sector, not two)
-  float4* ray;
+  // Ray at position i of extend queue q: ray[q][2 i] = origin, tmin;
+  // ray[q][2 i + 1] = direction, tmax (one 32-byte record, so a ray is one
+  // memory sector, not two).  Stored by queue position, not by slot: regen
+  // and shading write the rays they append in block-contiguous runs, and
+  // extend, shading and inference read them in queue order.
+  float4* ray[2];
   // the path record of slot s: four fields in one 64-byte record
   SlotField<float4> thr;   // throughput, eta
   SlotField<float4> lsum;  // radiance of the sample in flight, previous BSDF pdf
