@@ -19,8 +19,8 @@ __global__ void k_init_pass(Wavefront w, int first) {
     w.pacc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
     w.pacc2[s] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  if (s < w.Q) {
-    w.dead[s] = s;
+  if (s < w.Q) w.dead[s] = s;  // read only as a count: the first regen of a pass is told so
+  if (s < 2 * w.Q) {
     if (w.glast) w.glast[s] = w.gpend[s] = 0xffffffffu;
     if (w.susp) w.susp[s] = kNoSusp;
   }
@@ -64,10 +64,12 @@ __device__ __forceinline__ void finish_records(const Wavefront& w, uint32_t s, f
   w.glast[s] = w.gpend[s] = 0xffffffffu;
 }
 
-// Store the sample a dead slot finished into the result entries and start the
-// next camera sample of the pass on it (sample_renderer.cpp:97-99).  Each
-// thread takes kRegenPerThread dead slots per block iteration (entries
-// base + e * 256 + thread), so one set of block atomics covers them all.
+// Store the samples that ended last iteration into the result entries and
+// start one new camera sample of the pass for each (sample_renderer.cpp:97-99),
+// appended to this iteration's extend queue with its state at its position
+// there.  Each thread takes kRegenPerThread dead-list entries per block
+// iteration (entries base + e * 256 + thread), so one set of block atomics
+// covers them all.
 constexpr int kRegenPerThread = 4;
 
 // fresh: the first regen of a pass, whose dead slots hold no finished sample.
@@ -120,20 +122,6 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
     for (int e = 0; e < E; e++) {
       const bool start = cls[e] >= 0 && id[e] < pass;
       qcls[e] = start ? 0 : -1;
-      if (start) {
-        const uint32_t sl = s[e];
-        const uint32_t pixel = id[e] % w.npix, sample = w.row0 + id[e] / w.npix;
-        const uint32_t key = sample_key(w.seed, w.request, pixel, sample);
-        w.thr[sl] = make_float4(1.f, 1.f, 1.f, 1.f);
-        w.lsum[sl] = make_float4(0.f, 0.f, 0.f, 1.f);
-        w.prev[sl] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
-        w.meta[sl] = make_uint4(pixel, sample, 0u, 2u);
-        w.key[sl] = key;
-        if (w.flags & WF_RECORD) w.glast[sl] = w.gpend[sl] = 0xffffffffu;
-        if (w.medium) w.medium[sl] = 0xffffffffu;  // vol_path: medium / channel chosen at shade time
-      } else if (cls[e] >= 0) {
-        w.meta[s[e]] = make_uint4(0u, SAMPLE_NONE, 0u, 0u);  // idle for the rest of the pass
-      }
     }
     {
       uint32_t* c1[1] = {&w.cnt[in]};
@@ -142,10 +130,18 @@ __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int 
 #pragma unroll
       for (int e = 0; e < E; e++) {
         if (qcls[e] < 0) continue;
-        w.queue[in][pos[e]] = s[e];
-        // the camera ray, stored at the slot's queue position
+        // the new path's state at its queue position
+        const uint32_t st = state_index(w.Q, in, pos[e]);
         const uint32_t pixel = id[e] % w.npix, sample = w.row0 + id[e] / w.npix;
         const uint32_t key = sample_key(w.seed, w.request, pixel, sample);
+        w.thr[st] = make_float4(1.f, 1.f, 1.f, 1.f);
+        w.lsum[st] = make_float4(0.f, 0.f, 0.f, 1.f);
+        w.prev[st] = make_float4(0.f, 0.f, 0.f, __uint_as_float(PATH_SPECULAR));
+        w.meta[st] = make_uint4(pixel, sample, 0u, 2u);
+        w.key[st] = key;
+        if (w.flags & WF_RECORD) w.glast[st] = w.gpend[st] = 0xffffffffu;
+        if (w.medium) w.medium[st] = 0xffffffffu;  // vol_path: medium / channel chosen at shade time
+        if (w.susp) w.susp[st] = kNoSusp;
         const uint32_t px = pixel % W, py = pixel / W;
         const float jx = rng_u(key, 0), jy = rng_u(key, 1);
         f3 o, d;
@@ -172,11 +168,11 @@ __global__ void k_reset(uint32_t* cnt, uint32_t* gcnt, int out) {
   }
 }
 
-// The tag of an extend ray is its queue position; its slot is queue[tag].
+// The tag of an extend ray is its queue position; its state index is base + tag.
 struct ExtendJob {
   static constexpr bool kSuspend = true;
   static constexpr bool kTagIsIndex = true;
-  const uint32_t* queue;
+  uint32_t base;
   const float4* ray;
   float4* hit;
   uint32_t* hit_geom;
@@ -187,7 +183,7 @@ struct ExtendJob {
   uint32_t cap, budget;
   __device__ __forceinline__ void resume(uint32_t i, TraceRay& r, TraceStack& st) {
     if (!susp) return;
-    const uint32_t s = queue[i];
+    const uint32_t s = base + i;
     const uint32_t j = susp[s];
     if (j == kNoSusp) return;
     restore_trace(rin[j], r, st);
@@ -197,7 +193,7 @@ struct ExtendJob {
     const uint32_t j = atomicAdd(rcount, 1u);
     if (j >= cap) return false;  // pool full: the ray finishes in this launch
     save_trace(rout[j], r, st);
-    susp[queue[i]] = j;
+    susp[base + i] = j;
     return true;
   }
   __device__ __forceinline__ bool load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& tag) {
@@ -222,7 +218,7 @@ struct ExtendJob {
 // the pooled primitive tests need registers; per 1080p large-mesh frame
 // 78.2 ms at 6 waves, 84.8 ms at 5.
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))) k_extend(Wavefront w, DevScene scene, int in) {
-  ExtendJob job{w.queue[in], w.ray[in], w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
+  ExtendJob job{state_index(w.Q, in, 0), w.ray[in], w.hit, w.hit_geom, w.susp, w.resume[in], w.resume[1 - in],
                 &w.cnt[CNT_SUSP + 1 - in], w.resume_cap, w.susp ? w.visit_budget : 0xffffffffu};
   trace_stream<false>(scene, w.cnt[in], &w.cnt[CNT_FETCH_EXT], job);
 }
@@ -254,7 +250,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   __shared__ float2 gpw[GUIDE ? guide::kLobes * 256 : 1];
   constexpr bool kSplit = GUIDE && MAT >= 0;  // a guided material queue, read from the back
   const uint32_t n = MAT >= 0 ? w.cnt[(GUIDE ? CNT_GMAT : CNT_MAT) + MAT] : w.cnt[in];
-  const uint32_t* const queue = MAT >= 0 ? w.mq[MAT] : w.queue[in];
+  const uint32_t* const queue = MAT >= 0 ? w.mq[MAT >= 0 ? MAT : 0] : nullptr;
   const uint32_t stride = gridDim.x * blockDim.x;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
   const bool guideOn = GUIDE && (w.flags & WF_GUIDE) != 0;
@@ -264,21 +260,32 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     uint32_t i = base + threadIdx.x;
     bool alive = false, emit = false, want_rec = false, done = false;
-    uint32_t s = 0, p = 0;  // slot, and its position in the extend queue (which indexes hits and rays)
+    uint32_t s = 0, p = 0;  // state index, and position in the extend queue (which indexes hits and rays)
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
-    float4 nr0 = make_float4(0, 0, 0, 0), nr1 = make_float4(0, 0, 0, 0);  // the continuing ray
+    // the continuing path's ray and state, written at its next-queue position
+    float4 nr0 = make_float4(0, 0, 0, 0), nr1 = make_float4(0, 0, 0, 0);
+    float4 nthr = nr0, nlsum = nr0, nprev = nr0;
+    uint4 nmeta = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t nkey = 0, nsusp = kNoSusp, glast = 0xffffffffu, gpend = 0xffffffffu;
     guide::TrainRecord rec;
     if (i < n) {
       p = MAT < 0 ? i : kSplit ? queue[w.Q - 1 - i] : queue[i];
-      s = w.queue[in][p];
+      s = state_index(w.Q, in, p);
     }
     // a traversal suspended by this iteration's extend continues next iteration
-    // (the material queues never hold one)
+    // (the material queues never hold one): its state moves to the next queue
     const bool held = MAT < 0 && i < n && w.susp && w.susp[s] != kNoSusp;
     if (held) {
       alive = true;
       nr0 = w.ray[in][2 * size_t(p)];
       nr1 = w.ray[in][2 * size_t(p) + 1];
+      nthr = w.thr[s];
+      nlsum = w.lsum[s];
+      nprev = w.prev[s];
+      nmeta = w.meta[s];
+      nkey = w.key[s];
+      nsusp = w.susp[s];
+      if (w.glast) glast = w.glast[s], gpend = w.gpend[s];
     }
     if (i < n && !held) {
       float4 ro4 = w.ray[in][2 * size_t(p)], rd4 = w.ray[in][2 * size_t(p) + 1];
@@ -299,6 +306,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       uint32_t depth = m.z, dim = m.w;
       uint32_t key = w.key[s];
       if (record) {  // the previous vertex's record: radiance gathered before its bounce
+        glast = w.glast[s];
         uint32_t pr = w.gpend[s];
         if (pr != 0xffffffffu) {
           w.recs[pr].lbase[0] = L.x;
@@ -351,7 +359,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
           const bool guided = kSplit || (GUIDE && guided_vertex(w, scene, geom, depth, quirks));
           const guide::Mixture mx = guide::lane_mixture(gkm, gpw, threadIdx.x, 256);
           if (guided && guideOn)
-            guide::decode(mx, w.graw + size_t(kSplit ? guided_row(w.cnt, MAT, i) : s) * guide::kOut, si.p, mk(w.net.center[0], w.net.center[1], w.net.center[2]),
+            guide::decode(mx, w.graw + size_t(kSplit ? guided_row(w.cnt, MAT, i) : p) * guide::kOut, si.p, mk(w.net.center[0], w.net.center[1], w.net.center[2]),
                           mk(w.net.half[0], w.net.half[1], w.net.half[2]));
           const float a = (guided && guideOn) ? alpha : 0.f;
           if (b.lobes & LOBE_NO_DELTA) {  // NEE (path.cpp:173-191)
@@ -446,14 +454,17 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       // and regen reads nothing for the slot.  Of an ended sample regen reads
       // only the radiance and the pixel and sample, which stay as they are.
       if (alive) {
-        w.thr[s] = make_float4(T.x, T.y, T.z, eta);
-        w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
-        w.meta[s] = make_uint4(m.x, m.y, depth + 1, dim);
+        nthr = make_float4(T.x, T.y, T.z, eta);
+        nlsum = make_float4(L.x, L.y, L.z, prevPdf);
+        nprev = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
+        nmeta = make_uint4(m.x, m.y, depth + 1, dim);
+        nkey = key;
       } else if (!emit && !record) {
         w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(L.x, L.y, L.z, 0.f);
         done = true;
+      } else {
+        w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);  // in place: the shadow ray and regen finish it
       }
-      if (!done) w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
     }
     bool valid = i < n;
     if (GUIDE && record) {
@@ -464,8 +475,8 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       const uint32_t r = r1[0];
       if (want_rec && r < w.rec_cap) {
         w.recs[r] = rec;
-        w.glast[s] = r;
-        w.gpend[s] = r;  // baseline filled by the next bounce (or the path's end)
+        glast = gpend = r;  // baseline filled by the next bounce (or the path's end)
+        if (!alive) w.glast[s] = w.gpend[s] = r;
       }
     }
     uint32_t* c3[3] = {&w.cnt[out], &w.cnt[CNT_DEAD], &w.cnt[CNT_SHADOW]};
@@ -473,9 +484,17 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
     uint32_t r3[3];
     block_append<3>(c3, p3, r3, lds);
     if (valid && alive) {
-      w.queue[out][r3[0]] = s;
+      const uint32_t d = state_index(w.Q, out, r3[0]);
       w.ray[out][2 * size_t(r3[0])] = nr0;
       w.ray[out][2 * size_t(r3[0]) + 1] = nr1;
+      w.thr[d] = nthr;
+      w.lsum[d] = nlsum;
+      w.prev[d] = nprev;
+      w.meta[d] = nmeta;
+      w.key[d] = nkey;
+      if (w.susp) w.susp[d] = nsusp;
+      if (record || (held && w.glast)) w.glast[d] = glast, w.gpend[d] = gpend;
+      sdir.w = __uint_as_float(d);  // this vertex's shadow ray adds to the moved radiance
     }
     if (valid && !alive) w.dead[r3[1]] = s | (done ? kDeadDone : 0u);
     if (emit) {
@@ -503,7 +522,7 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
     bool g = false;
     uint32_t s = 0;
     if (i < n) {
-      s = w.queue[in][i];
+      s = state_index(w.Q, in, i);
       g = !(w.susp && w.susp[s] != kNoSusp) && guided_hit(w, scene, w.hit_geom[i], quirks, s);
     }
     uint32_t* c1[1] = {&w.gcnt[G_QUEUE]};
@@ -518,10 +537,10 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
 // geometric normal, outgoing direction) are formed from the slot's ray and
 // hit, and the 40 used raw outputs go to the vertex's graw row.  SORTED: the
 // vertices are the guided material queues' entries in row order (entry i of
-// the concatenation is row i); otherwise the guide queue, row = slot.  Both
-// hold extend-queue positions, which index the hits.  A wave takes 64
+// the concatenation is row i); otherwise the guide queue, row = position.
+// Both hold extend-queue positions, which index the rays and hits.  A wave takes 64
 // entries at a time, one per lane, so the chain of dependent loads behind
-// the inputs (queue -> position -> slot and hit -> ray, shape -> triangle)
+// the inputs (queue -> position -> ray and hit -> shape -> triangle)
 // is walked for 64 vertices at once; the MLP then runs as four
 // 16-vertex tiles whose inputs are shuffled in from the lanes that gathered
 // them.  The weights sit in LDS once per block of kInferBlock threads.
@@ -546,7 +565,7 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
   for (uint32_t base = (blockIdx.x * waves + wave) * 64; base < n; base += gridDim.x * waves * 64) {
     const uint32_t i = base + lane;
     float f9[guide::kFeat];
-    uint32_t s = 0, pos = 0;
+    uint32_t pos = 0;
     if (i < n) {
       if (SORTED) {
         int k = 0;
@@ -559,7 +578,6 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
       } else {
         pos = w.gq[i];
       }
-      if (!SORTED) s = w.queue[in][pos];  // the unsorted rows are slots
       float4 ro = w.ray[in][2 * size_t(pos)], rd = w.ray[in][2 * size_t(pos) + 1], hh = w.hit[pos];
       f3 d = mk(rd.x, rd.y, rd.z);
       f3 p = d * hh.x + mk(ro.x, ro.y, ro.z);
@@ -577,7 +595,7 @@ __global__ void __launch_bounds__(kInferBlock) k_guide_infer_hits(Wavefront w, D
       guide::FeatScalars fs;
 #pragma unroll
       for (int k = 0; k < 7; k++) fs.s[k] = __shfl(mine.s[k], src);
-      const uint32_t row = SORTED ? base + src : __shfl(s, src);
+      const uint32_t row = SORTED ? base + src : __shfl(pos, src);
       guide::h4 out[3];
       guide::infer_tile(sW, sB, fs, out);
       if (base + src < n) {
@@ -640,7 +658,7 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
       s[e] = 0;
       cls[e] = -1;
       if (i < n) {
-        s[e] = w.queue[in][i];
+        s[e] = state_index(w.Q, in, i);
         // traversal suspended: straight to the next extend queue
         const bool held = w.susp && w.susp[s[e]] != kNoSusp;
         if (held) {
@@ -660,20 +678,25 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
       if (c >= 0) {
         const bool g = GUIDE && c > kMatQueues;  // guided: from the back of the material's queue
         const int k = g ? c - kMatQueues - 1 : c;
-        uint32_t* q = k == kMatQueues ? w.queue[1 - in]
-                      : k == 0        ? w.mq[0]
-                      : k == 1        ? w.mq[1]
-                      : k == 2        ? w.mq[2]
-                      : k == 3        ? w.mq[3]
-                                      : w.mq[4];
-        // material queues hold extend-queue positions (which index the hits);
-        // the next extend queue, slots, and a passed-through ray moves to its
-        // new position there
+        uint32_t* q = k == 0 ? w.mq[0] : k == 1 ? w.mq[1] : k == 2 ? w.mq[2] : k == 3 ? w.mq[3] : w.mq[4];
+        // material queues hold extend-queue positions (which index rays,
+        // hits and, with the queue, path state); a passed-through path's ray
+        // and state move to its position in the next extend queue
         const uint32_t i = base + e * blockDim.x + threadIdx.x;
-        q[g ? w.Q - 1 - pos[e] : pos[e]] = k == kMatQueues ? s[e] : i;
         if (k == kMatQueues) {
+          const uint32_t d = state_index(w.Q, 1 - in, pos[e]), src = s[e];
           w.ray[1 - in][2 * size_t(pos[e])] = w.ray[in][2 * size_t(i)];
           w.ray[1 - in][2 * size_t(pos[e]) + 1] = w.ray[in][2 * size_t(i) + 1];
+          w.thr[d] = w.thr[src];
+          w.lsum[d] = w.lsum[src];
+          w.prev[d] = w.prev[src];
+          w.meta[d] = w.meta[src];
+          w.key[d] = w.key[src];
+          w.susp[d] = w.susp[src];
+          if (w.glast) w.glast[d] = w.glast[src], w.gpend[d] = w.gpend[src];
+          if (w.passsrc) w.passsrc[pos[e]] = i;
+        } else {
+          q[g ? w.Q - 1 - pos[e] : pos[e]] = i;
         }
       }
     }
@@ -681,14 +704,15 @@ __global__ void __launch_bounds__(256) k_classify(Wavefront w, DevScene scene, i
 }
 
 // Queue check (RAD_RENDER_CHECK_QUEUES).  The extension queue adds `add` to
-// each of its slots' tally words, every material queue and the pass-through
-// entries of the next extension queue subtract 1, and any word left nonzero
-// is a slot lost, duplicated or misplaced; an entry in the wrong material
-// queue, or passed through without a suspended traversal, counts as well.
+// the tally word of each of its positions, every material queue and the
+// pass-through entries of the next extension queue subtract 1, and any word
+// left nonzero is a path lost, duplicated or misplaced; an entry in the wrong
+// material queue, or passed through without a suspended traversal, counts as
+// well.
 __global__ void k_queue_tally(Wavefront w, int in, uint32_t* tally, uint32_t add) {
   const uint32_t n = w.cnt[in];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    atomicAdd(&tally[w.queue[in][i]], add);
+    atomicAdd(&tally[i], add);
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&w.stats[ST_QCHECK], (unsigned long long)n);
 }
 __global__ void k_queue_drain(Wavefront w, DevScene scene, int in, uint32_t* tally) {
@@ -697,23 +721,20 @@ __global__ void k_queue_drain(Wavefront w, DevScene scene, int in, uint32_t* tal
   const bool split = (w.flags & (WF_GUIDE | WF_RECORD)) != 0;
   const bool quirks = (w.flags & WF_QUIRKS) != 0;
   for (int m = 0; m <= 2 * kMatQueues; m++) {
-    const bool pass = m == kMatQueues;  // the suspended slots classify passed through
+    const bool pass = m == kMatQueues;  // the suspended paths classify passed through
     const bool g = m > kMatQueues;      // a material's guided vertices, from the back of its queue
     const int k = g ? m - kMatQueues - 1 : m;
     const uint32_t n = pass ? w.cnt[1 - in] : w.cnt[(g ? CNT_GMAT : CNT_MAT) + k];
-    const uint32_t* q = pass ? w.queue[1 - in] : w.mq[k];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      // material entries are extend-queue positions, pass-through entries slots
-      const uint32_t e = g ? q[w.Q - 1 - i] : q[i];
-      if (pass ? e >= w.Q : e >= w.cnt[in]) {
+      // the extend-queue position each entry stands for: a material entry
+      // holds it; a passed-through path sits at position i of the next queue
+      // and classify recorded where it came from
+      const uint32_t e = pass ? w.passsrc[i] : g ? w.mq[k][w.Q - 1 - i] : w.mq[k][i];
+      if (e >= w.cnt[in]) {
         bad++;
         continue;
       }
-      const uint32_t s = pass ? e : w.queue[in][e];
-      if (s >= w.Q) {
-        bad++;
-        continue;
-      }
+      const uint32_t s = pass ? state_index(w.Q, 1 - in, i) : state_index(w.Q, in, e);
       const bool held = w.susp && w.susp[s] != kNoSusp;
       if (pass) {
         if (!held) bad++;
@@ -721,17 +742,17 @@ __global__ void k_queue_drain(Wavefront w, DevScene scene, int in, uint32_t* tal
                  g != (split && k != BSDF_ROUGH_GLASS && guided_hit(w, scene, w.hit_geom[e], quirks, s))) {
         bad++;
       }
-      atomicSub(&tally[s], 1u);
+      atomicSub(&tally[e], 1u);
     }
   }
   if (bad) atomicAdd(&w.stats[ST_QERR], (unsigned long long)bad);
 }
 __global__ void k_queue_verify(Wavefront w, uint32_t* tally) {
   uint32_t bad = 0;
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < w.Q; s += gridDim.x * blockDim.x) {
-    if (tally[s] != 0u) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.Q; i += gridDim.x * blockDim.x) {
+    if (tally[i] != 0u) {
       bad++;
-      tally[s] = 0u;
+      tally[i] = 0u;
     }
   }
   if (bad) atomicAdd(&w.stats[ST_QERR], (unsigned long long)bad);
@@ -895,7 +916,7 @@ __global__ void k_bsdf_sample(BsdfRec b, bool quirks, const float* in, uint32_t 
 static inline unsigned Blocks(uint64_t n, int t) { return unsigned((n + t - 1) / t); }
 
 void LaunchInitPass(const Wavefront& w, bool first, hipStream_t s) {
-  uint32_t n = first ? std::max(w.Q, w.npix) : w.Q;
+  uint32_t n = first ? std::max(2 * w.Q, w.npix) : 2 * w.Q;
   hipLaunchKernelGGL(k_init_pass, dim3(Blocks(n, 256)), dim3(256), 0, s, w, first ? 1 : 0);
 }
 void LaunchRegen(const Wavefront& w, const DevScene& scene, int in, bool fresh, const LaunchCfg& c, hipStream_t s) {

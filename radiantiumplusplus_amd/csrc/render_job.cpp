@@ -149,16 +149,16 @@ JobShape Shape(const rad_scene* scene, const rad_render_desc& desc) {
   j.Q = uint32_t(Q);
   j.guided = desc.guiding != 0;
   j.resume_cap = desc.integrator == RAD_INTEGRATOR_PATH && VisitBudget() > 0 ? uint32_t(std::min<uint64_t>(Q, kResumeCap)) : 0;
-  // per slot: rays of the two extend queues (32 B each), path record (64 B),
-  // hit, shadow origin / direction / contribution (16 B each); key, medium,
-  // hit geometry and queues (4 B)
-  j.bytes = Align(Q * 32) * 2 + Align(Q * 64) + Align(Q * 16) * 4 + Align(Q * 4) * (6 + kMatQueues) +
+  // per path: rays and path records of the two extend queues (32 B and 64 B
+  // each), hit, shadow origin / direction / contribution (16 B each); keys and
+  // media of the two queues, hit geometry, dead list and material queues (4 B)
+  j.bytes = Align(Q * 32) * 2 + Align(Q * 64) * 2 + Align(Q * 16) * 4 + Align(Q * 4) * (6 + kMatQueues) +
             Align(pass * 16) + Align(uint64_t(j.npix) * 16) * 2 +
             2 * Align(uint64_t(j.npix) * 12) + 4096;
-  if (j.resume_cap) j.bytes += Align(Q * 4) + 2 * Align(uint64_t(j.resume_cap) * sizeof(ResumeState));
-  if (desc.flags & RAD_RENDER_CHECK_QUEUES) j.bytes += Align(Q * 4);
+  if (j.resume_cap) j.bytes += Align(Q * 4) * 2 + 2 * Align(uint64_t(j.resume_cap) * sizeof(ResumeState));
+  if (desc.flags & RAD_RENDER_CHECK_QUEUES) j.bytes += Align(Q * 4) * 2;
   if (j.guided)
-    j.bytes += Align(Q * 4) * 3 + Align(Q * 2 * guide::kOut) +
+    j.bytes += Align(Q * 4) * 5 + Align(Q * 2 * guide::kOut) +
                Align(uint64_t(kRecordCap) * sizeof(guide::TrainRecord)) + Align(kTrainBatch * 4) + 256 +
                Align(guide::kWeights * 2) + Align(guide::kBiases * 4);
   return j;
@@ -452,7 +452,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   const size_t f4 = size_t(Q) * 16, u4 = size_t(Q) * 4;
   w.ray[0] = (float4*)take(2 * f4);
   w.ray[1] = (float4*)take(2 * f4);
-  float4* rec = (float4*)take(4 * f4);
+  float4* rec = (float4*)take(8 * f4);  // 2 Q records
   w.thr.p = rec;
   w.lsum.p = rec + 1;
   w.prev.p = rec + 2;
@@ -461,11 +461,9 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.sh_o = (float4*)take(f4);
   w.sh_d = (float4*)take(f4);
   w.sh_c = (float4*)take(f4);
-  w.key = (uint32_t*)take(u4);
-  w.medium = (uint32_t*)take(u4);
+  w.key = (uint32_t*)take(2 * u4);
+  w.medium = (uint32_t*)take(2 * u4);
   w.hit_geom = (uint32_t*)take(u4);
-  w.queue[0] = (uint32_t*)take(u4);
-  w.queue[1] = (uint32_t*)take(u4);
   w.dead = (uint32_t*)take(u4);
   for (int k = 0; k < kMatQueues; k++) w.mq[k] = (uint32_t*)take(u4);
   w.res = (float4*)take(size_t(js.rows) * npix * 16);
@@ -474,7 +472,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.cnt = (uint32_t*)take(CNT_WORDS * 4 + ST_WORDS * 8);
   w.stats = reinterpret_cast<unsigned long long*>(w.cnt + CNT_WORDS);
   if (js.resume_cap) {
-    w.susp = (uint32_t*)take(u4);
+    w.susp = (uint32_t*)take(2 * u4);
     w.resume[0] = (ResumeState*)take(size_t(js.resume_cap) * sizeof(ResumeState));
     w.resume[1] = (ResumeState*)take(size_t(js.resume_cap) * sizeof(ResumeState));
     w.resume_cap = js.resume_cap;
@@ -482,13 +480,14 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   }
   if (desc.flags & RAD_RENDER_CHECK_QUEUES) {
     job->tally = (uint32_t*)take(u4);
+    w.passsrc = (uint32_t*)take(u4);
     job->tally_selftest = (desc.flags & RAD_RENDER_CHECK_SELFTEST) != 0;
     HipCheck(hipMemset(job->tally, 0, u4), "queue check tally");
   }
   if (js.guided) {
     w.gq = (uint32_t*)take(u4);
-    w.gpend = (uint32_t*)take(u4);
-    w.glast = (uint32_t*)take(u4);
+    w.gpend = (uint32_t*)take(2 * u4);
+    w.glast = (uint32_t*)take(2 * u4);
     w.graw = (_Float16*)take(size_t(Q) * 2 * guide::kOut);
     w.recs = (guide::TrainRecord*)take(size_t(kRecordCap) * sizeof(guide::TrainRecord));
     job->batch = (uint32_t*)take(kTrainBatch * 4);

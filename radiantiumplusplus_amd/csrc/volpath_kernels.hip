@@ -52,11 +52,16 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     bool alive = false, emit = false, msample = false, done = false;
     uint32_t s = 0;
     float4 so = make_float4(0, 0, 0, 0), sdir = make_float4(0, 0, 0, 0), sc = make_float4(0, 0, 0, 0);
-    float4 nr0 = make_float4(0, 0, 0, 0), nr1 = make_float4(0, 0, 0, 0);  // the continuing ray
+    // the continuing path's ray and state, written at its next-queue position
+    float4 nr0 = make_float4(0, 0, 0, 0), nr1 = make_float4(0, 0, 0, 0);
+    float4 nthr = nr0, nlsum = nr0, nprev = nr0;
+    uint4 nmeta = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t nkey = 0, nmed = 0;
     if (i < n) {
-      // material queues hold extend-queue positions, which index hits and rays
+      // material queues hold extend-queue positions, which index rays, hits
+      // and (with the queue) path state
       const uint32_t p = MAT >= 0 ? w.mq[MAT][i] : i;
-      s = w.queue[in][p];
+      s = state_index(w.Q, in, p);
       float4 ro4 = w.ray[in][2 * size_t(p)], rd4 = w.ray[in][2 * size_t(p) + 1];
       f3 ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
       float tmin = ro4.w, tmax = rd4.w;
@@ -247,18 +252,20 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
       if (alive) {
         nr0 = make_float4(ro.x, ro.y, ro.z, tmin);
         nr1 = make_float4(rd.x, rd.y, rd.z, tmax);
-        w.thr[s] = make_float4(T.x, T.y, T.z, eta);
-        w.prev[s] = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
-        w.meta[s] = make_uint4(m.x, m.y, next_depth, dim);
-        w.medium[s] = uint32_t(medium + 1) | (ch << 24);
+        nthr = make_float4(T.x, T.y, T.z, eta);
+        nlsum = make_float4(L.x, L.y, L.z, prevPdf);
+        nprev = make_float4(prevP.x, prevP.y, prevP.z, __uint_as_float(spec ? PATH_SPECULAR : 0u));
+        nmeta = make_uint4(m.x, m.y, next_depth, dim);
+        nmed = uint32_t(medium + 1) | (ch << 24);
+        nkey = key;
       }
       // a sample that ends here with no shadow ray still to add to it is
       // final: its result entry is written now, and regen reads nothing for it
       if (!alive && !emit) {
         w.res[size_t(m.y - w.row0) * w.npix + m.x] = make_float4(L.x, L.y, L.z, 0.f);
         done = true;
-      } else {
-        w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);
+      } else if (!alive) {
+        w.lsum[s] = make_float4(L.x, L.y, L.z, prevPdf);  // in place: the shadow ray and regen finish it
       }
     }
     bool valid = i < n;
@@ -267,9 +274,16 @@ __global__ void __launch_bounds__(256) k_shade_vol(Wavefront w, DevScene scene, 
     uint32_t r3[3];
     block_append<3>(c3, p3, r3, lds);
     if (valid && alive) {
-      w.queue[out][r3[0]] = s;
+      const uint32_t d = state_index(w.Q, out, r3[0]);
       w.ray[out][2 * size_t(r3[0])] = nr0;
       w.ray[out][2 * size_t(r3[0]) + 1] = nr1;
+      w.thr[d] = nthr;
+      w.lsum[d] = nlsum;
+      w.prev[d] = nprev;
+      w.meta[d] = nmeta;
+      w.medium[d] = nmed;
+      w.key[d] = nkey;
+      sdir.w = __uint_as_float(d);  // this vertex's shadow ray adds to the moved radiance
     }
     if (valid && !alive) w.dead[r3[1]] = s | (done ? kDeadDone : 0u);
     if (emit) {

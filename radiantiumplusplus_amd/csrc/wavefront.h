@@ -1,7 +1,13 @@
 // Wavefront state of a render job: structure-of-arrays path records in HBM,
 // compacted work queues and the per-sample result entries.
 //
-// A job owns Q path slots that are not tied to pixels.  Camera samples are
+// A job has room for Q paths in flight, not tied to pixels.  A path's state
+// lives at its position in the extend queue it is in: queue q (the two
+// alternate by iteration) holds positions [0, cnt[q]), and every per-path
+// array holds 2 Q entries, state index q Q + position (rays and hits are
+// per queue, by position).  Whoever appends a path to the next queue moves
+// its state there, so every kernel reads and writes path state in queue
+// order, never scattered by a slot number.  Camera samples are
 // handed out from a global counter in sample-major order (id -> pixel
 // id % pixels, sample row0 + id / pixels), so every slot stays busy until
 // the pass runs out of samples and the primary rays of a wave are adjacent
@@ -53,6 +59,9 @@ constexpr uint32_t kNoSusp = 0xffffffffu;
 constexpr uint32_t kDeadDone = 0x80000000u;
 constexpr int kMatQueues = 5;
 
+// State index of position i of extend queue q.
+__device__ __forceinline__ uint32_t state_index(uint32_t Q, int q, uint32_t i) { return uint32_t(q) * Q + i; }
+
 // Row of entry j of material k's guided queue in Wavefront::graw: the
 // materials' guided vertices one after another, in material order.
 __device__ __forceinline__ uint32_t guided_row(const uint32_t* cnt, int k, uint32_t j) {
@@ -76,20 +85,19 @@ struct Wavefront {
   // and shading write the rays they append in block-contiguous runs, and
   // extend, shading and inference read them in queue order.
   float4* ray[2];
-  // the path record of slot s: four fields in one 64-byte record
+  // the path record at state index s: four fields in one 64-byte record
   SlotField<float4> thr;   // throughput, eta
   SlotField<float4> lsum;  // radiance of the sample in flight, previous BSDF pdf
   SlotField<float4> prev;  // previous vertex position, path flags (bits)
   SlotField<uint4> meta;   // pixel, sample index (SAMPLE_NONE = idle), depth, rng dimension
-  uint32_t* key;   // rng key of the sample in flight
-  uint32_t* medium;  // vol_path: MED_UNSET, or current medium + 1 (0 = none) | channel << 24
+  uint32_t* key;   // rng key of the sample in flight (by state index)
+  uint32_t* medium;  // vol_path, by state index: MED_UNSET, or current medium + 1 (0 = none) | channel << 24
   // hit of the ray at position i of the extend queue (not slot-indexed, so
   // extend writes and classify reads consecutive entries)
   float4* hit;     // t, u, v, prim (bits)
   uint32_t* hit_geom;
-  uint32_t* queue[2];
   // Traversals cut off at visit_budget node visits (path jobs; null = never):
-  // per slot the resume-pool index, or kNoSusp.  Iteration `in` resumes from
+  // by state index the resume-pool index, or kNoSusp.  Iteration `in` resumes from
   // resume[in] and suspends into resume[1 - in] (count in CNT_SUSP + 1 - in);
   // a suspended slot skips shading and goes straight to the next extend queue.
   uint32_t* susp;
@@ -100,7 +108,8 @@ struct Wavefront {
   // guided vertices is mq[k][Q - 1 - j] (count CNT_GMAT + k): both parts
   // together never exceed the Q slots.
   uint32_t* mq[5];
-  uint32_t* dead;  // slots whose sample ended this iteration (| kDeadDone: result already written)
+  uint32_t* dead;  // state indices of the samples that ended this iteration (| kDeadDone: result already written)
+  uint32_t* passsrc;  // queue check only: extend-queue position each passed-through entry came from
   float4* sh_o;    // shadow origin, tmax
   float4* sh_d;    // shadow direction, slot (bits)
   float4* sh_c;    // shadow contribution
@@ -115,8 +124,8 @@ struct Wavefront {
   // guided_row(cnt, k, j) for entry j of material k's guided queue, so
   // inference writes and shading reads consecutive rows; unsorted: row = slot.
   _Float16* graw;
-  uint32_t* gpend;   // per slot: training record still waiting for its radiance baseline
-  uint32_t* glast;   // per slot: newest training record of the path in flight
+  uint32_t* gpend;   // by state index: training record still waiting for its radiance baseline
+  uint32_t* glast;   // by state index: newest training record of the path in flight
   guide::TrainRecord* recs;
   uint32_t* gcnt;    // G_* words (G_WSUM holds float bits)
   uint32_t rec_cap, rec_stride;
