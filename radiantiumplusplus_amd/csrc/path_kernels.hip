@@ -10,9 +10,9 @@
 
 namespace rad {
 
-// Start of a pass: every slot on the dead list, so the first regen hands
-// each one a sample (that regen is told that no slot holds a finished
-// sample, so the path records need no clearing here).
+// Start of a pass: Q entries on the dead list, so the first regen starts Q
+// samples (that regen is told that no entry holds a finished sample, so the
+// path records need no clearing here).
 __global__ void k_init_pass(Wavefront w, int first) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (first && s < w.npix) {
@@ -72,7 +72,7 @@ __device__ __forceinline__ void finish_records(const Wavefront& w, uint32_t s, f
 // covers them all.
 constexpr int kRegenPerThread = 4;
 
-// fresh: the first regen of a pass, whose dead slots hold no finished sample.
+// fresh: the first regen of a pass, whose dead-list entries hold no finished sample.
 __global__ void __launch_bounds__(256) k_regen(Wavefront w, DevScene scene, int in, int fresh) {
   constexpr int E = kRegenPerThread;
   __shared__ uint32_t lds[32];
@@ -238,7 +238,7 @@ __device__ __forceinline__ bool guided_vertex(const Wavefront& w, const DevScene
 // at guided vertices the direction is drawn from the learned vMF mixture
 // (probability alpha) or the BSDF, and weighted by the one-sample MIS
 // density alpha q + (1 - alpha) p_bsdf; NEE uses the same density in its MIS
-// weight; training records of the vertices are kept for 1 in rec_stride slots.
+// weight; training records are kept for the vertices at 1 in rec_stride state indices.
 // Sorted (MAT >= 0), the guided kernels shade the material's guided vertices
 // and the plain ones the rest, so unguided vertices never pay for the
 // mixture's registers and LDS; unsorted (MAT < 0), one kernel shades all.
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(256) k_shade_path(Wavefront w, DevScene scene,
       }
       // A sample that ends here with no shadow ray still to add to it (and no
       // training records to close) is final: its result entry is written now,
-      // and regen reads nothing for the slot.  Of an ended sample regen reads
+      // and regen reads nothing for the path.  Of an ended sample regen reads
       // only the radiance and the pixel and sample, which stay as they are.
       if (alive) {
         nthr = make_float4(T.x, T.y, T.z, eta);
@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(256) k_guide_prep(Wavefront w, DevScene scene,
 }
 
 // Network inference for the guided vertices: the inputs (hit position,
-// geometric normal, outgoing direction) are formed from the slot's ray and
+// geometric normal, outgoing direction) are formed from the vertex's ray and
 // hit, and the 40 used raw outputs go to the vertex's graw row.  SORTED: the
 // vertices are the guided material queues' entries in row order (entry i of
 // the concatenation is row i); otherwise the guide queue, row = position.
@@ -625,7 +625,7 @@ __device__ __forceinline__ int material_key(const DevScene& scene, uint32_t geom
 }
 
 // Each thread classifies kClassifyPerThread entries per block iteration
-// (entries base + e * 256 + thread): their slot gathers are in flight
+// (entries base + e * 256 + thread): their gathers are in flight
 // together and one set of block atomics covers them all.
 constexpr int kClassifyPerThread = 4;
 

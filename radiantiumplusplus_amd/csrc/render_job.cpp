@@ -45,7 +45,7 @@ struct rad_job {
   bool sorted = false;        // material-sorted shading
   uint32_t blocks_per_cu = 0, trace_blocks_per_cu = 0;
   uint32_t mat_mask = 0;      // BSDF types present in the scene
-  uint32_t* tally = nullptr;  // RAD_RENDER_CHECK_QUEUES: per-slot tally words (inside the arena)
+  uint32_t* tally = nullptr;  // RAD_RENDER_CHECK_QUEUES: tally words per extend-queue position (inside the arena)
   bool tally_selftest = false;
   bool joined = false;
 };
@@ -57,7 +57,7 @@ namespace {
 constexpr uint64_t kDefaultSlots = 4u << 20;     // paths in flight
 constexpr uint64_t kResultBudget = 3ull << 30;   // bytes of per-sample results per pass
 constexpr uint32_t kRecordCap = 2u << 20;        // guide training records per pass
-constexpr uint32_t kRecordSlots = 512u << 10;    // slots that record training data
+constexpr uint32_t kRecordSlots = 512u << 10;    // 1 in Q / kRecordSlots state indices record training data
 constexpr uint32_t kTrainBatch = 8192;           // records per training step
 constexpr uint32_t kTrainSteps = 96;             // training steps after each training pass
 constexpr float kTrainLr = 4e-3f;
@@ -269,7 +269,7 @@ void RunJob(rad_job* job) {
       job->guide_alpha = (w.flags & WF_GUIDE) ? w.net.alpha : 0.f;
       LaunchInitPass(w, pi == 0, st);
       int in = 0;
-      bool fresh = true;  // the pass's first regen: no slot holds a finished sample
+      bool fresh = true;  // the pass's first regen: no dead-list entry holds a finished sample
       for (;;) {
         if (prof) HipCheck(hipEventRecord(ev[0], st), "event");
         LaunchRegen(w, scene, in, fresh, cfg, st);
@@ -433,7 +433,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   const uint32_t npix = js.npix, Q = js.Q;
   job->total = uint64_t(npix) * desc.spp;
   PlanPasses(job.get(), js.npix, js.rows);
-  // one arena for every per-slot, per-sample and per-pixel array
+  // one arena for every per-path, per-sample and per-pixel array
   if (reuse && reuse->bytes >= js.bytes) {
     job->arena = std::move(reuse);
   } else {

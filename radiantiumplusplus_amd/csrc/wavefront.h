@@ -9,13 +9,13 @@
 // its state there, so every kernel reads and writes path state in queue
 // order, never scattered by a slot number.  Camera samples are
 // handed out from a global counter in sample-major order (id -> pixel
-// id % pixels, sample row0 + id / pixels), so every slot stays busy until
+// id % pixels, sample row0 + id / pixels), so the queue stays full until
 // the pass runs out of samples and the primary rays of a wave are adjacent
 // pixels.  A finished sample's radiance goes to its own 16-byte result
 // entry; after the pass, k_accum adds the entries of each pixel in sample
 // order (the order of sample_renderer.cpp:97-105), so the image does not
-// depend on which slot ran which sample.  Each iteration runs
-//   regen   (finished samples -> result entries, dead slots -> next sample)
+// depend on which path ran which sample.  Each iteration runs
+//   regen   (finished samples -> result entries, one new sample for each)
 //   extend  (closest hit for every queued path)
 //   shade   (emission + MIS, NEE shadow-ray emission, BSDF sampling, RR)
 //   shadow  (occlusion test, add unoccluded NEE contributions)
@@ -54,7 +54,7 @@ enum : uint32_t { MED_UNSET = 0xffffffffu };
 // Disney) have their own; kMatOther takes every other type.
 constexpr int kMatOther = 4;
 constexpr uint32_t kNoSusp = 0xffffffffu;
-// Dead-list flag: the shading kernel already wrote this slot's finished
+// Dead-list flag: the shading kernel already wrote this path's finished
 // sample to the result entries, so regen reads nothing for it.
 constexpr uint32_t kDeadDone = 0x80000000u;
 constexpr int kMatQueues = 5;
@@ -70,8 +70,8 @@ __device__ __forceinline__ uint32_t guided_row(const uint32_t* cnt, int k, uint3
   return r;
 }
 
-// Field k of a 64-byte per-slot record of four 16-byte fields: element s is
-// p[4 s], so the four fields of a slot share one memory sector.
+// Field k of a 64-byte per-path record of four 16-byte fields: element s is
+// p[4 s], so the four fields of a path share one memory sector.
 template <class T>
 struct SlotField {
   T* p;
@@ -99,19 +99,19 @@ struct Wavefront {
   // Traversals cut off at visit_budget node visits (path jobs; null = never):
   // by state index the resume-pool index, or kNoSusp.  Iteration `in` resumes from
   // resume[in] and suspends into resume[1 - in] (count in CNT_SUSP + 1 - in);
-  // a suspended slot skips shading and goes straight to the next extend queue.
+  // a suspended path skips shading and goes straight to the next extend queue.
   uint32_t* susp;
   ResumeState* resume[2];
   uint32_t resume_cap, visit_budget;
   // The extend queue partitioned by material (k_classify), as extend-queue
   // positions; [4]: every other type.  With guiding or training records on, entry j of a material's
   // guided vertices is mq[k][Q - 1 - j] (count CNT_GMAT + k): both parts
-  // together never exceed the Q slots.
+  // together never exceed the Q paths.
   uint32_t* mq[5];
   uint32_t* dead;  // state indices of the samples that ended this iteration (| kDeadDone: result already written)
   uint32_t* passsrc;  // queue check only: extend-queue position each passed-through entry came from
   float4* sh_o;    // shadow origin, tmax
-  float4* sh_d;    // shadow direction, slot (bits)
+  float4* sh_d;    // shadow direction, state index of the path it adds to (bits)
   float4* sh_c;    // shadow contribution
   float4* res;     // result entries: RGB (+ padding) of rows * npix samples
   float4* pacc;    // per-pixel radiance sum
@@ -122,7 +122,8 @@ struct Wavefront {
   uint32_t* gq;      // unsorted shading: extend-queue positions of this iteration's guided vertices
   // 48 raw network outputs (40 used) per guided vertex.  Sorted shading: row
   // guided_row(cnt, k, j) for entry j of material k's guided queue, so
-  // inference writes and shading reads consecutive rows; unsorted: row = slot.
+  // inference writes and shading reads consecutive rows; unsorted: row =
+  // extend-queue position.
   _Float16* graw;
   uint32_t* gpend;   // by state index: training record still waiting for its radiance baseline
   uint32_t* glast;   // by state index: newest training record of the path in flight
