@@ -68,6 +68,11 @@ constexpr float kGuideMinRough = 0.f;            // glossy BSDFs below this alph
 constexpr uint32_t kGuideMaxDepth = 2;          // deepest guided vertex (bounces before it)
 constexpr uint32_t kVisitBudget = 1024;          // node visits of a path ray per extend launch
 constexpr uint32_t kResumeCap = 16384;           // suspended traversals per extend launch
+constexpr int kIterBatch = 4;                    // wavefront iterations enqueued per host wait
+// pinned readback words: counters + stats + guide words, then one counter and
+// stats area per iteration of a batch
+constexpr uint32_t kIterWords = CNT_WORDS + 2 * ST_WORDS;
+constexpr uint32_t kIterReadback = CNT_WORDS + 2 * ST_WORDS + 2 * G_WORDS;
 
 // Training hyper-parameters; RAD_GUIDE_STEPS / RAD_GUIDE_LR / RAD_GUIDE_ALPHA
 // in the environment override the defaults above.
@@ -241,9 +246,10 @@ void RunJob(rad_job* job) {
     const int inferBlocks = std::max(64, job->scene->ctx->cu_count * 2);  // two 512-thread blocks per CU
     const bool prof = (job->desc.flags & RAD_RENDER_PROFILE) != 0;
     const size_t readback = CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t) + G_WORDS * 4;
-    hipEvent_t ev[7] = {};
+    hipEvent_t evs[kIterBatch][7] = {};  // stage boundaries of each iteration of a batch
     if (prof)
-      for (auto& e : ev) HipCheck(hipEventCreate(&e), "hipEventCreate");
+      for (auto& row : evs)
+        for (auto& e : row) HipCheck(hipEventCreate(&e), "hipEventCreate");
     auto t0 = std::chrono::steady_clock::now();
     const uint32_t baseFlags = w.flags & WF_QUIRKS;
     const float* lo = job->scene->host.world_lo;
@@ -270,75 +276,92 @@ void RunJob(rad_job* job) {
       LaunchInitPass(w, pi == 0, st);
       int in = 0;
       bool fresh = true;  // the pass's first regen: no dead-list entry holds a finished sample
-      for (;;) {
-        if (prof) HipCheck(hipEventRecord(ev[0], st), "event");
-        LaunchRegen(w, scene, in, fresh, cfg, st);
-        fresh = false;
-        LaunchReset(w, 1 - in, st);
-        if (prof) HipCheck(hipEventRecord(ev[1], st), "event");
-        LaunchExtend(w, scene, in, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[2], st), "event");
-        const bool vol = w.integrator == RAD_INTEGRATOR_VOL_PATH;
-        const bool guided = (w.flags & WF_GUIDE) != 0;
-        if (job->sorted) {
-          LaunchClassify(w, scene, in, cfg, st);  // with guiding or records, also the guided queues
-          if (job->tally) LaunchQueueCheck(w, scene, in, job->tally, job->tally_selftest, cfg, st);
-        }
-        else if (!vol && guided)
-          LaunchGuidePrep(w, scene, in, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
-        if (!vol && guided) LaunchGuideInfer(w, scene, in, inferBlocks, job->sorted, st);
-        if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
-        if (vol && job->sorted)
-          LaunchShadeVolMaterials(w, scene, in, job->mat_mask, cfg, st);
-        else if (vol)
-          LaunchShadeVol(w, scene, in, cfg, st);
-        else if (job->sorted)
-          LaunchShadeMaterials(w, scene, in, job->mat_mask, cfg, st);
-        else
-          LaunchShadePath(w, scene, in, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[5], st), "event");
-        if (vol)
-          LaunchVolShadow(w, scene, cfg, st);
-        else
-          LaunchShadow(w, scene, cfg, st);
-        if (prof) HipCheck(hipEventRecord(ev[6], st), "event");
-        HipCheck(hipGetLastError(), "wavefront launch");
-        HipCheck(hipMemcpyAsync(job->h_cnt, w.cnt, CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t),
-                                hipMemcpyDeviceToHost, st),
-                 "counter readback");
-        if (w.gcnt)
-          HipCheck(hipMemcpyAsync(job->h_cnt + CNT_WORDS + 2 * ST_WORDS, w.gcnt, G_WORDS * 4, hipMemcpyDeviceToHost, st),
-                   "guide counter readback");
-        HipCheck(hipStreamSynchronize(st), "wavefront iteration");
-        job->iterations++;
-        if (prof) {
-          // event intervals -> stages; classify (or guide-queue gathering) counts as shading
-          const int slot[6] = {RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_SHADE, RAD_KERNEL_GUIDE, RAD_KERNEL_SHADE,
-                               RAD_KERNEL_SHADOW};
-          for (int k = 0; k < 6; k++) {
-            if (slot[k] == RAD_KERNEL_GUIDE && !guided) continue;
-            float ms = 0.f;
-            HipCheck(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "event time");
-            job->kernel_ms[slot[k]] += ms;
-            if (k != 2) job->kernel_launches[slot[k]]++;
+      for (bool passDone = false; !passDone;) {
+        // Iterations are enqueued kIterBatch at a time and the host waits once
+        // per batch: every kernel takes its work count from device counters,
+        // so an iteration enqueued after the pass ran out of work does nothing.
+        // Each iteration's counters go to their own readback area.
+        const int batch = kIterBatch;
+        int ins[kIterBatch];
+        for (int b = 0; b < batch; b++) {
+          ins[b] = in;
+          uint32_t* const hc = job->h_cnt + kIterReadback + b * kIterWords;
+          hipEvent_t* const ev = evs[b];
+          if (prof) HipCheck(hipEventRecord(ev[0], st), "event");
+          LaunchRegen(w, scene, in, fresh, cfg, st);
+          fresh = false;
+          LaunchReset(w, 1 - in, st);
+          if (prof) HipCheck(hipEventRecord(ev[1], st), "event");
+          LaunchExtend(w, scene, in, cfg, st);
+          if (prof) HipCheck(hipEventRecord(ev[2], st), "event");
+          const bool vol = w.integrator == RAD_INTEGRATOR_VOL_PATH;
+          const bool guided = (w.flags & WF_GUIDE) != 0;
+          if (job->sorted) {
+            LaunchClassify(w, scene, in, cfg, st);  // with guiding or records, also the guided queues
+            if (job->tally) LaunchQueueCheck(w, scene, in, job->tally, job->tally_selftest, cfg, st);
           }
+          else if (!vol && guided)
+            LaunchGuidePrep(w, scene, in, cfg, st);
+          if (prof) HipCheck(hipEventRecord(ev[3], st), "event");
+          if (!vol && guided) LaunchGuideInfer(w, scene, in, inferBlocks, job->sorted, st);
+          if (prof) HipCheck(hipEventRecord(ev[4], st), "event");
+          if (vol && job->sorted)
+            LaunchShadeVolMaterials(w, scene, in, job->mat_mask, cfg, st);
+          else if (vol)
+            LaunchShadeVol(w, scene, in, cfg, st);
+          else if (job->sorted)
+            LaunchShadeMaterials(w, scene, in, job->mat_mask, cfg, st);
+          else
+            LaunchShadePath(w, scene, in, cfg, st);
+          if (prof) HipCheck(hipEventRecord(ev[5], st), "event");
+          if (vol)
+            LaunchVolShadow(w, scene, cfg, st);
+          else
+            LaunchShadow(w, scene, cfg, st);
+          if (prof) HipCheck(hipEventRecord(ev[6], st), "event");
+          HipCheck(hipGetLastError(), "wavefront launch");
+          HipCheck(hipMemcpyAsync(hc, w.cnt, CNT_WORDS * sizeof(uint32_t) + ST_WORDS * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, st),
+                   "counter readback");
+          if (w.gcnt)
+            HipCheck(hipMemcpyAsync(job->h_cnt + CNT_WORDS + 2 * ST_WORDS, w.gcnt, G_WORDS * 4, hipMemcpyDeviceToHost, st),
+                     "guide counter readback");
+          in = 1 - in;
         }
-        const uint64_t* hs = reinterpret_cast<const uint64_t*>(job->h_cnt + CNT_WORDS);
-        job->done.store(hs[ST_SAMPLES]);
-        // a suspended traversal counts once, in the launch that finishes it
-        job->segments += job->h_cnt[in] - std::min(job->h_cnt[CNT_SUSP + 1 - in], w.resume_cap);
-        job->suspended += std::min(job->h_cnt[CNT_SUSP + 1 - in], w.resume_cap);
-        job->shadow_rays += job->h_cnt[CNT_SHADOW];
-        uint32_t cont = job->h_cnt[1 - in], dead = job->h_cnt[CNT_DEAD];
-        if (cont == 0 && dead == 0) break;
+        HipCheck(hipStreamSynchronize(st), "wavefront iterations");
+        for (int b = 0; b < batch && !passDone; b++) {
+          const uint32_t* const hc = job->h_cnt + kIterReadback + b * kIterWords;
+          const int in = ins[b];
+          const bool guided = (w.flags & WF_GUIDE) != 0;
+          const hipEvent_t* const ev = evs[b];
+          job->iterations++;
+          if (prof) {
+            // event intervals -> stages; classify (or guide-queue gathering) counts as shading
+            const int slot[6] = {RAD_KERNEL_REGEN, RAD_KERNEL_EXTEND, RAD_KERNEL_SHADE, RAD_KERNEL_GUIDE, RAD_KERNEL_SHADE,
+                                 RAD_KERNEL_SHADOW};
+            for (int k = 0; k < 6; k++) {
+              if (slot[k] == RAD_KERNEL_GUIDE && !guided) continue;
+              float ms = 0.f;
+              HipCheck(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "event time");
+              job->kernel_ms[slot[k]] += ms;
+              if (k != 2) job->kernel_launches[slot[k]]++;
+            }
+          }
+          const uint64_t* hs = reinterpret_cast<const uint64_t*>(hc + CNT_WORDS);
+          job->done.store(hs[ST_SAMPLES]);
+          // a suspended traversal counts once, in the launch that finishes it
+          job->segments += hc[in] - std::min(hc[CNT_SUSP + 1 - in], w.resume_cap);
+          job->suspended += std::min(hc[CNT_SUSP + 1 - in], w.resume_cap);
+          job->shadow_rays += hc[CNT_SHADOW];
+          const uint32_t cont = hc[1 - in], dead = hc[CNT_DEAD];
+          if (cont == 0 && dead == 0) passDone = true;
+        }
         if (job->iterations > 1000000u)
           throw StatusError(RAD_ERR_STATE, "path lengths exceed one million bounces (no termination criterion)");
-        if (job->cancel.load()) {
+        if (!passDone && job->cancel.load()) {
           job->status = RAD_ERR_CANCELLED;
           break;
         }
-        in = 1 - in;
       }
       if (job->status != RAD_OK) break;
       LaunchAccum(w, st);
@@ -364,7 +387,8 @@ void RunJob(rad_job* job) {
     const uint64_t* hs = reinterpret_cast<const uint64_t*>(job->h_cnt + CNT_WORDS);
     for (int i = 0; i < ST_WORDS; i++) job->stats[i] = hs[i];
     if (prof)
-      for (auto& e : ev) (void)hipEventDestroy(e);
+      for (auto& row : evs)
+        for (auto& e : row) (void)hipEventDestroy(e);
   } catch (const StatusError& e) {
     job->status = e.code;
     job->error = e.what();
@@ -524,7 +548,7 @@ rad_job* StartJob(rad_scene* scene, const rad_render_desc& desc, std::unique_ptr
   w.integrator = desc.integrator;
   job->sorted = !(desc.flags & RAD_RENDER_UNSORTED);
   for (const BsdfRec& b : scene->host.bsdfs) job->mat_mask |= 1u << (b.tex_any ? kMatOther : std::min<int32_t>(b.type, kMatOther));
-  HipCheck(hipHostMalloc((void**)&job->h_cnt, CNT_WORDS * 4 + ST_WORDS * 8 + 2 * G_WORDS * 4, hipHostMallocDefault),
+  HipCheck(hipHostMalloc((void**)&job->h_cnt, (kIterReadback + kIterBatch * kIterWords) * 4, hipHostMallocDefault),
            "hipHostMalloc");
   HipCheck(hipStreamCreateWithFlags(&job->stream, hipStreamNonBlocking), "hipStreamCreate");
   rad_job* raw = job.release();
