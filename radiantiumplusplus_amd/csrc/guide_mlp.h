@@ -1,5 +1,6 @@
-// The guiding network's forward pass on FP16 MFMA, shared by the inference
-// kernels (path vertices, probe inputs) and the training kernel.
+// The guiding network's forward pass on FP16 MFMA: forward_tile for the
+// training kernel (which keeps every layer's activations for the backward
+// pass), infer_tile for the inference kernels (path vertices, probe inputs).
 //
 // Every layer is computed transposed, H_{l+1}^T = W_l H_l^T, with
 // v_mfma_f32_16x16x16_f16 over 16 vertices per wave: the accumulator layout
@@ -60,11 +61,11 @@ __device__ __forceinline__ h4 weight_operand(const _Float16* sW, int l, int t, i
 
 __device__ __forceinline__ int bias_base(int l) { return l * kHid; }
 
-// Forward pass of one 16-vertex tile.  fs: the lane's vertex scalars.
-// act[l][t]: post-ReLU outputs of hidden layer l (l = 0..3), tile t, in
-// B-operand form; out[t]: the 48 raw outputs (3 tiles).  When xs != nullptr
-// the layer-0 B operands (the one-blob inputs) are stored there as
-// xs[chunk][lane].  FRAG: sW holds the fragment-major copy.
+// Forward pass of one 16-vertex tile for training.  fs: the lane's vertex
+// scalars.  act[l][t]: post-ReLU outputs of hidden layer l (l = 0..3), tile
+// t, in B-operand form; out[t]: the 48 raw outputs (3 tiles).  When
+// xs != nullptr the layer-0 B operands (the one-blob inputs) are stored
+// there as xs[chunk][lane].  FRAG: sW holds the fragment-major copy.
 template <bool FRAG>
 __device__ __forceinline__ void forward_tile(const _Float16* sW, const float* sB, const FeatScalars& fs, h4 act[4][4],
                                              f4v out[3], h4* xs) {
