@@ -208,6 +208,26 @@ def test_guided_queues_are_a_permutation(hip_ctx):
     assert np.array_equal(img, ref)
 
 
+def test_unsorted_guided_render_matches_sorted(hip_ctx):
+    """The single-kernel guided path (guide queue of extend-queue positions,
+    outputs in rows by position, one shading kernel for every vertex)
+    converges to the material-sorted guided path's image: per-tile z-test at
+    4 sigma, fixed trained weights, different seeds."""
+    b = small_room()
+    s = hip.Scene.from_builder(hip_ctx, b)
+    rd = b.render_desc(spp=256, seed=3)
+    rd.guiding = 1
+    _, _, st = s.render(rd)
+    assert st.guide_steps > 0
+    fa, qa, sa = render(hip_ctx, s, b, 1024, seed=4, guiding=2)
+    fu, qu, su = render(hip_ctx, s, b, 1024, seed=5, guiding=2, flags=F.RAD_RENDER_UNSORTED)
+    assert sa.guide_alpha > 0 and su.guide_alpha > 0
+    z, rel = stats.compare(fa, qa, 1024, fu, qu, 1024, tile=8)
+    print(f"max |z| {np.abs(z).max():.2f}  relMSE {rel:.2e}")
+    assert np.abs(z).max() < 4.0
+    assert rel < 2e-3
+
+
 def test_trained_guide_reduces_error(hip_ctx):
     """At equal spp the trained guide's error against a converged unguided
     reference is well below that of BSDF sampling alone (quirks off, so the
